@@ -1,0 +1,199 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so built from gravmag_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
+bench.py's CPU-baseline leg.  The product package jif3d_amd never imports it.
+
+Each wrapper restates one reference entry point (citations are to the
+mmoorkamp/jif3D tree):
+
+* grav_box_term     -- CalcGravBoxTerm,   jif3D/Gravity/BasicGravElements.cpp:82-109
+* tensor_box_term   -- CalcTensorBoxTerm, jif3D/Gravity/BasicGravElements.cpp:216-244
+* inf_sheet_term    -- CalcInfSheetTerm,  jif3D/Gravity/BasicGravElements.h:51-63
+* calculate         -- ThreeDGravMagImplementation::Calculate (+ the OMP imps),
+                       jif3D/GravMag/ThreeDGravMagImplementation.h:116-168
+* lqderivative      -- ThreeDGravMagImplementation::LQDerivative, :177-231
+* calculate_ld      -- the forward problem in 80-bit long double (truth oracle)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBPATH = os.path.join(_HERE, "liboracle.so")
+
+SCALAR, TENSOR, MAGNETIC, MAGGRAD, MAGNETIZATION = 1, 2, 3, 4, 5
+TF_NONE, TF_FTG_INVARIANT, TF_TOTAL_FIELD, TF_TOTAL_FIELD_ANOMALY = 0, 1, 2, 3
+RAW_PER_SITE = {SCALAR: 1, TENSOR: 9, MAGNETIC: 3, MAGGRAD: 1, MAGNETIZATION: 3}
+
+_lib = None
+
+
+def build():
+    """Compile liboracle.so with the Makefile beside this file."""
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle.so"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIBPATH):
+            build()
+        _lib = ctypes.CDLL(_LIBPATH)
+        d = ctypes.c_double
+        p = ctypes.POINTER(ctypes.c_double)
+        sz = ctypes.c_size_t
+        _lib.orc_grav_box_term.restype = d
+        _lib.orc_grav_box_term.argtypes = [d] * 9
+        _lib.orc_grav_box_term_ld.restype = d
+        _lib.orc_grav_box_term_ld.argtypes = [d] * 9
+        _lib.orc_tensor_box_term.restype = None
+        _lib.orc_tensor_box_term.argtypes = [d] * 9 + [p]
+        _lib.orc_tensor_box_term_ld.restype = None
+        _lib.orc_tensor_box_term_ld.argtypes = [d] * 9 + [p]
+        _lib.orc_inf_sheet_term.restype = d
+        _lib.orc_inf_sheet_term.argtypes = [d] * 3
+        _lib.orc_uzz_inf_sheet_term.restype = d
+        _lib.orc_uzz_inf_sheet_term.argtypes = [d] * 3
+        _lib.orc_num_threads.restype = ctypes.c_int
+        common = [ctypes.c_int, sz, sz, sz, p, p, p, p, p, p, sz, p, p, sz, p, p, p, d, d, d]
+        _lib.orc_calculate.restype = ctypes.c_int
+        _lib.orc_calculate.argtypes = common + [ctypes.c_int, d, d, p, p]
+        _lib.orc_lqderivative.restype = ctypes.c_int
+        _lib.orc_lqderivative.argtypes = common + [ctypes.c_int, d, d, p, p]
+        _lib.orc_calculate_ld.restype = ctypes.c_int
+        _lib.orc_calculate_ld.argtypes = common + [p]
+        _lib.orc_j_times.restype = None
+        _lib.orc_j_times.argtypes = [sz, sz, p, p, p]
+        _lib.orc_jt_times2.restype = None
+        _lib.orc_jt_times2.argtypes = [sz, sz, p, p, p]
+        _lib.orc_transform.restype = None
+        _lib.orc_transform.argtypes = [ctypes.c_int, d, d, p, p]
+        _lib.orc_transform_derivative.restype = None
+        _lib.orc_transform_derivative.argtypes = [ctypes.c_int, d, d, p, p]
+    return _lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def num_threads():
+    return lib().orc_num_threads()
+
+
+def grav_box_term(*args, long_double=False):
+    f = lib().orc_grav_box_term_ld if long_double else lib().orc_grav_box_term
+    return f(*[float(a) for a in args])
+
+
+def tensor_box_term(*args, long_double=False):
+    out = np.zeros(9)
+    f = lib().orc_tensor_box_term_ld if long_double else lib().orc_tensor_box_term
+    f(*[float(a) for a in args], _ptr(out))
+    return out.reshape(3, 3)
+
+
+def inf_sheet_term(measz, top, bottom):
+    return lib().orc_inf_sheet_term(measz, top, bottom)
+
+
+def uzz_inf_sheet_term(measz, top, bottom):
+    return lib().orc_uzz_inf_sheet_term(measz, top, bottom)
+
+
+def _common(kind, mesh, model, sites, field):
+    xc, yc, zc = (_f64(v) for v in mesh.coords)
+    xs, ys, zs = (_f64(v) for v in mesh.sizes)
+    bg = _f64(mesh.bg_thicknesses if len(mesh.bg_thicknesses) else np.zeros(1))
+    model = _f64(model)
+    sx, sy, sz = (_f64(v) for v in sites)
+    inc, dec, fs = field if field is not None else (0.0, 0.0, 0.0)
+    keep = (xc, yc, zc, xs, ys, zs, bg, model, sx, sy, sz)
+    args = [kind, mesh.nx, mesh.ny, mesh.nz, _ptr(xc), _ptr(yc), _ptr(zc), _ptr(xs), _ptr(ys),
+            _ptr(zs), len(mesh.bg_thicknesses), _ptr(bg), _ptr(model), len(sx), _ptr(sx),
+            _ptr(sy), _ptr(sz), float(inc), float(dec), float(fs)]
+    return args, keep
+
+
+def n_params(kind, mesh):
+    n = mesh.nx * mesh.ny * mesh.nz
+    return 3 * n if kind == MAGNETIZATION else n + len(mesh.bg_thicknesses)
+
+
+def calculate(kind, mesh, model, sites, field=None, transform=TF_NONE, tf_angles=(0.0, 0.0),
+              want_sens=False):
+    """Forward response (nsites*k) and optionally the raw sensitivity matrix."""
+    args, keep = _common(kind, mesh, model, sites, field)
+    ns = len(keep[8])
+    kraw = RAW_PER_SITE[kind]
+    kout = kraw if transform == TF_NONE else 1
+    out = np.zeros(ns * kout)
+    J = np.zeros((ns * kraw, n_params(kind, mesh))) if want_sens else None
+    rc = lib().orc_calculate(*args, transform, float(tf_angles[0]), float(tf_angles[1]),
+                             _ptr(out), _ptr(J))
+    if rc != 0:
+        raise RuntimeError(f"orc_calculate failed ({rc})")
+    return (out, J) if want_sens else out
+
+
+def lqderivative(kind, mesh, model, sites, misfit, field=None, transform=TF_NONE,
+                 tf_angles=(0.0, 0.0)):
+    args, keep = _common(kind, mesh, model, sites, field)
+    misfit = _f64(misfit)
+    grad = np.zeros(n_params(kind, mesh))
+    rc = lib().orc_lqderivative(*args, transform, float(tf_angles[0]), float(tf_angles[1]),
+                                _ptr(misfit), _ptr(grad))
+    if rc != 0:
+        raise RuntimeError(f"orc_lqderivative failed ({rc})")
+    return grad
+
+
+def calculate_ld(kind, mesh, model, sites, field=None):
+    """Forward response evaluated in 80-bit long double (no transform)."""
+    args, keep = _common(kind, mesh, model, sites, field)
+    ns = len(keep[8])
+    out = np.zeros(ns * RAW_PER_SITE[kind])
+    rc = lib().orc_calculate_ld(*args, _ptr(out))
+    if rc != 0:
+        raise RuntimeError(f"orc_calculate_ld failed ({rc})")
+    return out
+
+
+def j_times(J, m):
+    J = _f64(J)
+    m = _f64(m)
+    out = np.zeros(J.shape[0])
+    lib().orc_j_times(J.shape[0], J.shape[1], _ptr(J), _ptr(m), _ptr(out))
+    return out
+
+
+def jt_times2(J, r):
+    J = _f64(J)
+    r = _f64(r)
+    out = np.zeros(J.shape[1])
+    lib().orc_jt_times2(J.shape[0], J.shape[1], _ptr(J), _ptr(r), _ptr(out))
+    return out
+
+
+def transform(tf, values, inc=0.0, dec=0.0):
+    """FTGInvariant / TotalField / TotalFieldAnomaly::Transform of one site."""
+    v = _f64(values)
+    out = np.zeros(1)
+    lib().orc_transform(tf, inc, dec, _ptr(v), _ptr(out))
+    return out[0]
+
+
+def transform_derivative(tf, values, inc=0.0, dec=0.0):
+    """The 1 x k Jacobian returned by VectorTransform::Derivative."""
+    v = _f64(values)
+    out = np.zeros(len(v))
+    lib().orc_transform_derivative(tf, inc, dec, _ptr(v), _ptr(out))
+    return out
