@@ -1,0 +1,163 @@
+/*
+ * jif3d_gravmag.h -- C ABI of the MI355X prism gravity / FTG / magnetic library.
+ *
+ * One shared library (libjif3d_amd.so) replaces the compute behind jif3D's
+ * ThreeDGravMagImplementation (jif3D/GravMag/ThreeDGravMagImplementation.h:39-84)
+ * for the scalar, tensor and magnetic-susceptibility imps and the stored-J
+ * products of FullSensitivityGravMagCalculator.  Plain C types only; cell values
+ * are z-fastest (offset = (xi*ny + yi)*nz + zi, jif3D/ModelBase/ThreeDModelBase.h:157-168);
+ * SI units (kg/m^3, m, nT, radians); z positive down.
+ *
+ * Pointer conventions: geometry setters take host arrays and copy them.  The
+ * plain compute calls take device pointers on the context's device and run on
+ * the context's stream (asynchronously); the *_host variants take host arrays
+ * and return after the result is copied back.
+ *
+ * Errors: every call returns a jgm_status; jgm_last_error() gives the message of
+ * the last failure on the calling thread.  Nothing calls exit() and nothing resets
+ * the device (compare jif3D/Global/CudaError.h:13-22 and the cudaDeviceReset in
+ * jif3D/Gravity/ScalarCudaGravityImp.cpp:52).
+ *
+ * Threading: a context is used by one thread at a time; distinct contexts are
+ * independent.
+ */
+#ifndef JIF3D_GRAVMAG_H_
+#define JIF3D_GRAVMAG_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JGM_API_VERSION 1
+
+typedef struct jgm_context jgm_context;
+
+typedef enum jgm_status {
+  JGM_OK = 0,
+  JGM_ERR_INVALID_ARGUMENT = 1, /* sizes, null pointers, unknown enum values */
+  JGM_ERR_NOT_READY = 2,        /* mesh or sites not set */
+  JGM_ERR_DEVICE = 3,           /* HIP runtime error */
+  JGM_ERR_OUT_OF_MEMORY = 4,
+  JGM_ERR_UNSUPPORTED = 5
+} jgm_status;
+
+/* Raw data per site: 1 (gz, m/s^2), 9 (FTG, 1/s^2, row-major Uxx..Uzz as
+ * jif3D/Gravity/TensorOMPGravityImp.cpp:255-264), 3 (Bx, By, Bz in nT), or 10
+ * (gz followed by the 9 FTG values: one pass for a joint scalar+tensor survey). */
+typedef enum jgm_kind {
+  JGM_KIND_SCALAR = 1,
+  JGM_KIND_TENSOR = 2,
+  JGM_KIND_MAGNETIC = 3,
+  JGM_KIND_SCALAR_TENSOR = 4
+} jgm_kind;
+
+/* JGM_FP64: closed-form box terms in FP64 for every site-cell pair.
+ * JGM_MIXED: FP64 closed form for cells inside each site's near box (all cells
+ * closer than q0 cell half-diagonals), FP32 multipole (monopole + quadrupole of the
+ * prism about its centre) for the rest, FP64 accumulation. */
+typedef enum jgm_precision { JGM_FP64 = 0, JGM_MIXED = 1 } jgm_precision;
+
+/* Per-site data transforms (jif3D/Gravity/GravityTransforms.h:25-100,
+ * jif3D/Magnetics/MagneticTransforms.h:19-156). */
+typedef enum jgm_transform {
+  JGM_TF_NONE = 0,
+  JGM_TF_FTG_INVARIANT = 1,     /* tensor kind only */
+  JGM_TF_TOTAL_FIELD = 2,       /* magnetic kind only */
+  JGM_TF_TOTAL_FIELD_ANOMALY = 3 /* magnetic kind only */
+} jgm_transform;
+
+/* Precision of the stored-J products. */
+typedef enum jgm_gemm_precision {
+  JGM_GEMM_FP32 = 0,        /* FP32 MFMA, FP32 accumulate */
+  JGM_GEMM_BF16 = 1,        /* BF16 operands, FP32 accumulate */
+  JGM_GEMM_BF16_REFINED = 2 /* BF16 hi/lo split of both operands, three products */
+} jgm_gemm_precision;
+
+const char *jgm_last_error(void);
+int jgm_api_version(void);
+size_t jgm_raw_per_site(jgm_kind kind);
+
+/* Create a context on HIP device `device`; `stream` is a hipStream_t or NULL for
+ * the device's null stream.  Replaces PrepareData's device pick
+ * (jif3D/Gravity/gravcuda.cu:38-107). */
+jgm_status jgm_create(int device, void *stream, jgm_context **out);
+jgm_status jgm_destroy(jgm_context *ctx);
+jgm_status jgm_set_stream(jgm_context *ctx, void *stream);
+jgm_status jgm_synchronize(jgm_context *ctx);
+
+/* Mesh: n+1 corner coordinates and n cell sizes per axis, as
+ * ThreeDModelBase::GetX/Y/ZCoordinates and Get*CellSizes return them; mirrors the
+ * argument list of PrepareData (jif3D/Gravity/gravcuda.cu:38-44).  Cell i spans
+ * [coord[i], coord[i] + size[i]].  Any of nx, ny, nz may be zero (background only). */
+jgm_status jgm_set_mesh(jgm_context *ctx, size_t nx, size_t ny, size_t nz, const double *xcoords,
+                        const double *ycoords, const double *zcoords, const double *xsizes,
+                        const double *ysizes, const double *zsizes);
+/* 1-D background layer thicknesses (ThreeDGravityModel::SetBackgroundThicknesses);
+ * the layer values are the last nlayers entries of the model vector. */
+jgm_status jgm_set_background(jgm_context *ctx, size_t nlayers, const double *thicknesses);
+/* Measurement sites (GeneralData::GetMeasPosX/Y/Z, jif3D/DataBase/GeneralData.h:95-108). */
+jgm_status jgm_set_sites(jgm_context *ctx, size_t nsites, const double *x, const double *y,
+                         const double *z);
+/* Inducing field of the susceptibility imp (OMPMagneticSusceptibilityImp(Inc, Dec, Fs),
+ * jif3D/Magnetics/OMPMagneticSusceptibilityImp.cpp:196-201): radians, radians, nT. */
+jgm_status jgm_set_field(jgm_context *ctx, double inclination, double declination,
+                         double fieldstrength);
+/* Data transform (ThreeDGravMagImplementation::SetDataTransform, :67-70). */
+jgm_status jgm_set_transform(jgm_context *ctx, jgm_transform tf, double inclination,
+                             double declination);
+/* Near-field radius of JGM_MIXED in cell half-diagonals (default 12). */
+jgm_status jgm_set_mixed_threshold(jgm_context *ctx, double q0);
+
+/* Values per site after the transform, and the model vector length
+ * (ncells + nlayers, ThreeDGravityModel::GetNModelParm). */
+size_t jgm_data_per_site(const jgm_context *ctx, jgm_kind kind);
+size_t jgm_n_params(const jgm_context *ctx);
+
+/* Forward response of every site: ThreeDGravMagImplementation::Calculate
+ * (jif3D/GravMag/ThreeDGravMagImplementation.h:116-168) with the OMP imps'
+ * CalcGridded/CalcBackground (jif3D/Gravity/ScalarOMPGravityImp.cpp:124-180,
+ * TensorOMPGravityImp.cpp:112-182, OMPMagneticSusceptibilityImp.cpp:31-194).
+ * model: n_params doubles (device); out: nsites * data_per_site doubles (device). */
+jgm_status jgm_forward(jgm_context *ctx, jgm_kind kind, jgm_precision prec, const double *model,
+                       double *out);
+jgm_status jgm_forward_host(jgm_context *ctx, jgm_kind kind, jgm_precision prec,
+                            const double *model, double *out);
+
+/* Least-squares derivative 2 J^T D^T misfit, matrix free (J recomputed, never stored):
+ * ThreeDGravMagImplementation::LQDerivative (:177-231).  model is needed for the
+ * transform Jacobian D.  misfit: nsites * data_per_site; grad: n_params (device). */
+jgm_status jgm_adjoint(jgm_context *ctx, jgm_kind kind, jgm_precision prec, const double *model,
+                       const double *misfit, double *grad);
+jgm_status jgm_adjoint_host(jgm_context *ctx, jgm_kind kind, jgm_precision prec,
+                            const double *model, const double *misfit, double *grad);
+
+/* Raw (untransformed) FP64 sensitivity rows of sites [site0, site0+nsites) in the
+ * layout FullSensitivityGravMagCalculator::HandleSensitivities builds
+ * (jif3D/GravMag/FullSensitivityGravMagCalculator.h:99-114): nsites*raw_per_site rows
+ * of n_params doubles, row-major (device). */
+jgm_status jgm_sensitivity_rows(jgm_context *ctx, jgm_kind kind, size_t site0, size_t nsites,
+                                double *rows);
+
+/* Stored J in HBM, FP32, compact layout: rows = nsites * raw components
+ * (gz: 1; tensor and magnetic: the 6 independent tensor values; scalar+tensor: 7),
+ * columns = ncells, column-major (rows fastest).  Replaces the ublas matrix of
+ * FullSensitivityGravMagCalculator (:40-41, :116-130); prec selects how it is built. */
+jgm_status jgm_storedj_build(jgm_context *ctx, jgm_kind kind, jgm_precision prec);
+size_t jgm_storedj_rows(const jgm_context *ctx);
+/* Y = J M for nrhs model vectors (M: ncells x nrhs column-major, Y: rows x nrhs),
+ * batching CalculateRawData (:137-152) over L-BFGS trial models. */
+jgm_status jgm_storedj_apply(jgm_context *ctx, jgm_gemm_precision prec, const float *M,
+                             size_t nrhs, float *Y);
+/* G = J^T R for nrhs residual vectors (R: rows x nrhs, G: ncells x nrhs), the batched
+ * form of CalculateRawLQDerivative (:214-221) without its factor 2. */
+jgm_status jgm_storedj_apply_t(jgm_context *ctx, jgm_gemm_precision prec, const float *R,
+                               size_t nrhs, float *G);
+jgm_status jgm_storedj_release(jgm_context *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JIF3D_GRAVMAG_H_ */

@@ -1,0 +1,224 @@
+"""ctypes binding of libjif3d_amd.so (C ABI in include/jif3d_gravmag.h).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback for any entry point.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libjif3d_amd.so")
+
+JGM_OK = 0
+KIND_SCALAR, KIND_TENSOR, KIND_MAGNETIC, KIND_SCALAR_TENSOR = 1, 2, 3, 4
+FP64, MIXED = 0, 1
+TF_NONE, TF_FTG_INVARIANT, TF_TOTAL_FIELD, TF_TOTAL_FIELD_ANOMALY = 0, 1, 2, 3
+GEMM_FP32, GEMM_BF16, GEMM_BF16_REFINED = 0, 1, 2
+RAW_PER_SITE = {KIND_SCALAR: 1, KIND_TENSOR: 9, KIND_MAGNETIC: 3, KIND_SCALAR_TENSOR: 10}
+
+EXPORTS = [
+    "jgm_last_error", "jgm_api_version", "jgm_raw_per_site", "jgm_create", "jgm_destroy",
+    "jgm_set_stream", "jgm_synchronize", "jgm_set_mesh", "jgm_set_background", "jgm_set_sites",
+    "jgm_set_field", "jgm_set_transform", "jgm_set_mixed_threshold", "jgm_data_per_site",
+    "jgm_n_params", "jgm_forward", "jgm_forward_host", "jgm_adjoint", "jgm_adjoint_host",
+    "jgm_sensitivity_rows", "jgm_storedj_build", "jgm_storedj_rows", "jgm_storedj_apply",
+    "jgm_storedj_apply_t", "jgm_storedj_release",
+]
+
+
+class JgmError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"jif3d_amd error {status}: {msg}")
+        self.status = status
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise ImportError(
+            f"{LIBPATH} is missing: build it with `python -m jif3d_amd.build` "
+            "(the prism kernels have no CPU fallback)")
+    lib = ctypes.CDLL(LIBPATH)
+    vp, sz, d, i = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_double, ctypes.c_int
+    pd = ctypes.POINTER(ctypes.c_double)
+    lib.jgm_last_error.restype = ctypes.c_char_p
+    lib.jgm_last_error.argtypes = []
+    lib.jgm_api_version.restype = i
+    lib.jgm_raw_per_site.restype = sz
+    lib.jgm_raw_per_site.argtypes = [i]
+    sigs = {
+        "jgm_create": [i, vp, ctypes.POINTER(vp)],
+        "jgm_destroy": [vp],
+        "jgm_set_stream": [vp, vp],
+        "jgm_synchronize": [vp],
+        "jgm_set_mesh": [vp, sz, sz, sz, pd, pd, pd, pd, pd, pd],
+        "jgm_set_background": [vp, sz, pd],
+        "jgm_set_sites": [vp, sz, pd, pd, pd],
+        "jgm_set_field": [vp, d, d, d],
+        "jgm_set_transform": [vp, i, d, d],
+        "jgm_set_mixed_threshold": [vp, d],
+        "jgm_forward": [vp, i, i, vp, vp],
+        "jgm_forward_host": [vp, i, i, pd, pd],
+        "jgm_adjoint": [vp, i, i, vp, vp, vp],
+        "jgm_adjoint_host": [vp, i, i, pd, pd, pd],
+        "jgm_sensitivity_rows": [vp, i, sz, sz, vp],
+        "jgm_storedj_build": [vp, i, i],
+        "jgm_storedj_apply": [vp, i, vp, sz, vp],
+        "jgm_storedj_apply_t": [vp, i, vp, sz, vp],
+        "jgm_storedj_release": [vp],
+    }
+    for name, args in sigs.items():
+        f = getattr(lib, name)
+        f.restype = i
+        f.argtypes = args
+    lib.jgm_data_per_site.restype = sz
+    lib.jgm_data_per_site.argtypes = [vp, i]
+    lib.jgm_n_params.restype = sz
+    lib.jgm_n_params.argtypes = [vp]
+    lib.jgm_storedj_rows.restype = sz
+    lib.jgm_storedj_rows.argtypes = [vp]
+    _lib = lib
+    return lib
+
+
+def check(status):
+    if status != JGM_OK:
+        raise JgmError(status, load().jgm_last_error().decode())
+
+
+def _pd(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _devptr(t):
+    """Device address of a torch tensor (torch is only used for device memory)."""
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _is_device(a):
+    return hasattr(a, "data_ptr") and getattr(a, "is_cuda", False)
+
+
+class Context:
+    """Owns one jgm_context: mesh, background, sites, field, transform."""
+
+    def __init__(self, device=0, stream=None):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        check(self.lib.jgm_create(int(device), stream, ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+        self.nsites = 0
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.jgm_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream):
+        check(self.lib.jgm_set_stream(self.handle, stream))
+
+    def synchronize(self):
+        check(self.lib.jgm_synchronize(self.handle))
+
+    def set_mesh(self, mesh):
+        xc, yc, zc = (_f64(v) for v in mesh.coords)
+        xs, ys, zs = (_f64(v) for v in mesh.sizes)
+        check(self.lib.jgm_set_mesh(self.handle, mesh.nx, mesh.ny, mesh.nz, _pd(xc), _pd(yc),
+                                    _pd(zc), _pd(xs), _pd(ys), _pd(zs)))
+        t = _f64(getattr(mesh, "bg_thicknesses", np.zeros(0)))
+        check(self.lib.jgm_set_background(self.handle, len(t), _pd(t) if len(t) else None))
+
+    def set_sites(self, x, y, z):
+        x, y, z = _f64(x), _f64(y), _f64(z)
+        check(self.lib.jgm_set_sites(self.handle, len(x), _pd(x), _pd(y), _pd(z)))
+        self.nsites = len(x)
+
+    def set_field(self, inclination, declination, fieldstrength):
+        check(self.lib.jgm_set_field(self.handle, float(inclination), float(declination),
+                                     float(fieldstrength)))
+
+    def set_transform(self, tf, inclination=0.0, declination=0.0):
+        check(self.lib.jgm_set_transform(self.handle, int(tf), float(inclination),
+                                         float(declination)))
+
+    def set_mixed_threshold(self, q0):
+        check(self.lib.jgm_set_mixed_threshold(self.handle, float(q0)))
+
+    def data_per_site(self, kind):
+        return self.lib.jgm_data_per_site(self.handle, kind)
+
+    def n_params(self):
+        return self.lib.jgm_n_params(self.handle)
+
+    def forward(self, kind, prec, model, out=None):
+        """Host arrays in -> host array out; torch device tensors -> device result."""
+        if _is_device(model):
+            if out is None:
+                raise ValueError("device forward needs an output tensor")
+            check(self.lib.jgm_forward(self.handle, kind, prec, _devptr(model), _devptr(out)))
+            return out
+        m = _f64(model)
+        res = np.zeros(self.nsites * self.data_per_site(kind))
+        check(self.lib.jgm_forward_host(self.handle, kind, prec, _pd(m), _pd(res)))
+        return res
+
+    def adjoint(self, kind, prec, model, misfit, out=None):
+        if _is_device(misfit):
+            if out is None:
+                raise ValueError("device adjoint needs an output tensor")
+            mp = _devptr(model) if model is not None else None
+            check(self.lib.jgm_adjoint(self.handle, kind, prec, mp, _devptr(misfit),
+                                       _devptr(out)))
+            return out
+        m = _f64(model) if model is not None else None
+        r = _f64(misfit)
+        g = np.zeros(self.n_params())
+        check(self.lib.jgm_adjoint_host(self.handle, kind, prec, _pd(m), _pd(r), _pd(g)))
+        return g
+
+    def sensitivity_rows(self, kind, site0, nsites, out):
+        check(self.lib.jgm_sensitivity_rows(self.handle, kind, site0, nsites, _devptr(out)))
+        return out
+
+    def storedj_build(self, kind, prec):
+        check(self.lib.jgm_storedj_build(self.handle, kind, prec))
+
+    def storedj_rows(self):
+        return self.lib.jgm_storedj_rows(self.handle)
+
+    def storedj_apply(self, prec, M, Y):
+        check(self.lib.jgm_storedj_apply(self.handle, prec, _devptr(M), M.shape[1], _devptr(Y)))
+        return Y
+
+    def storedj_apply_t(self, prec, R, G):
+        check(self.lib.jgm_storedj_apply_t(self.handle, prec, _devptr(R), R.shape[1],
+                                           _devptr(G)))
+        return G
+
+    def storedj_release(self):
+        check(self.lib.jgm_storedj_release(self.handle))
+
+    def configure(self, mesh, data, field=None):
+        self.set_mesh(mesh)
+        x, y, z = data.positions
+        self.set_sites(x, y, z)
+        if field is not None:
+            self.set_field(*field)

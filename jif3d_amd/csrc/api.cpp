@@ -1,0 +1,484 @@
+// C-ABI layer: argument checking, context state, staging of host arrays and the
+// dispatch to the HIP kernels.  See include/jif3d_gravmag.h for the contract.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "context.hpp"
+#include "jif3d_gravmag.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+jgm_status fail(jgm_status st, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return st;
+}
+
+jgm_status hip_fail(hipError_t e, const char* where) {
+  if (e == hipErrorOutOfMemory)
+    return fail(JGM_ERR_OUT_OF_MEMORY, "%s: out of device memory", where);
+  return fail(JGM_ERR_DEVICE, "%s: HIP error %d (%s)", where, (int)e, hipGetErrorString(e));
+}
+
+// Makes the context's device current for the duration of a call.
+struct DeviceGuard {
+  int prev = -1;
+  hipError_t err = hipSuccess;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) err = hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+bool kind_ok(int kind) {
+  return kind == JGM_KIND_SCALAR || kind == JGM_KIND_TENSOR || kind == JGM_KIND_MAGNETIC ||
+         kind == JGM_KIND_SCALAR_TENSOR;
+}
+
+jgm_status check_ready(const jgm_context* ctx, int kind, int prec) {
+  if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
+  if (!kind_ok(kind)) return fail(JGM_ERR_INVALID_ARGUMENT, "unknown kind %d", kind);
+  if (prec != JGM_FP64 && prec != JGM_MIXED)
+    return fail(JGM_ERR_INVALID_ARGUMENT, "unknown precision %d", prec);
+  if (!ctx->mesh_set) return fail(JGM_ERR_NOT_READY, "mesh not set (jgm_set_mesh)");
+  if (!ctx->sites_set) return fail(JGM_ERR_NOT_READY, "sites not set (jgm_set_sites)");
+  const int tf = ctx->transform;
+  if (tf == JGM_TF_FTG_INVARIANT && kind != JGM_KIND_TENSOR)
+    return fail(JGM_ERR_INVALID_ARGUMENT, "the FTG invariant transform needs the tensor kind");
+  if ((tf == JGM_TF_TOTAL_FIELD || tf == JGM_TF_TOTAL_FIELD_ANOMALY) && kind != JGM_KIND_MAGNETIC)
+    return fail(JGM_ERR_INVALID_ARGUMENT, "total-field transforms need the magnetic kind");
+  if (jgm::ncells(ctx) >= (size_t)1 << 31 || jgm::ncorners(ctx) >= ((size_t)1 << 40))
+    return fail(JGM_ERR_UNSUPPORTED, "mesh too large");
+  return JGM_OK;
+}
+
+jgm_status upload(jgm::DeviceBuffer& b, const double* h, size_t n, hipStream_t st,
+                  const char* what) {
+  hipError_t e = b.ensure(sizeof(double) * (n ? n : 1));
+  if (e != hipSuccess) return hip_fail(e, what);
+  if (n) {
+    e = hipMemcpyAsync(b.ptr, h, sizeof(double) * n, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, what);
+  }
+  return JGM_OK;
+}
+
+jgm_status ensure_background(jgm_context* ctx) {
+  if (ctx->bg_valid) return JGM_OK;
+  hipError_t e = jgm::launch_background_table(ctx);
+  if (e != hipSuccess) return hip_fail(e, "background table");
+  return JGM_OK;
+}
+
+size_t data_per_site(const jgm_context* ctx, int kind) {
+  if (!kind_ok(kind)) return 0;
+  return ctx->transform == JGM_TF_NONE ? jgm::raw_per_site(kind) : 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* jgm_last_error(void) { return g_last_error.c_str(); }
+
+int jgm_api_version(void) { return JGM_API_VERSION; }
+
+size_t jgm_raw_per_site(jgm_kind kind) { return jgm::raw_per_site(kind); }
+
+jgm_status jgm_create(int device, void* stream, jgm_context** out) {
+  if (!out) return fail(JGM_ERR_INVALID_ARGUMENT, "null output pointer");
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
+  if (device < 0 || device >= ndev)
+    return fail(JGM_ERR_INVALID_ARGUMENT, "device %d out of range (%d devices)", device, ndev);
+  DeviceGuard g(device);
+  if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+  jgm_context* ctx = new (std::nothrow) jgm_context();
+  if (!ctx) return fail(JGM_ERR_OUT_OF_MEMORY, "host allocation failed");
+  ctx->device = device;
+  ctx->stream = static_cast<hipStream_t>(stream);
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+      cus > 0)
+    ctx->cu_count = cus;
+  *out = ctx;
+  return JGM_OK;
+}
+
+jgm_status jgm_destroy(jgm_context* ctx) {
+  if (!ctx) return JGM_OK;
+  DeviceGuard g(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  jgm::DeviceBuffer* bufs[] = {&ctx->d_Xc, &ctx->d_Yc, &ctx->d_Zc, &ctx->d_xlo, &ctx->d_ylo,
+                               &ctx->d_zlo, &ctx->d_xsz, &ctx->d_ysz, &ctx->d_zsz, &ctx->d_sx,
+                               &ctx->d_sy, &ctx->d_sz, &ctx->bgT, &ctx->near.box,
+                               &ctx->near.tiles, &ctx->weights, &ctx->partial, &ctx->raw,
+                               &ctx->coef, &ctx->hpart, &ctx->hsum, &ctx->model_stage,
+                               &ctx->out_stage, &ctx->misfit_stage, &ctx->grad_stage,
+                               &ctx->farpart, &ctx->nearpart, &ctx->storedJ};
+  for (auto* b : bufs) b->release();
+  delete ctx;
+  return JGM_OK;
+}
+
+jgm_status jgm_set_stream(jgm_context* ctx, void* stream) {
+  if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
+  ctx->stream = static_cast<hipStream_t>(stream);
+  return JGM_OK;
+}
+
+jgm_status jgm_synchronize(jgm_context* ctx) {
+  if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
+  DeviceGuard g(ctx->device);
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  return JGM_OK;
+}
+
+jgm_status jgm_set_mesh(jgm_context* ctx, size_t nx, size_t ny, size_t nz, const double* xcoords,
+                        const double* ycoords, const double* zcoords, const double* xsizes,
+                        const double* ysizes, const double* zsizes) {
+  if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
+  const size_t n[3] = {nx, ny, nz};
+  const double* c[3] = {xcoords, ycoords, zcoords};
+  const double* s[3] = {xsizes, ysizes, zsizes};
+  for (int a = 0; a < 3; ++a) {
+    if (n[a] > 0 && (!c[a] || !s[a]))
+      return fail(JGM_ERR_INVALID_ARGUMENT, "axis %d: null coordinate or size array", a);
+    if (n[a] > (1u << 20)) return fail(JGM_ERR_UNSUPPORTED, "axis %d: too many cells", a);
+    for (size_t i = 0; i < n[a]; ++i)
+      if (!std::isfinite(s[a][i]) || !std::isfinite(c[a][i]))
+        return fail(JGM_ERR_INVALID_ARGUMENT, "axis %d: non-finite coordinate or size", a);
+  }
+  DeviceGuard g(ctx->device);
+  if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+  ctx->nx = nx;
+  ctx->ny = ny;
+  ctx->nz = nz;
+  std::vector<double>* hc[3] = {&ctx->xc, &ctx->yc, &ctx->zc};
+  std::vector<double>* hs[3] = {&ctx->xs, &ctx->ys, &ctx->zs};
+  double width[3];
+  for (int a = 0; a < 3; ++a) {
+    hc[a]->assign(n[a] + 1, 0.0);
+    if (n[a] > 0)
+      hs[a]->assign(s[a], s[a] + n[a]);
+    else
+      hs[a]->clear();
+    // corner a of the mesh: cell i spans [coord[i], coord[i] + size[i]]; the last
+    // corner is the far face of the last cell as CalcGravBoxTerm forms it.
+    if (n[a] > 0) {
+      for (size_t i = 0; i < n[a]; ++i) (*hc[a])[i] = c[a][i];
+      (*hc[a])[n[a]] = c[a][n[a] - 1] + s[a][n[a] - 1];
+    } else {
+      (*hc[a])[0] = c[a] ? c[a][0] : 0.0;
+    }
+    double w = 0.0;  // std::accumulate(sizes, 0.0), ThreeDGravMagImplementation.h:132-137
+    for (size_t i = 0; i < n[a]; ++i) w += s[a][i];
+    width[a] = w;
+  }
+  ctx->xwidth = width[0];
+  ctx->ywidth = width[1];
+  ctx->zwidth = width[2];
+  jgm_status st;
+  if ((st = upload(ctx->d_Xc, ctx->xc.data(), nx + 1, ctx->stream, "mesh")) != JGM_OK) return st;
+  if ((st = upload(ctx->d_Yc, ctx->yc.data(), ny + 1, ctx->stream, "mesh")) != JGM_OK) return st;
+  if ((st = upload(ctx->d_Zc, ctx->zc.data(), nz + 1, ctx->stream, "mesh")) != JGM_OK) return st;
+  if ((st = upload(ctx->d_xlo, ctx->xc.data(), nx, ctx->stream, "mesh")) != JGM_OK) return st;
+  if ((st = upload(ctx->d_ylo, ctx->yc.data(), ny, ctx->stream, "mesh")) != JGM_OK) return st;
+  if ((st = upload(ctx->d_zlo, ctx->zc.data(), nz, ctx->stream, "mesh")) != JGM_OK) return st;
+  if ((st = upload(ctx->d_xsz, ctx->xs.data(), nx, ctx->stream, "mesh")) != JGM_OK) return st;
+  if ((st = upload(ctx->d_ysz, ctx->ys.data(), ny, ctx->stream, "mesh")) != JGM_OK) return st;
+  if ((st = upload(ctx->d_zsz, ctx->zs.data(), nz, ctx->stream, "mesh")) != JGM_OK) return st;
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_fail(e, "mesh upload");
+  ctx->mesh_set = true;
+  ctx->bg_valid = false;
+  ctx->near.valid = false;
+  ctx->storedJ_kind = 0;
+  return JGM_OK;
+}
+
+jgm_status jgm_set_background(jgm_context* ctx, size_t nlayers, const double* thicknesses) {
+  if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
+  if (nlayers > 0 && !thicknesses) return fail(JGM_ERR_INVALID_ARGUMENT, "null thicknesses");
+  if (nlayers > 4096) return fail(JGM_ERR_UNSUPPORTED, "too many background layers");
+  ctx->bgthick.assign(thicknesses, thicknesses + nlayers);
+  ctx->bg_valid = false;
+  ctx->storedJ_kind = 0;
+  return JGM_OK;
+}
+
+jgm_status jgm_set_sites(jgm_context* ctx, size_t nsites, const double* x, const double* y,
+                         const double* z) {
+  if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
+  if (nsites > 0 && (!x || !y || !z)) return fail(JGM_ERR_INVALID_ARGUMENT, "null site array");
+  if (nsites >= (1u << 30)) return fail(JGM_ERR_UNSUPPORTED, "too many sites");
+  for (size_t i = 0; i < nsites; ++i)
+    if (!std::isfinite(x[i]) || !std::isfinite(y[i]) || !std::isfinite(z[i]))
+      return fail(JGM_ERR_INVALID_ARGUMENT, "site %zu has a non-finite coordinate", i);
+  DeviceGuard g(ctx->device);
+  if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+  ctx->S = nsites;
+  ctx->hsx.assign(x, x + nsites);
+  ctx->hsy.assign(y, y + nsites);
+  ctx->hsz.assign(z, z + nsites);
+  jgm_status st;
+  if ((st = upload(ctx->d_sx, x, nsites, ctx->stream, "sites")) != JGM_OK) return st;
+  if ((st = upload(ctx->d_sy, y, nsites, ctx->stream, "sites")) != JGM_OK) return st;
+  if ((st = upload(ctx->d_sz, z, nsites, ctx->stream, "sites")) != JGM_OK) return st;
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_fail(e, "site upload");
+  ctx->sites_set = true;
+  ctx->bg_valid = false;
+  ctx->near.valid = false;
+  ctx->storedJ_kind = 0;
+  return JGM_OK;
+}
+
+jgm_status jgm_set_field(jgm_context* ctx, double inclination, double declination,
+                         double fieldstrength) {
+  if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
+  if (!std::isfinite(inclination) || !std::isfinite(declination) || !std::isfinite(fieldstrength))
+    return fail(JGM_ERR_INVALID_ARGUMENT, "non-finite field parameter");
+  ctx->inclination = inclination;
+  ctx->declination = declination;
+  ctx->fieldstrength = fieldstrength;
+  if (ctx->storedJ_kind == JGM_KIND_MAGNETIC) ctx->storedJ_kind = 0;
+  return JGM_OK;
+}
+
+jgm_status jgm_set_transform(jgm_context* ctx, jgm_transform tf, double inclination,
+                             double declination) {
+  if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
+  if (tf != JGM_TF_NONE && tf != JGM_TF_FTG_INVARIANT && tf != JGM_TF_TOTAL_FIELD &&
+      tf != JGM_TF_TOTAL_FIELD_ANOMALY)
+    return fail(JGM_ERR_INVALID_ARGUMENT, "unknown transform %d", (int)tf);
+  ctx->transform = tf;
+  ctx->tf_inc = inclination;
+  ctx->tf_dec = declination;
+  return JGM_OK;
+}
+
+jgm_status jgm_set_mixed_threshold(jgm_context* ctx, double q0) {
+  if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
+  if (!(q0 >= 2.0 && q0 <= 1e6)) return fail(JGM_ERR_INVALID_ARGUMENT, "q0 must be in [2, 1e6]");
+  ctx->mixed_q0 = q0;
+  ctx->near.valid = false;
+  return JGM_OK;
+}
+
+size_t jgm_data_per_site(const jgm_context* ctx, jgm_kind kind) {
+  return ctx ? data_per_site(ctx, kind) : 0;
+}
+
+size_t jgm_n_params(const jgm_context* ctx) { return ctx ? jgm::nparams(ctx) : 0; }
+
+jgm_status jgm_forward(jgm_context* ctx, jgm_kind kind, jgm_precision prec, const double* model,
+                       double* out) {
+  jgm_status st = check_ready(ctx, kind, prec);
+  if (st != JGM_OK) return st;
+  if (!model || !out) return fail(JGM_ERR_INVALID_ARGUMENT, "null model or output pointer");
+  DeviceGuard g(ctx->device);
+  if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+  if (ctx->S == 0) return JGM_OK;
+  if ((st = ensure_background(ctx)) != JGM_OK) return st;
+  hipError_t e;
+  if (prec == JGM_MIXED)
+    e = jgm::launch_forward_mixed(ctx, kind, model, out, nullptr);
+  else
+    e = jgm::launch_forward_fp64(ctx, kind, model, out, nullptr);
+  if (e != hipSuccess) return hip_fail(e, "forward");
+  return JGM_OK;
+}
+
+jgm_status jgm_adjoint(jgm_context* ctx, jgm_kind kind, jgm_precision prec, const double* model,
+                       const double* misfit, double* grad) {
+  jgm_status st = check_ready(ctx, kind, prec);
+  if (st != JGM_OK) return st;
+  if (!misfit || !grad) return fail(JGM_ERR_INVALID_ARGUMENT, "null misfit or gradient pointer");
+  if (ctx->transform != JGM_TF_NONE && !model)
+    return fail(JGM_ERR_INVALID_ARGUMENT, "a transform needs the model for its Jacobian");
+  DeviceGuard g(ctx->device);
+  if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+  if ((st = ensure_background(ctx)) != JGM_OK) return st;
+  const size_t S = ctx->S;
+  const size_t kraw = jgm::raw_per_site(kind);
+  hipError_t e;
+  if (S == 0) {
+    e = jgm::launch_fill_zero(ctx, grad, jgm::nparams(ctx));
+    if (e != hipSuccess) return hip_fail(e, "adjoint");
+    return JGM_OK;
+  }
+  const double* rm = misfit;
+  if (ctx->transform != JGM_TF_NONE) {
+    // raw data at the model for the transform Jacobian (LQDerivative recomputes it
+    // per site, ThreeDGravMagImplementation.h:209-214)
+    if ((e = ctx->raw.ensure(sizeof(double) * S * kraw)) != hipSuccess) return hip_fail(e, "adjoint");
+    if ((e = ctx->out_stage.ensure(sizeof(double) * S)) != hipSuccess) return hip_fail(e, "adjoint");
+    if ((e = ctx->misfit_stage.ensure(sizeof(double) * S * kraw)) != hipSuccess)
+      return hip_fail(e, "adjoint");
+    e = jgm::launch_forward_fp64(ctx, kind, model, ctx->out_stage.as<double>(),
+                                 ctx->raw.as<double>());
+    if (e != hipSuccess) return hip_fail(e, "adjoint (raw forward)");
+    e = jgm::launch_misfit_to_raw(ctx, kind, misfit, ctx->raw.as<const double>(),
+                                  ctx->misfit_stage.as<double>());
+    if (e != hipSuccess) return hip_fail(e, "adjoint (transform)");
+    rm = ctx->misfit_stage.as<const double>();
+  }
+  if (prec == JGM_MIXED)
+    e = jgm::launch_adjoint_mixed(ctx, kind, model, rm, grad);
+  else
+    e = jgm::launch_adjoint_fp64(ctx, kind, model, rm, grad);
+  if (e != hipSuccess) return hip_fail(e, "adjoint");
+  return JGM_OK;
+}
+
+jgm_status jgm_forward_host(jgm_context* ctx, jgm_kind kind, jgm_precision prec,
+                            const double* model, double* out) {
+  jgm_status st = check_ready(ctx, kind, prec);
+  if (st != JGM_OK) return st;
+  if (!model || !out) return fail(JGM_ERR_INVALID_ARGUMENT, "null model or output pointer");
+  DeviceGuard g(ctx->device);
+  const size_t np = jgm::nparams(ctx);
+  const size_t nout = ctx->S * data_per_site(ctx, kind);
+  hipError_t e;
+  if ((e = ctx->model_stage.ensure(sizeof(double) * (np ? np : 1))) != hipSuccess)
+    return hip_fail(e, "forward_host");
+  if ((e = ctx->grad_stage.ensure(sizeof(double) * (nout ? nout : 1))) != hipSuccess)
+    return hip_fail(e, "forward_host");
+  if (np && (e = hipMemcpyAsync(ctx->model_stage.ptr, model, sizeof(double) * np,
+                                hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
+    return hip_fail(e, "forward_host");
+  st = jgm_forward(ctx, kind, prec, ctx->model_stage.as<double>(), ctx->grad_stage.as<double>());
+  if (st != JGM_OK) return st;
+  if (nout && (e = hipMemcpyAsync(out, ctx->grad_stage.ptr, sizeof(double) * nout,
+                                  hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess)
+    return hip_fail(e, "forward_host");
+  if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return hip_fail(e, "forward_host");
+  return JGM_OK;
+}
+
+jgm_status jgm_adjoint_host(jgm_context* ctx, jgm_kind kind, jgm_precision prec,
+                            const double* model, const double* misfit, double* grad) {
+  jgm_status st = check_ready(ctx, kind, prec);
+  if (st != JGM_OK) return st;
+  if (!misfit || !grad) return fail(JGM_ERR_INVALID_ARGUMENT, "null misfit or gradient pointer");
+  DeviceGuard g(ctx->device);
+  const size_t np = jgm::nparams(ctx);
+  const size_t nmis = ctx->S * data_per_site(ctx, kind);
+  jgm::DeviceBuffer dmis;  // separate from the internal misfit_stage used by transforms
+  hipError_t e;
+  if ((e = ctx->model_stage.ensure(sizeof(double) * (np ? np : 1))) != hipSuccess)
+    return hip_fail(e, "adjoint_host");
+  if ((e = ctx->grad_stage.ensure(sizeof(double) * (np ? np : 1))) != hipSuccess)
+    return hip_fail(e, "adjoint_host");
+  if ((e = dmis.ensure(sizeof(double) * (nmis ? nmis : 1))) != hipSuccess)
+    return hip_fail(e, "adjoint_host");
+  if (model && np &&
+      (e = hipMemcpyAsync(ctx->model_stage.ptr, model, sizeof(double) * np, hipMemcpyHostToDevice,
+                          ctx->stream)) != hipSuccess) {
+    dmis.release();
+    return hip_fail(e, "adjoint_host");
+  }
+  if (nmis && (e = hipMemcpyAsync(dmis.ptr, misfit, sizeof(double) * nmis, hipMemcpyHostToDevice,
+                                  ctx->stream)) != hipSuccess) {
+    dmis.release();
+    return hip_fail(e, "adjoint_host");
+  }
+  st = jgm_adjoint(ctx, kind, prec, model ? ctx->model_stage.as<double>() : nullptr,
+                   dmis.as<double>(), ctx->grad_stage.as<double>());
+  if (st == JGM_OK && np)
+    e = hipMemcpyAsync(grad, ctx->grad_stage.ptr, sizeof(double) * np, hipMemcpyDeviceToHost,
+                       ctx->stream);
+  hipError_t e2 = hipStreamSynchronize(ctx->stream);
+  dmis.release();
+  if (st != JGM_OK) return st;
+  if (e != hipSuccess) return hip_fail(e, "adjoint_host");
+  if (e2 != hipSuccess) return hip_fail(e2, "adjoint_host");
+  return JGM_OK;
+}
+
+jgm_status jgm_sensitivity_rows(jgm_context* ctx, jgm_kind kind, size_t site0, size_t nsites,
+                                double* rows) {
+  jgm_status st = check_ready(ctx, kind, JGM_FP64);
+  if (st != JGM_OK) return st;
+  if (!rows && nsites) return fail(JGM_ERR_INVALID_ARGUMENT, "null rows pointer");
+  if (site0 > ctx->S || nsites > ctx->S - site0)
+    return fail(JGM_ERR_INVALID_ARGUMENT, "site range [%zu, %zu) outside [0, %zu)", site0,
+                site0 + nsites, ctx->S);
+  if (nsites > 65535) return fail(JGM_ERR_UNSUPPORTED, "at most 65535 sites per call");
+  if (nsites == 0) return JGM_OK;
+  DeviceGuard g(ctx->device);
+  if ((st = ensure_background(ctx)) != JGM_OK) return st;
+  hipError_t e = jgm::launch_sensitivity_rows(ctx, kind, site0, nsites, rows);
+  if (e != hipSuccess) return hip_fail(e, "sensitivity rows");
+  return JGM_OK;
+}
+
+jgm_status jgm_storedj_build(jgm_context* ctx, jgm_kind kind, jgm_precision prec) {
+  jgm_status st = check_ready(ctx, kind, prec);
+  if (st != JGM_OK) return st;
+  DeviceGuard g(ctx->device);
+  hipError_t e = jgm::launch_storedj_build(ctx, kind, prec);
+  if (e == hipErrorNotSupported) return fail(JGM_ERR_UNSUPPORTED, "stored J not available");
+  if (e != hipSuccess) return hip_fail(e, "stored J build");
+  return JGM_OK;
+}
+
+size_t jgm_storedj_rows(const jgm_context* ctx) { return ctx ? ctx->storedJ_rows : 0; }
+
+static jgm_status storedj_apply(jgm_context* ctx, jgm_gemm_precision prec, int transpose,
+                                const float* in, size_t nrhs, float* out) {
+  if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
+  if (ctx->storedJ_kind == 0) return fail(JGM_ERR_NOT_READY, "no stored J (jgm_storedj_build)");
+  if (prec != JGM_GEMM_FP32 && prec != JGM_GEMM_BF16 && prec != JGM_GEMM_BF16_REFINED)
+    return fail(JGM_ERR_INVALID_ARGUMENT, "unknown GEMM precision %d", (int)prec);
+  if (!in || !out) return fail(JGM_ERR_INVALID_ARGUMENT, "null operand");
+  if (nrhs == 0) return JGM_OK;
+  if (nrhs > 4096) return fail(JGM_ERR_UNSUPPORTED, "at most 4096 right-hand sides");
+  DeviceGuard g(ctx->device);
+  hipError_t e = jgm::launch_storedj_apply(ctx, prec, transpose, in, nrhs, out);
+  if (e == hipErrorNotSupported) return fail(JGM_ERR_UNSUPPORTED, "stored J product not available");
+  if (e != hipSuccess) return hip_fail(e, "stored J product");
+  return JGM_OK;
+}
+
+jgm_status jgm_storedj_apply(jgm_context* ctx, jgm_gemm_precision prec, const float* M,
+                             size_t nrhs, float* Y) {
+  return storedj_apply(ctx, prec, 0, M, nrhs, Y);
+}
+
+jgm_status jgm_storedj_apply_t(jgm_context* ctx, jgm_gemm_precision prec, const float* R,
+                               size_t nrhs, float* G) {
+  return storedj_apply(ctx, prec, 1, R, nrhs, G);
+}
+
+jgm_status jgm_storedj_release(jgm_context* ctx) {
+  if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
+  DeviceGuard g(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->storedJ.release();
+  ctx->storedJ_kind = 0;
+  ctx->storedJ_rows = ctx->storedJ_cols = 0;
+  return JGM_OK;
+}
+
+}  // extern "C"

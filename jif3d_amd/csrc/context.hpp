@@ -1,0 +1,140 @@
+// Internal state behind the C-ABI handle (include/jif3d_gravmag.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "jif3d_gravmag.h"
+
+namespace jgm {
+
+// Grow-only device allocation owned by a context.
+struct DeviceBuffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t need) {
+    if (need <= bytes) return hipSuccess;
+    if (ptr) {
+      hipError_t e = hipFree(ptr);
+      if (e != hipSuccess) return e;
+      ptr = nullptr;
+      bytes = 0;
+    }
+    hipError_t e = hipMalloc(&ptr, need < 256 ? 256 : need);
+    if (e == hipSuccess) bytes = need < 256 ? 256 : need;
+    return e;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Near-field description for the mixed-precision kernels: per site an index box
+// of cells [lo, hi] (inclusive) evaluated with the FP64 closed form; every cell
+// outside it is at least q0 half-diagonals away from the site.
+struct NearBoxes {
+  DeviceBuffer box;   // int32 [S][6]: i0, i1, j0, j1, k0, k1 (empty: i0 > i1)
+  DeviceBuffer tiles; // adjoint near pass: CSR of candidate sites per cell column block
+  double q0 = 0.0;
+  bool valid = false;
+  size_t max_box_cells = 0;
+};
+
+}  // namespace jgm
+
+struct jgm_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int cu_count = 256;
+
+  // mesh (host copies)
+  size_t nx = 0, ny = 0, nz = 0;
+  std::vector<double> xc, yc, zc;  // corner coordinates, n+1 each
+  std::vector<double> xs, ys, zs;  // cell sizes, n each
+  double xwidth = 0, ywidth = 0, zwidth = 0;
+  bool mesh_set = false;
+  // device mesh: corner coordinates used by the corner-first kernels (n+1 each),
+  // and the per-cell origin/size arrays used by the per-cell kernels.
+  jgm::DeviceBuffer d_Xc, d_Yc, d_Zc;
+  jgm::DeviceBuffer d_xlo, d_ylo, d_zlo, d_xsz, d_ysz, d_zsz;
+
+  // background layers
+  std::vector<double> bgthick;
+
+  // sites
+  size_t S = 0;
+  std::vector<double> hsx, hsy, hsz;
+  jgm::DeviceBuffer d_sx, d_sy, d_sz;
+  bool sites_set = false;
+
+  // magnetic inducing field (radians, radians, nT)
+  double inclination = 0.0, declination = 0.0, fieldstrength = 0.0;
+
+  // data transform
+  int transform = JGM_TF_NONE;
+  double tf_inc = 0.0, tf_dec = 0.0;
+
+  // mixed precision near-field threshold in cell half-diagonals
+  double mixed_q0 = 12.0;
+
+  // caches
+  jgm::DeviceBuffer bgT;  // [S][L][7] background box terms, valid when bg_valid
+  bool bg_valid = false;
+  jgm::NearBoxes near;
+
+  // scratch
+  jgm::DeviceBuffer weights, partial, raw, coef, hpart, hsum, model_stage, out_stage, misfit_stage,
+      grad_stage, farpart, nearpart;
+
+  // stored sensitivity matrix
+  jgm::DeviceBuffer storedJ;
+  int storedJ_kind = 0;
+  size_t storedJ_rows = 0, storedJ_cols = 0;
+};
+
+namespace jgm {
+
+// Launch helpers implemented in the kernel translation units.  All return a HIP
+// error code; arguments have been validated by the API layer.
+hipError_t launch_background_table(jgm_context* ctx);
+hipError_t launch_forward_fp64(jgm_context* ctx, int kind, const double* model, double* out,
+                               double* raw_out);
+hipError_t launch_adjoint_fp64(jgm_context* ctx, int kind, const double* model,
+                               const double* raw_misfit, double* grad);
+hipError_t launch_misfit_to_raw(jgm_context* ctx, int kind, const double* misfit,
+                                const double* raw, double* raw_misfit);
+hipError_t launch_sensitivity_rows(jgm_context* ctx, int kind, size_t site0, size_t nsites,
+                                   double* rows);
+hipError_t launch_forward_mixed(jgm_context* ctx, int kind, const double* model, double* out,
+                                double* raw_out);
+hipError_t launch_adjoint_mixed(jgm_context* ctx, int kind, const double* model,
+                                const double* raw_misfit, double* grad);
+hipError_t build_near_boxes(jgm_context* ctx);
+hipError_t launch_fill_zero(jgm_context* ctx, double* p, size_t n);
+hipError_t launch_storedj_build(jgm_context* ctx, int kind, int precision);
+hipError_t launch_storedj_apply(jgm_context* ctx, int gemm_precision, int transpose,
+                                const float* in, size_t nrhs, float* out);
+
+__host__ __device__ inline size_t raw_per_site(int kind) {
+  switch (kind) {
+    case JGM_KIND_SCALAR: return 1;
+    case JGM_KIND_TENSOR: return 9;
+    case JGM_KIND_MAGNETIC: return 3;
+    case JGM_KIND_SCALAR_TENSOR: return 10;
+    default: return 0;
+  }
+}
+
+inline size_t ncells(const jgm_context* c) { return c->nx * c->ny * c->nz; }
+inline size_t ncorners(const jgm_context* c) { return (c->nx + 1) * (c->ny + 1) * (c->nz + 1); }
+inline size_t nparams(const jgm_context* c) { return ncells(c) + c->bgthick.size(); }
+
+}  // namespace jgm

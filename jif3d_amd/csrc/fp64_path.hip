@@ -1,0 +1,624 @@
+// FP64 closed-form path: corner-first forward and adjoint, 1-D background,
+// epilogues (component assembly, magnetic projection, data transforms) and the
+// per-cell sensitivity rows.
+//
+// Layouts in HBM
+//   corner coordinates Xc[nx+1], Yc[ny+1], Zc[nz+1]      (FP64)
+//   corner weights     w[(I*(ny+1) + J)*(nz+1) + K]       (FP64, K fastest)
+//   forward partials   partial[(chunk*NC + m)*S + s]      (FP64, sites fastest)
+//   adjoint partials   hpart[chunk*Ncorner + q]           (FP64, corners fastest)
+//   background table   bgT[(s*L + j)*7 + m]               (FP64, m = gz,xx,yy,zz,xy,xz,yz)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "context.hpp"
+#include "prism_device.hpp"
+
+namespace jgm {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kTargetWaves = 16384;  // ~16 waves per SIMD over 256 CUs x 4 SIMDs
+
+// w(I,J,K) = sum_{a,b,c} sigma(a,b,c) rho(I-a, J-b, K-c), rho zero outside the mesh.
+__global__ __launch_bounds__(kBlock) void k_corner_weights(const double* __restrict__ rho,
+                                                           int nx, int ny, int nz,
+                                                           double* __restrict__ w) {
+  const long ncx = nx + 1, ncy = ny + 1, ncz = nz + 1;
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ncx * ncy * ncz) return;
+  const int K = (int)(q % ncz);
+  const long t = q / ncz;
+  const int J = (int)(t % ncy);
+  const int I = (int)(t / ncy);
+  double acc = 0.0;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int i = I - a, j = J - b, k = K - c;
+        if (i < 0 || j < 0 || k < 0 || i >= nx || j >= ny || k >= nz) continue;
+        const double v = rho[((long)i * ny + j) * nz + k];
+        acc += ((a + b + c) & 1) ? v : -v;
+      }
+  w[q] = acc;
+}
+
+// Forward: one lane per site; the wave walks a chunk of corner columns (I, J)
+// and all K.  Corner coordinates and weights are wave-uniform (scalar loads).
+template <int MASK>
+__global__ __launch_bounds__(kBlock) void k_forward_corners(
+    const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
+    int S, const double* __restrict__ Xc, const double* __restrict__ Yc,
+    const double* __restrict__ Zc, int ncy, int ncz, const double* __restrict__ w, long ncols,
+    long cols_per_chunk, double* __restrict__ partial) {
+  constexpr int NC = Ncomp<MASK>::value;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sl = s < S ? s : S - 1;
+  const double px = sx[sl], py = sy[sl], pz = sz[sl];
+  const long col0 = (long)blockIdx.y * cols_per_chunk;
+  const long col1 = col0 + cols_per_chunk < ncols ? col0 + cols_per_chunk : ncols;
+  double acc[NC];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) acc[m] = 0.0;
+  for (long col = col0; col < col1; ++col) {
+    const int I = (int)(col / ncy);
+    const int J = (int)(col - (long)I * ncy);
+    const double x = px - Xc[I];
+    const double y = py - Yc[J];
+    const double* wc = w + col * ncz;
+    for (int K = 0; K < ncz; ++K) {
+      const double z = pz - Zc[K];
+      const double wk = wc[K];
+      double t[NC];
+      corner_terms_f64<MASK>(x, y, z, t);
+#pragma unroll
+      for (int m = 0; m < NC; ++m) acc[m] = fma(wk, t[m], acc[m]);
+    }
+  }
+  if (s < S) {
+#pragma unroll
+    for (int m = 0; m < NC; ++m) partial[((long)blockIdx.y * NC + m) * S + s] = acc[m];
+  }
+}
+
+// Background table: per site and layer the 7 box-term values of the layer minus
+// its part inside the mesh (jif3D/Gravity/GravityBackground.cpp:24-137).
+__device__ inline double inf_sheet(double measz, double top, double bottom) {
+  const double thick = bottom - top;
+  if ((measz <= top) || (measz >= bottom))
+    return 2.0 * kPi * kGravConst * thick * (top - measz >= 0 ? 1.0 : -1.0);
+  return 2.0 * kPi * kGravConst * (thick - 2.0 * (measz - top));
+}
+__device__ inline double uzz_inf_sheet(double measz, double top, double bottom) {
+  if ((measz <= top) || (measz >= bottom)) return 0.0;
+  return 4.0 * kPi * kGravConst;
+}
+
+__global__ __launch_bounds__(kBlock) void k_background_table(
+    const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
+    int S, const double* __restrict__ thick, int L, double x0, double y0, double z0,
+    double xwidth, double ywidth, double zwidth, double* __restrict__ bgT) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const double px = sx[s], py = sy[s], pz = sz[s];
+  double currtop = z0;
+  for (int j = 0; j < L; ++j) {
+    const double th = thick[j];
+    const double currbottom = currtop + th;
+    double v[7];
+    v[0] = inf_sheet(pz, currtop, currbottom);
+    for (int m = 1; m < 7; ++m) v[m] = 0.0;
+    v[3] = uzz_inf_sheet(pz, currtop, currbottom);
+    double boxh = -1.0;
+    if (currtop < zwidth && currbottom <= zwidth) boxh = th;
+    if (currtop < zwidth && currbottom > zwidth) boxh = zwidth - currtop;
+    if (boxh >= 0.0) {
+      double g[1], t[6];
+      box_terms_f64<kMaskScalar>(px, py, pz, x0, y0, currtop, xwidth, ywidth, boxh, g);
+      box_terms_f64<kMaskTensor>(px, py, pz, x0, y0, currtop, xwidth, ywidth, boxh, t);
+      v[0] -= g[0];
+      for (int m = 0; m < 6; ++m) v[1 + m] -= t[m];
+    }
+    double* o = bgT + ((long)s * L + j) * 7;
+    for (int m = 0; m < 7; ++m) o[m] = v[m];
+    currtop += th;
+  }
+}
+
+struct FieldDir {
+  double bx, by, bz, scale;  // direction cosines and F / (4 pi G)
+};
+
+__host__ inline FieldDir field_dir(const jgm_context* ctx) {
+  FieldDir f;
+  f.bx = std::cos(ctx->inclination) * std::cos(ctx->declination);
+  f.by = std::cos(ctx->inclination) * std::sin(ctx->declination);
+  f.bz = std::sin(ctx->inclination);
+  f.scale = ctx->fieldstrength * (1.0 / (4 * kPi * kGravConst));
+  return f;
+}
+
+// Sum the chunk partials, scale, add the background and assemble the raw data of
+// one site; then apply the transform.  u[] holds gz, Uxx, Uyy, Uzz, Uxy, Uxz, Uyz.
+template <int MASK>
+__global__ __launch_bounds__(kBlock) void k_forward_finish(
+    const double* __restrict__ partial, int nchunks, int S, const double* __restrict__ bgT,
+    const double* __restrict__ bgvals, int L, int kind, FieldDir fd, int tf, double tfbx,
+    double tfby, double tfbz, double* __restrict__ out, double* __restrict__ raw_out) {
+  constexpr int NC = Ncomp<MASK>::value;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  double acc[NC];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) acc[m] = 0.0;
+  for (int c = 0; c < nchunks; ++c)
+#pragma unroll
+    for (int m = 0; m < NC; ++m) acc[m] += partial[((long)c * NC + m) * S + s];
+  double sc[NC];
+  comp_scales<MASK>(sc);
+  double u[7] = {0, 0, 0, 0, 0, 0, 0};
+  {
+    int k = 0;
+    if (MASK & kGz) u[0] = acc[k] * sc[k], ++k;
+    if (MASK & kDiag) {
+      u[1] = acc[k] * sc[k]; ++k;
+      u[2] = acc[k] * sc[k]; ++k;
+      u[3] = acc[k] * sc[k]; ++k;
+    }
+    if (MASK & kOff) {
+      u[4] = acc[k] * sc[k]; ++k;
+      u[5] = acc[k] * sc[k]; ++k;
+      u[6] = acc[k] * sc[k]; ++k;
+    }
+  }
+  for (int j = 0; j < L; ++j) {
+    const double* b = bgT + ((long)s * L + j) * 7;
+    const double v = bgvals[j];
+    for (int m = 0; m < 7; ++m) u[m] += b[m] * v;
+  }
+  double raw[10];
+  int kraw = 0;
+  if (kind == JGM_KIND_SCALAR) {
+    raw[0] = u[0];
+    kraw = 1;
+  } else if (kind == JGM_KIND_TENSOR || kind == JGM_KIND_SCALAR_TENSOR) {
+    int o = 0;
+    if (kind == JGM_KIND_SCALAR_TENSOR) raw[o++] = u[0];
+    raw[o + 0] = u[1]; raw[o + 1] = u[4]; raw[o + 2] = u[5];
+    raw[o + 3] = u[4]; raw[o + 4] = u[2]; raw[o + 5] = u[6];
+    raw[o + 6] = u[5]; raw[o + 7] = u[6]; raw[o + 8] = u[3];
+    kraw = o + 9;
+  } else {  // magnetic: B = (T b) F / (4 pi G)
+    raw[0] = (u[1] * fd.bx + u[4] * fd.by + u[5] * fd.bz) * fd.scale;
+    raw[1] = (u[4] * fd.bx + u[2] * fd.by + u[6] * fd.bz) * fd.scale;
+    raw[2] = (u[5] * fd.bx + u[6] * fd.by + u[3] * fd.bz) * fd.scale;
+    kraw = 3;
+  }
+  if (raw_out) {
+    for (int m = 0; m < kraw; ++m) raw_out[(long)s * kraw + m] = raw[m];
+  }
+  if (tf == JGM_TF_FTG_INVARIANT) {
+    const double* d = raw;  // GravityTransforms.h:36-40
+    out[s] = d[0] * d[4] + d[4] * d[8] + d[0] * d[8] - d[3] * d[1] - d[7] * d[5] - d[2] * d[6];
+  } else if (tf == JGM_TF_TOTAL_FIELD) {
+    out[s] = sqrt(raw[0] * raw[0] + raw[1] * raw[1] + raw[2] * raw[2]);
+  } else if (tf == JGM_TF_TOTAL_FIELD_ANOMALY) {
+    out[s] = (tfbx * raw[0] + tfby * raw[1] + tfbz * raw[2]);
+  } else {
+    for (int m = 0; m < kraw; ++m) out[(long)s * kraw + m] = raw[m];
+  }
+}
+
+// Misfit of transformed data -> misfit of raw data: D^T r per site
+// (FullSensitivityGravMagCalculator.h:184-208; ThreeDGravMagImplementation.h:212-219).
+__global__ __launch_bounds__(kBlock) void k_misfit_to_raw(const double* __restrict__ misfit,
+                                                          const double* __restrict__ raw, int S,
+                                                          int kraw, int tf, double tfbx,
+                                                          double tfby, double tfbz,
+                                                          double* __restrict__ rm) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  if (tf == JGM_TF_NONE) {
+    for (int m = 0; m < kraw; ++m) rm[(long)s * kraw + m] = misfit[(long)s * kraw + m];
+    return;
+  }
+  const double r = misfit[s];
+  const double* in = raw + (long)s * kraw;
+  double d[9];
+  if (tf == JGM_TF_FTG_INVARIANT) {  // GravityTransforms.h:80-91
+    d[0] = in[4] + in[8]; d[1] = -in[3]; d[2] = -in[6];
+    d[3] = -in[1]; d[4] = in[0] + in[8]; d[5] = -in[7];
+    d[6] = -in[2]; d[7] = -in[5]; d[8] = in[0] + in[4];
+  } else if (tf == JGM_TF_TOTAL_FIELD) {  // MagneticTransforms.h:66-72
+    const double t = sqrt(in[0] * in[0] + in[1] * in[1] + in[2] * in[2]);
+    d[0] = in[0] / t; d[1] = in[1] / t; d[2] = in[2] / t;
+  } else {
+    d[0] = tfbx; d[1] = tfby; d[2] = tfbz;
+  }
+  for (int m = 0; m < kraw; ++m) rm[(long)s * kraw + m] = d[m] * r;
+}
+
+// Per-site coefficients of the 7 corner sums for the adjoint:
+// coef[s][m] = sum_k rraw[s][k] * dJ_k/dS_m where S_m is the corner sum of component m
+// (including its sign and G).  Stored in the kernel's component order for MASK.
+template <int MASK>
+__global__ __launch_bounds__(kBlock) void k_adjoint_coef(const double* __restrict__ rm, int S,
+                                                         int kind, FieldDir fd,
+                                                         double* __restrict__ coef) {
+  constexpr int NC = Ncomp<MASK>::value;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  double c[7] = {0, 0, 0, 0, 0, 0, 0};  // gz, xx, yy, zz, xy, xz, yz (before sign*G)
+  if (kind == JGM_KIND_SCALAR) {
+    c[0] = rm[s];
+  } else if (kind == JGM_KIND_TENSOR || kind == JGM_KIND_SCALAR_TENSOR) {
+    const int k = kind == JGM_KIND_SCALAR_TENSOR ? 10 : 9;
+    const double* r = rm + (long)s * k;
+    int o = 0;
+    if (kind == JGM_KIND_SCALAR_TENSOR) c[0] = r[o++];
+    c[1] = r[o + 0];
+    c[2] = r[o + 4];
+    c[3] = r[o + 8];
+    c[4] = r[o + 1] + r[o + 3];
+    c[5] = r[o + 2] + r[o + 6];
+    c[6] = r[o + 5] + r[o + 7];
+  } else {
+    const double* r = rm + (long)s * 3;
+    const double f = fd.scale;
+    c[1] = r[0] * fd.bx * f;
+    c[2] = r[1] * fd.by * f;
+    c[3] = r[2] * fd.bz * f;
+    c[4] = (r[0] * fd.by + r[1] * fd.bx) * f;
+    c[5] = (r[0] * fd.bz + r[2] * fd.bx) * f;
+    c[6] = (r[1] * fd.bz + r[2] * fd.by) * f;
+  }
+  double sc[NC];
+  comp_scales<MASK>(sc);
+  int k = 0;
+  if (MASK & kGz) coef[(long)s * NC + k] = c[0] * sc[k], ++k;
+  if (MASK & kDiag)
+    for (int m = 1; m <= 3; ++m) coef[(long)s * NC + k] = c[m] * sc[k], ++k;
+  if (MASK & kOff)
+    for (int m = 4; m <= 6; ++m) coef[(long)s * NC + k] = c[m] * sc[k], ++k;
+}
+
+// Adjoint: one lane per corner; the wave walks a chunk of sites (wave-uniform).
+template <int MASK>
+__global__ __launch_bounds__(kBlock) void k_adjoint_corners(
+    const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
+    const double* __restrict__ coef, int S, int sites_per_chunk,
+    const double* __restrict__ Xc, const double* __restrict__ Yc,
+    const double* __restrict__ Zc, int ncy, int ncz, long ncorner, double* __restrict__ hpart) {
+  constexpr int NC = Ncomp<MASK>::value;
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long ql = q < ncorner ? q : ncorner - 1;
+  const int K = (int)(ql % ncz);
+  const long t = ql / ncz;
+  const int J = (int)(t % ncy);
+  const int I = (int)(t / ncy);
+  const double cx = Xc[I], cy = Yc[J], cz = Zc[K];
+  const int s0 = blockIdx.y * sites_per_chunk;
+  const int s1 = s0 + sites_per_chunk < S ? s0 + sites_per_chunk : S;
+  double h = 0.0;
+  for (int s = s0; s < s1; ++s) {
+    const double x = sx[s] - cx, y = sy[s] - cy, z = sz[s] - cz;
+    double tv[NC];
+    corner_terms_f64<MASK>(x, y, z, tv);
+    const double* cs = coef + (long)s * NC;
+#pragma unroll
+    for (int m = 0; m < NC; ++m) h = fma(cs[m], tv[m], h);
+  }
+  if (q < ncorner) hpart[(long)blockIdx.y * ncorner + q] = h;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_chunks(const double* __restrict__ part,
+                                                       int nchunks, long n,
+                                                       double* __restrict__ out) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  double a = 0.0;
+  for (int c = 0; c < nchunks; ++c) a += part[(long)c * n + q];
+  out[q] = a;
+}
+
+// g_c = 2 sum_{a,b,c} sigma(a,b,c) H(i+a, j+b, k+c)
+__global__ __launch_bounds__(kBlock) void k_corner_difference(const double* __restrict__ H,
+                                                              int nx, int ny, int nz,
+                                                              double* __restrict__ grad) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = (long)nx * ny * nz;
+  if (c >= n) return;
+  const int k = (int)(c % nz);
+  const long t = c / nz;
+  const int j = (int)(t % ny);
+  const int i = (int)(t / ny);
+  const long ncy = ny + 1, ncz = nz + 1;
+  double acc = 0.0;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int cc = 0; cc < 2; ++cc) {
+        const double v = H[((long)(i + a) * ncy + (j + b)) * ncz + (k + cc)];
+        acc += ((a + b + cc) & 1) ? v : -v;
+      }
+  grad[c] = 2.0 * acc;
+}
+
+// Background gradient: g[N + j] = 2 sum_s sum_k rraw[s][k] * Jbg[s][k][j].
+__global__ __launch_bounds__(kBlock) void k_background_gradient(
+    const double* __restrict__ rm, const double* __restrict__ bgT, int S, int L, int kind,
+    FieldDir fd, double* __restrict__ gbg) {
+  const int j = blockIdx.x;
+  __shared__ double red[kBlock];
+  double acc = 0.0;
+  for (int s = threadIdx.x; s < S; s += blockDim.x) {
+    const double* b = bgT + ((long)s * L + j) * 7;
+    if (kind == JGM_KIND_SCALAR) {
+      acc += rm[s] * b[0];
+    } else if (kind == JGM_KIND_TENSOR || kind == JGM_KIND_SCALAR_TENSOR) {
+      const int k = kind == JGM_KIND_SCALAR_TENSOR ? 10 : 9;
+      const double* r = rm + (long)s * k;
+      int o = 0;
+      if (kind == JGM_KIND_SCALAR_TENSOR) acc += r[o++] * b[0];
+      acc += r[o + 0] * b[1] + r[o + 1] * b[4] + r[o + 2] * b[5] + r[o + 3] * b[4] +
+             r[o + 4] * b[2] + r[o + 5] * b[6] + r[o + 6] * b[5] + r[o + 7] * b[6] +
+             r[o + 8] * b[3];
+    } else {
+      const double* r = rm + (long)s * 3;
+      const double Bx = (b[1] * fd.bx + b[4] * fd.by + b[5] * fd.bz) * fd.scale;
+      const double By = (b[4] * fd.bx + b[2] * fd.by + b[6] * fd.bz) * fd.scale;
+      const double Bz = (b[5] * fd.bx + b[6] * fd.by + b[3] * fd.bz) * fd.scale;
+      acc += r[0] * Bx + r[1] * By + r[2] * Bz;
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) gbg[j] = 2.0 * red[0];
+}
+
+// Raw sensitivity rows for a range of sites (reference layout), one lane per
+// parameter column, per-cell 8-corner evaluation like the OMP imps.
+template <int MASK>
+__global__ __launch_bounds__(kBlock) void k_sensitivity_rows(
+    const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
+    int site0, const double* __restrict__ xlo, const double* __restrict__ ylo,
+    const double* __restrict__ zlo, const double* __restrict__ xsz,
+    const double* __restrict__ ysz, const double* __restrict__ zsz, int nx, int ny, int nz,
+    const double* __restrict__ bgT, int L, int kind, FieldDir fd, double* __restrict__ rows) {
+  constexpr int NC = Ncomp<MASK>::value;
+  const long N = (long)nx * ny * nz;
+  const long np = N + L;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= np) return;
+  const int sl = blockIdx.y;  // local site index
+  const int s = site0 + sl;
+  double u[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (c < N) {
+    const int k = (int)(c % nz);
+    const long t = c / nz;
+    const int j = (int)(t % ny);
+    const int i = (int)(t / ny);
+    double v[NC];
+    box_terms_f64<MASK>(sx[s], sy[s], sz[s], xlo[i], ylo[j], zlo[k], xsz[i], ysz[j], zsz[k], v);
+    int m = 0;
+    if (MASK & kGz) u[0] = v[m++];
+    if (MASK & kDiag) { u[1] = v[m++]; u[2] = v[m++]; u[3] = v[m++]; }
+    if (MASK & kOff) { u[4] = v[m++]; u[5] = v[m++]; u[6] = v[m++]; }
+  } else {
+    const double* b = bgT + ((long)s * L + (c - N)) * 7;
+    for (int m = 0; m < 7; ++m) u[m] = b[m];
+  }
+  const int kraw = (int)raw_per_site(kind);
+  double* base = rows + (long)sl * kraw * np + c;
+  if (kind == JGM_KIND_SCALAR) {
+    base[0] = u[0];
+  } else if (kind == JGM_KIND_TENSOR || kind == JGM_KIND_SCALAR_TENSOR) {
+    int o = 0;
+    if (kind == JGM_KIND_SCALAR_TENSOR) base[(long)(o++) * np] = u[0];
+    const double t9[9] = {u[1], u[4], u[5], u[4], u[2], u[6], u[5], u[6], u[3]};
+    for (int m = 0; m < 9; ++m) base[(long)(o + m) * np] = t9[m];
+  } else {
+    base[0] = (u[1] * fd.bx + u[4] * fd.by + u[5] * fd.bz) * fd.scale;
+    base[np] = (u[4] * fd.bx + u[2] * fd.by + u[6] * fd.bz) * fd.scale;
+    base[2 * np] = (u[5] * fd.bx + u[6] * fd.by + u[3] * fd.bz) * fd.scale;
+  }
+}
+
+inline unsigned blocks_for(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+int mask_of(int kind) {
+  switch (kind) {
+    case JGM_KIND_SCALAR: return kMaskScalar;
+    case JGM_KIND_SCALAR_TENSOR: return kMaskAll;
+    default: return kMaskTensor;
+  }
+}
+
+int ncomp_of(int mask) {
+  return ((mask & kGz) ? 1 : 0) + ((mask & kDiag) ? 3 : 0) + ((mask & kOff) ? 3 : 0);
+}
+
+template <int MASK>
+hipError_t forward_impl(jgm_context* ctx, int kind, const double* model, double* out,
+                        double* raw_out) {
+  constexpr int NC = Ncomp<MASK>::value;
+  hipStream_t st = ctx->stream;
+  const int S = (int)ctx->S;
+  const long N = (long)ncells(ctx);
+  const int L = (int)ctx->bgthick.size();
+  const int ncx = (int)ctx->nx + 1, ncy = (int)ctx->ny + 1, ncz = (int)ctx->nz + 1;
+  const long ncols = (long)ncx * ncy;
+  const int site_blocks = (S + kBlock - 1) / kBlock;
+  int nchunks = 1;
+  if (N > 0) {
+    const long waves_per_chunk = (long)site_blocks * (kBlock / 64);
+    long want = (kTargetWaves + waves_per_chunk - 1) / waves_per_chunk;
+    want = std::min<long>(want, ncols);
+    nchunks = (int)std::max<long>(1, want);
+  }
+  const long cols_per_chunk = (ncols + nchunks - 1) / nchunks;
+  nchunks = (int)((ncols + cols_per_chunk - 1) / cols_per_chunk);
+  hipError_t e;
+  if (N > 0) {
+    e = ctx->weights.ensure(sizeof(double) * (size_t)ncols * ncz);
+    if (e != hipSuccess) return e;
+    e = ctx->partial.ensure(sizeof(double) * (size_t)nchunks * NC * S);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_corner_weights, dim3(blocks_for(ncols * ncz)), dim3(kBlock), 0, st,
+                       model, (int)ctx->nx, (int)ctx->ny, (int)ctx->nz,
+                       ctx->weights.as<double>());
+    hipLaunchKernelGGL((k_forward_corners<MASK>), dim3(site_blocks, nchunks), dim3(kBlock), 0, st,
+                       ctx->d_sx.as<double>(), ctx->d_sy.as<double>(), ctx->d_sz.as<double>(), S,
+                       ctx->d_Xc.as<double>(), ctx->d_Yc.as<double>(), ctx->d_Zc.as<double>(),
+                       ncy, ncz, ctx->weights.as<const double>(), ncols, cols_per_chunk,
+                       ctx->partial.as<double>());
+  } else {
+    nchunks = 0;
+  }
+  const double tf_bx = std::cos(ctx->tf_inc) * std::cos(ctx->tf_dec);
+  const double tf_by = std::cos(ctx->tf_inc) * std::sin(ctx->tf_dec);
+  const double tf_bz = std::sin(ctx->tf_inc);
+  hipLaunchKernelGGL((k_forward_finish<MASK>), dim3(blocks_for(S)), dim3(kBlock), 0, st,
+                     ctx->partial.as<const double>(), nchunks, S, ctx->bgT.as<const double>(),
+                     model + N, L, kind, field_dir(ctx), ctx->transform, tf_bx, tf_by, tf_bz,
+                     out, raw_out);
+  return hipGetLastError();
+}
+
+template <int MASK>
+hipError_t adjoint_impl(jgm_context* ctx, int kind, const double* rm, double* grad) {
+  constexpr int NC = Ncomp<MASK>::value;
+  hipStream_t st = ctx->stream;
+  const int S = (int)ctx->S;
+  const long N = (long)ncells(ctx);
+  const int L = (int)ctx->bgthick.size();
+  const int ncy = (int)ctx->ny + 1, ncz = (int)ctx->nz + 1;
+  const long nq = (long)ncorners(ctx);
+  hipError_t e = ctx->coef.ensure(sizeof(double) * (size_t)S * NC);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_adjoint_coef<MASK>), dim3(blocks_for(S)), dim3(kBlock), 0, st, rm, S,
+                     kind, field_dir(ctx), ctx->coef.as<double>());
+  if (N > 0) {
+    const long corner_waves = (nq + 63) / 64;
+    int nchunks = (int)std::min<long>(S, std::max<long>(1, (kTargetWaves + corner_waves - 1) / corner_waves));
+    const int per = (S + nchunks - 1) / nchunks;
+    nchunks = (S + per - 1) / per;
+    e = ctx->hpart.ensure(sizeof(double) * (size_t)nchunks * nq);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_adjoint_corners<MASK>), dim3(blocks_for(nq), nchunks), dim3(kBlock), 0,
+                       st, ctx->d_sx.as<double>(), ctx->d_sy.as<double>(), ctx->d_sz.as<double>(),
+                       ctx->coef.as<const double>(), S, per, ctx->d_Xc.as<double>(),
+                       ctx->d_Yc.as<double>(), ctx->d_Zc.as<double>(), ncy, ncz, nq,
+                       ctx->hpart.as<double>());
+    const double* H = ctx->hpart.as<const double>();
+    if (nchunks > 1) {
+      e = ctx->hsum.ensure(sizeof(double) * (size_t)nq);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(k_sum_chunks, dim3(blocks_for(nq)), dim3(kBlock), 0, st, H, nchunks, nq,
+                         ctx->hsum.as<double>());
+      H = ctx->hsum.as<const double>();
+    }
+    hipLaunchKernelGGL(k_corner_difference, dim3(blocks_for(N)), dim3(kBlock), 0, st, H,
+                       (int)ctx->nx, (int)ctx->ny, (int)ctx->nz, grad);
+  }
+  if (L > 0) {
+    hipLaunchKernelGGL(k_background_gradient, dim3(L), dim3(kBlock), 0, st, rm,
+                       ctx->bgT.as<const double>(), S, L, kind, field_dir(ctx), grad + N);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_background_table(jgm_context* ctx) {
+  const int S = (int)ctx->S;
+  const int L = (int)ctx->bgthick.size();
+  hipError_t e = ctx->bgT.ensure(sizeof(double) * (size_t)std::max(1, S) * std::max(1, L) * 7);
+  if (e != hipSuccess) return e;
+  if (L == 0 || S == 0) {
+    ctx->bg_valid = true;
+    return hipSuccess;
+  }
+  DeviceBuffer thick;
+  e = thick.ensure(sizeof(double) * L);
+  if (e != hipSuccess) return e;
+  e = hipMemcpyAsync(thick.ptr, ctx->bgthick.data(), sizeof(double) * L, hipMemcpyHostToDevice,
+                     ctx->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_background_table, dim3(blocks_for(S)), dim3(kBlock), 0, ctx->stream,
+                       ctx->d_sx.as<double>(), ctx->d_sy.as<double>(), ctx->d_sz.as<double>(), S,
+                       thick.as<const double>(), L, ctx->xc[0], ctx->yc[0], ctx->zc[0],
+                       ctx->xwidth, ctx->ywidth, ctx->zwidth, ctx->bgT.as<double>());
+    e = hipGetLastError();
+  }
+  hipError_t e2 = hipStreamSynchronize(ctx->stream);
+  thick.release();
+  if (e != hipSuccess) return e;
+  if (e2 != hipSuccess) return e2;
+  ctx->bg_valid = true;
+  return hipSuccess;
+}
+
+hipError_t launch_forward_fp64(jgm_context* ctx, int kind, const double* model, double* out,
+                               double* raw_out) {
+  switch (mask_of(kind)) {
+    case kMaskScalar: return forward_impl<kMaskScalar>(ctx, kind, model, out, raw_out);
+    case kMaskAll: return forward_impl<kMaskAll>(ctx, kind, model, out, raw_out);
+    default: return forward_impl<kMaskTensor>(ctx, kind, model, out, raw_out);
+  }
+}
+
+hipError_t launch_misfit_to_raw(jgm_context* ctx, int kind, const double* misfit,
+                                const double* raw, double* raw_misfit) {
+  const double tf_bx = std::cos(ctx->tf_inc) * std::cos(ctx->tf_dec);
+  const double tf_by = std::cos(ctx->tf_inc) * std::sin(ctx->tf_dec);
+  const double tf_bz = std::sin(ctx->tf_inc);
+  hipLaunchKernelGGL(k_misfit_to_raw, dim3(blocks_for((long)ctx->S)), dim3(kBlock), 0,
+                     ctx->stream, misfit, raw, (int)ctx->S, (int)raw_per_site(kind),
+                     ctx->transform, tf_bx, tf_by, tf_bz, raw_misfit);
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint_fp64(jgm_context* ctx, int kind, const double* /*model*/,
+                               const double* raw_misfit, double* grad) {
+  switch (mask_of(kind)) {
+    case kMaskScalar: return adjoint_impl<kMaskScalar>(ctx, kind, raw_misfit, grad);
+    case kMaskAll: return adjoint_impl<kMaskAll>(ctx, kind, raw_misfit, grad);
+    default: return adjoint_impl<kMaskTensor>(ctx, kind, raw_misfit, grad);
+  }
+}
+
+hipError_t launch_sensitivity_rows(jgm_context* ctx, int kind, size_t site0, size_t nsites,
+                                   double* rows) {
+  const long np = (long)nparams(ctx);
+  const int L = (int)ctx->bgthick.size();
+  dim3 grid(blocks_for(np), (unsigned)nsites);
+  const double* a[6] = {ctx->d_xlo.as<double>(), ctx->d_ylo.as<double>(), ctx->d_zlo.as<double>(),
+                        ctx->d_xsz.as<double>(), ctx->d_ysz.as<double>(), ctx->d_zsz.as<double>()};
+#define JGM_ROWS(M)                                                                              \
+  hipLaunchKernelGGL((k_sensitivity_rows<M>), grid, dim3(kBlock), 0, ctx->stream,                \
+                     ctx->d_sx.as<double>(), ctx->d_sy.as<double>(), ctx->d_sz.as<double>(),     \
+                     (int)site0, a[0], a[1], a[2], a[3], a[4], a[5], (int)ctx->nx, (int)ctx->ny, \
+                     (int)ctx->nz, ctx->bgT.as<const double>(), L, kind, field_dir(ctx), rows)
+  switch (mask_of(kind)) {
+    case kMaskScalar: JGM_ROWS(kMaskScalar); break;
+    case kMaskAll: JGM_ROWS(kMaskAll); break;
+    default: JGM_ROWS(kMaskTensor); break;
+  }
+#undef JGM_ROWS
+  return hipGetLastError();
+}
+
+}  // namespace jgm

@@ -1,0 +1,135 @@
+// Device-side prism mathematics for gfx950.
+//
+// The closed form of jif3D's box terms is an 8-corner alternating sum of a corner
+// function F(x, y, z) of the site-to-corner offset (BasicGravElements.cpp:26-59,
+// :82-244).  Every corner of the mesh is shared by up to eight cells with the same
+// sign pattern sigma(a,b,c) = (-1)^(a+b+c+1) for the corner at offset (a,b,c) of a
+// cell.  So for one site
+//
+//   sum_c rho_c J[s,c] = sign_m * G * sum_corners F_m(s, corner) * w(corner),
+//   w(I,J,K) = sum_{a,b,c in {0,1}} sigma(a,b,c) rho(I-a, J-b, K-c),
+//
+// and for the adjoint g_c = 2 sum_s r_s J[s,c] = 2 sum_{corners of c} sigma H(corner)
+// with H(corner) = sum_s sum_m coef[s][m] F_m(s, corner).  Each corner function is
+// evaluated once per (site, corner) pair: (nx+1)(ny+1)(nz+1) evaluations per site
+// instead of 8 nx ny nz.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <cstdint>
+
+namespace jgm {
+
+constexpr double kGravConst = 6.67428e-11;  // jif3D/Gravity/BasicGravElements.h:22
+constexpr double kPi = 3.14159265358979323846;
+
+// Component masks of the corner function.  Component order inside kernels:
+// gz, Uxx, Uyy, Uzz, Uxy, Uxz, Uyz.
+enum : int { kGz = 1, kDiag = 2, kOff = 4 };
+constexpr int kMaskScalar = kGz;
+constexpr int kMaskTensor = kDiag | kOff;
+constexpr int kMaskAll = kGz | kDiag | kOff;
+
+template <int MASK>
+struct Ncomp {
+  static constexpr int value = ((MASK & kGz) ? 1 : 0) + ((MASK & kDiag) ? 3 : 0) + ((MASK & kOff) ? 3 : 0);
+};
+
+// Corner function in FP64.  x, y, z = site - corner.
+//  gz  : x log(y+r) + y log(x+r) + z atan2(z r, x y), r floored at DBL_EPSILON (:26-37)
+//  Uxx : atan2(y z, x r); Uyy: atan2(x z, y r); Uzz: atan2(x y, z r)         (:43-47, :223-229)
+//  Uxy : log(max(z + r, eps)); Uxz: log(max(y + r, eps)); Uyz: log(max(x + r, eps))
+//                                                                            (:53-59, :231-241)
+// With the gravity term alone, the logs keep the unguarded form of :32 so that a
+// site on a mesh line reproduces the reference bit for bit in its special cases;
+// with the tensor terms present gz reuses the guarded logs of the off-diagonal
+// terms (they coincide unless x = z = 0 and y < 0, where the reference's gz term
+// is 0 * log(0) = NaN).
+template <int MASK>
+__device__ __forceinline__ void corner_terms_f64(double x, double y, double z, double* t) {
+  const double r = sqrt(x * x + y * y + z * z);
+  int k = 0;
+  if constexpr ((MASK & kOff) != 0) {
+    const double lz = log(fmax(z + r, DBL_EPSILON));
+    const double ly = log(fmax(y + r, DBL_EPSILON));
+    const double lx = log(fmax(x + r, DBL_EPSILON));
+    if constexpr ((MASK & kGz) != 0) {
+      const double rg = fmax(r, DBL_EPSILON);
+      double v = x * ly + y * lx;
+      v += z * atan2(z * rg, x * y);
+      t[k++] = v;
+    }
+    if constexpr ((MASK & kDiag) != 0) {
+      t[k++] = atan2(y * z, x * r);
+      t[k++] = atan2(x * z, y * r);
+      t[k++] = atan2(x * y, z * r);
+    }
+    t[k++] = lz;
+    t[k++] = ly;
+    t[k++] = lx;
+  } else {
+    if constexpr ((MASK & kGz) != 0) {
+      const double rg = fmax(r, DBL_EPSILON);
+      double v = x * log(y + rg) + y * log(x + rg);
+      v += z * atan2(z * rg, x * y);
+      t[k++] = v;
+    }
+    if constexpr ((MASK & kDiag) != 0) {
+      t[k++] = atan2(y * z, x * r);
+      t[k++] = atan2(x * z, y * r);
+      t[k++] = atan2(x * y, z * r);
+    }
+  }
+}
+
+// Sign and scale applied to each accumulated corner sum, in the kernel's
+// component order: gz and the off-diagonal (log) terms are returned as -G * sum
+// (BasicGravElements.cpp:108, :201), the diagonal (atan2) terms as +G * sum (:151).
+template <int MASK>
+__host__ __device__ inline void comp_scales(double* s) {
+  int k = 0;
+  if (MASK & kGz) s[k++] = -kGravConst;
+  if (MASK & kDiag) { s[k++] = kGravConst; s[k++] = kGravConst; s[k++] = kGravConst; }
+  if (MASK & kOff) { s[k++] = -kGravConst; s[k++] = -kGravConst; s[k++] = -kGravConst; }
+}
+
+// Full per-cell box term (8 corners) for the background boxes and the
+// per-cell sensitivity rows.  Returns sign-applied, G-scaled values in the
+// kernel's component order, like CalcGravBoxTerm / CalcTensorBoxTerm.
+template <int MASK>
+__device__ inline void box_terms_f64(double mx, double my, double mz, double ux, double uy,
+                                     double uz, double sx, double sy, double sz, double* out) {
+  constexpr int NC = Ncomp<MASK>::value;
+  double acc[NC];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) acc[m] = 0.0;
+  const double xs[2] = {mx - ux, mx - (ux + sx)};
+  const double ys[2] = {my - uy, my - (uy + sy)};
+  const double zs[2] = {mz - uz, mz - (uz + sz)};
+  // corners in the reference's summation order (BasicGravElements.cpp:88-106)
+  constexpr int order[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
+                               {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+#pragma unroll
+  for (int n = 0; n < 8; ++n) {
+    const int a = order[n][0], b = order[n][1], c = order[n][2];
+    double t[NC];
+    corner_terms_f64<MASK>(xs[a], ys[b], zs[c], t);
+    if (n == 0) {
+#pragma unroll
+      for (int m = 0; m < NC; ++m) acc[m] = -t[m];
+    } else if ((a + b + c) & 1) {
+#pragma unroll
+      for (int m = 0; m < NC; ++m) acc[m] += t[m];
+    } else {
+#pragma unroll
+      for (int m = 0; m < NC; ++m) acc[m] -= t[m];
+    }
+  }
+  double sc[NC];
+  comp_scales<MASK>(sc);
+#pragma unroll
+  for (int m = 0; m < NC; ++m) out[m] = acc[m] * sc[m];
+}
+
+}  // namespace jgm

@@ -1,0 +1,178 @@
+"""Implementation and calculator classes mirroring jif3D's GravMag bridge.
+
+* ScalarHipGravityImp, TensorHipGravityImp, HipMagneticSusceptibilityImp and
+  ScalarTensorHipGravityImp play the role of ScalarOMPGravityImp /
+  TensorOMPGravityImp / OMPMagneticSusceptibilityImp
+  (jif3D/Gravity/ScalarOMPGravityImp.cpp:124-282, TensorOMPGravityImp.cpp:112-267,
+  jif3D/Magnetics/OMPMagneticSusceptibilityImp.cpp:31-201): Calculate() and
+  LQDerivative() of ThreeDGravMagImplementation (jif3D/GravMag/
+  ThreeDGravMagImplementation.h:116-231) run on the GPU through the C ABI.
+* MinMemGravMagCalculator mirrors jif3D/GravMag/MinMemGravMagCalculator.h:25-79.
+* FTGInvariant / TotalField / TotalFieldAnomaly select the per-site transforms of
+  jif3D/Gravity/GravityTransforms.h:25-100 and jif3D/Magnetics/MagneticTransforms.h:19-156.
+
+Arrays are numpy on the host; the device work happens inside libjif3d_amd.so.
+"""
+import numpy as np
+
+from . import _native as nat
+
+
+class FTGInvariant:
+    code = nat.TF_FTG_INVARIANT
+    angles = (0.0, 0.0)
+
+    def GetInputSize(self):
+        return 9
+
+    def GetOutputSize(self):
+        return 1
+
+
+class TotalField:
+    code = nat.TF_TOTAL_FIELD
+    angles = (0.0, 0.0)
+
+    def GetInputSize(self):
+        return 3
+
+    def GetOutputSize(self):
+        return 1
+
+
+class TotalFieldAnomaly:
+    code = nat.TF_TOTAL_FIELD_ANOMALY
+
+    def __init__(self, inc=0.0, dec=0.0):
+        self.angles = (float(inc), float(dec))
+
+    def GetInputSize(self):
+        return 3
+
+    def GetOutputSize(self):
+        return 1
+
+
+def _geometry_key(model, data):
+    parts = list(model.coords) + list(model.sizes) + [np.asarray(model.bg_thicknesses)]
+    parts += list(data.positions)
+    return tuple(np.asarray(p, dtype=np.float64).tobytes() for p in parts)
+
+
+class HipGravMagImplementation:
+    """GPU implementation object; kind and precision fixed at construction."""
+
+    kind = None
+
+    def __init__(self, precision=nat.FP64, device=0, stream=None):
+        self.precision = precision
+        self.Transform = None
+        self._ctx = None
+        self._key = None
+        self._device = device
+        self._stream = stream
+
+    def RawDataPerMeasurement(self):
+        return nat.RAW_PER_SITE[self.kind]
+
+    def SetDataTransform(self, transform):
+        self.Transform = transform
+
+    def GetDataPerMeasurement(self):
+        return self.Transform.GetOutputSize() if self.Transform else self.RawDataPerMeasurement()
+
+    def _field(self):
+        return None
+
+    def context(self, model, data):
+        if self._ctx is None:
+            self._ctx = nat.Context(self._device, self._stream)
+        key = _geometry_key(model, data)
+        if key != self._key:
+            self._ctx.configure(model, data, self._field())
+            self._key = key
+        elif self._field() is not None:
+            self._ctx.set_field(*self._field())
+        tf = self.Transform
+        self._ctx.set_transform(tf.code if tf else nat.TF_NONE, *(tf.angles if tf else (0, 0)))
+        return self._ctx
+
+    def Calculate(self, model, data):
+        ctx = self.context(model, data)
+        return ctx.forward(self.kind, self.precision, model.GetModelParameters())
+
+    def LQDerivative(self, model, data, misfit):
+        ctx = self.context(model, data)
+        misfit = np.asarray(misfit, dtype=np.float64)
+        need = data.nsites * self.GetDataPerMeasurement()
+        if misfit.size != need:
+            raise ValueError(f"misfit has {misfit.size} entries, expected {need}")
+        return ctx.adjoint(self.kind, self.precision, model.GetModelParameters(), misfit)
+
+    def SensitivityRows(self, model, data, site0=0, nsites=None):
+        """Raw FP64 J rows (reference layout) of a site range, copied to the host."""
+        import torch
+        ctx = self.context(model, data)
+        nsites = data.nsites - site0 if nsites is None else nsites
+        k = self.RawDataPerMeasurement()
+        out = torch.empty((nsites * k, ctx.n_params()), dtype=torch.float64,
+                          device=f"cuda:{self._device}")
+        ctx.sensitivity_rows(self.kind, site0, nsites, out)
+        ctx.synchronize()
+        return out.cpu().numpy()
+
+
+class ScalarHipGravityImp(HipGravMagImplementation):
+    kind = nat.KIND_SCALAR
+
+
+class TensorHipGravityImp(HipGravMagImplementation):
+    kind = nat.KIND_TENSOR
+
+
+class ScalarTensorHipGravityImp(HipGravMagImplementation):
+    """gz and the 9 FTG values per site in one pass (10 raw values per site)."""
+    kind = nat.KIND_SCALAR_TENSOR
+
+
+class HipMagneticSusceptibilityImp(HipGravMagImplementation):
+    """OMPMagneticSusceptibilityImp(Inc, Dec, Fs): radians, radians, nT."""
+    kind = nat.KIND_MAGNETIC
+
+    def __init__(self, Inc, Dec, Fs, precision=nat.FP64, device=0, stream=None):
+        super().__init__(precision, device, stream)
+        self.Inclination, self.Declination, self.FieldStrength = float(Inc), float(Dec), float(Fs)
+
+    def _field(self):
+        return (self.Inclination, self.Declination, self.FieldStrength)
+
+
+class ThreeDGravMagCalculator:
+    """Calculator base (jif3D/GravMag/ThreeDGravMagCalculator.h:48-173)."""
+
+    def __init__(self, imp):
+        self.Imp = imp
+        self.Transform = None
+
+    def SetDataTransform(self, transform):
+        self.Transform = transform
+        self.Imp.SetDataTransform(transform)
+
+    def GetDataPerMeasurement(self):
+        return self.Imp.GetDataPerMeasurement()
+
+    def CheckModelConsistency(self, model):
+        model.CheckConsistency()
+
+    def Calculate(self, model, data):
+        self.CheckModelConsistency(model)
+        return self.Imp.Calculate(model, data)
+
+    def LQDerivative(self, model, data, misfit):
+        self.CheckModelConsistency(model)
+        return self.Imp.LQDerivative(model, data, misfit)
+
+
+class MinMemGravMagCalculator(ThreeDGravMagCalculator):
+    """No stored sensitivities: forward and gradient are recomputed matrix-free."""
+    pass

@@ -1,0 +1,68 @@
+"""CPU-only checks of the C ABI: the library loads, exports what include/*.h declares,
+and rejects bad arguments in host code before touching a device."""
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+from golden_paths import ROOT
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names.update(re.findall(r"\b(jgm_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    from jif3d_amd import _native
+    lib = _native.load()
+    syms = declared_symbols()
+    assert len(syms) >= 20
+    missing = [s for s in sorted(syms) if not hasattr(lib, s)]
+    assert not missing, missing
+    assert set(_native.EXPORTS) <= syms
+
+
+def test_api_version_and_kinds():
+    from jif3d_amd import _native
+    lib = _native.load()
+    assert lib.jgm_api_version() == 1
+    assert [lib.jgm_raw_per_site(k) for k in (1, 2, 3, 4, 99)] == [1, 9, 3, 10, 0]
+
+
+def test_null_context_is_rejected():
+    from jif3d_amd import _native
+    lib = _native.load()
+    assert lib.jgm_forward(None, 1, 0, None, None) == 1
+    assert b"null context" in lib.jgm_last_error()
+    assert lib.jgm_set_mesh(None, 1, 1, 1, None, None, None, None, None, None) == 1
+    assert lib.jgm_storedj_apply(None, 0, None, 1, None) == 1
+    assert lib.jgm_n_params(None) == 0
+
+
+def test_create_without_device_fails_cleanly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from jif3d_amd import _native
+    lib = _native.load()
+    h = ctypes.c_void_p()
+    st = lib.jgm_create(0, None, ctypes.byref(h))
+    assert st != 0
+    assert len(lib.jgm_last_error()) > 0
+    assert not h.value
+
+
+def test_product_package_does_not_import_the_oracle():
+    """The product path must never route through the CPU oracle."""
+    pkg = os.path.join(ROOT, "jif3d_amd")
+    pat = re.compile(r"(import\s+oracle|from\s+oracle|liboracle|gravmag_oracle)")
+    for path in glob.glob(os.path.join(pkg, "**", "*.*"), recursive=True):
+        if path.endswith((".py", ".cpp", ".hip", ".hpp", ".h")):
+            assert not pat.search(open(path).read()), path
