@@ -1,0 +1,197 @@
+"""Headline benchmark: gz + FTG forward and adjoint on one MI355X.
+
+Workload: 128x128x64 cubic cells of 20 m (1,048,576 cells), 64x64 sites at
+z = -30 m on cell centres, density 2670 + N(0, 20) + five dipping dikes
+(+/-300 kg/m^3), two background layers.  One step = one forward (gz + the 9 FTG
+values per site) plus one matrix-free adjoint (gradient of a least-squares misfit
+over all 10 values per site) with all inputs resident in HBM.
+
+value = site-cell pairs per second of a forward+adjoint step (4096 x 1048576 pairs
+per step) for the mixed-precision path; the plain FP64 path is timed beside it.
+The CPU baseline is the oracle (the OpenMP restatement of the reference's OMP imps:
+scalar and tensor calculators run separately, as jif3D does) on a bounded sample of
+sites, scaled linearly (work per site is identical).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp64|both]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("site·cell box-term evals/s fwd+adjoint (mixed vs FP64) at stated error; "
+          "% vector peak")
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12}  # MI355X vector FP64 / FP32 (flop/s)
+
+
+def fe_per_pair(rho_c):
+    """Flop-equivalent contract per site-cell pair for gz + FTG, forward only."""
+    return 133.0 * rho_c + 70.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_gpu(args, model, data, prec_name):
+    from jif3d_amd import _native as nat
+    prec = nat.MIXED if prec_name == "mixed" else nat.FP64
+    kind = nat.KIND_SCALAR_TENSOR
+    ctx = nat.Context(0)
+    ctx.configure(model, data)
+    S, N = data.nsites, model.ncells
+    mv = ctx.to_device(model.GetModelParameters())
+    out = ctx.alloc(S * 10)
+    rng = np.random.default_rng(3)
+    resid = ctx.to_device(rng.normal(size=S * 10) * 1e-9)
+    grad = ctx.alloc(ctx.n_params())
+
+    def step():
+        ctx.forward(kind, prec, mv, out)
+        ctx.adjoint(kind, prec, mv, resid, grad)
+
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    log(f"[bench] {prec_name}: warmup done")
+    ctx.profile_enable(True)
+    ctx.profile_reset()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    prof = ctx.profile()
+    ctx.profile_enable(False)
+    ms = (t1 - t0) * 1e3 / args.steps
+    log(f"[bench] {prec_name}: {ms:.2f} ms/step")
+    result = {"ms_per_step": ms, "pairs_per_s": S * N / (ms * 1e-3), "kernels": prof,
+              "out": out.to_host(), "grad": grad.to_host()}
+    return result
+
+
+def cpu_baseline(model, data, budget_s):
+    from oracle import oracle as orc
+    mv = model.GetModelParameters()
+    x, y, z = data.positions
+    S, N = data.nsites, model.ncells
+    rng = np.random.default_rng(11)
+
+    def run(idx):
+        sites = (x[idx], y[idx], z[idx])
+        t = time.perf_counter()
+        g = orc.calculate(orc.SCALAR, model, mv, sites)
+        tt = orc.calculate(orc.TENSOR, model, mv, sites)
+        r1 = rng.normal(size=len(idx))
+        r9 = rng.normal(size=9 * len(idx))
+        orc.lqderivative(orc.SCALAR, model, mv, sites, r1)
+        orc.lqderivative(orc.TENSOR, model, mv, sites, r9)
+        return time.perf_counter() - t, g, tt
+
+    order = rng.permutation(S)
+    t1, g1, t9 = run(order[:1])
+    n = int(max(1, min(S - 1, budget_s / max(t1, 1e-3))))
+    idx = order[1:1 + n]
+    t, g, tt = run(idx)
+    pairs_per_s = len(idx) * N / t
+    return {"value": pairs_per_s, "unit": "site·cell pairs/s (forward + adjoint)",
+            "cores": orc.num_threads(), "kind": "port",
+            "sample": f"{len(idx)} random sites x {N} cells, gz and FTG calculators separately, "
+                      f"Calculate + LQDerivative each; {t:.1f} s",
+            "sites": idx, "gz": g, "ftg": tt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", default="both", choices=["mixed", "fp64", "both"])
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--small", action="store_true", help="quarter-size mesh for quick checks")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards "
+              "by sites (forward) and cells (adjoint), see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    from jif3d_amd import synthetic
+    if args.small:
+        model, data = synthetic.config4_gravity(n=64, nzc=32, nsite=32)
+    else:
+        model, data = synthetic.config4_gravity()
+    S, N = data.nsites, model.ncells
+    rho_c = (model.nx + 1) * (model.ny + 1) * (model.nz + 1) / N
+    log(f"[bench] workload: {model.nx}x{model.ny}x{model.nz} cells, {S} sites")
+
+    precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    res = {p: run_gpu(args, model, data, p) for p in precs}
+    main_p = "mixed" if "mixed" in res else "fp64"
+    r = res[main_p]
+
+    # dominant kernel and its roofline (flop-equivalent contract per pair)
+    kern = max(r["kernels"].items(), key=lambda kv: kv[1][0])
+    kname, (kms, klaunch) = kern
+    avg_s = kms / klaunch * 1e-3
+    fe = S * N * fe_per_pair(rho_c)  # forward-equivalent work of one launch
+    peak = PEAK["fp64"] if "fp64" in kname else PEAK["fp32"]
+    achieved = fe / avg_s
+    line = {
+        "metric": METRIC,
+        "value": r["pairs_per_s"],
+        "unit": "site·cell pairs/s (forward + adjoint per pair)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if main_p == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"gz+FTG forward+adjoint, {model.nx}x{model.ny}x{model.nz} cells "
+                               f"(20 m), {int(np.sqrt(S))}x{int(np.sqrt(S))} sites at z=-30 m",
+                   "cells": N, "sites": S, "components": 10, "precision": main_p},
+        "roofline": {"bound": "valu", "kernel": kname, "achieved": achieved / 1e12,
+                     "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": None,
+                     "work": f"{fe_per_pair(rho_c):.1f} flop-equivalents per pair x {S * N} pairs"},
+    }
+    if "fp64" in res and main_p != "fp64":
+        line["fp64_pairs_per_s"] = res["fp64"]["pairs_per_s"]
+        line["speedup_vs_fp64"] = res["fp64"]["ms_per_step"] / r["ms_per_step"]
+        ref = res["fp64"]["out"].reshape(-1, 10)
+        mix = r["out"].reshape(-1, 10)
+        comp_err = [float(np.max(np.abs(mix[:, c] - ref[:, c])) / np.max(np.abs(ref[:, c])))
+                    for c in range(10)]
+        gz_pt = float(np.max(np.abs(mix[:, 0] - ref[:, 0]) / np.abs(ref[:, 0])))
+        gref, gmix = res["fp64"]["grad"], r["grad"]
+        line["error_vs_fp64"] = {
+            "max_rel_per_component_inf_norm": max(comp_err),
+            "gz_max_pointwise_rel": gz_pt,
+            "gradient_rel_inf_norm": float(np.max(np.abs(gmix - gref)) / np.max(np.abs(gref)))}
+    line["kernels_ms_per_step"] = {k: v[0] / args.steps for k, v in r["kernels"].items()}
+    if not args.no_cpu:
+        cb = cpu_baseline(model, data, args.cpu_budget)
+        idx = cb.pop("sites")
+        g, t9 = cb.pop("gz"), cb.pop("ftg")
+        ref64 = res["fp64"]["out"].reshape(-1, 10) if "fp64" in res else None
+        if ref64 is not None:
+            e_gz = float(np.max(np.abs(ref64[idx, 0] - g) / np.abs(g)))
+            t9 = t9.reshape(-1, 9)
+            e_t = float(np.max(np.abs(ref64[idx, 1:] - t9)) / np.max(np.abs(t9)))
+            cb["gpu_fp64_vs_oracle"] = {"gz_max_rel": e_gz, "ftg_rel_inf_norm": e_t}
+        line["cpu_baseline"] = cb
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -156,6 +156,26 @@ jgm_status jgm_storedj_apply_t(jgm_context *ctx, jgm_gemm_precision prec, const 
                                size_t nrhs, float *G);
 jgm_status jgm_storedj_release(jgm_context *ctx);
 
+/* Device memory on the context's device, for callers without their own HIP
+ * allocator (the CUDA host wrappers did this inside PrepareData/FreeData,
+ * jif3D/Gravity/gravcuda.cu:8-107).  Copies are ordered on the context's stream
+ * and return when complete. */
+jgm_status jgm_device_alloc(jgm_context *ctx, size_t bytes, void **ptr);
+jgm_status jgm_device_free(jgm_context *ctx, void *ptr);
+jgm_status jgm_copy_to_device(jgm_context *ctx, void *dst, const void *src, size_t bytes);
+jgm_status jgm_copy_to_host(jgm_context *ctx, void *dst, const void *src, size_t bytes);
+
+/* Kernel timing with HIP events recorded on the context's stream around every
+ * kernel launch while enabled (replaces the wall-clock harness of
+ * jif3D/Profiling/time_gravity_size.cpp:107-157).  jgm_profile_get reports the
+ * summed time and launch count of entry `index` (0 .. jgm_profile_count-1);
+ * the name pointer stays valid until the next reset. */
+jgm_status jgm_profile_enable(jgm_context *ctx, int enable);
+jgm_status jgm_profile_reset(jgm_context *ctx);
+size_t jgm_profile_count(jgm_context *ctx);
+jgm_status jgm_profile_get(jgm_context *ctx, size_t index, const char **name, double *total_ms,
+                           size_t *launches);
+
 #ifdef __cplusplus
 }
 #endif

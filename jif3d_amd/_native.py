@@ -24,7 +24,9 @@ EXPORTS = [
     "jgm_set_field", "jgm_set_transform", "jgm_set_mixed_threshold", "jgm_data_per_site",
     "jgm_n_params", "jgm_forward", "jgm_forward_host", "jgm_adjoint", "jgm_adjoint_host",
     "jgm_sensitivity_rows", "jgm_storedj_build", "jgm_storedj_rows", "jgm_storedj_apply",
-    "jgm_storedj_apply_t", "jgm_storedj_release",
+    "jgm_storedj_apply_t", "jgm_storedj_release", "jgm_device_alloc", "jgm_device_free",
+    "jgm_copy_to_device", "jgm_copy_to_host", "jgm_profile_enable", "jgm_profile_reset",
+    "jgm_profile_count", "jgm_profile_get",
 ]
 
 
@@ -73,6 +75,14 @@ def load():
         "jgm_storedj_apply": [vp, i, vp, sz, vp],
         "jgm_storedj_apply_t": [vp, i, vp, sz, vp],
         "jgm_storedj_release": [vp],
+        "jgm_device_alloc": [vp, sz, ctypes.POINTER(vp)],
+        "jgm_device_free": [vp, vp],
+        "jgm_copy_to_device": [vp, vp, vp, sz],
+        "jgm_copy_to_host": [vp, vp, vp, sz],
+        "jgm_profile_enable": [vp, i],
+        "jgm_profile_reset": [vp],
+        "jgm_profile_get": [vp, sz, ctypes.POINTER(ctypes.c_char_p), pd,
+                            ctypes.POINTER(sz)],
     }
     for name, args in sigs.items():
         f = getattr(lib, name)
@@ -84,6 +94,8 @@ def load():
     lib.jgm_n_params.argtypes = [vp]
     lib.jgm_storedj_rows.restype = sz
     lib.jgm_storedj_rows.argtypes = [vp]
+    lib.jgm_profile_count.restype = sz
+    lib.jgm_profile_count.argtypes = [vp]
     _lib = lib
     return lib
 
@@ -102,12 +114,58 @@ def _f64(a):
 
 
 def _devptr(t):
-    """Device address of a torch tensor (torch is only used for device memory)."""
+    """Device address of a DeviceArray (or of any object exposing data_ptr())."""
+    if isinstance(t, DeviceArray):
+        return ctypes.c_void_p(t.ptr)
     return ctypes.c_void_p(t.data_ptr())
 
 
 def _is_device(a):
-    return hasattr(a, "data_ptr") and getattr(a, "is_cuda", False)
+    return isinstance(a, DeviceArray) or (hasattr(a, "data_ptr") and getattr(a, "is_cuda", False))
+
+
+class DeviceArray:
+    """A device allocation made through the library (jgm_device_alloc)."""
+
+    def __init__(self, ctx, shape, dtype=np.float64):
+        self.ctx = ctx
+        self.shape = tuple(int(v) for v in np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        check(ctx.lib.jgm_device_alloc(ctx.handle, self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, ctx, arr, dtype=None):
+        arr = np.ascontiguousarray(arr, dtype=dtype or np.asarray(arr).dtype)
+        d = cls(ctx, arr.shape, arr.dtype)
+        d.upload(arr)
+        return d
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        if arr.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        check(self.ctx.lib.jgm_copy_to_device(self.ctx.handle, ctypes.c_void_p(self.ptr),
+                                              arr.ctypes.data_as(ctypes.c_void_p), self.nbytes))
+
+    def to_host(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(self.ctx.lib.jgm_copy_to_host(self.ctx.handle, out.ctypes.data_as(ctypes.c_void_p),
+                                            ctypes.c_void_p(self.ptr), self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr and self.ctx.handle:
+            self.ctx.lib.jgm_device_free(self.ctx.handle, ctypes.c_void_p(self.ptr))
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Context:
@@ -204,8 +262,33 @@ class Context:
     def storedj_rows(self):
         return self.lib.jgm_storedj_rows(self.handle)
 
+    def alloc(self, shape, dtype=np.float64):
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, arr, dtype=None):
+        return DeviceArray.from_host(self, arr, dtype)
+
+    def profile_enable(self, on=True):
+        check(self.lib.jgm_profile_enable(self.handle, 1 if on else 0))
+
+    def profile_reset(self):
+        check(self.lib.jgm_profile_reset(self.handle))
+
+    def profile(self):
+        """{kernel name: (total ms, launches)} of the timed launches so far."""
+        res = {}
+        for i in range(self.lib.jgm_profile_count(self.handle)):
+            name = ctypes.c_char_p()
+            ms = ctypes.c_double()
+            n = ctypes.c_size_t()
+            check(self.lib.jgm_profile_get(self.handle, i, ctypes.byref(name), ctypes.byref(ms),
+                                           ctypes.byref(n)))
+            res[name.value.decode()] = (ms.value, n.value)
+        return res
+
     def storedj_apply(self, prec, M, Y):
-        check(self.lib.jgm_storedj_apply(self.handle, prec, _devptr(M), M.shape[1], _devptr(Y)))
+        check(self.lib.jgm_storedj_apply(self.handle, prec, _devptr(M), M.shape[1],
+                                         _devptr(Y)))
         return Y
 
     def storedj_apply_t(self, prec, R, G):

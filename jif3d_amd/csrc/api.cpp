@@ -135,6 +135,7 @@ jgm_status jgm_destroy(jgm_context* ctx) {
                                &ctx->out_stage, &ctx->misfit_stage, &ctx->grad_stage,
                                &ctx->farpart, &ctx->nearpart, &ctx->storedJ};
   for (auto* b : bufs) b->release();
+  jgm::prof_clear(ctx);
   delete ctx;
   return JGM_OK;
 }
@@ -478,6 +479,81 @@ jgm_status jgm_storedj_release(jgm_context* ctx) {
   ctx->storedJ.release();
   ctx->storedJ_kind = 0;
   ctx->storedJ_rows = ctx->storedJ_cols = 0;
+  return JGM_OK;
+}
+
+jgm_status jgm_device_alloc(jgm_context* ctx, size_t bytes, void** ptr) {
+  if (!ctx || !ptr) return fail(JGM_ERR_INVALID_ARGUMENT, "null context or output pointer");
+  *ptr = nullptr;
+  DeviceGuard g(ctx->device);
+  if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+  hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+  if (e != hipSuccess) return hip_fail(e, "jgm_device_alloc");
+  return JGM_OK;
+}
+
+jgm_status jgm_device_free(jgm_context* ctx, void* ptr) {
+  if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
+  if (!ptr) return JGM_OK;
+  DeviceGuard g(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  hipError_t e = hipFree(ptr);
+  if (e != hipSuccess) return hip_fail(e, "jgm_device_free");
+  return JGM_OK;
+}
+
+static jgm_status copy(jgm_context* ctx, void* dst, const void* src, size_t bytes,
+                       hipMemcpyKind kind, const char* what) {
+  if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
+  if (bytes == 0) return JGM_OK;
+  if (!dst || !src) return fail(JGM_ERR_INVALID_ARGUMENT, "%s: null pointer", what);
+  DeviceGuard g(ctx->device);
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_fail(e, what);
+  return JGM_OK;
+}
+
+jgm_status jgm_copy_to_device(jgm_context* ctx, void* dst, const void* src, size_t bytes) {
+  return copy(ctx, dst, src, bytes, hipMemcpyHostToDevice, "jgm_copy_to_device");
+}
+
+jgm_status jgm_copy_to_host(jgm_context* ctx, void* dst, const void* src, size_t bytes) {
+  return copy(ctx, dst, src, bytes, hipMemcpyDeviceToHost, "jgm_copy_to_host");
+}
+
+jgm_status jgm_profile_enable(jgm_context* ctx, int enable) {
+  if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
+  ctx->prof.enabled = enable != 0;
+  return JGM_OK;
+}
+
+jgm_status jgm_profile_reset(jgm_context* ctx) {
+  if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
+  DeviceGuard g(ctx->device);
+  jgm::prof_clear(ctx);
+  return JGM_OK;
+}
+
+size_t jgm_profile_count(jgm_context* ctx) {
+  if (!ctx) return 0;
+  DeviceGuard g(ctx->device);
+  (void)jgm::prof_harvest(ctx);
+  return ctx->prof.entries.size();
+}
+
+jgm_status jgm_profile_get(jgm_context* ctx, size_t index, const char** name, double* total_ms,
+                           size_t* launches) {
+  if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
+  DeviceGuard g(ctx->device);
+  hipError_t e = jgm::prof_harvest(ctx);
+  if (e != hipSuccess) return hip_fail(e, "profile");
+  if (index >= ctx->prof.entries.size())
+    return fail(JGM_ERR_INVALID_ARGUMENT, "profile index %zu out of range", index);
+  const auto& en = ctx->prof.entries[index];
+  if (name) *name = en.name.c_str();
+  if (total_ms) *total_ms = en.total_ms;
+  if (launches) *launches = en.launches;
   return JGM_OK;
 }
 

@@ -48,6 +48,23 @@ struct NearBoxes {
   size_t max_box_cells = 0;
 };
 
+// Optional per-kernel timing: HIP events recorded on the context's stream around
+// each launch, harvested lazily.
+struct Profiler {
+  struct Entry {
+    std::string name;
+    double total_ms = 0.0;
+    size_t launches = 0;
+  };
+  struct Pending {
+    size_t entry;
+    hipEvent_t start, stop;
+  };
+  bool enabled = false;
+  std::vector<Entry> entries;
+  std::vector<Pending> pending;
+};
+
 }  // namespace jgm
 
 struct jgm_context {
@@ -94,6 +111,8 @@ struct jgm_context {
   jgm::DeviceBuffer weights, partial, raw, coef, hpart, hsum, model_stage, out_stage, misfit_stage,
       grad_stage, farpart, nearpart;
 
+  jgm::Profiler prof;
+
   // stored sensitivity matrix
   jgm::DeviceBuffer storedJ;
   int storedJ_kind = 0;
@@ -132,6 +151,20 @@ __host__ __device__ inline size_t raw_per_site(int kind) {
     default: return 0;
   }
 }
+
+// Kernel timing hooks (profiler.cpp).  prof_begin returns a token for prof_end,
+// or -1 when profiling is off.
+long prof_begin(jgm_context* ctx, const char* name);
+void prof_end(jgm_context* ctx, long token);
+hipError_t prof_harvest(jgm_context* ctx);
+void prof_clear(jgm_context* ctx);
+
+struct ProfScope {
+  jgm_context* ctx;
+  long token;
+  ProfScope(jgm_context* c, const char* name) : ctx(c), token(prof_begin(c, name)) {}
+  ~ProfScope() { prof_end(ctx, token); }
+};
 
 inline size_t ncells(const jgm_context* c) { return c->nx * c->ny * c->nz; }
 inline size_t ncorners(const jgm_context* c) { return (c->nx + 1) * (c->ny + 1) * (c->nz + 1); }

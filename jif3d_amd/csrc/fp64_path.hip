@@ -479,6 +479,9 @@ hipError_t forward_impl(jgm_context* ctx, int kind, const double* model, double*
     hipLaunchKernelGGL(k_corner_weights, dim3(blocks_for(ncols * ncz)), dim3(kBlock), 0, st,
                        model, (int)ctx->nx, (int)ctx->ny, (int)ctx->nz,
                        ctx->weights.as<double>());
+    ProfScope ps(ctx, MASK == kMaskScalar ? "forward_corners_fp64_gz"
+                      : MASK == kMaskAll  ? "forward_corners_fp64_gz_ftg"
+                                          : "forward_corners_fp64_ftg");
     hipLaunchKernelGGL((k_forward_corners<MASK>), dim3(site_blocks, nchunks), dim3(kBlock), 0, st,
                        ctx->d_sx.as<double>(), ctx->d_sy.as<double>(), ctx->d_sz.as<double>(), S,
                        ctx->d_Xc.as<double>(), ctx->d_Yc.as<double>(), ctx->d_Zc.as<double>(),
@@ -517,11 +520,16 @@ hipError_t adjoint_impl(jgm_context* ctx, int kind, const double* rm, double* gr
     nchunks = (S + per - 1) / per;
     e = ctx->hpart.ensure(sizeof(double) * (size_t)nchunks * nq);
     if (e != hipSuccess) return e;
+    {
+    ProfScope ps(ctx, MASK == kMaskScalar ? "adjoint_corners_fp64_gz"
+                      : MASK == kMaskAll  ? "adjoint_corners_fp64_gz_ftg"
+                                          : "adjoint_corners_fp64_ftg");
     hipLaunchKernelGGL((k_adjoint_corners<MASK>), dim3(blocks_for(nq), nchunks), dim3(kBlock), 0,
                        st, ctx->d_sx.as<double>(), ctx->d_sy.as<double>(), ctx->d_sz.as<double>(),
                        ctx->coef.as<const double>(), S, per, ctx->d_Xc.as<double>(),
                        ctx->d_Yc.as<double>(), ctx->d_Zc.as<double>(), ncy, ncz, nq,
                        ctx->hpart.as<double>());
+    }
     const double* H = ctx->hpart.as<const double>();
     if (nchunks > 1) {
       e = ctx->hsum.ensure(sizeof(double) * (size_t)nq);

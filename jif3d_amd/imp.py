@@ -111,15 +111,15 @@ class HipGravMagImplementation:
 
     def SensitivityRows(self, model, data, site0=0, nsites=None):
         """Raw FP64 J rows (reference layout) of a site range, copied to the host."""
-        import torch
         ctx = self.context(model, data)
         nsites = data.nsites - site0 if nsites is None else nsites
         k = self.RawDataPerMeasurement()
-        out = torch.empty((nsites * k, ctx.n_params()), dtype=torch.float64,
-                          device=f"cuda:{self._device}")
+        out = ctx.alloc((nsites * k, ctx.n_params()))
         ctx.sensitivity_rows(self.kind, site0, nsites, out)
         ctx.synchronize()
-        return out.cpu().numpy()
+        host = out.to_host()
+        out.free()
+        return host
 
 
 class ScalarHipGravityImp(HipGravMagImplementation):
