@@ -130,10 +130,14 @@ jgm_status jgm_destroy(jgm_context* ctx) {
   jgm::DeviceBuffer* bufs[] = {&ctx->d_Xc, &ctx->d_Yc, &ctx->d_Zc, &ctx->d_xlo, &ctx->d_ylo,
                                &ctx->d_zlo, &ctx->d_xsz, &ctx->d_ysz, &ctx->d_zsz, &ctx->d_sx,
                                &ctx->d_sy, &ctx->d_sz, &ctx->bgT, &ctx->near.box,
-                               &ctx->near.tiles, &ctx->weights, &ctx->partial, &ctx->raw,
+                               &ctx->weights, &ctx->partial, &ctx->raw,
                                &ctx->coef, &ctx->hpart, &ctx->hsum, &ctx->model_stage,
                                &ctx->out_stage, &ctx->misfit_stage, &ctx->grad_stage,
-                               &ctx->farpart, &ctx->nearpart, &ctx->storedJ};
+                               &ctx->farpart, &ctx->nearpart, &ctx->coef32, &ctx->storedJ,
+                               &ctx->near.col_ptr, &ctx->near.col_sites, &ctx->near.col_ids,
+                               &ctx->near.col_kmax, &ctx->near.xcen, &ctx->near.ycen,
+                               &ctx->near.zcen, &ctx->near.offx, &ctx->near.offy,
+                               &ctx->near.offz, &ctx->near.adj_table};
   for (auto* b : bufs) b->release();
   jgm::prof_clear(ctx);
   delete ctx;
@@ -336,8 +340,12 @@ jgm_status jgm_adjoint(jgm_context* ctx, jgm_kind kind, jgm_precision prec, cons
     if ((e = ctx->out_stage.ensure(sizeof(double) * S)) != hipSuccess) return hip_fail(e, "adjoint");
     if ((e = ctx->misfit_stage.ensure(sizeof(double) * S * kraw)) != hipSuccess)
       return hip_fail(e, "adjoint");
-    e = jgm::launch_forward_fp64(ctx, kind, model, ctx->out_stage.as<double>(),
-                                 ctx->raw.as<double>());
+    if (prec == JGM_MIXED)
+      e = jgm::launch_forward_mixed(ctx, kind, model, ctx->out_stage.as<double>(),
+                                    ctx->raw.as<double>());
+    else
+      e = jgm::launch_forward_fp64(ctx, kind, model, ctx->out_stage.as<double>(),
+                                   ctx->raw.as<double>());
     if (e != hipSuccess) return hip_fail(e, "adjoint (raw forward)");
     e = jgm::launch_misfit_to_raw(ctx, kind, misfit, ctx->raw.as<const double>(),
                                   ctx->misfit_stage.as<double>());

@@ -41,11 +41,21 @@ struct DeviceBuffer {
 // of cells [lo, hi] (inclusive) evaluated with the FP64 closed form; every cell
 // outside it is at least q0 half-diagonals away from the site.
 struct NearBoxes {
-  DeviceBuffer box;   // int32 [S][6]: i0, i1, j0, j1, k0, k1 (empty: i0 > i1)
-  DeviceBuffer tiles; // adjoint near pass: CSR of candidate sites per cell column block
-  double q0 = 0.0;
+  DeviceBuffer box;       // int32 [S][6]: i0, i1, j0, j1, k0, k1 (empty: i0 > i1)
+  DeviceBuffer col_ptr;   // int32 [ncols+1]: CSR of sites whose box covers column (i, j)
+  DeviceBuffer col_sites; // int32 site indices, ascending within a column
+  DeviceBuffer col_ids;   // int32 column index i*ny + j of each listed column
+  DeviceBuffer col_kmax;  // int32 largest k1 over the column's sites
+  size_t ncols = 0;
+  size_t total_near_cells = 0;  // sum over sites of box sizes
+  double radius = 0.0;          // q0 * largest cell half-diagonal (m)
   bool valid = false;
-  size_t max_box_cells = 0;
+  // far-field geometry: cell centres (FP64) and FP32 offsets of the centres from
+  // the first centre of their 4-cell tile along each axis
+  DeviceBuffer xcen, ycen, zcen, offx, offy, offz;
+  bool quad = true;  // false when every cell is a cube of one size (quadrupole zero)
+  DeviceBuffer adj_table;  // FP32 per-cell table with unit density (G V, B...)
+  bool adj_table_valid = false;
 };
 
 // Optional per-kernel timing: HIP events recorded on the context's stream around
@@ -109,7 +119,7 @@ struct jgm_context {
 
   // scratch
   jgm::DeviceBuffer weights, partial, raw, coef, hpart, hsum, model_stage, out_stage, misfit_stage,
-      grad_stage, farpart, nearpart;
+      grad_stage, farpart, nearpart, coef32;
 
   jgm::Profiler prof;
 
@@ -138,6 +148,8 @@ hipError_t launch_adjoint_mixed(jgm_context* ctx, int kind, const double* model,
                                 const double* raw_misfit, double* grad);
 hipError_t build_near_boxes(jgm_context* ctx);
 hipError_t launch_fill_zero(jgm_context* ctx, double* p, size_t n);
+hipError_t launch_background_gradient(jgm_context* ctx, int kind, const double* raw_misfit,
+                                      double* grad_bg);
 hipError_t launch_storedj_build(jgm_context* ctx, int kind, int precision);
 hipError_t launch_storedj_apply(jgm_context* ctx, int gemm_precision, int transpose,
                                 const float* in, size_t nrhs, float* out);

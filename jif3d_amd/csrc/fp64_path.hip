@@ -15,6 +15,7 @@
 
 #include "context.hpp"
 #include "prism_device.hpp"
+#include "epilogue.hpp"
 
 namespace jgm {
 
@@ -131,19 +132,6 @@ __global__ __launch_bounds__(kBlock) void k_background_table(
   }
 }
 
-struct FieldDir {
-  double bx, by, bz, scale;  // direction cosines and F / (4 pi G)
-};
-
-__host__ inline FieldDir field_dir(const jgm_context* ctx) {
-  FieldDir f;
-  f.bx = std::cos(ctx->inclination) * std::cos(ctx->declination);
-  f.by = std::cos(ctx->inclination) * std::sin(ctx->declination);
-  f.bz = std::sin(ctx->inclination);
-  f.scale = ctx->fieldstrength * (1.0 / (4 * kPi * kGravConst));
-  return f;
-}
-
 // Sum the chunk partials, scale, add the background and assemble the raw data of
 // one site; then apply the transform.  u[] holds gz, Uxx, Uyy, Uzz, Uxy, Uxz, Uyz.
 template <int MASK>
@@ -162,57 +150,11 @@ __global__ __launch_bounds__(kBlock) void k_forward_finish(
     for (int m = 0; m < NC; ++m) acc[m] += partial[((long)c * NC + m) * S + s];
   double sc[NC];
   comp_scales<MASK>(sc);
-  double u[7] = {0, 0, 0, 0, 0, 0, 0};
-  {
-    int k = 0;
-    if (MASK & kGz) u[0] = acc[k] * sc[k], ++k;
-    if (MASK & kDiag) {
-      u[1] = acc[k] * sc[k]; ++k;
-      u[2] = acc[k] * sc[k]; ++k;
-      u[3] = acc[k] * sc[k]; ++k;
-    }
-    if (MASK & kOff) {
-      u[4] = acc[k] * sc[k]; ++k;
-      u[5] = acc[k] * sc[k]; ++k;
-      u[6] = acc[k] * sc[k]; ++k;
-    }
-  }
-  for (int j = 0; j < L; ++j) {
-    const double* b = bgT + ((long)s * L + j) * 7;
-    const double v = bgvals[j];
-    for (int m = 0; m < 7; ++m) u[m] += b[m] * v;
-  }
-  double raw[10];
-  int kraw = 0;
-  if (kind == JGM_KIND_SCALAR) {
-    raw[0] = u[0];
-    kraw = 1;
-  } else if (kind == JGM_KIND_TENSOR || kind == JGM_KIND_SCALAR_TENSOR) {
-    int o = 0;
-    if (kind == JGM_KIND_SCALAR_TENSOR) raw[o++] = u[0];
-    raw[o + 0] = u[1]; raw[o + 1] = u[4]; raw[o + 2] = u[5];
-    raw[o + 3] = u[4]; raw[o + 4] = u[2]; raw[o + 5] = u[6];
-    raw[o + 6] = u[5]; raw[o + 7] = u[6]; raw[o + 8] = u[3];
-    kraw = o + 9;
-  } else {  // magnetic: B = (T b) F / (4 pi G)
-    raw[0] = (u[1] * fd.bx + u[4] * fd.by + u[5] * fd.bz) * fd.scale;
-    raw[1] = (u[4] * fd.bx + u[2] * fd.by + u[6] * fd.bz) * fd.scale;
-    raw[2] = (u[5] * fd.bx + u[6] * fd.by + u[3] * fd.bz) * fd.scale;
-    kraw = 3;
-  }
-  if (raw_out) {
-    for (int m = 0; m < kraw; ++m) raw_out[(long)s * kraw + m] = raw[m];
-  }
-  if (tf == JGM_TF_FTG_INVARIANT) {
-    const double* d = raw;  // GravityTransforms.h:36-40
-    out[s] = d[0] * d[4] + d[4] * d[8] + d[0] * d[8] - d[3] * d[1] - d[7] * d[5] - d[2] * d[6];
-  } else if (tf == JGM_TF_TOTAL_FIELD) {
-    out[s] = sqrt(raw[0] * raw[0] + raw[1] * raw[1] + raw[2] * raw[2]);
-  } else if (tf == JGM_TF_TOTAL_FIELD_ANOMALY) {
-    out[s] = (tfbx * raw[0] + tfby * raw[1] + tfbz * raw[2]);
-  } else {
-    for (int m = 0; m < kraw; ++m) out[(long)s * kraw + m] = raw[m];
-  }
+#pragma unroll
+  for (int m = 0; m < NC; ++m) acc[m] *= sc[m];
+  double u[7];
+  spread_components<MASK>(acc, u);
+  finish_site(u, s, bgT, bgvals, L, kind, fd, tf, tfbx, tfby, tfbz, out, raw_out);
 }
 
 // Misfit of transformed data -> misfit of raw data: D^T r per site
@@ -254,30 +196,8 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_coef(const double* __restric
   constexpr int NC = Ncomp<MASK>::value;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S) return;
-  double c[7] = {0, 0, 0, 0, 0, 0, 0};  // gz, xx, yy, zz, xy, xz, yz (before sign*G)
-  if (kind == JGM_KIND_SCALAR) {
-    c[0] = rm[s];
-  } else if (kind == JGM_KIND_TENSOR || kind == JGM_KIND_SCALAR_TENSOR) {
-    const int k = kind == JGM_KIND_SCALAR_TENSOR ? 10 : 9;
-    const double* r = rm + (long)s * k;
-    int o = 0;
-    if (kind == JGM_KIND_SCALAR_TENSOR) c[0] = r[o++];
-    c[1] = r[o + 0];
-    c[2] = r[o + 4];
-    c[3] = r[o + 8];
-    c[4] = r[o + 1] + r[o + 3];
-    c[5] = r[o + 2] + r[o + 6];
-    c[6] = r[o + 5] + r[o + 7];
-  } else {
-    const double* r = rm + (long)s * 3;
-    const double f = fd.scale;
-    c[1] = r[0] * fd.bx * f;
-    c[2] = r[1] * fd.by * f;
-    c[3] = r[2] * fd.bz * f;
-    c[4] = (r[0] * fd.by + r[1] * fd.bx) * f;
-    c[5] = (r[0] * fd.bz + r[2] * fd.bx) * f;
-    c[6] = (r[1] * fd.bz + r[2] * fd.by) * f;
-  }
+  double c[7];  // gz, xx, yy, zz, xy, xz, yz (before sign*G)
+  site_coefficients(rm, s, kind, fd, c);
   double sc[NC];
   comp_scales<MASK>(sc);
   int k = 0;
@@ -542,13 +462,22 @@ hipError_t adjoint_impl(jgm_context* ctx, int kind, const double* rm, double* gr
                        (int)ctx->nx, (int)ctx->ny, (int)ctx->nz, grad);
   }
   if (L > 0) {
-    hipLaunchKernelGGL(k_background_gradient, dim3(L), dim3(kBlock), 0, st, rm,
-                       ctx->bgT.as<const double>(), S, L, kind, field_dir(ctx), grad + N);
+    hipError_t eb = launch_background_gradient(ctx, kind, rm, grad + N);
+    if (eb != hipSuccess) return eb;
   }
   return hipGetLastError();
 }
 
 }  // namespace
+
+hipError_t launch_background_gradient(jgm_context* ctx, int kind, const double* rm,
+                                      double* grad_bg) {
+  const int L = (int)ctx->bgthick.size();
+  if (L == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_background_gradient, dim3(L), dim3(kBlock), 0, ctx->stream, rm,
+                     ctx->bgT.as<const double>(), (int)ctx->S, L, kind, field_dir(ctx), grad_bg);
+  return hipGetLastError();
+}
 
 hipError_t launch_background_table(jgm_context* ctx) {
   const int S = (int)ctx->S;
