@@ -55,8 +55,8 @@ typedef enum jgm_kind {
 
 /* JGM_FP64: closed-form box terms in FP64 for every site-cell pair.
  * JGM_MIXED: FP64 closed form for cells inside each site's near box (all cells
- * closer than q0 cell half-diagonals), FP32 multipole (monopole + quadrupole of the
- * prism about its centre) for the rest, FP64 accumulation. */
+ * closer than q0 cell half-diagonals), FP32 multipole of the prism about its centre
+ * through the hexadecapole for the rest, FP64 accumulation across tiles. */
 typedef enum jgm_precision { JGM_FP64 = 0, JGM_MIXED = 1 } jgm_precision;
 
 /* Per-site data transforms (jif3D/Gravity/GravityTransforms.h:25-100,
@@ -107,7 +107,7 @@ jgm_status jgm_set_field(jgm_context *ctx, double inclination, double declinatio
 /* Data transform (ThreeDGravMagImplementation::SetDataTransform, :67-70). */
 jgm_status jgm_set_transform(jgm_context *ctx, jgm_transform tf, double inclination,
                              double declination);
-/* Near-field radius of JGM_MIXED in cell half-diagonals (default 12). */
+/* Near-field radius of JGM_MIXED in cell half-diagonals (default 8). */
 jgm_status jgm_set_mixed_threshold(jgm_context *ctx, double q0);
 
 /* Values per site after the transform, and the model vector length

@@ -136,8 +136,8 @@ jgm_status jgm_destroy(jgm_context* ctx) {
                                &ctx->farpart, &ctx->nearpart, &ctx->coef32, &ctx->storedJ,
                                &ctx->near.col_ptr, &ctx->near.col_sites, &ctx->near.col_ids,
                                &ctx->near.col_kmax, &ctx->near.xcen, &ctx->near.ycen,
-                               &ctx->near.zcen, &ctx->near.offx, &ctx->near.offy,
-                               &ctx->near.offz, &ctx->near.adj_table};
+                               &ctx->near.zcen, &ctx->near.adj_table, &ctx->near.corner_off,
+                               &ctx->near.cell_off, &ctx->near_h, &ctx->near_v};
   for (auto* b : bufs) b->release();
   jgm::prof_clear(ctx);
   delete ctx;

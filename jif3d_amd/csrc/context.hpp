@@ -46,14 +46,16 @@ struct NearBoxes {
   DeviceBuffer col_sites; // int32 site indices, ascending within a column
   DeviceBuffer col_ids;   // int32 column index i*ny + j of each listed column
   DeviceBuffer col_kmax;  // int32 largest k1 over the column's sites
+  DeviceBuffer corner_off; // int64 per-site offset of its box corners (adjoint near pass)
+  DeviceBuffer cell_off;   // int64 per-site offset of its box cells (adjoint near pass)
   size_t ncols = 0;
-  size_t total_near_cells = 0;  // sum over sites of box sizes
+  size_t total_near_cells = 0;    // sum over sites of box cells
+  size_t total_near_corners = 0;  // sum over sites of box corners
   double radius = 0.0;          // q0 * largest cell half-diagonal (m)
   bool valid = false;
-  // far-field geometry: cell centres (FP64) and FP32 offsets of the centres from
-  // the first centre of their 4-cell tile along each axis
-  DeviceBuffer xcen, ycen, zcen, offx, offy, offz;
-  bool quad = true;  // false when every cell is a cube of one size (quadrupole zero)
+  // far-field geometry: cell centres (FP64)
+  DeviceBuffer xcen, ycen, zcen;
+  int mesh_class = 2;  // 0: cubes of one size, 1: one cell shape, 2: general
   DeviceBuffer adj_table;  // FP32 per-cell table with unit density (G V, B...)
   bool adj_table_valid = false;
 };
@@ -110,7 +112,7 @@ struct jgm_context {
   double tf_inc = 0.0, tf_dec = 0.0;
 
   // mixed precision near-field threshold in cell half-diagonals
-  double mixed_q0 = 12.0;
+  double mixed_q0 = 8.0;
 
   // caches
   jgm::DeviceBuffer bgT;  // [S][L][7] background box terms, valid when bg_valid
@@ -119,7 +121,7 @@ struct jgm_context {
 
   // scratch
   jgm::DeviceBuffer weights, partial, raw, coef, hpart, hsum, model_stage, out_stage, misfit_stage,
-      grad_stage, farpart, nearpart, coef32;
+      grad_stage, farpart, nearpart, coef32, near_h, near_v;
 
   jgm::Profiler prof;
 

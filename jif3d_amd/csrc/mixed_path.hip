@@ -2,22 +2,31 @@
 //
 // Split per site: cells inside the site's near box (every cell closer than q0
 // cell half-diagonals lies inside it) use the FP64 closed form; all other cells use
-// the FP32 multipole expansion of the uniform prism about its centre, monopole plus
-// quadrupole:
+// the FP32 multipole expansion of the uniform prism about its centre through the
+// hexadecapole:
 //
-//   Phi/V = 1/r + Q/r^5,  Q = Bx X^2 + By Y^2 + Bz Z^2,  B_m = (h_m^2 - hbar^2)/2,
-//   hbar^2 = (hx^2 + hy^2 + hz^2)/3, (X, Y, Z) = site - cell centre, h = half sizes,
+//   Phi/V = 1/r + Q(x)/r^5 + H(x)/r^9,
+//   Q = sum_m B_m x_m^2,           B_m = (h_m^2 - hbar^2)/2, hbar^2 = mean of h^2,
+//   H = harmonic part of <(u.x)^4> over the box times 35/8
+//     = a1 x^4 + a2 y^4 + a3 z^4 + b12 x^2 y^2 + b13 x^2 z^2 + b23 y^2 z^2,
 //
-// gz = G rho V dPhi/dZ and U_ij = G rho V d2Phi/dx_i dx_j (the octupole vanishes by
-// symmetry; the neglected hexadecapole is O((h/d)^4) relative).  Cubes have B = 0.
-// Accumulation per site (forward) and per cell (adjoint) runs in FP32 within a
-// 4x4x4-cell tile and in FP64 across tiles.
+// with x = site - cell centre and h the half sizes; gz = G rho V dPhi/dz and
+// U_ij = G rho V d2Phi/dx_i dx_j.  Odd orders vanish by symmetry, so the neglected
+// term is O((h/d)^6) relative.  Cubes have B = 0.  The formulas are evaluated in
+// direction cosines u = x/r so that no power of 1/r below r^-7 is formed (FP32
+// range).  Each site-centre difference is formed in FP64 and rounded once, so the
+// FP32 rounding errors of different pairs are independent.  Accumulation per site
+// (forward) and per cell (adjoint) runs in FP32 over short runs, flushed to
+// FP64 every 4 cells (forward) or 8 sites (adjoint): the rounding error of an FP32
+// run grows with the run length times its partial sum.
 //
 // Layouts in HBM
-//   cell table   FP32, QUAD: [c][8] = {w, Bx, By, Bz, Bx+By, Bx+Bz, By+Bz, 0}; else [c] = {w}
+//   cell table   FP32: uniform mesh [c] = {w}; otherwise [c][12] =
+//                {w, Bx, By, Bz, a1, a2, a3, b12, b13, b23, 0, 0}
 //                w = G rho V (forward) or G V (adjoint)
 //   far partials forward [chunk][NC][S] FP64; adjoint [chunk][N] FP64
-//   near boxes   int32 [S][6]; per-column CSR of sites for the adjoint near pass
+//   near boxes   int32 [S][6]; per-column CSR of sites; per-site offsets of the
+//                near corner values and near cell values of the adjoint
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,6 +44,10 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kTile = 4;  // far-field tile edge (cells)
 constexpr int kTargetWaves = 16384;
+constexpr int kTableWidth = 12;  // floats per cell of a non-uniform mesh
+
+// Mesh classes of the far-field kernels.
+enum : int { kMeshCubes = 0, kMeshUniform = 1, kMeshGeneral = 2 };
 
 inline unsigned blocks_for(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
@@ -43,59 +56,121 @@ __global__ void k_fill_zero(double* __restrict__ p, size_t n) {
   if (i < n) p[i] = 0.0;
 }
 
-// ---------------------------------------------------------------------------
-// Far field: FP32 multipole of a prism of unit G*rho*V.  Outputs in the kernel
-// component order of MASK (gz, Uxx, Uyy, Uzz, Uxy, Uxz, Uyz).
-// ---------------------------------------------------------------------------
+// Multipole constants of one prism shape (see the file comment).
+struct MpConst {
+  float Bx, By, Bz, a1, a2, a3, b12, b13, b23;
+};
+
+__host__ __device__ inline MpConst mp_const(double dx, double dy, double dz) {
+  const double hx2 = 0.25 * dx * dx, hy2 = 0.25 * dy * dy, hz2 = 0.25 * dz * dz;
+  const double hb = (hx2 + hy2 + hz2) / 3.0;
+  // fourth moments <u^4> = h^4/5 and multinomial x^2 y^2 terms 6 <u^2><v^2> = 2 h^2 k^2 / 3
+  const double A = hx2 * hx2 / 5, B = hy2 * hy2 / 5, C = hz2 * hz2 / 5;
+  const double D = 2 * hx2 * hy2 / 3, E = 2 * hx2 * hz2 / 3, F = 2 * hy2 * hz2 / 3;
+  // harmonic projection P - r^2 lap(P)/14 + r^4 lap^2(P)/280
+  const double p = 12 * A + 2 * D + 2 * E, q = 12 * B + 2 * D + 2 * F, t = 12 * C + 2 * E + 2 * F;
+  const double w = 2 * (p + q + t);
+  const double c = 35.0 / 8.0;  // (2n-1)!!/n! for n = 4
+  MpConst m;
+  m.Bx = (float)(0.5 * (hx2 - hb));
+  m.By = (float)(0.5 * (hy2 - hb));
+  m.Bz = (float)(0.5 * (hz2 - hb));
+  m.a1 = (float)(c * (A - p / 14 + w / 280));
+  m.a2 = (float)(c * (B - q / 14 + w / 280));
+  m.a3 = (float)(c * (C - t / 14 + w / 280));
+  m.b12 = (float)(c * (D - (p + q) / 14 + w / 140));
+  m.b13 = (float)(c * (E - (p + t) / 14 + w / 140));
+  m.b23 = (float)(c * (F - (q + t) / 14 + w / 140));
+  return m;
+}
+
+// Field of a prism of unit G*rho*V at offset (X, Y, Z) = site - centre, in the
+// kernel component order of MASK.  QUAD = false drops the quadrupole (B = 0).
 template <int MASK, bool QUAD>
-__device__ __forceinline__ void multipole_f32(float X, float Y, float Z, float X2, float Y2,
-                                              float XY, const float* B, float* T) {
-  const float Z2 = Z * Z;
-  const float r2 = X2 + Y2 + Z2;
-  const float s = __builtin_amdgcn_rsqf(r2);
+__device__ __forceinline__ void multipole_f32(float X, float Y, float Z, const MpConst& c,
+                                              float* T) {
+  const float r2 = fmaf(X, X, fmaf(Y, Y, Z * Z));
+  float s = __builtin_amdgcn_rsqf(r2);
+  const float e = fmaf(-(r2 * s), s, 1.0f);  // one Newton step on the hardware estimate
+  s = fmaf(0.5f * s, e, s);
   const float s2 = s * s;
   const float s3 = s2 * s;
   const float s5 = s3 * s2;
+  const float s7 = s5 * s2;
+  const float ux = X * s, uy = Y * s, uz = Z * s;
+  const float vx = ux * ux, vy = uy * uy, vz = uz * uz;
+  const float hh = fmaf(c.a1 * vx, vx,
+                        fmaf(c.a2 * vy, vy,
+                             fmaf(c.a3 * vz, vz,
+                                  fmaf(c.b12 * vx, vy, fmaf(c.b13 * vx, vz, c.b23 * vy * vz)))));
+  float P = fmaf(99.0f * hh, s7, 3.0f * s3);   // coefficient of u_i u_j
+  float base = fmaf(-9.0f * hh, s7, -s3);      // isotropic diagonal part
+  float q = 0.0f;
+  if constexpr (QUAD) {
+    q = fmaf(c.Bx, vx, fmaf(c.By, vy, c.Bz * vz));
+    P = fmaf(35.0f * q, s5, P);
+    base = fmaf(-5.0f * q, s5, base);
+  }
+  const float t2b = 2.0f * c.b12, t2c = 2.0f * c.b13, t2d = 2.0f * c.b23;
+  const float hx = fmaf(4.0f * c.a1, vx, fmaf(t2b, vy, t2c * vz));
+  const float hy = fmaf(4.0f * c.a2, vy, fmaf(t2b, vx, t2d * vz));
+  const float hz = fmaf(4.0f * c.a3, vz, fmaf(t2c, vx, t2d * vy));
   int k = 0;
-  if constexpr (!QUAD) {
-    const float t5 = 3.0f * s5;
-    if constexpr ((MASK & kGz) != 0) T[k++] = -Z * s3;
-    if constexpr ((MASK & kDiag) != 0) {
-      T[k++] = fmaf(X2, t5, -s3);
-      T[k++] = fmaf(Y2, t5, -s3);
-      T[k++] = fmaf(Z2, t5, -s3);
+  if constexpr ((MASK & kGz) != 0) {
+    float g = fmaf(hz, s7, base);
+    if constexpr (QUAD) g = fmaf(2.0f * c.Bz, s5, g);
+    T[k++] = Z * g;
+  }
+  if constexpr ((MASK & kDiag) != 0) {
+    const float m18 = -18.0f * s7;
+    const float hxx = fmaf(8.0f * c.a1, vx, hx), hyy = fmaf(8.0f * c.a2, vy, hy),
+                hzz = fmaf(8.0f * c.a3, vz, hz);
+    float px = fmaf(hx, m18, P), py = fmaf(hy, m18, P), pz = fmaf(hz, m18, P);
+    float ox = fmaf(hxx, s7, base), oy = fmaf(hyy, s7, base), oz = fmaf(hzz, s7, base);
+    if constexpr (QUAD) {
+      const float m20 = -20.0f * s5, t2 = 2.0f * s5;
+      px = fmaf(c.Bx, m20, px); py = fmaf(c.By, m20, py); pz = fmaf(c.Bz, m20, pz);
+      ox = fmaf(c.Bx, t2, ox); oy = fmaf(c.By, t2, oy); oz = fmaf(c.Bz, t2, oz);
     }
-    if constexpr ((MASK & kOff) != 0) {
-      T[k++] = XY * t5;
-      T[k++] = (X * Z) * t5;
-      T[k++] = (Y * Z) * t5;
+    T[k++] = fmaf(vx, px, ox);
+    T[k++] = fmaf(vy, py, oy);
+    T[k++] = fmaf(vz, pz, oz);
+  }
+  if constexpr ((MASK & kOff) != 0) {
+    const float m9 = -9.0f * s7, f4 = 4.0f * s7;
+    float cxy = fmaf(hx + hy, m9, fmaf(c.b12, f4, P));
+    float cxz = fmaf(hx + hz, m9, fmaf(c.b13, f4, P));
+    float cyz = fmaf(hy + hz, m9, fmaf(c.b23, f4, P));
+    if constexpr (QUAD) {
+      const float m10 = -10.0f * s5;
+      cxy = fmaf(c.Bx + c.By, m10, cxy);
+      cxz = fmaf(c.Bx + c.Bz, m10, cxz);
+      cyz = fmaf(c.By + c.Bz, m10, cyz);
     }
-  } else {
-    const float Bx = B[0], By = B[1], Bz = B[2];
-    const float s7 = s5 * s2;
-    const float s9 = s7 * s2;
-    const float Q = fmaf(Bx, X2, fmaf(By, Y2, Bz * Z2));
-    const float base = fmaf(-5.0f * Q, s7, -s3);        // -s3 - 5 Q s7
-    const float P = fmaf(35.0f * Q, s9, 3.0f * s5);      // 3 s5 + 35 Q s9
-    const float t2s5 = 2.0f * s5;
-    if constexpr ((MASK & kGz) != 0) T[k++] = Z * fmaf(Bz, t2s5, base);
-    if constexpr ((MASK & kDiag) != 0) {
-      const float m20s7 = -20.0f * s7;
-      T[k++] = fmaf(X2, fmaf(Bx, m20s7, P), fmaf(Bx, t2s5, base));
-      T[k++] = fmaf(Y2, fmaf(By, m20s7, P), fmaf(By, t2s5, base));
-      T[k++] = fmaf(Z2, fmaf(Bz, m20s7, P), fmaf(Bz, t2s5, base));
-    }
-    if constexpr ((MASK & kOff) != 0) {
-      const float m10s7 = -10.0f * s7;
-      T[k++] = XY * fmaf(B[3], m10s7, P);
-      T[k++] = (X * Z) * fmaf(B[4], m10s7, P);
-      T[k++] = (Y * Z) * fmaf(B[5], m10s7, P);
-    }
+    T[k++] = (ux * uy) * cxy;
+    T[k++] = (ux * uz) * cxz;
+    T[k++] = (uy * uz) * cyz;
   }
 }
 
-// Cell table: w = G * rho * V (rho = 1 when rho is null) and the quadrupole constants.
-template <bool QUAD>
+template <int MESH>
+__device__ __forceinline__ MpConst cell_const(const float* __restrict__ table, long c,
+                                              const MpConst& uniform) {
+  if constexpr (MESH != kMeshGeneral) {
+    return uniform;
+  } else {
+    const float* t = table + c * kTableWidth + 1;
+    MpConst m;
+    m.Bx = t[0]; m.By = t[1]; m.Bz = t[2];
+    m.a1 = t[3]; m.a2 = t[4]; m.a3 = t[5];
+    m.b12 = t[6]; m.b13 = t[7]; m.b23 = t[8];
+    return m;
+  }
+}
+
+// Cell table: w = G * rho * V (rho = 1 when rho is null); the shape constants
+// for non-uniform meshes.
+template <int MESH>
 __global__ __launch_bounds__(kBlock) void k_cell_table(const double* __restrict__ rho,
                                                        const double* __restrict__ xsz,
                                                        const double* __restrict__ ysz,
@@ -109,23 +184,17 @@ __global__ __launch_bounds__(kBlock) void k_cell_table(const double* __restrict_
   const int j = (int)(t % ny);
   const int i = (int)(t / ny);
   const double dx = xsz[i], dy = ysz[j], dz = zsz[k];
-  const double V = dx * dy * dz;
-  const double w = kGravConst * (rho ? rho[c] : 1.0) * V;
-  if constexpr (!QUAD) {
+  const double w = kGravConst * (rho ? rho[c] : 1.0) * (dx * dy * dz);
+  if constexpr (MESH != kMeshGeneral) {
     table[c] = (float)w;
   } else {
-    const double hx2 = 0.25 * dx * dx, hy2 = 0.25 * dy * dy, hz2 = 0.25 * dz * dz;
-    const double hb = (hx2 + hy2 + hz2) / 3.0;
-    const double Bx = 0.5 * (hx2 - hb), By = 0.5 * (hy2 - hb), Bz = 0.5 * (hz2 - hb);
-    float* o = table + c * 8;
+    const MpConst m = mp_const(dx, dy, dz);
+    float* o = table + c * kTableWidth;
     o[0] = (float)w;
-    o[1] = (float)Bx;
-    o[2] = (float)By;
-    o[3] = (float)Bz;
-    o[4] = (float)(Bx + By);
-    o[5] = (float)(Bx + Bz);
-    o[6] = (float)(By + Bz);
-    o[7] = 0.0f;
+    o[1] = m.Bx; o[2] = m.By; o[3] = m.Bz;
+    o[4] = m.a1; o[5] = m.a2; o[6] = m.a3;
+    o[7] = m.b12; o[8] = m.b13; o[9] = m.b23;
+    o[10] = 0.0f; o[11] = 0.0f;
   }
 }
 
@@ -133,22 +202,25 @@ struct FarGeom {
   const double* xcen;
   const double* ycen;
   const double* zcen;
-  const float* offx;
-  const float* offy;
-  const float* offz;
   int nx, ny, nz;
   int ntx, nty, ntz;
 };
 
+template <int MESH>
+struct TableWidth {
+  static constexpr int value = MESH == kMeshGeneral ? kTableWidth : 1;
+};
+
 // Forward far field: one lane per site; each wave walks a chunk of 4x4x4 tiles.
 // Cells inside the lane's near box contribute nothing here (FP64 near pass).
-template <int MASK, bool QUAD>
+template <int MASK, int MESH>
 __global__ __launch_bounds__(kBlock) void k_far_forward(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
-    int S, const int* __restrict__ box, FarGeom g, const float* __restrict__ table, long ntiles,
-    long tiles_per_chunk, double* __restrict__ farpart) {
+    int S, const int* __restrict__ box, FarGeom g, const float* __restrict__ table,
+    MpConst uc, long ntiles, long tiles_per_chunk, double* __restrict__ farpart) {
   constexpr int NC = Ncomp<MASK>::value;
-  constexpr int TW = QUAD ? 8 : 1;
+  constexpr int TW = TableWidth<MESH>::value;
+  constexpr bool QUAD = MESH != kMeshCubes;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int sl = s < S ? s : S - 1;
   const double px = sx[sl], py = sy[sl], pz = sz[sl];
@@ -168,60 +240,60 @@ __global__ __launch_bounds__(kBlock) void k_far_forward(
     const int ti = (int)(tt / g.nty);
     const int i0 = ti * kTile, j0 = tj * kTile, k0 = tk * kTile;
     const int ni = min(kTile, g.nx - i0), nj = min(kTile, ny - j0), nk = min(kTile, nz - k0);
-    // site relative to the tile's first cell centre, rounded once to FP32
-    const float pxr = (float)(px - g.xcen[i0]);
-    const float pyr = (float)(py - g.ycen[j0]);
-    const float pzr = (float)(pz - g.zcen[k0]);
     const bool hit = !(b_i1 < i0 || b_i0 > i0 + ni - 1 || b_j1 < j0 || b_j0 > j0 + nj - 1 ||
                        b_k1 < k0 || b_k0 > k0 + nk - 1);
-    float acc[NC];
+    float acc[NC];  // FP32 partial of one tile column (4 cells), flushed to FP64
 #pragma unroll
     for (int m = 0; m < NC; ++m) acc[m] = 0.0f;
     if (!__any(hit)) {
       for (int ii = 0; ii < ni; ++ii) {
-        const float X = pxr - g.offx[i0 + ii];
-        const float X2 = X * X;
+        const float X = (float)(px - g.xcen[i0 + ii]);
         for (int jj = 0; jj < nj; ++jj) {
-          const float Y = pyr - g.offy[j0 + jj];
-          const float Y2 = Y * Y, XY = X * Y;
-          const float* tb = table + (((long)(i0 + ii) * ny + (j0 + jj)) * nz + k0) * TW;
+          const float Y = (float)(py - g.ycen[j0 + jj]);
+          const long cb = ((long)(i0 + ii) * ny + (j0 + jj)) * nz + k0;
           for (int kk = 0; kk < nk; ++kk) {
-            const float Z = pzr - g.offz[k0 + kk];
-            const float* cw = tb + kk * TW;
+            const float Z = (float)(pz - g.zcen[k0 + kk]);
+            const MpConst cc = cell_const<MESH>(table, cb + kk, uc);
             float T[NC];
-            multipole_f32<MASK, QUAD>(X, Y, Z, X2, Y2, XY, cw + 1, T);
-            const float w = cw[0];
+            multipole_f32<MASK, QUAD>(X, Y, Z, cc, T);
+            const float w = table[(cb + kk) * TW];
 #pragma unroll
             for (int m = 0; m < NC; ++m) acc[m] = fmaf(w, T[m], acc[m]);
+          }
+#pragma unroll
+          for (int m = 0; m < NC; ++m) {
+            acc64[m] += (double)acc[m];
+            acc[m] = 0.0f;
           }
         }
       }
     } else {
       for (int ii = 0; ii < ni; ++ii) {
-        const float X = pxr - g.offx[i0 + ii];
-        const float X2 = X * X;
+        const float X = (float)(px - g.xcen[i0 + ii]);
         const bool inx = (i0 + ii >= b_i0) && (i0 + ii <= b_i1);
         for (int jj = 0; jj < nj; ++jj) {
-          const float Y = pyr - g.offy[j0 + jj];
-          const float Y2 = Y * Y, XY = X * Y;
+          const float Y = (float)(py - g.ycen[j0 + jj]);
           const bool inxy = inx && (j0 + jj >= b_j0) && (j0 + jj <= b_j1);
-          const float* tb = table + (((long)(i0 + ii) * ny + (j0 + jj)) * nz + k0) * TW;
+          const long cb = ((long)(i0 + ii) * ny + (j0 + jj)) * nz + k0;
           for (int kk = 0; kk < nk; ++kk) {
             const bool near = inxy && (k0 + kk >= b_k0) && (k0 + kk <= b_k1);
             // a near cell is evaluated at a harmless far point and weighted by zero
-            const float Z = near ? 1.0e6f : pzr - g.offz[k0 + kk];
-            const float* cw = tb + kk * TW;
+            const float Z = near ? 1.0e6f : (float)(pz - g.zcen[k0 + kk]);
+            const MpConst cc = cell_const<MESH>(table, cb + kk, uc);
             float T[NC];
-            multipole_f32<MASK, QUAD>(X, Y, Z, X2, Y2, XY, cw + 1, T);
-            const float w = near ? 0.0f : cw[0];
+            multipole_f32<MASK, QUAD>(X, Y, Z, cc, T);
+            const float w = near ? 0.0f : table[(cb + kk) * TW];
 #pragma unroll
             for (int m = 0; m < NC; ++m) acc[m] = fmaf(w, T[m], acc[m]);
+          }
+#pragma unroll
+          for (int m = 0; m < NC; ++m) {
+            acc64[m] += (double)acc[m];
+            acc[m] = 0.0f;
           }
         }
       }
     }
-#pragma unroll
-    for (int m = 0; m < NC; ++m) acc64[m] += (double)acc[m];
   }
   if (s < S) {
 #pragma unroll
@@ -306,33 +378,37 @@ __global__ __launch_bounds__(kBlock) void k_mixed_forward_finish(
   finish_site(u, s, bgT, bgvals, L, kind, fd, tf, tfbx, tfby, tfbz, out, raw_out);
 }
 
-// Physical coefficients of the adjoint (before sign*G), FP64 and FP32 copies.
+// Adjoint coefficients: coef (FP64, with the sign and G of each corner sum, for
+// the near pass) and coef32 (FP32 physical values, for the far pass).
 template <int MASK>
-__global__ __launch_bounds__(kBlock) void k_phys_coef(const double* __restrict__ rm, int S,
-                                                      int kind, FieldDir fd,
-                                                      double* __restrict__ c64,
-                                                      float* __restrict__ c32) {
+__global__ __launch_bounds__(kBlock) void k_mixed_coef(const double* __restrict__ rm, int S,
+                                                       int kind, FieldDir fd,
+                                                       double* __restrict__ c64,
+                                                       float* __restrict__ c32) {
   constexpr int NC = Ncomp<MASK>::value;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S) return;
-  double c7[7], c[NC];
+  double c7[7], c[NC], sc[NC];
   site_coefficients(rm, s, kind, fd, c7);
   select_components<MASK>(c7, c);
+  comp_scales<MASK>(sc);
   for (int m = 0; m < NC; ++m) {
-    c64[(long)s * NC + m] = c[m];
+    c64[(long)s * NC + m] = c[m] * sc[m];
     c32[(long)s * NC + m] = (float)c[m];
   }
 }
 
 // Adjoint far field: one lane per cell of a 4x4x4 tile, the wave walks a chunk
 // of sites (wave-uniform); sites whose near box holds the cell are skipped.
-template <int MASK, bool QUAD>
+template <int MASK, int MESH>
 __global__ __launch_bounds__(kBlock) void k_far_adjoint(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     const float* __restrict__ coef, const int* __restrict__ box, int S, int sites_per_chunk,
-    FarGeom g, const float* __restrict__ table, long ntiles, double* __restrict__ gpart) {
+    FarGeom g, const float* __restrict__ table, MpConst uc, long ntiles,
+    double* __restrict__ gpart) {
   constexpr int NC = Ncomp<MASK>::value;
-  constexpr int TW = QUAD ? 8 : 1;
+  constexpr int TW = TableWidth<MESH>::value;
+  constexpr bool QUAD = MESH != kMeshCubes;
   const long t = ((long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
   const int lane = threadIdx.x & 63;
   if (t >= ntiles) return;
@@ -346,37 +422,31 @@ __global__ __launch_bounds__(kBlock) void k_far_adjoint(
   const bool valid = ii < ni && jj < nj && kk < nk;
   const int i = i0 + (valid ? ii : 0), j = j0 + (valid ? jj : 0), k = k0 + (valid ? kk : 0);
   const long c = ((long)i * g.ny + j) * g.nz + k;
-  const float dx = g.offx[i], dy = g.offy[j], dz = g.offz[k];
-  float B[6] = {0, 0, 0, 0, 0, 0};
-  if constexpr (QUAD) {
-    for (int q = 0; q < 6; ++q) B[q] = table[c * 8 + 1 + q];
-  }
-  const double xr = g.xcen[i0], yr = g.ycen[j0], zr = g.zcen[k0];
+  const double cx = g.xcen[i], cy = g.ycen[j], cz = g.zcen[k];
+  const MpConst cc = cell_const<MESH>(table, c, uc);
   const int s0 = blockIdx.y * sites_per_chunk;
   const int s1 = s0 + sites_per_chunk < S ? s0 + sites_per_chunk : S;
   double acc64 = 0.0;
   float acc = 0.0f;
   int nacc = 0;
   for (int s = s0; s < s1; ++s) {
-    const float pxr = (float)(sx[s] - xr);
-    const float pyr = (float)(sy[s] - yr);
-    const float pzr = (float)(sz[s] - zr);
     const int* bx = box + s * 6;
     const bool tile_hit = !(bx[1] < i0 || bx[0] > i0 + ni - 1 || bx[3] < j0 ||
                             bx[2] > j0 + nj - 1 || bx[5] < k0 || bx[4] > k0 + nk - 1);
     bool near = false;
     if (tile_hit)
       near = i >= bx[0] && i <= bx[1] && j >= bx[2] && j <= bx[3] && k >= bx[4] && k <= bx[5];
-    const float X = pxr - dx, Y = pyr - dy;
-    const float Z = near ? 1.0e6f : pzr - dz;
+    // site - centre in FP64, rounded once per pair
+    const float X = (float)(sx[s] - cx), Y = (float)(sy[s] - cy);
+    const float Z = near ? 1.0e6f : (float)(sz[s] - cz);
     float T[NC];
-    multipole_f32<MASK, QUAD>(X, Y, Z, X * X, Y * Y, X * Y, B, T);
+    multipole_f32<MASK, QUAD>(X, Y, Z, cc, T);
     const float* cs = coef + (long)s * NC;
     float v = 0.0f;
 #pragma unroll
     for (int m = 0; m < NC; ++m) v = fmaf(cs[m], T[m], v);
     acc += near ? 0.0f : v;
-    if (++nacc == 64) {
+    if (++nacc == 8) {
       acc64 += (double)acc;
       acc = 0.0f;
       nacc = 0;
@@ -389,19 +459,69 @@ __global__ __launch_bounds__(kBlock) void k_far_adjoint(
   }
 }
 
-// Adjoint near field: one block per listed column (i, j); threads over k; each
-// cell gathers its near sites (those whose box holds it) in ascending site order
-// with the FP64 8-corner closed form.
+// Adjoint near field, pass 1: one block per site.  H(corner) = sum_m coef_m F_m
+// over the corners of the site's near box (FP64 closed form, each corner once),
+// then v(cell) = 2 sum sigma H over the cell's corners for every cell of the box.
 template <int MASK>
-__global__ __launch_bounds__(64) void k_near_adjoint(
+__global__ __launch_bounds__(kBlock) void k_near_adjoint_site(
+    const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
+    const int* __restrict__ box, const double* __restrict__ coef,
+    const double* __restrict__ Xc, const double* __restrict__ Yc,
+    const double* __restrict__ Zc, const long* __restrict__ corner_off,
+    const long* __restrict__ cell_off, double* __restrict__ hbuf,
+    double* __restrict__ vbuf) {
+  constexpr int NC = Ncomp<MASK>::value;
+  const int s = blockIdx.x;
+  const int i0 = box[s * 6 + 0], i1 = box[s * 6 + 1];
+  const int j0 = box[s * 6 + 2], j1 = box[s * 6 + 3];
+  const int k0 = box[s * 6 + 4], k1 = box[s * 6 + 5];
+  if (i0 > i1 || j0 > j1 || k0 > k1) return;
+  const double px = sx[s], py = sy[s], pz = sz[s];
+  double cs[NC];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) cs[m] = coef[(long)s * NC + m];
+  const int ci = i1 - i0 + 2, cj = j1 - j0 + 2, ck = k1 - k0 + 2;
+  double* H = hbuf + corner_off[s];
+  for (int q = threadIdx.x; q < ci * cj * ck; q += blockDim.x) {
+    const int K = k0 + q % ck;
+    const int J = j0 + (q / ck) % cj;
+    const int I = i0 + q / (ck * cj);
+    double t[NC];
+    corner_terms_f64<MASK>(px - Xc[I], py - Yc[J], pz - Zc[K], t);
+    double h = 0.0;
+#pragma unroll
+    for (int m = 0; m < NC; ++m) h = fma(cs[m], t[m], h);
+    H[q] = h;
+  }
+  __threadfence_block();
+  __syncthreads();
+  const int ni = ci - 1, nj = cj - 1, nk = ck - 1;
+  double* V = vbuf + cell_off[s];
+  for (int q = threadIdx.x; q < ni * nj * nk; q += blockDim.x) {
+    const int kk = q % nk;
+    const int jj = (q / nk) % nj;
+    const int ii = q / (nk * nj);
+    double a = 0.0;
+#pragma unroll
+    for (int da = 0; da < 2; ++da)
+#pragma unroll
+      for (int db = 0; db < 2; ++db)
+#pragma unroll
+        for (int dc = 0; dc < 2; ++dc) {
+          const double h = H[((ii + da) * cj + (jj + db)) * ck + (kk + dc)];
+          a += ((da + db + dc) & 1) ? h : -h;
+        }
+    V[q] = 2.0 * a;
+  }
+}
+
+// Adjoint near field, pass 2: one block per listed column (i, j); each cell sums
+// the values of the sites whose box holds it, in ascending site order.
+__global__ __launch_bounds__(64) void k_near_adjoint_gather(
     const int* __restrict__ col_ptr, const int* __restrict__ col_sites,
     const int* __restrict__ col_ids, const int* __restrict__ col_kmax,
-    const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
-    const int* __restrict__ box, const double* __restrict__ coef, const double* __restrict__ xlo,
-    const double* __restrict__ ylo, const double* __restrict__ zlo,
-    const double* __restrict__ xsz, const double* __restrict__ ysz,
-    const double* __restrict__ zsz, int ny, int nz, double* __restrict__ gnear) {
-  constexpr int NC = Ncomp<MASK>::value;
+    const int* __restrict__ box, const long* __restrict__ cell_off,
+    const double* __restrict__ vbuf, int ny, int nz, double* __restrict__ gnear) {
   const int col = col_ids[blockIdx.x];
   const int i = col / ny, j = col % ny;
   const int p0 = col_ptr[blockIdx.x], p1 = col_ptr[blockIdx.x + 1];
@@ -410,14 +530,12 @@ __global__ __launch_bounds__(64) void k_near_adjoint(
     double acc = 0.0;
     for (int p = p0; p < p1; ++p) {
       const int s = col_sites[p];
-      if (k < box[s * 6 + 4] || k > box[s * 6 + 5]) continue;
-      double v[NC];
-      box_terms_f64<MASK>(sx[s], sy[s], sz[s], xlo[i], ylo[j], zlo[k], xsz[i], ysz[j], zsz[k], v);
-      const double* cs = coef + (long)s * NC;
-#pragma unroll
-      for (int m = 0; m < NC; ++m) acc = fma(cs[m], v[m], acc);
+      const int* b = box + s * 6;
+      if (k < b[4] || k > b[5]) continue;
+      const int nj = b[3] - b[2] + 1, nk = b[5] - b[4] + 1;
+      acc += vbuf[cell_off[s] + ((long)(i - b[0]) * nj + (j - b[2])) * nk + (k - b[4])];
     }
-    gnear[((long)i * ny + j) * nz + k] = 2.0 * acc;
+    gnear[((long)i * ny + j) * nz + k] = acc;
   }
 }
 
@@ -446,9 +564,6 @@ FarGeom far_geom(const jgm_context* ctx) {
   g.xcen = ctx->near.xcen.as<const double>();
   g.ycen = ctx->near.ycen.as<const double>();
   g.zcen = ctx->near.zcen.as<const double>();
-  g.offx = ctx->near.offx.as<const float>();
-  g.offy = ctx->near.offy.as<const float>();
-  g.offz = ctx->near.offz.as<const float>();
   g.nx = (int)ctx->nx;
   g.ny = (int)ctx->ny;
   g.nz = (int)ctx->nz;
@@ -456,6 +571,11 @@ FarGeom far_geom(const jgm_context* ctx) {
   g.nty = (g.ny + kTile - 1) / kTile;
   g.ntz = (g.nz + kTile - 1) / kTile;
   return g;
+}
+
+MpConst uniform_const(const jgm_context* ctx) {
+  if (ctx->nx == 0 || ctx->ny == 0 || ctx->nz == 0) return MpConst{};
+  return mp_const(ctx->xs[0], ctx->ys[0], ctx->zs[0]);
 }
 
 template <class T>
@@ -473,7 +593,17 @@ void axis_range(const std::vector<double>& cen, double p, double R, int* lo, int
   *hi = (int)(b - cen.begin()) - 1;
 }
 
-template <int MASK, bool QUAD>
+const char* far_name(int mask, bool adjoint) {
+  if (adjoint)
+    return mask == kMaskScalar ? "far_adjoint_fp32_gz"
+           : mask == kMaskAll  ? "far_adjoint_fp32_gz_ftg"
+                               : "far_adjoint_fp32_ftg";
+  return mask == kMaskScalar ? "far_forward_fp32_gz"
+         : mask == kMaskAll  ? "far_forward_fp32_gz_ftg"
+                             : "far_forward_fp32_ftg";
+}
+
+template <int MASK, int MESH>
 hipError_t forward_mixed_impl(jgm_context* ctx, int kind, const double* model, double* out,
                               double* raw_out) {
   constexpr int NC = Ncomp<MASK>::value;
@@ -485,6 +615,7 @@ hipError_t forward_mixed_impl(jgm_context* ctx, int kind, const double* model, d
   const long ntiles = (long)g.ntx * g.nty * g.ntz;
   hipError_t e;
   int nchunks = 0;
+  if ((e = ctx->nearpart.ensure(sizeof(double) * (size_t)S * NC)) != hipSuccess) return e;
   if (N > 0) {
     const int site_blocks = (S + kBlock - 1) / kBlock;
     const long waves_per_chunk = (long)site_blocks * (kBlock / 64);
@@ -492,22 +623,21 @@ hipError_t forward_mixed_impl(jgm_context* ctx, int kind, const double* model, d
     want = std::max<long>(1, std::min<long>(want, ntiles));
     const long per = (ntiles + want - 1) / want;
     nchunks = (int)((ntiles + per - 1) / per);
-    if ((e = ctx->weights.ensure(sizeof(float) * (size_t)N * (QUAD ? 8 : 1))) != hipSuccess)
+    if ((e = ctx->weights.ensure(sizeof(float) * (size_t)N * TableWidth<MESH>::value)) !=
+        hipSuccess)
       return e;
     if ((e = ctx->farpart.ensure(sizeof(double) * (size_t)nchunks * NC * S)) != hipSuccess)
       return e;
-    if ((e = ctx->nearpart.ensure(sizeof(double) * (size_t)S * NC)) != hipSuccess) return e;
-    hipLaunchKernelGGL((k_cell_table<QUAD>), dim3(blocks_for(N)), dim3(kBlock), 0, st, model,
+    hipLaunchKernelGGL((k_cell_table<MESH>), dim3(blocks_for(N)), dim3(kBlock), 0, st, model,
                        ctx->d_xsz.as<const double>(), ctx->d_ysz.as<const double>(),
                        ctx->d_zsz.as<const double>(), g.nx, g.ny, g.nz, ctx->weights.as<float>());
     {
-      ProfScope ps(ctx, MASK == kMaskScalar ? "far_forward_fp32_gz"
-                        : MASK == kMaskAll  ? "far_forward_fp32_gz_ftg"
-                                            : "far_forward_fp32_ftg");
-      hipLaunchKernelGGL((k_far_forward<MASK, QUAD>), dim3(site_blocks, nchunks), dim3(kBlock), 0,
+      ProfScope ps(ctx, far_name(MASK, false));
+      hipLaunchKernelGGL((k_far_forward<MASK, MESH>), dim3(site_blocks, nchunks), dim3(kBlock), 0,
                          st, ctx->d_sx.as<double>(), ctx->d_sy.as<double>(),
                          ctx->d_sz.as<double>(), S, ctx->near.box.as<const int>(), g,
-                         ctx->weights.as<const float>(), ntiles, per, ctx->farpart.as<double>());
+                         ctx->weights.as<const float>(), uniform_const(ctx), ntiles, per,
+                         ctx->farpart.as<double>());
     }
     {
       ProfScope ps(ctx, "near_forward_fp64");
@@ -518,7 +648,6 @@ hipError_t forward_mixed_impl(jgm_context* ctx, int kind, const double* model, d
                          ctx->nearpart.as<double>());
     }
   } else {
-    if ((e = ctx->nearpart.ensure(sizeof(double) * (size_t)S * NC)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_fill_zero, dim3(blocks_for((long)S * NC)), dim3(kBlock), 0, st,
                        ctx->nearpart.as<double>(), (size_t)S * NC);
   }
@@ -532,7 +661,7 @@ hipError_t forward_mixed_impl(jgm_context* ctx, int kind, const double* model, d
   return hipGetLastError();
 }
 
-template <int MASK, bool QUAD>
+template <int MASK, int MESH>
 hipError_t adjoint_mixed_impl(jgm_context* ctx, int kind, const double* rm, double* grad) {
   constexpr int NC = Ncomp<MASK>::value;
   hipStream_t st = ctx->stream;
@@ -540,50 +669,58 @@ hipError_t adjoint_mixed_impl(jgm_context* ctx, int kind, const double* rm, doub
   const long N = (long)ncells(ctx);
   const FarGeom g = far_geom(ctx);
   const long ntiles = (long)g.ntx * g.nty * g.ntz;
+  NearBoxes& nb = ctx->near;
   hipError_t e;
   if (N > 0) {
     if ((e = ctx->coef.ensure(sizeof(double) * (size_t)S * NC)) != hipSuccess) return e;
     if ((e = ctx->coef32.ensure(sizeof(float) * (size_t)S * NC)) != hipSuccess) return e;
-    hipLaunchKernelGGL((k_phys_coef<MASK>), dim3(blocks_for(S)), dim3(kBlock), 0, st, rm, S, kind,
-                       field_dir(ctx), ctx->coef.as<double>(), ctx->coef32.as<float>());
-    if (!ctx->near.adj_table_valid) {
-      if ((e = ctx->near.adj_table.ensure(sizeof(float) * (size_t)N * 8)) != hipSuccess) return e;
-      hipLaunchKernelGGL((k_cell_table<QUAD>), dim3(blocks_for(N)), dim3(kBlock), 0, st,
+    hipLaunchKernelGGL((k_mixed_coef<MASK>), dim3(blocks_for(S)), dim3(kBlock), 0, st, rm, S,
+                       kind, field_dir(ctx), ctx->coef.as<double>(), ctx->coef32.as<float>());
+    if (!nb.adj_table_valid) {
+      if ((e = nb.adj_table.ensure(sizeof(float) * (size_t)N * TableWidth<MESH>::value)) !=
+          hipSuccess)
+        return e;
+      hipLaunchKernelGGL((k_cell_table<MESH>), dim3(blocks_for(N)), dim3(kBlock), 0, st,
                          (const double*)nullptr, ctx->d_xsz.as<const double>(),
                          ctx->d_ysz.as<const double>(), ctx->d_zsz.as<const double>(), g.nx, g.ny,
-                         g.nz, ctx->near.adj_table.as<float>());
-      ctx->near.adj_table_valid = true;
+                         g.nz, nb.adj_table.as<float>());
+      nb.adj_table_valid = true;
     }
-    const long tile_waves = ntiles;
     int nchunks = (int)std::min<long>(
-        S, std::max<long>(1, (kTargetWaves + tile_waves - 1) / tile_waves));
+        S, std::max<long>(1, (kTargetWaves + ntiles - 1) / ntiles));
     const int per = (S + nchunks - 1) / nchunks;
     nchunks = (S + per - 1) / per;
     if ((e = ctx->hpart.ensure(sizeof(double) * (size_t)nchunks * N)) != hipSuccess) return e;
     if ((e = ctx->hsum.ensure(sizeof(double) * (size_t)N)) != hipSuccess) return e;
     {
-      ProfScope ps(ctx, MASK == kMaskScalar ? "far_adjoint_fp32_gz"
-                        : MASK == kMaskAll  ? "far_adjoint_fp32_gz_ftg"
-                                            : "far_adjoint_fp32_ftg");
-      hipLaunchKernelGGL((k_far_adjoint<MASK, QUAD>), dim3(blocks_for(ntiles * 64), nchunks),
+      ProfScope ps(ctx, far_name(MASK, true));
+      hipLaunchKernelGGL((k_far_adjoint<MASK, MESH>), dim3(blocks_for(ntiles * 64), nchunks),
                          dim3(kBlock), 0, st, ctx->d_sx.as<double>(), ctx->d_sy.as<double>(),
                          ctx->d_sz.as<double>(), ctx->coef32.as<const float>(),
-                         ctx->near.box.as<const int>(), S, per, g,
-                         ctx->near.adj_table.as<const float>(), ntiles, ctx->hpart.as<double>());
+                         nb.box.as<const int>(), S, per, g, nb.adj_table.as<const float>(),
+                         uniform_const(ctx), ntiles, ctx->hpart.as<double>());
     }
     hipLaunchKernelGGL(k_fill_zero, dim3(blocks_for(N)), dim3(kBlock), 0, st,
                        ctx->hsum.as<double>(), (size_t)N);
-    if (ctx->near.ncols > 0) {
+    if (nb.ncols > 0) {
+      if ((e = ctx->near_h.ensure(sizeof(double) * std::max<size_t>(1, nb.total_near_corners))) !=
+          hipSuccess)
+        return e;
+      if ((e = ctx->near_v.ensure(sizeof(double) * std::max<size_t>(1, nb.total_near_cells))) !=
+          hipSuccess)
+        return e;
       ProfScope ps(ctx, "near_adjoint_fp64");
-      hipLaunchKernelGGL((k_near_adjoint<MASK>), dim3((unsigned)ctx->near.ncols), dim3(64), 0, st,
-                         ctx->near.col_ptr.as<const int>(), ctx->near.col_sites.as<const int>(),
-                         ctx->near.col_ids.as<const int>(), ctx->near.col_kmax.as<const int>(),
+      hipLaunchKernelGGL((k_near_adjoint_site<MASK>), dim3(S), dim3(kBlock), 0, st,
                          ctx->d_sx.as<double>(), ctx->d_sy.as<double>(), ctx->d_sz.as<double>(),
-                         ctx->near.box.as<const int>(), ctx->coef.as<const double>(),
-                         ctx->d_xlo.as<double>(), ctx->d_ylo.as<double>(),
-                         ctx->d_zlo.as<double>(), ctx->d_xsz.as<double>(),
-                         ctx->d_ysz.as<double>(), ctx->d_zsz.as<double>(), g.ny, g.nz,
-                         ctx->hsum.as<double>());
+                         nb.box.as<const int>(), ctx->coef.as<const double>(),
+                         ctx->d_Xc.as<double>(), ctx->d_Yc.as<double>(), ctx->d_Zc.as<double>(),
+                         nb.corner_off.as<const long>(), nb.cell_off.as<const long>(),
+                         ctx->near_h.as<double>(), ctx->near_v.as<double>());
+      hipLaunchKernelGGL(k_near_adjoint_gather, dim3((unsigned)nb.ncols), dim3(64), 0, st,
+                         nb.col_ptr.as<const int>(), nb.col_sites.as<const int>(),
+                         nb.col_ids.as<const int>(), nb.col_kmax.as<const int>(),
+                         nb.box.as<const int>(), nb.cell_off.as<const long>(),
+                         ctx->near_v.as<const double>(), g.ny, g.nz, ctx->hsum.as<double>());
     }
     hipLaunchKernelGGL(k_adjoint_mixed_finish, dim3(blocks_for(N)), dim3(kBlock), 0, st,
                        ctx->hpart.as<const double>(), nchunks, ctx->hsum.as<const double>(), N,
@@ -596,6 +733,26 @@ hipError_t adjoint_mixed_impl(jgm_context* ctx, int kind, const double* rm, doub
   return hipGetLastError();
 }
 
+template <int MASK>
+hipError_t forward_dispatch(jgm_context* ctx, int kind, const double* model, double* out,
+                            double* raw_out) {
+  switch (ctx->near.mesh_class) {
+    case kMeshCubes: return forward_mixed_impl<MASK, kMeshCubes>(ctx, kind, model, out, raw_out);
+    case kMeshUniform:
+      return forward_mixed_impl<MASK, kMeshUniform>(ctx, kind, model, out, raw_out);
+    default: return forward_mixed_impl<MASK, kMeshGeneral>(ctx, kind, model, out, raw_out);
+  }
+}
+
+template <int MASK>
+hipError_t adjoint_dispatch(jgm_context* ctx, int kind, const double* rm, double* grad) {
+  switch (ctx->near.mesh_class) {
+    case kMeshCubes: return adjoint_mixed_impl<MASK, kMeshCubes>(ctx, kind, rm, grad);
+    case kMeshUniform: return adjoint_mixed_impl<MASK, kMeshUniform>(ctx, kind, rm, grad);
+    default: return adjoint_mixed_impl<MASK, kMeshGeneral>(ctx, kind, rm, grad);
+  }
+}
+
 }  // namespace
 
 hipError_t launch_fill_zero(jgm_context* ctx, double* p, size_t n) {
@@ -606,7 +763,8 @@ hipError_t launch_fill_zero(jgm_context* ctx, double* p, size_t n) {
 }
 
 // Host-side planning of the near/far split (geometry only): per-site index boxes,
-// the per-column site lists of the adjoint near pass, and the far-field geometry.
+// the per-column site lists and per-site buffer offsets of the adjoint near pass,
+// and the far-field geometry.
 hipError_t build_near_boxes(jgm_context* ctx) {
   NearBoxes& nb = ctx->near;
   const int nx = (int)ctx->nx, ny = (int)ctx->ny, nz = (int)ctx->nz;
@@ -622,12 +780,12 @@ hipError_t build_near_boxes(jgm_context* ctx) {
   // any cell outside the box is >= R away along some axis, and R >= q0 * its half-diagonal
   const double R = ctx->mixed_q0 * 0.5 * std::sqrt(mx * mx + my * my + mz * mz);
   nb.radius = R;
-  bool cubes = true;
-  const double c0 = nx ? ctx->xs[0] : 0.0;
-  for (double v : ctx->xs) cubes = cubes && v == c0;
-  for (double v : ctx->ys) cubes = cubes && v == c0;
-  for (double v : ctx->zs) cubes = cubes && v == c0;
-  nb.quad = !cubes;
+  bool uniform = nx > 0 && ny > 0 && nz > 0;
+  for (double v : ctx->xs) uniform = uniform && v == ctx->xs[0];
+  for (double v : ctx->ys) uniform = uniform && v == ctx->ys[0];
+  for (double v : ctx->zs) uniform = uniform && v == ctx->zs[0];
+  const bool cubes = uniform && ctx->xs[0] == ctx->ys[0] && ctx->xs[0] == ctx->zs[0];
+  nb.mesh_class = cubes ? kMeshCubes : uniform ? kMeshUniform : kMeshGeneral;
   // monotone centres are required by the binary searches
   bool monotone = true;
   for (int i = 1; i < nx; ++i) monotone = monotone && xcen[i] > xcen[i - 1];
@@ -636,7 +794,8 @@ hipError_t build_near_boxes(jgm_context* ctx) {
   if (!monotone) return hipErrorInvalidValue;
   std::vector<int> box(std::max<size_t>(1, S) * 6);
   std::vector<int> count((size_t)nx * ny + 1, 0);
-  size_t total_cells = 0;
+  std::vector<long> corner_off(std::max<size_t>(1, S), 0), cell_off(std::max<size_t>(1, S), 0);
+  size_t total_cells = 0, total_corners = 0;
   for (size_t s = 0; s < S; ++s) {
     int lo[3], hi[3];
     axis_range(xcen, ctx->hsx[s], R, &lo[0], &hi[0]);
@@ -644,11 +803,15 @@ hipError_t build_near_boxes(jgm_context* ctx) {
     axis_range(zcen, ctx->hsz[s], R, &lo[2], &hi[2]);
     bool empty = false;
     for (int a = 0; a < 3; ++a) empty = empty || lo[a] > hi[a];
+    corner_off[s] = (long)total_corners;
+    cell_off[s] = (long)total_cells;
     if (empty) {
       lo[0] = lo[1] = lo[2] = 1;
       hi[0] = hi[1] = hi[2] = 0;
     } else {
-      total_cells += (size_t)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
+      const size_t di = hi[0] - lo[0] + 1, dj = hi[1] - lo[1] + 1, dk = hi[2] - lo[2] + 1;
+      total_cells += di * dj * dk;
+      total_corners += (di + 1) * (dj + 1) * (dk + 1);
       for (int i = lo[0]; i <= hi[0]; ++i)
         for (int j = lo[1]; j <= hi[1]; ++j) count[(size_t)i * ny + j]++;
     }
@@ -658,6 +821,7 @@ hipError_t build_near_boxes(jgm_context* ctx) {
     }
   }
   nb.total_near_cells = total_cells;
+  nb.total_near_corners = total_corners;
   std::vector<int> col_ids, col_ptr(1, 0), col_kmax;
   std::vector<int> slot((size_t)nx * ny, -1);
   for (size_t col = 0; col < (size_t)nx * ny; ++col) {
@@ -680,11 +844,6 @@ hipError_t build_near_boxes(jgm_context* ctx) {
       }
   }
   nb.ncols = col_ids.size();
-  // far-field geometry: FP32 offsets of centres from the first centre of each tile
-  std::vector<float> offx(nx), offy(ny), offz(nz);
-  for (int i = 0; i < nx; ++i) offx[i] = (float)(xcen[i] - xcen[(i / kTile) * kTile]);
-  for (int j = 0; j < ny; ++j) offy[j] = (float)(ycen[j] - ycen[(j / kTile) * kTile]);
-  for (int k = 0; k < nz; ++k) offz[k] = (float)(zcen[k] - zcen[(k / kTile) * kTile]);
   hipStream_t st = ctx->stream;
   hipError_t e;
   if ((e = upload_vec(nb.box, box, st)) != hipSuccess) return e;
@@ -692,12 +851,11 @@ hipError_t build_near_boxes(jgm_context* ctx) {
   if ((e = upload_vec(nb.col_sites, col_sites, st)) != hipSuccess) return e;
   if ((e = upload_vec(nb.col_ids, col_ids, st)) != hipSuccess) return e;
   if ((e = upload_vec(nb.col_kmax, col_kmax, st)) != hipSuccess) return e;
+  if ((e = upload_vec(nb.corner_off, corner_off, st)) != hipSuccess) return e;
+  if ((e = upload_vec(nb.cell_off, cell_off, st)) != hipSuccess) return e;
   if ((e = upload_vec(nb.xcen, xcen, st)) != hipSuccess) return e;
   if ((e = upload_vec(nb.ycen, ycen, st)) != hipSuccess) return e;
   if ((e = upload_vec(nb.zcen, zcen, st)) != hipSuccess) return e;
-  if ((e = upload_vec(nb.offx, offx, st)) != hipSuccess) return e;
-  if ((e = upload_vec(nb.offy, offy, st)) != hipSuccess) return e;
-  if ((e = upload_vec(nb.offz, offz, st)) != hipSuccess) return e;
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
   nb.adj_table_valid = false;
   nb.valid = true;
@@ -710,17 +868,10 @@ hipError_t launch_forward_mixed(jgm_context* ctx, int kind, const double* model,
     hipError_t e = build_near_boxes(ctx);
     if (e != hipSuccess) return e;
   }
-  const bool q = ctx->near.quad;
   switch (mask_of(kind)) {
-    case kMaskScalar:
-      return q ? forward_mixed_impl<kMaskScalar, true>(ctx, kind, model, out, raw_out)
-               : forward_mixed_impl<kMaskScalar, false>(ctx, kind, model, out, raw_out);
-    case kMaskAll:
-      return q ? forward_mixed_impl<kMaskAll, true>(ctx, kind, model, out, raw_out)
-               : forward_mixed_impl<kMaskAll, false>(ctx, kind, model, out, raw_out);
-    default:
-      return q ? forward_mixed_impl<kMaskTensor, true>(ctx, kind, model, out, raw_out)
-               : forward_mixed_impl<kMaskTensor, false>(ctx, kind, model, out, raw_out);
+    case kMaskScalar: return forward_dispatch<kMaskScalar>(ctx, kind, model, out, raw_out);
+    case kMaskAll: return forward_dispatch<kMaskAll>(ctx, kind, model, out, raw_out);
+    default: return forward_dispatch<kMaskTensor>(ctx, kind, model, out, raw_out);
   }
 }
 
@@ -730,17 +881,10 @@ hipError_t launch_adjoint_mixed(jgm_context* ctx, int kind, const double* /*mode
     hipError_t e = build_near_boxes(ctx);
     if (e != hipSuccess) return e;
   }
-  const bool q = ctx->near.quad;
   switch (mask_of(kind)) {
-    case kMaskScalar:
-      return q ? adjoint_mixed_impl<kMaskScalar, true>(ctx, kind, raw_misfit, grad)
-               : adjoint_mixed_impl<kMaskScalar, false>(ctx, kind, raw_misfit, grad);
-    case kMaskAll:
-      return q ? adjoint_mixed_impl<kMaskAll, true>(ctx, kind, raw_misfit, grad)
-               : adjoint_mixed_impl<kMaskAll, false>(ctx, kind, raw_misfit, grad);
-    default:
-      return q ? adjoint_mixed_impl<kMaskTensor, true>(ctx, kind, raw_misfit, grad)
-               : adjoint_mixed_impl<kMaskTensor, false>(ctx, kind, raw_misfit, grad);
+    case kMaskScalar: return adjoint_dispatch<kMaskScalar>(ctx, kind, raw_misfit, grad);
+    case kMaskAll: return adjoint_dispatch<kMaskAll>(ctx, kind, raw_misfit, grad);
+    default: return adjoint_dispatch<kMaskTensor>(ctx, kind, raw_misfit, grad);
   }
 }
 

@@ -62,12 +62,12 @@ def test_mixed_against_oracle_nonuniform_cells(orc):
     X, Y = np.meshgrid(xs, ys, indexing="ij")
     d.SetMeasurementPoints(X.ravel(), Y.ravel(), np.full(X.size, -50.0))
     imp = TensorHipGravityImp(nat.MIXED)
-    imp.context(m, d).set_mixed_threshold(6.0)  # small mesh: force a real far field
+    imp.context(m, d).set_mixed_threshold(4.0)  # small mesh: force a real far field
     mix = imp.Calculate(m, d)
     ref = orc.calculate(orc.TENSOR, m, m.GetModelParameters(), d.positions)
-    assert comp_err(mix, ref, 9) <= 2e-5  # q0 = 6: truncation ~1e-5, stated here
-    imp12 = TensorHipGravityImp(nat.MIXED)
-    assert comp_err(imp12.Calculate(m, d), ref, 9) <= TOL
+    assert comp_err(mix, ref, 9) <= 2e-5  # q0 = 4: truncation O(4^-6), stated here
+    imp8 = TensorHipGravityImp(nat.MIXED)  # default q0 = 8
+    assert comp_err(imp8.Calculate(m, d), ref, 9) <= TOL
 
 
 def test_mixed_adjoint_dot_product_and_fp64():
@@ -136,9 +136,36 @@ def test_mixed_magnetic_with_transform(orc):
     assert rel_inf(g, g64) <= TOL
 
 
-def test_mixed_config0_is_exact_near_field(orc):
-    """On config 0 every cell is inside the near boxes: mixed equals FP64 closely."""
+def test_mixed_config0(orc):
+    """Config 0 (50 m cells, sites 1 m above): gz within 1e-9 relative of the oracle."""
     m, d = synthetic.config0()
     mix = ScalarHipGravityImp(nat.MIXED).Calculate(m, d)
     ref = orc.calculate(orc.SCALAR, m, m.GetModelParameters(), d.positions)
-    assert np.max(np.abs(mix - ref) / np.abs(ref)) <= 1e-12
+    assert np.max(np.abs(mix - ref) / np.abs(ref)) <= 1e-9
+
+
+def test_mixed_all_near_equals_fp64():
+    """A near radius larger than the mesh leaves only the FP64 corner pass."""
+    m, d = small_headline()
+    imp = ScalarTensorHipGravityImp(nat.MIXED)
+    imp.context(m, d).set_mixed_threshold(1e6)
+    mix = imp.Calculate(m, d)
+    ref = ScalarTensorHipGravityImp(nat.FP64).Calculate(m, d)
+    assert comp_err(mix, ref, 10) <= 1e-11
+    r = np.random.default_rng(5).normal(size=d.nsites * 10) * 1e-9
+    g = imp.LQDerivative(m, d, r)
+    g64 = ScalarTensorHipGravityImp(nat.FP64).LQDerivative(m, d, r)
+    assert rel_inf(g, g64) <= 1e-11
+
+
+def test_headline_accuracy_full_size():
+    """128x128x64 cubes, 64x64 sites, gz + FTG: mixed within 1e-6 of FP64 for every
+    component (relative to the component's largest magnitude over the survey) and
+    the gradient within 1e-6 of ||g||_inf."""
+    m, d = synthetic.config4_gravity()
+    f64 = ScalarTensorHipGravityImp(nat.FP64)
+    mix = ScalarTensorHipGravityImp(nat.MIXED)
+    a, b = mix.Calculate(m, d), f64.Calculate(m, d)
+    assert comp_err(a, b, 10) <= TOL
+    r = np.random.default_rng(8).normal(size=d.nsites * 10) * 1e-9
+    assert rel_inf(mix.LQDerivative(m, d, r), f64.LQDerivative(m, d, r)) <= TOL
