@@ -39,6 +39,28 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+# library timer names -> kernel symbols in the rocprofv3 summaries (cubic mesh)
+KERNEL_SYMBOL = {
+    "forward_corners_fp64_gz_ftg": "k_forward_corners<7>",
+    "adjoint_corners_fp64_gz_ftg": "k_adjoint_corners<7>",
+    "far_forward_fp32_gz_ftg": "k_far_forward<7, 0>",
+    "far_adjoint_fp32_gz_ftg": "k_far_adjoint<7, 0>",
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gz_ftg_128x128x64_hbm_traffic.md")
+
+
+def profiled_traffic(kname):
+    """HBM bytes per dispatch of a kernel from the committed counter summary, or None."""
+    sym = KERNEL_SYMBOL.get(kname)
+    if sym is None or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 4 and cells[0] == f"`{sym}`":
+            return float(cells[3]) * 1e6
+    return None
+
+
 def run_gpu(args, model, data, prec_name):
     from jif3d_amd import _native as nat
     prec = nat.MIXED if prec_name == "mixed" else nat.FP64
@@ -117,6 +139,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="quarter-size mesh for quick checks")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 512 MiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards "
@@ -132,6 +156,17 @@ def main():
     rho_c = (model.nx + 1) * (model.ny + 1) * (model.nz + 1) / N
     log(f"[bench] workload: {model.nx}x{model.ny}x{model.nz} cells, {S} sites")
 
+    if args.calibrate_copy:
+        from jif3d_amd import _native as nat
+        cctx = nat.Context(0)
+        a = cctx.alloc(64 * 1024 * 1024)  # 512 MiB of doubles
+        b = cctx.alloc(64 * 1024 * 1024)
+        for _ in range(2):
+            cctx.copy_device(b, a)
+        cctx.synchronize()
+        a.free()
+        b.free()
+        log("[bench] calibration copy: 2 x 512 MiB device-to-device")
     precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     res = {p: run_gpu(args, model, data, p) for p in precs}
     main_p = "mixed" if "mixed" in res else "fp64"
@@ -162,7 +197,7 @@ def main():
                    "cells": N, "sites": S, "components": 10, "precision": main_p},
         "roofline": {"bound": "valu", "kernel": kname, "achieved": achieved / 1e12,
                      "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": None,
+                     "traffic": profiled_traffic(kname),
                      "work": f"{fe_per_pair(rho_c):.1f} flop-equivalents per pair x {S * N} pairs"},
     }
     if "fp64" in res and main_p != "fp64":

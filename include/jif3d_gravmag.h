@@ -164,6 +164,7 @@ jgm_status jgm_device_alloc(jgm_context *ctx, size_t bytes, void **ptr);
 jgm_status jgm_device_free(jgm_context *ctx, void *ptr);
 jgm_status jgm_copy_to_device(jgm_context *ctx, void *dst, const void *src, size_t bytes);
 jgm_status jgm_copy_to_host(jgm_context *ctx, void *dst, const void *src, size_t bytes);
+jgm_status jgm_copy_device(jgm_context *ctx, void *dst, const void *src, size_t bytes);
 
 /* Kernel timing with HIP events recorded on the context's stream around every
  * kernel launch while enabled (replaces the wall-clock harness of

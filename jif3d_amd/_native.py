@@ -25,7 +25,7 @@ EXPORTS = [
     "jgm_n_params", "jgm_forward", "jgm_forward_host", "jgm_adjoint", "jgm_adjoint_host",
     "jgm_sensitivity_rows", "jgm_storedj_build", "jgm_storedj_rows", "jgm_storedj_apply",
     "jgm_storedj_apply_t", "jgm_storedj_release", "jgm_device_alloc", "jgm_device_free",
-    "jgm_copy_to_device", "jgm_copy_to_host", "jgm_profile_enable", "jgm_profile_reset",
+    "jgm_copy_to_device", "jgm_copy_to_host", "jgm_copy_device", "jgm_profile_enable", "jgm_profile_reset",
     "jgm_profile_count", "jgm_profile_get",
 ]
 
@@ -79,6 +79,7 @@ def load():
         "jgm_device_free": [vp, vp],
         "jgm_copy_to_device": [vp, vp, vp, sz],
         "jgm_copy_to_host": [vp, vp, vp, sz],
+        "jgm_copy_device": [vp, vp, vp, sz],
         "jgm_profile_enable": [vp, i],
         "jgm_profile_reset": [vp],
         "jgm_profile_get": [vp, sz, ctypes.POINTER(ctypes.c_char_p), pd,
@@ -267,6 +268,11 @@ class Context:
 
     def to_device(self, arr, dtype=None):
         return DeviceArray.from_host(self, arr, dtype)
+
+    def copy_device(self, dst, src):
+        n = min(dst.nbytes, src.nbytes)
+        check(self.lib.jgm_copy_device(self.handle, ctypes.c_void_p(dst.ptr),
+                                       ctypes.c_void_p(src.ptr), n))
 
     def profile_enable(self, on=True):
         check(self.lib.jgm_profile_enable(self.handle, 1 if on else 0))

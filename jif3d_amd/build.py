@@ -69,7 +69,23 @@ def build(verbose=False, asm=False):
     if res.returncode != 0:
         raise RuntimeError(f"linking libjif3d_amd.so failed:\n{res.stderr}")
     os.replace(tmp, LIB)
+    build_driver()
     return LIB
+
+
+DRIVER = os.path.join(HERE, "host", "adapter_driver")
+
+
+def build_driver():
+    """The C++ adapter's command-line driver, linked against libjif3d_amd.so."""
+    cxx = shutil.which("g++") or "g++"
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, "-I", os.path.join(HERE, "host"),
+           os.path.join(HERE, "host", "adapter_driver.cpp"), "-o", DRIVER + ".tmp",
+           "-L", HERE, "-ljif3d_amd", "-Wl,-rpath,$ORIGIN/.."]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"building the C++ adapter driver failed:\n{res.stderr}")
+    os.replace(DRIVER + ".tmp", DRIVER)
 
 
 if __name__ == "__main__":

@@ -530,6 +530,10 @@ jgm_status jgm_copy_to_host(jgm_context* ctx, void* dst, const void* src, size_t
   return copy(ctx, dst, src, bytes, hipMemcpyDeviceToHost, "jgm_copy_to_host");
 }
 
+jgm_status jgm_copy_device(jgm_context* ctx, void* dst, const void* src, size_t bytes) {
+  return copy(ctx, dst, src, bytes, hipMemcpyDeviceToDevice, "jgm_copy_device");
+}
+
 jgm_status jgm_profile_enable(jgm_context* ctx, int enable) {
   if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
   ctx->prof.enabled = enable != 0;

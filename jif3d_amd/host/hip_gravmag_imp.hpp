@@ -1,0 +1,140 @@
+// C++ host adapter over the C ABI of include/jif3d_gravmag.h.
+//
+// HipGravMagImplementation mirrors jif3D's ThreeDGravMagImplementation
+// (jif3D/GravMag/ThreeDGravMagImplementation.h:39-84): RawDataPerMeasurement,
+// SetDataTransform, GetDataPerMeasurement, Calculate and LQDerivative, with std::vector
+// in place of ublas and a plain mesh/site description in place of the jif3D model and
+// data classes.  Errors from the library are thrown as FatalException, as
+// jif3D::FatalException is (jif3D/Global/FatalException.h:27-41).  The object owns one
+// library context (device buffers, cached geometry); it is movable but not copyable,
+// like the reference's CUDA imps (jif3D/Gravity/ScalarCudaGravityImp.h:75-76).
+#pragma once
+
+#include <cstddef>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "jif3d_gravmag.h"
+
+namespace jif3d_amd {
+
+class FatalException : public std::runtime_error {
+ public:
+  explicit FatalException(const std::string& msg) : std::runtime_error(msg) {}
+};
+
+inline void check(jgm_status st, const char* what) {
+  if (st != JGM_OK)
+    throw FatalException(std::string(what) + ": " + jgm_last_error() + " (status " +
+                         std::to_string(static_cast<int>(st)) + ")");
+}
+
+// Rectilinear mesh as ThreeDModelBase stores it: n+1 corner coordinates and n cell
+// sizes per axis, plus the 1-D background layer thicknesses.
+struct Mesh {
+  std::vector<double> xcoords, ycoords, zcoords;
+  std::vector<double> xsizes, ysizes, zsizes;
+  std::vector<double> bg_thicknesses;
+  size_t ncells() const { return xsizes.size() * ysizes.size() * zsizes.size(); }
+  bool operator==(const Mesh& o) const {
+    return xcoords == o.xcoords && ycoords == o.ycoords && zcoords == o.zcoords &&
+           xsizes == o.xsizes && ysizes == o.ysizes && zsizes == o.zsizes &&
+           bg_thicknesses == o.bg_thicknesses;
+  }
+};
+
+struct Sites {
+  std::vector<double> x, y, z;
+  size_t size() const { return x.size(); }
+  bool operator==(const Sites& o) const { return x == o.x && y == o.y && z == o.z; }
+};
+
+class HipGravMagImplementation {
+ public:
+  HipGravMagImplementation(jgm_kind kind, jgm_precision prec = JGM_FP64, int device = 0,
+                           void* stream = nullptr)
+      : kind_(kind), prec_(prec) {
+    check(jgm_create(device, stream, &ctx_), "jgm_create");
+  }
+  ~HipGravMagImplementation() {
+    if (ctx_) jgm_destroy(ctx_);
+  }
+  HipGravMagImplementation(const HipGravMagImplementation&) = delete;
+  HipGravMagImplementation& operator=(const HipGravMagImplementation&) = delete;
+  HipGravMagImplementation(HipGravMagImplementation&& o) noexcept { *this = std::move(o); }
+  HipGravMagImplementation& operator=(HipGravMagImplementation&& o) noexcept {
+    std::swap(ctx_, o.ctx_);
+    std::swap(kind_, o.kind_);
+    std::swap(prec_, o.prec_);
+    std::swap(mesh_, o.mesh_);
+    std::swap(sites_, o.sites_);
+    std::swap(have_geometry_, o.have_geometry_);
+    return *this;
+  }
+
+  size_t RawDataPerMeasurement() const { return jgm_raw_per_site(kind_); }
+  void SetDataTransform(jgm_transform tf, double inclination = 0.0, double declination = 0.0) {
+    check(jgm_set_transform(ctx_, tf, inclination, declination), "jgm_set_transform");
+    tf_ = tf;
+  }
+  size_t GetDataPerMeasurement() const { return tf_ == JGM_TF_NONE ? RawDataPerMeasurement() : 1; }
+  // Inducing field of the susceptibility imp: radians, radians, nT.
+  void SetField(double inclination, double declination, double fieldstrength) {
+    check(jgm_set_field(ctx_, inclination, declination, fieldstrength), "jgm_set_field");
+  }
+  void SetPrecision(jgm_precision prec) { prec_ = prec; }
+
+  // Forward response of every site (nsites * GetDataPerMeasurement values).
+  std::vector<double> Calculate(const Mesh& mesh, const Sites& sites,
+                                const std::vector<double>& model) {
+    Configure(mesh, sites);
+    if (model.size() != jgm_n_params(ctx_))
+      throw FatalException("model vector length does not match the mesh and background");
+    std::vector<double> out(sites.size() * GetDataPerMeasurement());
+    check(jgm_forward_host(ctx_, kind_, prec_, model.data(), out.data()), "jgm_forward_host");
+    return out;
+  }
+
+  // 2 J^T D^T misfit, J recomputed (never stored).
+  std::vector<double> LQDerivative(const Mesh& mesh, const Sites& sites,
+                                   const std::vector<double>& model,
+                                   const std::vector<double>& misfit) {
+    Configure(mesh, sites);
+    if (misfit.size() != sites.size() * GetDataPerMeasurement())
+      throw FatalException("misfit length does not match the number of data");
+    std::vector<double> grad(jgm_n_params(ctx_));
+    check(jgm_adjoint_host(ctx_, kind_, prec_, model.data(), misfit.data(), grad.data()),
+          "jgm_adjoint_host");
+    return grad;
+  }
+
+  jgm_context* context() { return ctx_; }
+
+ private:
+  void Configure(const Mesh& mesh, const Sites& sites) {
+    if (have_geometry_ && mesh == mesh_ && sites == sites_) return;
+    check(jgm_set_mesh(ctx_, mesh.xsizes.size(), mesh.ysizes.size(), mesh.zsizes.size(),
+                       mesh.xcoords.data(), mesh.ycoords.data(), mesh.zcoords.data(),
+                       mesh.xsizes.data(), mesh.ysizes.data(), mesh.zsizes.data()),
+          "jgm_set_mesh");
+    check(jgm_set_background(ctx_, mesh.bg_thicknesses.size(), mesh.bg_thicknesses.data()),
+          "jgm_set_background");
+    check(jgm_set_sites(ctx_, sites.size(), sites.x.data(), sites.y.data(), sites.z.data()),
+          "jgm_set_sites");
+    mesh_ = mesh;
+    sites_ = sites;
+    have_geometry_ = true;
+  }
+
+  jgm_context* ctx_ = nullptr;
+  jgm_kind kind_ = JGM_KIND_SCALAR;
+  jgm_precision prec_ = JGM_FP64;
+  jgm_transform tf_ = JGM_TF_NONE;
+  Mesh mesh_;
+  Sites sites_;
+  bool have_geometry_ = false;
+};
+
+}  // namespace jif3d_amd
