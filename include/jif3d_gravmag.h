@@ -140,14 +140,20 @@ jgm_status jgm_adjoint_host(jgm_context *ctx, jgm_kind kind, jgm_precision prec,
 jgm_status jgm_sensitivity_rows(jgm_context *ctx, jgm_kind kind, size_t site0, size_t nsites,
                                 double *rows);
 
-/* Stored J in HBM, FP32, compact layout: rows = nsites * raw components
- * (gz: 1; tensor and magnetic: the 6 independent tensor values; scalar+tensor: 7),
- * columns = ncells, column-major (rows fastest).  Replaces the ublas matrix of
- * FullSensitivityGravMagCalculator (:40-41, :116-130); prec selects how it is built. */
+/* Stored J in HBM, FP32, compact layout: rows = nsites * k with, per site,
+ * gz (scalar, k = 1); Uxx, Uxy, Uxz, Uyy, Uyz, Uzz (tensor, k = 6); Bx, By, Bz per unit
+ * susceptibility (magnetic, k = 3); gz then the six tensor values (scalar+tensor, k = 7).
+ * Columns = ncells (background layers are not stored), column-major (rows fastest).
+ * Replaces the ublas matrix of FullSensitivityGravMagCalculator (:40-41, :116-130).
+ * Every entry is evaluated with the FP64 closed form and rounded once to FP32, so
+ * prec (JGM_FP64 or JGM_MIXED) does not change J. */
 jgm_status jgm_storedj_build(jgm_context *ctx, jgm_kind kind, jgm_precision prec);
 size_t jgm_storedj_rows(const jgm_context *ctx);
-/* Y = J M for nrhs model vectors (M: ncells x nrhs column-major, Y: rows x nrhs),
- * batching CalculateRawData (:137-152) over L-BFGS trial models. */
+/* Y = J M for nrhs model vectors (M: ncells x nrhs column-major, Y: rows x nrhs
+ * column-major; device pointers), batching CalculateRawData (:137-152) over L-BFGS
+ * trial models.  JGM_GEMM_FP32 multiplies FP32 operands; JGM_GEMM_BF16 rounds J and M
+ * to BF16 (FP32 accumulation); JGM_GEMM_BF16_REFINED adds the products of the BF16
+ * remainders (hi*hi + hi*lo + lo*hi), close to FP32 accuracy at BF16 matrix-core rate. */
 jgm_status jgm_storedj_apply(jgm_context *ctx, jgm_gemm_precision prec, const float *M,
                              size_t nrhs, float *Y);
 /* G = J^T R for nrhs residual vectors (R: rows x nrhs, G: ncells x nrhs), the batched

@@ -12,6 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libjif3d_amd.so")
 
 JGM_OK = 0
+(JGM_ERR_INVALID_ARGUMENT, JGM_ERR_NOT_READY, JGM_ERR_DEVICE, JGM_ERR_OUT_OF_MEMORY,
+ JGM_ERR_UNSUPPORTED) = 1, 2, 3, 4, 5
 KIND_SCALAR, KIND_TENSOR, KIND_MAGNETIC, KIND_SCALAR_TENSOR = 1, 2, 3, 4
 FP64, MIXED = 0, 1
 TF_NONE, TF_FTG_INVARIANT, TF_TOTAL_FIELD, TF_TOTAL_FIELD_ANOMALY = 0, 1, 2, 3
@@ -114,6 +116,15 @@ def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def _check_rhs(x, y, nin, nout):
+    """Shapes of a batched stored-J product: x (nrhs, nin) and y (nrhs, nout), FP32."""
+    if len(x.shape) != 2 or tuple(y.shape) != (x.shape[0], nout) or x.shape[1] != nin:
+        raise ValueError(f"stored-J operands must be (nrhs, {nin}) and (nrhs, {nout}), "
+                         f"got {tuple(x.shape)} and {tuple(y.shape)}")
+    if np.dtype(x.dtype) != np.float32 or np.dtype(y.dtype) != np.float32:
+        raise ValueError("stored-J operands are float32")
+
+
 def _devptr(t):
     """Device address of a DeviceArray (or of any object exposing data_ptr())."""
     if isinstance(t, DeviceArray):
@@ -202,6 +213,7 @@ class Context:
         xs, ys, zs = (_f64(v) for v in mesh.sizes)
         check(self.lib.jgm_set_mesh(self.handle, mesh.nx, mesh.ny, mesh.nz, _pd(xc), _pd(yc),
                                     _pd(zc), _pd(xs), _pd(ys), _pd(zs)))
+        self.ncells = int(mesh.nx) * int(mesh.ny) * int(mesh.nz)
         t = _f64(getattr(mesh, "bg_thicknesses", np.zeros(0)))
         check(self.lib.jgm_set_background(self.handle, len(t), _pd(t) if len(t) else None))
 
@@ -263,6 +275,9 @@ class Context:
     def storedj_rows(self):
         return self.lib.jgm_storedj_rows(self.handle)
 
+    def storedj_cols(self):
+        return self.ncells
+
     def alloc(self, shape, dtype=np.float64):
         return DeviceArray(self, shape, dtype)
 
@@ -293,12 +308,18 @@ class Context:
         return res
 
     def storedj_apply(self, prec, M, Y):
-        check(self.lib.jgm_storedj_apply(self.handle, prec, _devptr(M), M.shape[1],
+        """Y = J M for device arrays of FP32 right-hand sides, one per row: M of shape
+        (nrhs, ncells), Y of shape (nrhs, storedj_rows())."""
+        _check_rhs(M, Y, self.storedj_cols(), self.storedj_rows())
+        check(self.lib.jgm_storedj_apply(self.handle, prec, _devptr(M), M.shape[0],
                                          _devptr(Y)))
         return Y
 
     def storedj_apply_t(self, prec, R, G):
-        check(self.lib.jgm_storedj_apply_t(self.handle, prec, _devptr(R), R.shape[1],
+        """G = J^T R for device arrays: R of shape (nrhs, storedj_rows()), G of shape
+        (nrhs, ncells)."""
+        _check_rhs(R, G, self.storedj_rows(), self.storedj_cols())
+        check(self.lib.jgm_storedj_apply_t(self.handle, prec, _devptr(R), R.shape[0],
                                            _devptr(G)))
         return G
 

@@ -118,6 +118,20 @@ def config2(data_cls=TensorGravityData):
     return m, _sites(data_cls, s, s, -50.0)
 
 
+def config3(nrhs=64, seed=3):
+    """Batched stored-J case: the 64x64x32 mesh and 32x32 sites of config1, scalar+FTG
+    (7 compact rows per site); nrhs trial models (the seeded model perturbed by uniform
+    +/-5% per cell) and nrhs Gaussian residual vectors of the compact row length.
+    Returns (model, data, M, Rr) with M of shape (nrhs, ncells) and Rr of shape
+    (nrhs, 7 * nsites), one right-hand side per row."""
+    m, d = config1()
+    rng = np.random.default_rng(seed)
+    base = m.GetModelParameters()[:m.ncells]
+    M = base[None, :] * (1.0 + rng.uniform(-0.05, 0.05, size=(nrhs, m.ncells)))
+    Rr = rng.normal(size=(nrhs, 7 * d.nsites))
+    return m, d, M, Rr
+
+
 def config4_gravity(data_cls=TensorGravityData, n=128, nzc=64, nsite=64):
     """Headline geometry (128x128x64 cubes of 20 m, 64x64 sites at z=-30 m) with densities."""
     rng = np.random.default_rng(2)
