@@ -452,7 +452,9 @@ jgm_status jgm_storedj_build(jgm_context* ctx, jgm_kind kind, jgm_precision prec
   return JGM_OK;
 }
 
-size_t jgm_storedj_rows(const jgm_context* ctx) { return ctx ? ctx->storedJ_rows : 0; }
+size_t jgm_storedj_rows(const jgm_context* ctx) {
+  return ctx && ctx->storedJ_kind ? ctx->storedJ_rows : 0;
+}
 
 static jgm_status storedj_apply(jgm_context* ctx, jgm_gemm_precision prec, int transpose,
                                 const float* in, size_t nrhs, float* out) {

@@ -125,10 +125,12 @@ struct jgm_context {
 
   jgm::Profiler prof;
 
-  // stored sensitivity matrix
+  // stored sensitivity matrix: FP32, column-major with leading dimension storedJ_ld
+  // (rows padded to a multiple of 64) and storedJ_colpad columns (cells padded to a
+  // multiple of 256); the padding holds zeros
   jgm::DeviceBuffer storedJ;
   int storedJ_kind = 0;
-  size_t storedJ_rows = 0, storedJ_cols = 0;
+  size_t storedJ_rows = 0, storedJ_cols = 0, storedJ_ld = 0, storedJ_colpad = 0;
 };
 
 namespace jgm {

@@ -1,17 +1,26 @@
 // Stored-sensitivity path: J built once in HBM (FP32), then applied to many
 // right-hand sides at once on the matrix cores.
 //
-// J layout: column-major, J[c * R + row], rows = site * kJ + component fastest, with
+// J layout: column-major, J[c * ld + row], rows = site * kJ + component fastest, with
 // the compact components per site: gz (scalar); Uxx, Uxy, Uxz, Uyy, Uyz, Uzz
 // (tensor); Bx, By, Bz (magnetic); gz then the six tensor values (scalar+tensor).
-// Columns are the mesh cells (background layers are not stored).
+// Columns are the mesh cells (background layers are not stored).  The leading
+// dimension ld is the row count rounded up to 64 and the column count is padded to a
+// multiple of 256; padding entries are zero, so the product kernels read whole tiles
+// without bounds tests.
 //
 // Products (nrhs right-hand sides, column-major operands):
-//   Y = J M    (R x nrhs):  split over cells; A fragments of J are read straight from
-//                          HBM (32 consecutive rows per cell = one 128-byte line).
-//   G = J^T Rr (N x nrhs):  one 128-cell tile per workgroup over all rows; J tiles are
-//                          staged through LDS so global reads stay coalesced.
-// M and Rr are first transposed to [k][nrhs_padded] (FP32, nrhs padded to 64).
+//   Y = J M    (R x nrhs):  split over cells.  A wave owns 64 rows x 64 right-hand
+//                          sides (2 x 2 MFMA tiles); its A fragments are 128-byte row
+//                          segments of one J column, read straight from HBM with the
+//                          next 16-cell chunk in flight while the current one is
+//                          multiplied.  Partial sums per split are reduced in FP64.
+//   G = J^T Rr (N x nrhs):  a workgroup owns 256 cells x 64 right-hand sides and walks
+//                          all rows in stages of 32; each stage of J (32 KB, one
+//                          128-byte line per cell) and of Rr passes through LDS, the
+//                          next stage's global loads overlapping this stage's MFMAs.
+// M and Rr are first transposed to [k][nrhs_padded] (FP32, nrhs padded to 64, k padded
+// with zero rows to the padded J extent).
 // Precisions: FP32 operands on v_mfma_f32_32x32x2_f32; BF16 operands on
 // v_mfma_f32_32x32x16_bf16 (FP32 accumulate), with the "refined" variant splitting
 // both operands into hi + lo BF16 parts in registers and summing hi*hi + hi*lo +
@@ -31,10 +40,17 @@ namespace jgm {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kRowPad = 64;    // J leading dimension multiple
+constexpr int kColPad = 256;   // J column count multiple (one J^T R workgroup)
+constexpr int kChunk = 16;     // cells per pipeline chunk of J M
+constexpr int kStage = 32;     // rows per LDS stage of J^T R
+constexpr int kPitch = 257;    // LDS row pitch (floats) of the J^T R stage
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 inline unsigned blocks_for(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+inline long round_up(long v, long m) { return (v + m - 1) / m * m; }
 
 __host__ __device__ inline int compact_rows(int kind) {
   switch (kind) {
@@ -53,6 +69,11 @@ int mask_of(int kind) {
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_fill_zero_f32(float* __restrict__ p, long n) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0.0f;
+}
+
 // Build: one lane per site, the wave walks a chunk of cells (wave-uniform); FP64
 // closed form per cell (8 corners), stored as FP32 in the compact layout.
 template <int MASK>
@@ -61,13 +82,12 @@ __global__ __launch_bounds__(kBlock) void k_storedj_build(
     int S, const double* __restrict__ xlo, const double* __restrict__ ylo,
     const double* __restrict__ zlo, const double* __restrict__ xsz,
     const double* __restrict__ ysz, const double* __restrict__ zsz, int ny, int nz, long ncell,
-    long cells_per_chunk, int kind, FieldDir fd, float* __restrict__ J) {
+    long cells_per_chunk, int kind, FieldDir fd, long ld, float* __restrict__ J) {
   constexpr int NC = Ncomp<MASK>::value;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int sl = s < S ? s : S - 1;
   const double px = sx[sl], py = sy[sl], pz = sz[sl];
   const int kJ = compact_rows(kind);
-  const long R = (long)S * kJ;
   const long c0 = (long)blockIdx.y * cells_per_chunk;
   const long c1 = c0 + cells_per_chunk < ncell ? c0 + cells_per_chunk : ncell;
   for (long c = c0; c < c1; ++c) {
@@ -79,7 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_storedj_build(
     box_terms_f64<MASK>(px, py, pz, xlo[i], ylo[j], zlo[k], xsz[i], ysz[j], zsz[k], v);
     spread_components<MASK>(v, u);  // gz, Uxx, Uyy, Uzz, Uxy, Uxz, Uyz
     if (s >= S) continue;
-    float* o = J + c * R + (long)s * kJ;
+    float* o = J + c * ld + (long)s * kJ;
     if (kind == JGM_KIND_SCALAR) {
       o[0] = (float)u[0];
     } else if (kind == JGM_KIND_MAGNETIC) {
@@ -99,93 +119,177 @@ __global__ __launch_bounds__(kBlock) void k_storedj_build(
   }
 }
 
-// Column-major X (n x nrhs) -> Xt[k][P] (P = nrhs padded to a multiple of 64, zeros).
+// Column-major X (n x nrhs) -> Xt[k][P] for k < npad (P = nrhs padded to 64); rows
+// k >= n and columns b >= nrhs are zero.
 __global__ __launch_bounds__(kBlock) void k_transpose_pad(const float* __restrict__ X, long n,
-                                                          int nrhs, int P,
+                                                          long npad, int nrhs, int P,
                                                           float* __restrict__ Xt) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n * P) return;
+  if (idx >= npad * P) return;
   const long k = idx / P;
   const int b = (int)(idx % P);
-  Xt[idx] = b < nrhs ? X[k + (long)b * n] : 0.0f;
+  Xt[idx] = (k < n && b < nrhs) ? X[k + (long)b * n] : 0.0f;
+}
+
+__device__ __forceinline__ f32x16 mfma32(float a, float b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x16 mfma_bf16(bf16x8 a, bf16x8 b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 
 // x = hi + lo + e, |e| <= 2^-16 |x|: hi is x rounded to BF16, lo the rounded remainder
 // (x - hi is exact in FP32)
-__device__ __forceinline__ void split_bf16(float x, bf16x8& hi, bf16x8& lo, int e) {
-  const __bf16 h = (__bf16)x;
-  hi[e] = h;
-  lo[e] = (__bf16)(x - (float)h);
+template <bool LO>
+__device__ __forceinline__ void to_bf16(const float (&x)[8], bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const __bf16 h = (__bf16)x[e];
+    hi[e] = h;
+    if constexpr (LO) lo[e] = (__bf16)(x[e] - (float)h);
+  }
 }
 
-// Y partials: workgroup = 4 waves x 32 rows, 64 right-hand sides (two 32x32 tiles),
-// cells [c0, c1) of one split.  Yp[split][row][P].
+// acc[ta][tb] += A_ta B_tb for a 2 x 2 block of 32x32 tiles.  Element s of a lane's
+// operands is k-step s (FP32: the lane's k is lane/32 within each step) or, for BF16,
+// element s of the single 16-deep step (k = 8 * (lane/32) + s); A and B elements with
+// the same index always refer to the same k.
 template <int PREC>
-__global__ __launch_bounds__(kBlock) void k_gemm_jm(const float* __restrict__ J, long R, long N,
-                                                    const float* __restrict__ Mt, int P,
-                                                    long cells_per_split,
-                                                    float* __restrict__ Yp) {
-  const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  const long row0 = ((long)blockIdx.x * 4 + wave) * 32;
-  const int b0 = blockIdx.z * 64;
-  const long c0 = (long)blockIdx.y * cells_per_split;
-  const long c1 = c0 + cells_per_split < N ? c0 + cells_per_split : N;
-  const int li = lane & 31, lk = lane >> 5;
-  const long row = row0 + li;
-  const bool rv = row < R;
-  f32x16 acc0 = {0}, acc1 = {0};
+__device__ __forceinline__ void mma_block(const float (&a)[2][8], const float (&b)[2][8],
+                                          f32x16 (&acc)[2][2]) {
   if constexpr (PREC == JGM_GEMM_FP32) {
-    // every lane joins every MFMA: the k-slot past the end of the range reads zeros
-    for (long c = c0; c < c1; c += 2) {
-      const long cc = c + lk;
-      const bool cv = cc < c1;
-      const float a = (rv && cv) ? J[cc * R + row] : 0.0f;
-      const float bv0 = cv ? Mt[cc * P + b0 + li] : 0.0f;
-      const float bv1 = cv ? Mt[cc * P + b0 + 32 + li] : 0.0f;
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bv0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bv1, acc1, 0, 0, 0);
-    }
-  } else {
-    for (long c = c0; c < c1; c += 16) {
-      bf16x8 ahi, alo, bhi0, blo0, bhi1, blo1;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const long cc = c + 8 * lk + e;
-        const bool cv = cc < c1;
-        const float a = (rv && cv) ? J[cc * R + row] : 0.0f;
-        const float m0 = cv ? Mt[cc * P + b0 + li] : 0.0f;
-        const float m1 = cv ? Mt[cc * P + b0 + 32 + li] : 0.0f;
-        if constexpr (PREC == JGM_GEMM_BF16) {
-          ahi[e] = (__bf16)a;
-          bhi0[e] = (__bf16)m0;
-          bhi1[e] = (__bf16)m1;
-        } else {
-          split_bf16(a, ahi, alo, e);
-          split_bf16(m0, bhi0, blo0, e);
-          split_bf16(m1, bhi1, blo1, e);
+    for (int s = 0; s < 8; ++s)
+#pragma unroll
+      for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb) acc[ta][tb] = mfma32(a[ta][s], b[tb][s], acc[ta][tb]);
+  } else {
+    constexpr bool R = PREC == JGM_GEMM_BF16_REFINED;
+    bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      to_bf16<R>(a[t], ah[t], al[t]);
+      to_bf16<R>(b[t], bh[t], bl[t]);
+    }
+#pragma unroll
+    for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+      for (int tb = 0; tb < 2; ++tb) {
+        acc[ta][tb] = mfma_bf16(ah[ta], bh[tb], acc[ta][tb]);
+        if constexpr (R) {
+          acc[ta][tb] = mfma_bf16(ah[ta], bl[tb], acc[ta][tb]);
+          acc[ta][tb] = mfma_bf16(al[ta], bh[tb], acc[ta][tb]);
         }
       }
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bhi0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bhi1, acc1, 0, 0, 0);
-      if constexpr (PREC == JGM_GEMM_BF16_REFINED) {
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo0, acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo1, acc1, 0, 0, 0);
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi0, acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi1, acc1, 0, 0, 0);
+  }
+}
+
+// Two-level accumulation: the MFMA accumulators are added into FP32 totals and reset
+// every 256 k, so no FP32 running sum grows longer than 256 k (the error of one long
+// FP32 sum grows with its length).
+__device__ __forceinline__ void flush(f32x16 (&acc)[2][2], float (&tot)[2][2][16]) {
+#pragma unroll
+  for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        tot[ta][tb][r] += acc[ta][tb][r];
+        acc[ta][tb][r] = 0.0f;
+      }
+}
+
+// k index (within a 16-deep chunk) of operand element s for a lane in half lk
+template <int PREC>
+__device__ __forceinline__ int chunk_k(int s, int lk) {
+  return PREC == JGM_GEMM_FP32 ? 2 * s + lk : 8 * lk + s;
+}
+
+// Y partials: Yp[split][row][P].  Grid: x = nsplit * row_tiles workgroups (XCD-grouped
+// when nsplit is a multiple of 8 so that the workgroups of one split share an L2), y =
+// right-hand-side tiles of 64.  Each split covers `per` cells (a multiple of 32).
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_gemm_jm(const float* __restrict__ J, long ld, long R,
+                                                    long colpad, const float* __restrict__ Mt,
+                                                    int P, long per, int row_tiles, int nsplit,
+                                                    float* __restrict__ Yp) {
+  const int w = blockIdx.x;
+  int split, rt;
+  if ((nsplit & 7) == 0) {
+    const int slot = w >> 3;
+    split = (w & 7) * (nsplit >> 3) + slot / row_tiles;
+    rt = slot % row_tiles;
+  } else {
+    split = w / row_tiles;
+    rt = w % row_tiles;
+  }
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int li = lane & 31, lk = lane >> 5;
+  const long row0 = (long)rt * 256 + wave * 64;
+  if (row0 >= ld) return;  // whole wave outside the padded rows; no barriers follow
+  const int b0 = blockIdx.y * 64;
+  const long c0 = (long)split * per;
+  const long cend = c0 + per < colpad ? c0 + per : colpad;
+  const int nchunk = cend > c0 ? (int)((cend - c0) / kChunk) : 0;  // even
+  const float* Jw = J + row0 + li;
+  const float* Mw = Mt + b0 + li;
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[ta][tb][r] = 0.0f;
+  float tot[2][2][16];
+#pragma unroll
+  for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) tot[ta][tb][r] = 0.0f;
+  float xa[2][8], xb[2][8], ya[2][8], yb[2][8];
+  auto load = [&](float (&a)[2][8], float (&b)[2][8], long cb) {
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const long cell = cb + chunk_k<PREC>(s, lk);
+      a[0][s] = __builtin_nontemporal_load(Jw + cell * ld);
+      a[1][s] = __builtin_nontemporal_load(Jw + cell * ld + 32);
+      b[0][s] = Mw[cell * P];
+      b[1][s] = Mw[cell * P + 32];
+    }
+  };
+  // sched_barrier keeps the scheduler from sinking the next chunk's loads down to
+  // their uses, which would expose the full memory latency once per chunk
+  if (nchunk > 0) {
+    load(xa, xb, c0);
+    for (int q = 0; q < nchunk; q += 2) {
+      load(ya, yb, c0 + (long)(q + 1) * kChunk);
+      __builtin_amdgcn_sched_barrier(0);
+      mma_block<PREC>(xa, xb, acc);
+      __builtin_amdgcn_sched_barrier(0);
+      const int qn = q + 2 < nchunk ? q + 2 : nchunk - 1;  // the last reload is unused
+      load(xa, xb, c0 + (long)qn * kChunk);
+      __builtin_amdgcn_sched_barrier(0);
+      mma_block<PREC>(ya, yb, acc);
+      __builtin_amdgcn_sched_barrier(0);
+      if (((q + 2) & 15) == 0) flush(acc, tot);  // every 256 cells
+    }
+  }
+  flush(acc, tot);
+  // D layout of a 32x32 MFMA: register r of lane l holds (row 8*(r/4) + 4*(l/32) + r%4, col l%32)
+  float* out = Yp + (long)split * R * P;
+#pragma unroll
+  for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const long row = row0 + 32 * ta + 8 * (r >> 2) + 4 * lk + (r & 3);
+      if (row < R) {
+        out[row * P + b0 + li] = tot[ta][0][r];
+        out[row * P + b0 + 32 + li] = tot[ta][1][r];
       }
     }
-  }
-  // D layout of a 32x32 MFMA: register r of lane l holds (row 8*(r/4) + 4*(l/32) + r%4, col l%32)
-  float* out = Yp + (long)blockIdx.y * R * P;
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const long orow = row0 + 8 * (r >> 2) + 4 * lk + (r & 3);
-    if (orow < R) {
-      out[orow * P + b0 + li] = acc0[r];
-      out[orow * P + b0 + 32 + li] = acc1[r];
-    }
-  }
 }
 
 // Y[row + b*R] = sum over splits of Yp[split][row][b]
@@ -201,82 +305,88 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm_reduce(const float* __restri
   Y[row + (long)b * R] = (float)a;
 }
 
-// G = J^T Rt: workgroup = 128 cells (4 waves x 32) x 64 right-hand sides, all rows.
-// J tiles [128 cells][32 rows] pass through LDS (coalesced 128-byte reads per cell).
+// G^T = Rt^T J over all rows: workgroup = 256 cells (wave: 64 cells) x 64 right-hand
+// sides.  A operand = Rt stage (i = right-hand side, k = row), B operand = J stage
+// (k = row, j = cell), both read from LDS.
 template <int PREC>
-__global__ __launch_bounds__(kBlock) void k_gemm_jtr(const float* __restrict__ J, long R, long N,
-                                                     const float* __restrict__ Rt, int P,
-                                                     float* __restrict__ G, int nrhs) {
-  constexpr int TR = 32;        // rows per stage
-  constexpr int PITCH = TR + 1;  // LDS row pitch (floats) against bank conflicts
-  __shared__ float tile[128 * PITCH];
-  const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  const long cbase = (long)blockIdx.x * 128;
+__global__ __launch_bounds__(kBlock) void k_gemm_jtr(const float* __restrict__ J, long ld, long N,
+                                                     long nblk, const float* __restrict__ Rt,
+                                                     int P, float* __restrict__ G, int nrhs) {
+  __shared__ float jt[kStage * kPitch];                           // [row][cell]
+  __shared__ __attribute__((aligned(16))) float rt[kStage * 64];  // [row][rhs]
   const int b0 = blockIdx.y * 64;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
   const int li = lane & 31, lk = lane >> 5;
-  f32x16 acc0 = {0}, acc1 = {0};
-  for (long r0 = 0; r0 < R; r0 += TR) {
-    __syncthreads();
-    // 32 consecutive lanes read the 32 rows (one 128-byte line) of one cell
+  // loaders: J by 16-byte pieces (8 per cell line), Rt by 16-byte pieces (16 per row)
+  const int jpart = tid & 7, jcell = tid >> 3;
+  const int rpart = tid & 15, rrow = tid >> 4;
+  const float* Rl = Rt + (long)rrow * P + b0 + rpart * 4;
+  const long nst = ld / kStage;
+  for (long blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const long cbase = blk * kColPad;
+    const float* Jl = J + (cbase + jcell) * ld + jpart * 4;
+    f32x4 pj[8], pr[2];
+    auto gload = [&](long r0) {
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      const int idx = e * kBlock + threadIdx.x;
-      const int lc = idx >> 5, lr = idx & 31;
-      const long cell = cbase + lc, rr = r0 + lr;
-      tile[lc * PITCH + lr] = (cell < N && rr < R) ? J[cell * R + rr] : 0.0f;
-    }
-    __syncthreads();
-    const int ci = wave * 32 + li;  // this lane's cell row of the A fragment
-    if constexpr (PREC == JGM_GEMM_FP32) {
+      for (int q = 0; q < 8; ++q)
+        pj[q] = *(const f32x4*)(Jl + (long)q * 32 * ld + r0);
 #pragma unroll
-      for (int ks = 0; ks < TR; ks += 2) {
-        const float a = tile[ci * PITCH + ks + lk];
-        const long rr = r0 + ks + lk;
-        const float bv0 = rr < R ? Rt[rr * P + b0 + li] : 0.0f;
-        const float bv1 = rr < R ? Rt[rr * P + b0 + 32 + li] : 0.0f;
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bv0, acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bv1, acc1, 0, 0, 0);
+      for (int h = 0; h < 2; ++h) pr[h] = *(const f32x4*)(Rl + (r0 + 16 * h) * P);
+    };
+    f32x16 acc[2][2];  // [rhs tile][cell tile]
+    float tot[2][2][16];
+#pragma unroll
+    for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+      for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[ta][tb][r] = tot[ta][tb][r] = 0.0f;
+    gload(0);
+    for (long st = 0; st < nst; ++st) {
+      __syncthreads();  // the previous stage has been consumed
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        float* d = jt + jpart * 4 * kPitch + jcell + 32 * q;
+        d[0] = pj[q][0];
+        d[kPitch] = pj[q][1];
+        d[2 * kPitch] = pj[q][2];
+        d[3 * kPitch] = pj[q][3];
       }
-    } else {
 #pragma unroll
-      for (int ks = 0; ks < TR; ks += 16) {
-        bf16x8 ahi, alo, bhi0, blo0, bhi1, blo1;
+      for (int h = 0; h < 2; ++h) *(f32x4*)(rt + (rrow + 16 * h) * 64 + rpart * 4) = pr[h];
+      __syncthreads();
+      gload((st + 1 < nst ? st + 1 : st) * kStage);  // next stage in flight (last: unused)
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int kk = ks + 8 * lk + e;
-          const long rr = r0 + kk;
-          const float a = tile[ci * PITCH + kk];
-          const float m0 = rr < R ? Rt[rr * P + b0 + li] : 0.0f;
-          const float m1 = rr < R ? Rt[rr * P + b0 + 32 + li] : 0.0f;
-          if constexpr (PREC == JGM_GEMM_BF16) {
-            ahi[e] = (__bf16)a;
-            bhi0[e] = (__bf16)m0;
-            bhi1[e] = (__bf16)m1;
-          } else {
-            split_bf16(a, ahi, alo, e);
-            split_bf16(m0, bhi0, blo0, e);
-            split_bf16(m1, bhi1, blo1, e);
+      for (int half = 0; half < 2; ++half) {
+        float a[2][8], b[2][8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+          const int row = 16 * half + chunk_k<PREC>(s, lk);
+          a[0][s] = rt[row * 64 + li];
+          a[1][s] = rt[row * 64 + 32 + li];
+          b[0][s] = jt[row * kPitch + 64 * wave + li];
+          b[1][s] = jt[row * kPitch + 64 * wave + 32 + li];
+        }
+        mma_block<PREC>(a, b, acc);
+      }
+      if ((st & 7) == 7) flush(acc, tot);  // every 256 rows
+    }
+    flush(acc, tot);
+#pragma unroll
+    for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+      for (int tb = 0; tb < 2; ++tb) {
+        const long c = cbase + 64 * wave + 32 * tb + li;
+        if (c < N) {
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const int b = b0 + 32 * ta + 8 * (r >> 2) + 4 * lk + (r & 3);
+            if (b < nrhs) G[c + (long)b * N] = tot[ta][tb][r];
           }
         }
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bhi0, acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bhi1, acc1, 0, 0, 0);
-        if constexpr (PREC == JGM_GEMM_BF16_REFINED) {
-          acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo0, acc0, 0, 0, 0);
-          acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo1, acc1, 0, 0, 0);
-          acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi0, acc0, 0, 0, 0);
-          acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi1, acc1, 0, 0, 0);
-        }
       }
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const long c = cbase + wave * 32 + 8 * (r >> 2) + 4 * lk + (r & 3);
-    if (c < N) {
-      if (b0 + li < nrhs) G[c + (long)(b0 + li) * N] = acc0[r];
-      if (b0 + 32 + li < nrhs) G[c + (long)(b0 + 32 + li) * N] = acc1[r];
-    }
   }
 }
 
@@ -289,8 +399,12 @@ hipError_t launch_storedj_build(jgm_context* ctx, int kind, int precision) {
   const int kJ = compact_rows(kind);
   const long R = (long)S * kJ;
   if (S == 0 || N == 0) return hipErrorInvalidValue;
-  hipError_t e = ctx->storedJ.ensure(sizeof(float) * (size_t)R * N);
+  const long ld = round_up(R, kRowPad), colpad = round_up(N, kColPad);
+  ctx->storedJ_kind = 0;
+  hipError_t e = ctx->storedJ.ensure(sizeof(float) * (size_t)ld * colpad);
   if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_fill_zero_f32, dim3(8 * ctx->cu_count), dim3(kBlock), 0, ctx->stream,
+                     ctx->storedJ.as<float>(), ld * colpad);
   const int site_blocks = (S + kBlock - 1) / kBlock;
   const long waves_per_chunk = (long)site_blocks * (kBlock / 64);
   long want = std::max<long>(1, std::min<long>(N, (16384 + waves_per_chunk - 1) / waves_per_chunk));
@@ -305,7 +419,8 @@ hipError_t launch_storedj_build(jgm_context* ctx, int kind, int precision) {
                      ctx->stream, ctx->d_sx.as<double>(), ctx->d_sy.as<double>(),               \
                      ctx->d_sz.as<double>(), S, ctx->d_xlo.as<double>(), ctx->d_ylo.as<double>(), \
                      ctx->d_zlo.as<double>(), ctx->d_xsz.as<double>(), ctx->d_ysz.as<double>(), \
-                     ctx->d_zsz.as<double>(), ny, nz, N, per, kind, fd, ctx->storedJ.as<float>())
+                     ctx->d_zsz.as<double>(), ny, nz, N, per, kind, fd, ld,                    \
+                     ctx->storedJ.as<float>())
     switch (mask_of(kind)) {
       case kMaskScalar: JGM_BUILD(kMaskScalar); break;
       case kMaskAll: JGM_BUILD(kMaskAll); break;
@@ -318,71 +433,67 @@ hipError_t launch_storedj_build(jgm_context* ctx, int kind, int precision) {
   ctx->storedJ_kind = kind;
   ctx->storedJ_rows = (size_t)R;
   ctx->storedJ_cols = (size_t)N;
+  ctx->storedJ_ld = (size_t)ld;
+  ctx->storedJ_colpad = (size_t)colpad;
   return hipSuccess;
 }
 
 hipError_t launch_storedj_apply(jgm_context* ctx, int gemm_precision, int transpose,
                                 const float* in, size_t nrhs, float* out) {
   const long R = (long)ctx->storedJ_rows, N = (long)ctx->storedJ_cols;
-  const int P = (int)((nrhs + 63) / 64 * 64);
-  const long K = transpose ? R : N;  // length of each right-hand side
+  const long ld = (long)ctx->storedJ_ld, colpad = (long)ctx->storedJ_colpad;
+  const int P = (int)round_up((long)nrhs, 64);
+  const long K = transpose ? R : N, Kpad = transpose ? ld : colpad;
   hipStream_t st = ctx->stream;
-  hipError_t e = ctx->coef32.ensure(sizeof(float) * (size_t)K * P);
+  hipError_t e = ctx->coef32.ensure(sizeof(float) * (size_t)Kpad * P);
   if (e != hipSuccess) return e;
   float* Xt = ctx->coef32.as<float>();
-  hipLaunchKernelGGL(k_transpose_pad, dim3(blocks_for(K * P)), dim3(kBlock), 0, st, in, K,
-                     (int)nrhs, P, Xt);
+  hipLaunchKernelGGL(k_transpose_pad, dim3(blocks_for(Kpad * P)), dim3(kBlock), 0, st, in, K,
+                     Kpad, (int)nrhs, P, Xt);
   const float* J = ctx->storedJ.as<const float>();
+  const int rhs_tiles = P / 64;
   if (!transpose) {
-    const long row_tiles = (R + 127) / 128;
-    const int rhs_tiles = P / 64;
-    long nsplit = std::max<long>(1, 2048 / std::max<long>(1, row_tiles * rhs_tiles));
-    nsplit = std::min<long>(nsplit, std::max<long>(1, N / 256));
-    long per = (N + nsplit - 1) / nsplit;
-    per = (per + 15) / 16 * 16;  // whole K steps of every precision
-    nsplit = (N + per - 1) / per;
+    const int row_tiles = (int)((ld + 255) / 256);
+    // one workgroup per CU: a second co-resident workgroup slows both down (the CU's
+    // memory pipeline is the limit); splits in multiples of 8 when there are enough of
+    // them to group by XCD
+    long nsplit = std::max<long>(1, (long)ctx->cu_count / ((long)row_tiles * rhs_tiles));
+    nsplit = std::min<long>(nsplit, colpad / 32);
+    if (nsplit >= 16) nsplit = nsplit / 8 * 8;
+    const long per = round_up((colpad + nsplit - 1) / nsplit, 2 * kChunk);
     if ((e = ctx->partial.ensure(sizeof(float) * (size_t)nsplit * R * P)) != hipSuccess) return e;
-    dim3 grid((unsigned)row_tiles, (unsigned)nsplit, (unsigned)rhs_tiles);
+    dim3 grid((unsigned)(nsplit * row_tiles), (unsigned)rhs_tiles);
     {
       ProfScope ps(ctx, gemm_precision == JGM_GEMM_FP32   ? "storedj_jm_fp32"
                         : gemm_precision == JGM_GEMM_BF16 ? "storedj_jm_bf16"
                                                           : "storedj_jm_bf16_refined");
+#define JGM_JM(PR)                                                                             \
+  hipLaunchKernelGGL((k_gemm_jm<PR>), grid, dim3(kBlock), 0, st, J, ld, R, colpad, Xt, P, per, \
+                     row_tiles, (int)nsplit, ctx->partial.as<float>())
       switch (gemm_precision) {
-        case JGM_GEMM_FP32:
-          hipLaunchKernelGGL((k_gemm_jm<JGM_GEMM_FP32>), grid, dim3(kBlock), 0, st, J, R, N, Xt,
-                             P, per, ctx->partial.as<float>());
-          break;
-        case JGM_GEMM_BF16:
-          hipLaunchKernelGGL((k_gemm_jm<JGM_GEMM_BF16>), grid, dim3(kBlock), 0, st, J, R, N, Xt,
-                             P, per, ctx->partial.as<float>());
-          break;
-        default:
-          hipLaunchKernelGGL((k_gemm_jm<JGM_GEMM_BF16_REFINED>), grid, dim3(kBlock), 0, st, J, R,
-                             N, Xt, P, per, ctx->partial.as<float>());
-          break;
+        case JGM_GEMM_FP32: JGM_JM(JGM_GEMM_FP32); break;
+        case JGM_GEMM_BF16: JGM_JM(JGM_GEMM_BF16); break;
+        default: JGM_JM(JGM_GEMM_BF16_REFINED); break;
       }
+#undef JGM_JM
     }
     hipLaunchKernelGGL(k_gemm_jm_reduce, dim3(blocks_for(R * (long)nrhs)), dim3(kBlock), 0, st,
                        ctx->partial.as<const float>(), (int)nsplit, R, P, (int)nrhs, out);
   } else {
-    dim3 grid((unsigned)((N + 127) / 128), (unsigned)(P / 64));
+    const long nblk = colpad / kColPad;
+    dim3 grid((unsigned)std::min<long>(nblk, 65535), (unsigned)rhs_tiles);
     ProfScope ps(ctx, gemm_precision == JGM_GEMM_FP32   ? "storedj_jtr_fp32"
                       : gemm_precision == JGM_GEMM_BF16 ? "storedj_jtr_bf16"
                                                         : "storedj_jtr_bf16_refined");
+#define JGM_JTR(PR) \
+  hipLaunchKernelGGL((k_gemm_jtr<PR>), grid, dim3(kBlock), 0, st, J, ld, N, nblk, Xt, P, out, \
+                     (int)nrhs)
     switch (gemm_precision) {
-      case JGM_GEMM_FP32:
-        hipLaunchKernelGGL((k_gemm_jtr<JGM_GEMM_FP32>), grid, dim3(kBlock), 0, st, J, R, N, Xt, P,
-                           out, (int)nrhs);
-        break;
-      case JGM_GEMM_BF16:
-        hipLaunchKernelGGL((k_gemm_jtr<JGM_GEMM_BF16>), grid, dim3(kBlock), 0, st, J, R, N, Xt, P,
-                           out, (int)nrhs);
-        break;
-      default:
-        hipLaunchKernelGGL((k_gemm_jtr<JGM_GEMM_BF16_REFINED>), grid, dim3(kBlock), 0, st, J, R,
-                           N, Xt, P, out, (int)nrhs);
-        break;
+      case JGM_GEMM_FP32: JGM_JTR(JGM_GEMM_FP32); break;
+      case JGM_GEMM_BF16: JGM_JTR(JGM_GEMM_BF16); break;
+      default: JGM_JTR(JGM_GEMM_BF16_REFINED); break;
     }
+#undef JGM_JTR
   }
   return hipGetLastError();
 }
