@@ -12,7 +12,12 @@ The CPU baseline is the oracle (the OpenMP restatement of the reference's OMP im
 scalar and tensor calculators run separately, as jif3D does) on a bounded sample of
 sites, scaled linearly (work per site is identical).
 
+The same line carries "storedj": the batched stored-J products (J M and J^T R with 64
+right-hand sides on a 7168 x 131072 FP32 J) in FP32, BF16 and BF16-refined MFMA modes,
+with HBM and matrix-core fractions.
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp64|both]
+                    [--workload all|headline|storedj]
 """
 import argparse
 import json
@@ -99,6 +104,72 @@ def run_gpu(args, model, data, prec_name):
     return result
 
 
+MFMA_PEAK = {"fp32": 157.3e12, "bf16": 2.5e15}  # MI355X dense matrix peaks (flop/s)
+HBM_PEAK = 8.0e12
+
+
+def run_storedj(args):
+    """Batched stored-J: 64x64x32 cells x 32x32 sites x 7 components (scalar+FTG), J in
+    HBM as FP32 (3.76 GB); 64 trial models (J M) and 64 residual vectors (J^T R) per
+    product, each precision timed over `steps` products after `warmup`."""
+    from jif3d_amd import _native as nat, synthetic
+    m, d, M, Rr = synthetic.config3()
+    m.SetBackgroundThicknesses([])
+    m.SetBackgroundDensities([])
+    ctx = nat.Context(0)
+    ctx.configure(m, d)
+    ctx.profile_enable(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    ctx.storedj_build(nat.KIND_SCALAR_TENSOR, nat.FP64)
+    ctx.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    R, N = ctx.storedj_rows(), ctx.storedj_cols()
+    nrhs = M.shape[0]
+    Md = ctx.to_device(M.astype(np.float32))
+    Rd = ctx.to_device(Rr.astype(np.float32))
+    Y = ctx.alloc((nrhs, R), np.float32)
+    G = ctx.alloc((nrhs, N), np.float32)
+    jbytes = 4.0 * R * N
+    flops = 2.0 * R * N * nrhs
+    res = {"workload": f"J {R} x {N} FP32 ({jbytes / 1e9:.2f} GB), {nrhs} right-hand sides",
+           "build_ms": build_ms}
+    ref = None
+    for name, prec in (("fp32", nat.GEMM_FP32), ("bf16", nat.GEMM_BF16),
+                       ("bf16_refined", nat.GEMM_BF16_REFINED)):
+        for _ in range(args.warmup):
+            ctx.storedj_apply(prec, Md, Y)
+            ctx.storedj_apply_t(prec, Rd, G)
+        ctx.synchronize()
+        ctx.profile_reset()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.storedj_apply(prec, Md, Y)
+            ctx.storedj_apply_t(prec, Rd, G)
+        ctx.synchronize()
+        wall = (time.perf_counter() - t0) * 1e3 / args.steps
+        prof = ctx.profile()
+        peak = MFMA_PEAK["fp32" if prec == nat.GEMM_FP32 else "bf16"]
+        entry = {"ms_per_pair_of_products": wall}
+        for key, kname in (("J_M", f"storedj_jm_{name}"), ("JT_R", f"storedj_jtr_{name}")):
+            kms, n = prof[kname]
+            avg = kms / n * 1e-3
+            entry[key] = {"kernel_ms": avg * 1e3, "J_stream_GB_s": jbytes / avg / 1e9,
+                          "hbm_frac": jbytes / avg / HBM_PEAK,
+                          "TFLOP_s": flops / avg / 1e12, "mfma_frac": flops / avg / peak}
+        y, g = Y.to_host().astype(np.float64), G.to_host().astype(np.float64)
+        if ref is None:
+            ref = (y, g)
+        else:
+            entry["rel_err_vs_fp32"] = {
+                "J_M": float(np.max(np.abs(y - ref[0])) / np.max(np.abs(ref[0]))),
+                "JT_R": float(np.max(np.abs(g - ref[1])) / np.max(np.abs(ref[1])))}
+        res[name] = entry
+        log(f"[bench] stored J {name}: {wall:.3f} ms per J M + J^T R")
+    ctx.storedj_release()
+    return res
+
+
 def cpu_baseline(model, data, budget_s):
     from oracle import oracle as orc
     mv = model.GetModelParameters()
@@ -141,11 +212,17 @@ def main():
     ap.add_argument("--small", action="store_true", help="quarter-size mesh for quick checks")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 512 MiB device-to-device copy (counter calibration)")
+    ap.add_argument("--workload", default="all", choices=["all", "headline", "storedj"],
+                    help="headline: gz+FTG forward+adjoint only; storedj: batched stored-J only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards "
               "by sites (forward) and cells (adjoint), see DESIGN.md", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "storedj":
+        print(json.dumps({"storedj": run_storedj(args)}), flush=True)
+        return
 
     from jif3d_amd import synthetic
     if args.small:
@@ -214,6 +291,8 @@ def main():
             "gz_max_pointwise_rel": gz_pt,
             "gradient_rel_inf_norm": float(np.max(np.abs(gmix - gref)) / np.max(np.abs(gref)))}
     line["kernels_ms_per_step"] = {k: v[0] / args.steps for k, v in r["kernels"].items()}
+    if args.workload == "all":
+        line["storedj"] = run_storedj(args)
     if not args.no_cpu:
         cb = cpu_baseline(model, data, args.cpu_budget)
         idx = cb.pop("sites")
