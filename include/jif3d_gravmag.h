@@ -160,6 +160,24 @@ jgm_status jgm_storedj_apply(jgm_context *ctx, jgm_gemm_precision prec, const fl
  * form of CalculateRawLQDerivative (:214-221) without its factor 2. */
 jgm_status jgm_storedj_apply_t(jgm_context *ctx, jgm_gemm_precision prec, const float *R,
                                size_t nrhs, float *G);
+/* Cached forward over the stored J: the kind's data (transformed when a transform is
+ * set) of model = n_params FP64 values (cells, then background layers), i.e.
+ * FullSensitivityGravMagCalculator::CalculateCachedResult (:154-164) with the raw data
+ * J m of CalculateRawData (:137-152); the background columns come from the cached
+ * layer table.  J m runs on the matrix cores in `prec` (FP32 operands: model rounded
+ * to FP32, totals of at most 256 cells accumulated in FP32 and combined in FP64).
+ * Device pointers; the _host variants take host arrays. */
+jgm_status jgm_storedj_forward(jgm_context *ctx, jgm_gemm_precision prec, const double *model,
+                               double *out);
+/* Cached gradient 2 [J B]^T D^T misfit (CachedLQDerivative :166-211 and
+ * CalculateRawLQDerivative :214-221; D is the transform Jacobian at the model's data
+ * through the stored J, needed only with a transform). */
+jgm_status jgm_storedj_gradient(jgm_context *ctx, jgm_gemm_precision prec, const double *model,
+                                const double *misfit, double *grad);
+jgm_status jgm_storedj_forward_host(jgm_context *ctx, jgm_gemm_precision prec,
+                                    const double *model, double *out);
+jgm_status jgm_storedj_gradient_host(jgm_context *ctx, jgm_gemm_precision prec,
+                                     const double *model, const double *misfit, double *grad);
 jgm_status jgm_storedj_release(jgm_context *ctx);
 
 /* Device memory on the context's device, for callers without their own HIP

@@ -26,7 +26,8 @@ EXPORTS = [
     "jgm_set_field", "jgm_set_transform", "jgm_set_mixed_threshold", "jgm_data_per_site",
     "jgm_n_params", "jgm_forward", "jgm_forward_host", "jgm_adjoint", "jgm_adjoint_host",
     "jgm_sensitivity_rows", "jgm_storedj_build", "jgm_storedj_rows", "jgm_storedj_apply",
-    "jgm_storedj_apply_t", "jgm_storedj_release", "jgm_device_alloc", "jgm_device_free",
+    "jgm_storedj_apply_t", "jgm_storedj_forward", "jgm_storedj_gradient",
+    "jgm_storedj_forward_host", "jgm_storedj_gradient_host", "jgm_storedj_release", "jgm_device_alloc", "jgm_device_free",
     "jgm_copy_to_device", "jgm_copy_to_host", "jgm_copy_device", "jgm_profile_enable", "jgm_profile_reset",
     "jgm_profile_count", "jgm_profile_get",
 ]
@@ -76,6 +77,10 @@ def load():
         "jgm_storedj_build": [vp, i, i],
         "jgm_storedj_apply": [vp, i, vp, sz, vp],
         "jgm_storedj_apply_t": [vp, i, vp, sz, vp],
+        "jgm_storedj_forward": [vp, i, vp, vp],
+        "jgm_storedj_gradient": [vp, i, vp, vp, vp],
+        "jgm_storedj_forward_host": [vp, i, pd, pd],
+        "jgm_storedj_gradient_host": [vp, i, pd, pd, pd],
         "jgm_storedj_release": [vp],
         "jgm_device_alloc": [vp, sz, ctypes.POINTER(vp)],
         "jgm_device_free": [vp, vp],
@@ -271,6 +276,7 @@ class Context:
 
     def storedj_build(self, kind, prec):
         check(self.lib.jgm_storedj_build(self.handle, kind, prec))
+        self.storedj_kind = kind
 
     def storedj_rows(self):
         return self.lib.jgm_storedj_rows(self.handle)
@@ -322,6 +328,33 @@ class Context:
         check(self.lib.jgm_storedj_apply_t(self.handle, prec, _devptr(R), R.shape[0],
                                            _devptr(G)))
         return G
+
+    def storedj_forward(self, prec, model, out=None):
+        """Cached forward over the stored J (host arrays, or device arrays with out)."""
+        if _is_device(model):
+            if out is None:
+                raise ValueError("device forward needs an output array")
+            check(self.lib.jgm_storedj_forward(self.handle, prec, _devptr(model), _devptr(out)))
+            return out
+        m = _f64(model)
+        res = np.zeros(self.nsites * self.data_per_site(self.storedj_kind))
+        check(self.lib.jgm_storedj_forward_host(self.handle, prec, _pd(m), _pd(res)))
+        return res
+
+    def storedj_gradient(self, prec, model, misfit, out=None):
+        """Cached gradient 2 [J B]^T D^T misfit over the stored J."""
+        if _is_device(misfit):
+            if out is None:
+                raise ValueError("device gradient needs an output array")
+            mp = _devptr(model) if model is not None else None
+            check(self.lib.jgm_storedj_gradient(self.handle, prec, mp, _devptr(misfit),
+                                                _devptr(out)))
+            return out
+        m = _f64(model) if model is not None else None
+        r = _f64(misfit)
+        g = np.zeros(self.n_params())
+        check(self.lib.jgm_storedj_gradient_host(self.handle, prec, _pd(m), _pd(r), _pd(g)))
+        return g
 
     def storedj_release(self):
         check(self.lib.jgm_storedj_release(self.handle))

@@ -133,7 +133,7 @@ jgm_status jgm_destroy(jgm_context* ctx) {
                                &ctx->weights, &ctx->partial, &ctx->raw,
                                &ctx->coef, &ctx->hpart, &ctx->hsum, &ctx->model_stage,
                                &ctx->out_stage, &ctx->misfit_stage, &ctx->grad_stage,
-                               &ctx->farpart, &ctx->nearpart, &ctx->coef32, &ctx->storedJ,
+                               &ctx->farpart, &ctx->nearpart, &ctx->coef32, &ctx->storedJ, &ctx->storedJ_vec,
                                &ctx->near.col_ptr, &ctx->near.col_sites, &ctx->near.col_ids,
                                &ctx->near.col_kmax, &ctx->near.xcen, &ctx->near.ycen,
                                &ctx->near.zcen, &ctx->near.adj_table, &ctx->near.corner_off,
@@ -263,10 +263,12 @@ jgm_status jgm_set_field(jgm_context* ctx, double inclination, double declinatio
   if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
   if (!std::isfinite(inclination) || !std::isfinite(declination) || !std::isfinite(fieldstrength))
     return fail(JGM_ERR_INVALID_ARGUMENT, "non-finite field parameter");
+  const bool changed = inclination != ctx->inclination || declination != ctx->declination ||
+                       fieldstrength != ctx->fieldstrength;
   ctx->inclination = inclination;
   ctx->declination = declination;
   ctx->fieldstrength = fieldstrength;
-  if (ctx->storedJ_kind == JGM_KIND_MAGNETIC) ctx->storedJ_kind = 0;
+  if (changed && ctx->storedJ_kind == JGM_KIND_MAGNETIC) ctx->storedJ_kind = 0;
   return JGM_OK;
 }
 
@@ -480,6 +482,129 @@ jgm_status jgm_storedj_apply(jgm_context* ctx, jgm_gemm_precision prec, const fl
 jgm_status jgm_storedj_apply_t(jgm_context* ctx, jgm_gemm_precision prec, const float* R,
                                size_t nrhs, float* G) {
   return storedj_apply(ctx, prec, 1, R, nrhs, G);
+}
+
+static jgm_status storedj_ready(jgm_context* ctx, jgm_gemm_precision prec) {
+  if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
+  if (ctx->storedJ_kind == 0) return fail(JGM_ERR_NOT_READY, "no stored J (jgm_storedj_build)");
+  if (prec != JGM_GEMM_FP32 && prec != JGM_GEMM_BF16 && prec != JGM_GEMM_BF16_REFINED)
+    return fail(JGM_ERR_INVALID_ARGUMENT, "unknown GEMM precision %d", (int)prec);
+  return check_ready(ctx, ctx->storedJ_kind, JGM_FP64);
+}
+
+jgm_status jgm_storedj_forward(jgm_context* ctx, jgm_gemm_precision prec, const double* model,
+                               double* out) {
+  jgm_status st = storedj_ready(ctx, prec);
+  if (st != JGM_OK) return st;
+  if (!model || !out) return fail(JGM_ERR_INVALID_ARGUMENT, "null model or output pointer");
+  DeviceGuard g(ctx->device);
+  if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+  if ((st = ensure_background(ctx)) != JGM_OK) return st;
+  hipError_t e = jgm::launch_storedj_forward(ctx, prec, model, out, nullptr);
+  if (e != hipSuccess) return hip_fail(e, "stored-J forward");
+  return JGM_OK;
+}
+
+jgm_status jgm_storedj_gradient(jgm_context* ctx, jgm_gemm_precision prec, const double* model,
+                                const double* misfit, double* grad) {
+  jgm_status st = storedj_ready(ctx, prec);
+  if (st != JGM_OK) return st;
+  if (!misfit || !grad) return fail(JGM_ERR_INVALID_ARGUMENT, "null misfit or gradient pointer");
+  if (ctx->transform != JGM_TF_NONE && !model)
+    return fail(JGM_ERR_INVALID_ARGUMENT, "a transform needs the model for its Jacobian");
+  DeviceGuard g(ctx->device);
+  if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+  if ((st = ensure_background(ctx)) != JGM_OK) return st;
+  const int kind = ctx->storedJ_kind;
+  const size_t S = ctx->S;
+  const size_t kraw = jgm::raw_per_site(kind);
+  hipError_t e;
+  const double* rm = misfit;
+  if (ctx->transform != JGM_TF_NONE) {
+    // raw data at the model through the stored J (CachedLQDerivative,
+    // FullSensitivityGravMagCalculator.h:170-205)
+    if ((e = ctx->raw.ensure(sizeof(double) * S * kraw)) != hipSuccess)
+      return hip_fail(e, "stored-J gradient");
+    if ((e = ctx->out_stage.ensure(sizeof(double) * S)) != hipSuccess)
+      return hip_fail(e, "stored-J gradient");
+    if ((e = ctx->misfit_stage.ensure(sizeof(double) * S * kraw)) != hipSuccess)
+      return hip_fail(e, "stored-J gradient");
+    e = jgm::launch_storedj_forward(ctx, prec, model, ctx->out_stage.as<double>(),
+                                    ctx->raw.as<double>());
+    if (e != hipSuccess) return hip_fail(e, "stored-J gradient (raw forward)");
+    e = jgm::launch_misfit_to_raw(ctx, kind, misfit, ctx->raw.as<const double>(),
+                                  ctx->misfit_stage.as<double>());
+    if (e != hipSuccess) return hip_fail(e, "stored-J gradient (transform)");
+    rm = ctx->misfit_stage.as<const double>();
+  }
+  e = jgm::launch_storedj_gradient(ctx, prec, rm, grad);
+  if (e != hipSuccess) return hip_fail(e, "stored-J gradient");
+  return JGM_OK;
+}
+
+jgm_status jgm_storedj_forward_host(jgm_context* ctx, jgm_gemm_precision prec,
+                                    const double* model, double* out) {
+  jgm_status st = storedj_ready(ctx, prec);
+  if (st != JGM_OK) return st;
+  if (!model || !out) return fail(JGM_ERR_INVALID_ARGUMENT, "null model or output pointer");
+  DeviceGuard g(ctx->device);
+  const size_t np = jgm::nparams(ctx);
+  const size_t nout = ctx->S * data_per_site(ctx, ctx->storedJ_kind);
+  hipError_t e;
+  if ((e = ctx->model_stage.ensure(sizeof(double) * (np ? np : 1))) != hipSuccess)
+    return hip_fail(e, "storedj_forward_host");
+  if ((e = ctx->grad_stage.ensure(sizeof(double) * (nout ? nout : 1))) != hipSuccess)
+    return hip_fail(e, "storedj_forward_host");
+  if ((e = hipMemcpyAsync(ctx->model_stage.ptr, model, sizeof(double) * np,
+                          hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
+    return hip_fail(e, "storedj_forward_host");
+  st = jgm_storedj_forward(ctx, prec, ctx->model_stage.as<double>(), ctx->grad_stage.as<double>());
+  if (st != JGM_OK) return st;
+  if (nout && (e = hipMemcpyAsync(out, ctx->grad_stage.ptr, sizeof(double) * nout,
+                                  hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess)
+    return hip_fail(e, "storedj_forward_host");
+  if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+    return hip_fail(e, "storedj_forward_host");
+  return JGM_OK;
+}
+
+jgm_status jgm_storedj_gradient_host(jgm_context* ctx, jgm_gemm_precision prec,
+                                     const double* model, const double* misfit, double* grad) {
+  jgm_status st = storedj_ready(ctx, prec);
+  if (st != JGM_OK) return st;
+  if (!misfit || !grad) return fail(JGM_ERR_INVALID_ARGUMENT, "null misfit or gradient pointer");
+  DeviceGuard g(ctx->device);
+  const size_t np = jgm::nparams(ctx);
+  const size_t nmis = ctx->S * data_per_site(ctx, ctx->storedJ_kind);
+  jgm::DeviceBuffer dmis;  // separate from the internal misfit_stage used by transforms
+  hipError_t e = hipSuccess;
+  if ((e = ctx->model_stage.ensure(sizeof(double) * (np ? np : 1))) != hipSuccess)
+    return hip_fail(e, "storedj_gradient_host");
+  if ((e = ctx->grad_stage.ensure(sizeof(double) * (np ? np : 1))) != hipSuccess)
+    return hip_fail(e, "storedj_gradient_host");
+  if ((e = dmis.ensure(sizeof(double) * (nmis ? nmis : 1))) != hipSuccess)
+    return hip_fail(e, "storedj_gradient_host");
+  if (model && (e = hipMemcpyAsync(ctx->model_stage.ptr, model, sizeof(double) * np,
+                                   hipMemcpyHostToDevice, ctx->stream)) != hipSuccess) {
+    dmis.release();
+    return hip_fail(e, "storedj_gradient_host");
+  }
+  if (nmis && (e = hipMemcpyAsync(dmis.ptr, misfit, sizeof(double) * nmis, hipMemcpyHostToDevice,
+                                  ctx->stream)) != hipSuccess) {
+    dmis.release();
+    return hip_fail(e, "storedj_gradient_host");
+  }
+  st = jgm_storedj_gradient(ctx, prec, model ? ctx->model_stage.as<double>() : nullptr,
+                            dmis.as<double>(), ctx->grad_stage.as<double>());
+  if (st == JGM_OK)
+    e = hipMemcpyAsync(grad, ctx->grad_stage.ptr, sizeof(double) * np, hipMemcpyDeviceToHost,
+                       ctx->stream);
+  hipError_t e2 = hipStreamSynchronize(ctx->stream);
+  dmis.release();
+  if (st != JGM_OK) return st;
+  if (e != hipSuccess) return hip_fail(e, "storedj_gradient_host");
+  if (e2 != hipSuccess) return hip_fail(e2, "storedj_gradient_host");
+  return JGM_OK;
 }
 
 jgm_status jgm_storedj_release(jgm_context* ctx) {

@@ -128,7 +128,7 @@ struct jgm_context {
   // stored sensitivity matrix: FP32, column-major with leading dimension storedJ_ld
   // (rows padded to a multiple of 64) and storedJ_colpad columns (cells padded to a
   // multiple of 256); the padding holds zeros
-  jgm::DeviceBuffer storedJ;
+  jgm::DeviceBuffer storedJ, storedJ_vec;
   int storedJ_kind = 0;
   size_t storedJ_rows = 0, storedJ_cols = 0, storedJ_ld = 0, storedJ_colpad = 0;
 };
@@ -157,6 +157,10 @@ hipError_t launch_background_gradient(jgm_context* ctx, int kind, const double* 
 hipError_t launch_storedj_build(jgm_context* ctx, int kind, int precision);
 hipError_t launch_storedj_apply(jgm_context* ctx, int gemm_precision, int transpose,
                                 const float* in, size_t nrhs, float* out);
+hipError_t launch_storedj_forward(jgm_context* ctx, int gemm_precision, const double* model,
+                                  double* out, double* raw_out);
+hipError_t launch_storedj_gradient(jgm_context* ctx, int gemm_precision,
+                                   const double* raw_misfit, double* grad);
 
 __host__ __device__ inline size_t raw_per_site(int kind) {
   switch (kind) {

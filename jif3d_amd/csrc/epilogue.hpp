@@ -26,36 +26,39 @@ __host__ inline FieldDir field_dir(const jgm_context* ctx) {
 }
 
 // u[] holds gz, Uxx, Uyy, Uzz, Uxy, Uxz, Uyz of the gridded part (physical values).
-// Adds the background layers, assembles the raw data of the kind (gz; the 9 tensor
-// values row-major, TensorOMPGravityImp.cpp:255-264; or B = (T b) F/(4 pi G),
-// OMPMagneticSusceptibilityImp.cpp:164-170), applies the transform and stores.
-__device__ inline void finish_site(double* u, int s, const double* __restrict__ bgT,
+// Adds the background layers and assembles the raw data of the kind into raw[] (gz;
+// the 9 tensor values row-major, TensorOMPGravityImp.cpp:255-264; or
+// B = (T b) F/(4 pi G), OMPMagneticSusceptibilityImp.cpp:164-170); returns its length.
+__device__ inline int assemble_raw(double* u, int s, const double* __restrict__ bgT,
                                    const double* __restrict__ bgvals, int L, int kind,
-                                   FieldDir fd, int tf, double tfbx, double tfby, double tfbz,
-                                   double* __restrict__ out, double* __restrict__ raw_out) {
+                                   FieldDir fd, double* raw) {
   for (int j = 0; j < L; ++j) {
     const double* b = bgT + ((long)s * L + j) * 7;
     const double v = bgvals[j];
     for (int m = 0; m < 7; ++m) u[m] += b[m] * v;
   }
-  double raw[10];
-  int kraw = 0;
   if (kind == JGM_KIND_SCALAR) {
     raw[0] = u[0];
-    kraw = 1;
-  } else if (kind == JGM_KIND_TENSOR || kind == JGM_KIND_SCALAR_TENSOR) {
+    return 1;
+  }
+  if (kind == JGM_KIND_TENSOR || kind == JGM_KIND_SCALAR_TENSOR) {
     int o = 0;
     if (kind == JGM_KIND_SCALAR_TENSOR) raw[o++] = u[0];
     raw[o + 0] = u[1]; raw[o + 1] = u[4]; raw[o + 2] = u[5];
     raw[o + 3] = u[4]; raw[o + 4] = u[2]; raw[o + 5] = u[6];
     raw[o + 6] = u[5]; raw[o + 7] = u[6]; raw[o + 8] = u[3];
-    kraw = o + 9;
-  } else {
-    raw[0] = (u[1] * fd.bx + u[4] * fd.by + u[5] * fd.bz) * fd.scale;
-    raw[1] = (u[4] * fd.bx + u[2] * fd.by + u[6] * fd.bz) * fd.scale;
-    raw[2] = (u[5] * fd.bx + u[6] * fd.by + u[3] * fd.bz) * fd.scale;
-    kraw = 3;
+    return o + 9;
   }
+  raw[0] = (u[1] * fd.bx + u[4] * fd.by + u[5] * fd.bz) * fd.scale;
+  raw[1] = (u[4] * fd.bx + u[2] * fd.by + u[6] * fd.bz) * fd.scale;
+  raw[2] = (u[5] * fd.bx + u[6] * fd.by + u[3] * fd.bz) * fd.scale;
+  return 3;
+}
+
+// Stores the raw data of site s (if raw_out) and the transformed data into out.
+__device__ inline void store_site(const double* raw, int kraw, int s, int tf, double tfbx,
+                                  double tfby, double tfbz, double* __restrict__ out,
+                                  double* __restrict__ raw_out) {
   if (raw_out) {
     for (int m = 0; m < kraw; ++m) raw_out[(long)s * kraw + m] = raw[m];
   }
@@ -69,6 +72,16 @@ __device__ inline void finish_site(double* u, int s, const double* __restrict__ 
   } else {
     for (int m = 0; m < kraw; ++m) out[(long)s * kraw + m] = raw[m];
   }
+}
+
+// Background, raw assembly, transform and store for one site.
+__device__ inline void finish_site(double* u, int s, const double* __restrict__ bgT,
+                                   const double* __restrict__ bgvals, int L, int kind,
+                                   FieldDir fd, int tf, double tfbx, double tfby, double tfbz,
+                                   double* __restrict__ out, double* __restrict__ raw_out) {
+  double raw[10];
+  const int kraw = assemble_raw(u, s, bgT, bgvals, L, kind, fd, raw);
+  store_site(raw, kraw, s, tf, tfbx, tfby, tfbz, out, raw_out);
 }
 
 // Coefficients of the 7 physical field components (gz, Uxx, Uyy, Uzz, Uxy, Uxz, Uyz)

@@ -390,6 +390,99 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jtr(const float* __restrict__ J
   }
 }
 
+// One model (or gradient) vector as the first right-hand side of a padded FP32
+// operand Xt[npad][64]; the other 63 columns and the padding rows are zero.
+__global__ __launch_bounds__(kBlock) void k_vec_to_rhs(const double* __restrict__ v, long n,
+                                                       long npad, float* __restrict__ Xt) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= npad * 64) return;
+  const long k = idx >> 6;
+  Xt[idx] = ((idx & 63) == 0 && k < n) ? (float)v[k] : 0.0f;
+}
+
+// Raw misfit (raw layout per site) -> compact residual rows as the first right-hand
+// side of Xt[ld][64]: the symmetric tensor entries appear twice in the raw layout, so
+// their compact rows receive the sum of both residuals.
+__global__ __launch_bounds__(kBlock) void k_rm_to_compact(const double* __restrict__ rm, int S,
+                                                          int kind, long ld,
+                                                          float* __restrict__ Xt) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ld * 64) return;
+  const long row = idx >> 6;
+  const int kJ = compact_rows(kind);
+  float v = 0.0f;
+  if ((idx & 63) == 0 && row < (long)S * kJ) {
+    const long s = row / kJ;
+    int q = (int)(row % kJ);
+    if (kind == JGM_KIND_SCALAR) {
+      v = (float)rm[s];
+    } else if (kind == JGM_KIND_MAGNETIC) {
+      v = (float)rm[s * 3 + q];
+    } else {
+      const int kraw = kind == JGM_KIND_SCALAR_TENSOR ? 10 : 9;
+      const double* r = rm + s * kraw;
+      if (kind == JGM_KIND_SCALAR_TENSOR) {
+        if (q == 0) {
+          v = (float)r[0];
+          q = -1;
+        } else {
+          r += 1;
+          q -= 1;
+        }
+      }
+      switch (q) {  // compact Uxx, Uxy, Uxz, Uyy, Uyz, Uzz
+        case 0: v = (float)r[0]; break;
+        case 1: v = (float)(r[1] + r[3]); break;
+        case 2: v = (float)(r[2] + r[6]); break;
+        case 3: v = (float)r[4]; break;
+        case 4: v = (float)(r[5] + r[7]); break;
+        case 5: v = (float)r[8]; break;
+        default: break;
+      }
+    }
+  }
+  Xt[idx] = v;
+}
+
+// Cached forward epilogue: compact J m (FP32) of site s plus the background layers,
+// assembled into the raw layout and transformed like the matrix-free path.
+__global__ __launch_bounds__(kBlock) void k_storedj_finish(
+    const float* __restrict__ Y, int S, int kind, const double* __restrict__ bgT,
+    const double* __restrict__ bgvals, int L, FieldDir fd, int tf, double tfbx, double tfby,
+    double tfbz, double* __restrict__ out, double* __restrict__ raw_out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const int kJ = compact_rows(kind);
+  const float* y = Y + (long)s * kJ;
+  double u[7] = {0, 0, 0, 0, 0, 0, 0};  // gz, Uxx, Uyy, Uzz, Uxy, Uxz, Uyz
+  if (kind == JGM_KIND_SCALAR) {
+    u[0] = y[0];
+  } else if (kind == JGM_KIND_TENSOR || kind == JGM_KIND_SCALAR_TENSOR) {
+    const int o = kind == JGM_KIND_SCALAR_TENSOR ? 1 : 0;
+    if (o) u[0] = y[0];
+    u[1] = y[o + 0];
+    u[4] = y[o + 1];
+    u[5] = y[o + 2];
+    u[2] = y[o + 3];
+    u[6] = y[o + 4];
+    u[3] = y[o + 5];
+  }
+  double raw[10];
+  const int kraw = assemble_raw(u, s, bgT, bgvals, L, kind, fd, raw);
+  if (kind == JGM_KIND_MAGNETIC) {  // J rows are already the projected field
+    raw[0] += y[0];
+    raw[1] += y[1];
+    raw[2] += y[2];
+  }
+  store_site(raw, kraw, s, tf, tfbx, tfby, tfbz, out, raw_out);
+}
+
+__global__ __launch_bounds__(kBlock) void k_scale_to_f64(const float* __restrict__ g, long n,
+                                                         double scale, double* __restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = scale * (double)g[i];
+}
+
 }  // namespace
 
 hipError_t launch_storedj_build(jgm_context* ctx, int kind, int precision) {
@@ -438,64 +531,129 @@ hipError_t launch_storedj_build(jgm_context* ctx, int kind, int precision) {
   return hipSuccess;
 }
 
+namespace {
+
+// J M on a transposed, padded operand Xt[colpad][P] -> out (R x nrhs, column-major)
+hipError_t run_jm(jgm_context* ctx, int gemm_precision, const float* Xt, int P, int nrhs,
+                  float* out) {
+  const long R = (long)ctx->storedJ_rows;
+  const long ld = (long)ctx->storedJ_ld, colpad = (long)ctx->storedJ_colpad;
+  const float* J = ctx->storedJ.as<const float>();
+  hipStream_t st = ctx->stream;
+  const int rhs_tiles = P / 64;
+  const int row_tiles = (int)((ld + 255) / 256);
+  // one workgroup per CU: a second co-resident workgroup slows both down (the CU's
+  // memory pipeline is the limit); splits in multiples of 8 when there are enough of
+  // them to group by XCD
+  long nsplit = std::max<long>(1, (long)ctx->cu_count / ((long)row_tiles * rhs_tiles));
+  nsplit = std::min<long>(nsplit, colpad / 32);
+  if (nsplit >= 16) nsplit = nsplit / 8 * 8;
+  const long per = round_up((colpad + nsplit - 1) / nsplit, 2 * kChunk);
+  hipError_t e = ctx->partial.ensure(sizeof(float) * (size_t)nsplit * R * P);
+  if (e != hipSuccess) return e;
+  dim3 grid((unsigned)(nsplit * row_tiles), (unsigned)rhs_tiles);
+  {
+    ProfScope ps(ctx, gemm_precision == JGM_GEMM_FP32   ? "storedj_jm_fp32"
+                      : gemm_precision == JGM_GEMM_BF16 ? "storedj_jm_bf16"
+                                                        : "storedj_jm_bf16_refined");
+#define JGM_JM(PR)                                                                             \
+  hipLaunchKernelGGL((k_gemm_jm<PR>), grid, dim3(kBlock), 0, st, J, ld, R, colpad, Xt, P, per, \
+                     row_tiles, (int)nsplit, ctx->partial.as<float>())
+    switch (gemm_precision) {
+      case JGM_GEMM_FP32: JGM_JM(JGM_GEMM_FP32); break;
+      case JGM_GEMM_BF16: JGM_JM(JGM_GEMM_BF16); break;
+      default: JGM_JM(JGM_GEMM_BF16_REFINED); break;
+    }
+#undef JGM_JM
+  }
+  hipLaunchKernelGGL(k_gemm_jm_reduce, dim3(blocks_for(R * (long)nrhs)), dim3(kBlock), 0, st,
+                     ctx->partial.as<const float>(), (int)nsplit, R, P, nrhs, out);
+  return hipGetLastError();
+}
+
+// J^T R on a transposed, padded operand Xt[ld][P] -> out (N x nrhs, column-major)
+hipError_t run_jtr(jgm_context* ctx, int gemm_precision, const float* Xt, int P, int nrhs,
+                   float* out) {
+  const long N = (long)ctx->storedJ_cols;
+  const long ld = (long)ctx->storedJ_ld, colpad = (long)ctx->storedJ_colpad;
+  const float* J = ctx->storedJ.as<const float>();
+  hipStream_t st = ctx->stream;
+  const long nblk = colpad / kColPad;
+  dim3 grid((unsigned)std::min<long>(nblk, 65535), (unsigned)(P / 64));
+  ProfScope ps(ctx, gemm_precision == JGM_GEMM_FP32   ? "storedj_jtr_fp32"
+                    : gemm_precision == JGM_GEMM_BF16 ? "storedj_jtr_bf16"
+                                                      : "storedj_jtr_bf16_refined");
+#define JGM_JTR(PR)                                                                          \
+  hipLaunchKernelGGL((k_gemm_jtr<PR>), grid, dim3(kBlock), 0, st, J, ld, N, nblk, Xt, P, out, \
+                     nrhs)
+  switch (gemm_precision) {
+    case JGM_GEMM_FP32: JGM_JTR(JGM_GEMM_FP32); break;
+    case JGM_GEMM_BF16: JGM_JTR(JGM_GEMM_BF16); break;
+    default: JGM_JTR(JGM_GEMM_BF16_REFINED); break;
+  }
+#undef JGM_JTR
+  return hipGetLastError();
+}
+
+}  // namespace
+
 hipError_t launch_storedj_apply(jgm_context* ctx, int gemm_precision, int transpose,
                                 const float* in, size_t nrhs, float* out) {
   const long R = (long)ctx->storedJ_rows, N = (long)ctx->storedJ_cols;
   const long ld = (long)ctx->storedJ_ld, colpad = (long)ctx->storedJ_colpad;
   const int P = (int)round_up((long)nrhs, 64);
   const long K = transpose ? R : N, Kpad = transpose ? ld : colpad;
-  hipStream_t st = ctx->stream;
   hipError_t e = ctx->coef32.ensure(sizeof(float) * (size_t)Kpad * P);
   if (e != hipSuccess) return e;
   float* Xt = ctx->coef32.as<float>();
-  hipLaunchKernelGGL(k_transpose_pad, dim3(blocks_for(Kpad * P)), dim3(kBlock), 0, st, in, K,
-                     Kpad, (int)nrhs, P, Xt);
-  const float* J = ctx->storedJ.as<const float>();
-  const int rhs_tiles = P / 64;
-  if (!transpose) {
-    const int row_tiles = (int)((ld + 255) / 256);
-    // one workgroup per CU: a second co-resident workgroup slows both down (the CU's
-    // memory pipeline is the limit); splits in multiples of 8 when there are enough of
-    // them to group by XCD
-    long nsplit = std::max<long>(1, (long)ctx->cu_count / ((long)row_tiles * rhs_tiles));
-    nsplit = std::min<long>(nsplit, colpad / 32);
-    if (nsplit >= 16) nsplit = nsplit / 8 * 8;
-    const long per = round_up((colpad + nsplit - 1) / nsplit, 2 * kChunk);
-    if ((e = ctx->partial.ensure(sizeof(float) * (size_t)nsplit * R * P)) != hipSuccess) return e;
-    dim3 grid((unsigned)(nsplit * row_tiles), (unsigned)rhs_tiles);
-    {
-      ProfScope ps(ctx, gemm_precision == JGM_GEMM_FP32   ? "storedj_jm_fp32"
-                        : gemm_precision == JGM_GEMM_BF16 ? "storedj_jm_bf16"
-                                                          : "storedj_jm_bf16_refined");
-#define JGM_JM(PR)                                                                             \
-  hipLaunchKernelGGL((k_gemm_jm<PR>), grid, dim3(kBlock), 0, st, J, ld, R, colpad, Xt, P, per, \
-                     row_tiles, (int)nsplit, ctx->partial.as<float>())
-      switch (gemm_precision) {
-        case JGM_GEMM_FP32: JGM_JM(JGM_GEMM_FP32); break;
-        case JGM_GEMM_BF16: JGM_JM(JGM_GEMM_BF16); break;
-        default: JGM_JM(JGM_GEMM_BF16_REFINED); break;
-      }
-#undef JGM_JM
-    }
-    hipLaunchKernelGGL(k_gemm_jm_reduce, dim3(blocks_for(R * (long)nrhs)), dim3(kBlock), 0, st,
-                       ctx->partial.as<const float>(), (int)nsplit, R, P, (int)nrhs, out);
-  } else {
-    const long nblk = colpad / kColPad;
-    dim3 grid((unsigned)std::min<long>(nblk, 65535), (unsigned)rhs_tiles);
-    ProfScope ps(ctx, gemm_precision == JGM_GEMM_FP32   ? "storedj_jtr_fp32"
-                      : gemm_precision == JGM_GEMM_BF16 ? "storedj_jtr_bf16"
-                                                        : "storedj_jtr_bf16_refined");
-#define JGM_JTR(PR) \
-  hipLaunchKernelGGL((k_gemm_jtr<PR>), grid, dim3(kBlock), 0, st, J, ld, N, nblk, Xt, P, out, \
-                     (int)nrhs)
-    switch (gemm_precision) {
-      case JGM_GEMM_FP32: JGM_JTR(JGM_GEMM_FP32); break;
-      case JGM_GEMM_BF16: JGM_JTR(JGM_GEMM_BF16); break;
-      default: JGM_JTR(JGM_GEMM_BF16_REFINED); break;
-    }
-#undef JGM_JTR
-  }
+  hipLaunchKernelGGL(k_transpose_pad, dim3(blocks_for(Kpad * P)), dim3(kBlock), 0, ctx->stream,
+                     in, K, Kpad, (int)nrhs, P, Xt);
+  return transpose ? run_jtr(ctx, gemm_precision, Xt, P, (int)nrhs, out)
+                   : run_jm(ctx, gemm_precision, Xt, P, (int)nrhs, out);
+}
+
+hipError_t launch_storedj_forward(jgm_context* ctx, int gemm_precision, const double* model,
+                                  double* out, double* raw_out) {
+  const long R = (long)ctx->storedJ_rows, N = (long)ctx->storedJ_cols;
+  const long colpad = (long)ctx->storedJ_colpad;
+  const int kind = ctx->storedJ_kind;
+  const int S = (int)ctx->S;
+  const int L = (int)ctx->bgthick.size();
+  hipError_t e = ctx->coef32.ensure(sizeof(float) * (size_t)colpad * 64);
+  if (e != hipSuccess) return e;
+  if ((e = ctx->storedJ_vec.ensure(sizeof(float) * (size_t)std::max(R, N))) != hipSuccess) return e;
+  float* Xt = ctx->coef32.as<float>();
+  hipLaunchKernelGGL(k_vec_to_rhs, dim3(blocks_for(colpad * 64)), dim3(kBlock), 0, ctx->stream,
+                     model, N, colpad, Xt);
+  if ((e = run_jm(ctx, gemm_precision, Xt, 64, 1, ctx->storedJ_vec.as<float>())) != hipSuccess)
+    return e;
+  const double tf_bx = std::cos(ctx->tf_inc) * std::cos(ctx->tf_dec);
+  const double tf_by = std::cos(ctx->tf_inc) * std::sin(ctx->tf_dec);
+  const double tf_bz = std::sin(ctx->tf_inc);
+  hipLaunchKernelGGL(k_storedj_finish, dim3(blocks_for(S)), dim3(kBlock), 0, ctx->stream,
+                     ctx->storedJ_vec.as<const float>(), S, kind, ctx->bgT.as<const double>(),
+                     model + N, L, field_dir(ctx), ctx->transform, tf_bx, tf_by, tf_bz, out,
+                     raw_out);
   return hipGetLastError();
+}
+
+hipError_t launch_storedj_gradient(jgm_context* ctx, int gemm_precision,
+                                   const double* raw_misfit, double* grad) {
+  const long R = (long)ctx->storedJ_rows, N = (long)ctx->storedJ_cols;
+  const long ld = (long)ctx->storedJ_ld;
+  const int kind = ctx->storedJ_kind;
+  hipError_t e = ctx->coef32.ensure(sizeof(float) * (size_t)ld * 64);
+  if (e != hipSuccess) return e;
+  if ((e = ctx->storedJ_vec.ensure(sizeof(float) * (size_t)std::max(R, N))) != hipSuccess) return e;
+  float* Xt = ctx->coef32.as<float>();
+  hipLaunchKernelGGL(k_rm_to_compact, dim3(blocks_for(ld * 64)), dim3(kBlock), 0, ctx->stream,
+                     raw_misfit, (int)ctx->S, kind, ld, Xt);
+  if ((e = run_jtr(ctx, gemm_precision, Xt, 64, 1, ctx->storedJ_vec.as<float>())) != hipSuccess)
+    return e;
+  hipLaunchKernelGGL(k_scale_to_f64, dim3(blocks_for(N)), dim3(kBlock), 0, ctx->stream,
+                     ctx->storedJ_vec.as<const float>(), N, 2.0, grad);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  return launch_background_gradient(ctx, kind, raw_misfit, grad + N);
 }
 
 }  // namespace jgm

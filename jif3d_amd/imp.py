@@ -7,7 +7,9 @@
   jif3D/Magnetics/OMPMagneticSusceptibilityImp.cpp:31-201): Calculate() and
   LQDerivative() of ThreeDGravMagImplementation (jif3D/GravMag/
   ThreeDGravMagImplementation.h:116-231) run on the GPU through the C ABI.
-* MinMemGravMagCalculator mirrors jif3D/GravMag/MinMemGravMagCalculator.h:25-79.
+* MinMemGravMagCalculator mirrors jif3D/GravMag/MinMemGravMagCalculator.h:25-79;
+  FullSensitivityGravMagCalculator mirrors jif3D/GravMag/FullSensitivityGravMagCalculator.h
+  with the geometry cache of CachedGravMagCalculator.h:27-135, keeping J in HBM.
 * FTGInvariant / TotalField / TotalFieldAnomaly select the per-site transforms of
   jif3D/Gravity/GravityTransforms.h:25-100 and jif3D/Magnetics/MagneticTransforms.h:19-156.
 
@@ -176,3 +178,62 @@ class ThreeDGravMagCalculator:
 class MinMemGravMagCalculator(ThreeDGravMagCalculator):
     """No stored sensitivities: forward and gradient are recomputed matrix-free."""
     pass
+
+
+class CachedGravMagCalculator(ThreeDGravMagCalculator):
+    """Geometry cache (jif3D/GravMag/CachedGravMagCalculator.h:27-135): while mesh,
+    background layers, sites (and inducing field) are unchanged, Calculate and
+    LQDerivative use the cached quantities of the derived class."""
+
+    def __init__(self, imp):
+        super().__init__(imp)
+        self._cache_key = None
+
+    def _key(self, model, data):
+        return _geometry_key(model, data) + (repr(self.Imp._field()),)
+
+    def InvalidateCache(self):
+        self._cache_key = None
+
+    def Calculate(self, model, data):
+        self.CheckModelConsistency(model)
+        key = self._key(model, data)
+        if self._cache_key == key:
+            return self.CalculateCachedResult(model, data)
+        result = self.CalculateNewModel(model, data)
+        self._cache_key = key
+        return result
+
+    def LQDerivative(self, model, data, misfit):
+        self.CheckModelConsistency(model)
+        if self._cache_key == self._key(model, data):
+            return self.CachedLQDerivative(model, data, misfit)
+        return self.Imp.LQDerivative(model, data, misfit)
+
+
+class FullSensitivityGravMagCalculator(CachedGravMagCalculator):
+    """Stored sensitivities (jif3D/GravMag/FullSensitivityGravMagCalculator.h:99-221),
+    kept in HBM as FP32 in the compact row layout: the first Calculate for a geometry
+    builds J and returns the imp's direct result; later calls are J m (plus the
+    background layers) and 2 [J B]^T D^T misfit on the matrix cores in `gemm_precision`."""
+
+    def __init__(self, imp, gemm_precision=nat.GEMM_FP32):
+        super().__init__(imp)
+        self.gemm_precision = gemm_precision
+
+    def CalculateNewModel(self, model, data):
+        ctx = self.Imp.context(model, data)
+        ctx.storedj_build(self.Imp.kind, self.Imp.precision)
+        return self.Imp.Calculate(model, data)
+
+    def CalculateCachedResult(self, model, data):
+        ctx = self.Imp.context(model, data)
+        return ctx.storedj_forward(self.gemm_precision, model.GetModelParameters())
+
+    def CachedLQDerivative(self, model, data, misfit):
+        ctx = self.Imp.context(model, data)
+        misfit = np.asarray(misfit, dtype=np.float64)
+        need = data.nsites * self.GetDataPerMeasurement()
+        if misfit.size != need:
+            raise ValueError(f"misfit has {misfit.size} entries, expected {need}")
+        return ctx.storedj_gradient(self.gemm_precision, model.GetModelParameters(), misfit)
