@@ -1,5 +1,7 @@
 // Command-line driver of the C++ adapter: reads a problem from a binary file, runs
-// Calculate and LQDerivative through HipGravMagImplementation and writes the results.
+// Calculate and LQDerivative through HipGravMagImplementation (or, with --full,
+// through HipFullSensitivityCalculator: a direct Calculate that stores J, then the
+// cached Calculate and LQDerivative) and writes the results.
 //
 // input  (little-endian): int32 kind, int32 precision, int32 nx, ny, nz, nbg, nsites;
 //        then doubles: xcoords[nx+1], ycoords[ny+1], zcoords[nz+1], xsizes[nx],
@@ -10,6 +12,7 @@
 #include <cstdio>
 #include <fstream>
 #include <iostream>
+#include <string>
 #include <vector>
 
 #include "hip_gravmag_imp.hpp"
@@ -24,8 +27,9 @@ std::vector<double> read_doubles(std::ifstream& in, size_t n) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  if (argc != 3) {
-    std::cerr << "usage: adapter_driver <input> <output>\n";
+  const bool full = argc == 4 && std::string(argv[3]) == "--full";
+  if (argc != 3 && !full) {
+    std::cerr << "usage: adapter_driver <input> <output> [--full]\n";
     return 2;
   }
   try {
@@ -53,8 +57,16 @@ int main(int argc, char** argv) {
     jif3d_amd::HipGravMagImplementation imp(kind, prec);
     if (kind == JGM_KIND_MAGNETIC) imp.SetField(field[0], field[1], field[2]);
     const std::vector<double> misfit = read_doubles(in, ns * imp.RawDataPerMeasurement());
-    const std::vector<double> fwd = imp.Calculate(mesh, sites, model);
-    const std::vector<double> grad = imp.LQDerivative(mesh, sites, model, misfit);
+    std::vector<double> fwd, grad;
+    if (full) {
+      jif3d_amd::HipFullSensitivityCalculator calc(imp);
+      calc.Calculate(mesh, sites, model);         // direct; stores J
+      fwd = calc.Calculate(mesh, sites, model);   // J m
+      grad = calc.LQDerivative(mesh, sites, model, misfit);
+    } else {
+      fwd = imp.Calculate(mesh, sites, model);
+      grad = imp.LQDerivative(mesh, sites, model, misfit);
+    }
     std::ofstream out(argv[2], std::ios::binary);
     out.write(reinterpret_cast<const char*>(fwd.data()),
               static_cast<std::streamsize>(fwd.size() * sizeof(double)));

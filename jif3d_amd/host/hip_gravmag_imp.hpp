@@ -111,10 +111,13 @@ class HipGravMagImplementation {
   }
 
   jgm_context* context() { return ctx_; }
+  jgm_kind kind() const { return kind_; }
+  jgm_precision precision() const { return prec_; }
 
- private:
-  void Configure(const Mesh& mesh, const Sites& sites) {
-    if (have_geometry_ && mesh == mesh_ && sites == sites_) return;
+  // Sets mesh, background and sites unless they are unchanged; returns true when the
+  // geometry was (re)loaded.
+  bool Configure(const Mesh& mesh, const Sites& sites) {
+    if (have_geometry_ && mesh == mesh_ && sites == sites_) return false;
     check(jgm_set_mesh(ctx_, mesh.xsizes.size(), mesh.ysizes.size(), mesh.zsizes.size(),
                        mesh.xcoords.data(), mesh.ycoords.data(), mesh.zcoords.data(),
                        mesh.xsizes.data(), mesh.ysizes.data(), mesh.zsizes.data()),
@@ -126,8 +129,10 @@ class HipGravMagImplementation {
     mesh_ = mesh;
     sites_ = sites;
     have_geometry_ = true;
+    return true;
   }
 
+ private:
   jgm_context* ctx_ = nullptr;
   jgm_kind kind_ = JGM_KIND_SCALAR;
   jgm_precision prec_ = JGM_FP64;
@@ -135,6 +140,56 @@ class HipGravMagImplementation {
   Mesh mesh_;
   Sites sites_;
   bool have_geometry_ = false;
+};
+
+// FullSensitivityGravMagCalculator with the geometry cache of CachedGravMagCalculator
+// (jif3D/GravMag/FullSensitivityGravMagCalculator.h:99-221,
+// CachedGravMagCalculator.h:27-135), J kept in HBM as FP32: the first Calculate for
+// a geometry returns the imp's direct result and builds J; while mesh, background,
+// sites and field are unchanged, Calculate is J m (plus the background layers) and
+// LQDerivative is 2 [J B]^T D^T misfit on the matrix cores.
+class HipFullSensitivityCalculator {
+ public:
+  explicit HipFullSensitivityCalculator(HipGravMagImplementation& imp,
+                                        jgm_gemm_precision prec = JGM_GEMM_FP32)
+      : imp_(imp), prec_(prec) {}
+
+  std::vector<double> Calculate(const Mesh& mesh, const Sites& sites,
+                                const std::vector<double>& model) {
+    if (Cached(mesh, sites)) {
+      std::vector<double> out(sites.size() * imp_.GetDataPerMeasurement());
+      check(jgm_storedj_forward_host(imp_.context(), prec_, model.data(), out.data()),
+            "jgm_storedj_forward_host");
+      return out;
+    }
+    std::vector<double> out = imp_.Calculate(mesh, sites, model);
+    check(jgm_storedj_build(imp_.context(), imp_.kind(), imp_.precision()), "jgm_storedj_build");
+    return out;
+  }
+
+  std::vector<double> LQDerivative(const Mesh& mesh, const Sites& sites,
+                                   const std::vector<double>& model,
+                                   const std::vector<double>& misfit) {
+    if (!Cached(mesh, sites)) return imp_.LQDerivative(mesh, sites, model, misfit);
+    if (misfit.size() != sites.size() * imp_.GetDataPerMeasurement())
+      throw FatalException("misfit length does not match the number of data");
+    std::vector<double> grad(jgm_n_params(imp_.context()));
+    check(jgm_storedj_gradient_host(imp_.context(), prec_, model.data(), misfit.data(),
+                                    grad.data()),
+          "jgm_storedj_gradient_host");
+    return grad;
+  }
+
+ private:
+  // The stored J is valid while the geometry is unchanged; the library drops it itself
+  // when the geometry or the inducing field changes (jgm_storedj_rows is then 0).
+  bool Cached(const Mesh& mesh, const Sites& sites) {
+    const bool reloaded = imp_.Configure(mesh, sites);
+    return !reloaded && jgm_storedj_rows(imp_.context()) != 0;
+  }
+
+  HipGravMagImplementation& imp_;
+  jgm_gemm_precision prec_;
 };
 
 }  // namespace jif3d_amd
