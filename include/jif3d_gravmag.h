@@ -45,12 +45,20 @@ typedef enum jgm_status {
 
 /* Raw data per site: 1 (gz, m/s^2), 9 (FTG, 1/s^2, row-major Uxx..Uzz as
  * jif3D/Gravity/TensorOMPGravityImp.cpp:255-264), 3 (Bx, By, Bz in nT), or 10
- * (gz followed by the 9 FTG values: one pass for a joint scalar+tensor survey). */
+ * (gz followed by the 9 FTG values: one pass for a joint scalar+tensor survey).
+ * JGM_KIND_MAGNETIC_GRADIENT: 1 value, Bz at the site minus Bz 1 m below with
+ * mu0/(4 pi) = 1e-7 (OMPMagneticGradImp, jif3D/Magnetics/OMPMagneticGradImp.cpp:13-106;
+ * no background).  JGM_KIND_MAGNETIZATION: Bx, By, Bz of a magnetization model with
+ * three values per cell, [Mx; My; Mz], B = 1e9 mu0/(4 pi G) T M (OMPMagnetizationImp,
+ * jif3D/Magnetics/OMPMagnetizationImp.cpp:25-117; no background, model length
+ * 3 ncells).  These two kinds run in JGM_FP64 only. */
 typedef enum jgm_kind {
   JGM_KIND_SCALAR = 1,
   JGM_KIND_TENSOR = 2,
   JGM_KIND_MAGNETIC = 3,
-  JGM_KIND_SCALAR_TENSOR = 4
+  JGM_KIND_SCALAR_TENSOR = 4,
+  JGM_KIND_MAGNETIC_GRADIENT = 5,
+  JGM_KIND_MAGNETIZATION = 6
 } jgm_kind;
 
 /* JGM_FP64: closed-form box terms in FP64 for every site-cell pair.
@@ -64,8 +72,8 @@ typedef enum jgm_precision { JGM_FP64 = 0, JGM_MIXED = 1 } jgm_precision;
 typedef enum jgm_transform {
   JGM_TF_NONE = 0,
   JGM_TF_FTG_INVARIANT = 1,     /* tensor kind only */
-  JGM_TF_TOTAL_FIELD = 2,       /* magnetic kind only */
-  JGM_TF_TOTAL_FIELD_ANOMALY = 3 /* magnetic kind only */
+  JGM_TF_TOTAL_FIELD = 2,       /* magnetic and magnetization kinds */
+  JGM_TF_TOTAL_FIELD_ANOMALY = 3 /* magnetic and magnetization kinds */
 } jgm_transform;
 
 /* Precision of the stored-J products. */
@@ -114,6 +122,9 @@ jgm_status jgm_set_mixed_threshold(jgm_context *ctx, double q0);
  * (ncells + nlayers, ThreeDGravityModel::GetNModelParm). */
 size_t jgm_data_per_site(const jgm_context *ctx, jgm_kind kind);
 size_t jgm_n_params(const jgm_context *ctx);
+/* Model vector length of a kind: 3 ncells for JGM_KIND_MAGNETIZATION
+ * (ThreeDMagnetizationModel), jgm_n_params otherwise. */
+size_t jgm_model_size(const jgm_context *ctx, jgm_kind kind);
 
 /* Forward response of every site: ThreeDGravMagImplementation::Calculate
  * (jif3D/GravMag/ThreeDGravMagImplementation.h:116-168) with the OMP imps'

@@ -15,17 +15,20 @@ JGM_OK = 0
 (JGM_ERR_INVALID_ARGUMENT, JGM_ERR_NOT_READY, JGM_ERR_DEVICE, JGM_ERR_OUT_OF_MEMORY,
  JGM_ERR_UNSUPPORTED) = 1, 2, 3, 4, 5
 KIND_SCALAR, KIND_TENSOR, KIND_MAGNETIC, KIND_SCALAR_TENSOR = 1, 2, 3, 4
+KIND_MAGNETIC_GRADIENT, KIND_MAGNETIZATION = 5, 6
 FP64, MIXED = 0, 1
 TF_NONE, TF_FTG_INVARIANT, TF_TOTAL_FIELD, TF_TOTAL_FIELD_ANOMALY = 0, 1, 2, 3
 GEMM_FP32, GEMM_BF16, GEMM_BF16_REFINED = 0, 1, 2
-RAW_PER_SITE = {KIND_SCALAR: 1, KIND_TENSOR: 9, KIND_MAGNETIC: 3, KIND_SCALAR_TENSOR: 10}
+RAW_PER_SITE = {KIND_SCALAR: 1, KIND_TENSOR: 9, KIND_MAGNETIC: 3, KIND_SCALAR_TENSOR: 10,
+                KIND_MAGNETIC_GRADIENT: 1, KIND_MAGNETIZATION: 3}
 
 EXPORTS = [
     "jgm_last_error", "jgm_api_version", "jgm_raw_per_site", "jgm_create", "jgm_destroy",
     "jgm_set_stream", "jgm_synchronize", "jgm_set_mesh", "jgm_set_background", "jgm_set_sites",
     "jgm_set_field", "jgm_set_transform", "jgm_set_mixed_threshold", "jgm_data_per_site",
     "jgm_n_params", "jgm_forward", "jgm_forward_host", "jgm_adjoint", "jgm_adjoint_host",
-    "jgm_sensitivity_rows", "jgm_storedj_build", "jgm_storedj_rows", "jgm_storedj_apply",
+    "jgm_sensitivity_rows", "jgm_model_size", "jgm_storedj_build", "jgm_storedj_rows",
+    "jgm_storedj_apply",
     "jgm_storedj_apply_t", "jgm_storedj_forward", "jgm_storedj_gradient",
     "jgm_storedj_forward_host", "jgm_storedj_gradient_host", "jgm_storedj_release", "jgm_device_alloc", "jgm_device_free",
     "jgm_copy_to_device", "jgm_copy_to_host", "jgm_copy_device", "jgm_profile_enable", "jgm_profile_reset",
@@ -100,6 +103,8 @@ def load():
     lib.jgm_data_per_site.argtypes = [vp, i]
     lib.jgm_n_params.restype = sz
     lib.jgm_n_params.argtypes = [vp]
+    lib.jgm_model_size.restype = sz
+    lib.jgm_model_size.argtypes = [vp, i]
     lib.jgm_storedj_rows.restype = sz
     lib.jgm_storedj_rows.argtypes = [vp]
     lib.jgm_profile_count.restype = sz
@@ -244,6 +249,9 @@ class Context:
     def n_params(self):
         return self.lib.jgm_n_params(self.handle)
 
+    def model_size(self, kind):
+        return self.lib.jgm_model_size(self.handle, kind)
+
     def forward(self, kind, prec, model, out=None):
         """Host arrays in -> host array out; torch device tensors -> device result."""
         if _is_device(model):
@@ -266,7 +274,7 @@ class Context:
             return out
         m = _f64(model) if model is not None else None
         r = _f64(misfit)
-        g = np.zeros(self.n_params())
+        g = np.zeros(self.model_size(kind))
         check(self.lib.jgm_adjoint_host(self.handle, kind, prec, _pd(m), _pd(r), _pd(g)))
         return g
 

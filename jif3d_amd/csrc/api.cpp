@@ -48,7 +48,13 @@ struct DeviceGuard {
 
 bool kind_ok(int kind) {
   return kind == JGM_KIND_SCALAR || kind == JGM_KIND_TENSOR || kind == JGM_KIND_MAGNETIC ||
-         kind == JGM_KIND_SCALAR_TENSOR;
+         kind == JGM_KIND_SCALAR_TENSOR || kind == JGM_KIND_MAGNETIC_GRADIENT ||
+         kind == JGM_KIND_MAGNETIZATION;
+}
+
+// the vertical-gradient and magnetization kinds have FP64 kernels only
+bool fp64_only(int kind) {
+  return kind == JGM_KIND_MAGNETIC_GRADIENT || kind == JGM_KIND_MAGNETIZATION;
 }
 
 jgm_status check_ready(const jgm_context* ctx, int kind, int prec) {
@@ -56,13 +62,17 @@ jgm_status check_ready(const jgm_context* ctx, int kind, int prec) {
   if (!kind_ok(kind)) return fail(JGM_ERR_INVALID_ARGUMENT, "unknown kind %d", kind);
   if (prec != JGM_FP64 && prec != JGM_MIXED)
     return fail(JGM_ERR_INVALID_ARGUMENT, "unknown precision %d", prec);
+  if (prec == JGM_MIXED && fp64_only(kind))
+    return fail(JGM_ERR_UNSUPPORTED, "kind %d runs in JGM_FP64 only", kind);
   if (!ctx->mesh_set) return fail(JGM_ERR_NOT_READY, "mesh not set (jgm_set_mesh)");
   if (!ctx->sites_set) return fail(JGM_ERR_NOT_READY, "sites not set (jgm_set_sites)");
   const int tf = ctx->transform;
   if (tf == JGM_TF_FTG_INVARIANT && kind != JGM_KIND_TENSOR)
     return fail(JGM_ERR_INVALID_ARGUMENT, "the FTG invariant transform needs the tensor kind");
-  if ((tf == JGM_TF_TOTAL_FIELD || tf == JGM_TF_TOTAL_FIELD_ANOMALY) && kind != JGM_KIND_MAGNETIC)
-    return fail(JGM_ERR_INVALID_ARGUMENT, "total-field transforms need the magnetic kind");
+  if ((tf == JGM_TF_TOTAL_FIELD || tf == JGM_TF_TOTAL_FIELD_ANOMALY) &&
+      kind != JGM_KIND_MAGNETIC && kind != JGM_KIND_MAGNETIZATION)
+    return fail(JGM_ERR_INVALID_ARGUMENT,
+                "total-field transforms need the magnetic or magnetization kind");
   if (jgm::ncells(ctx) >= (size_t)1 << 31 || jgm::ncorners(ctx) >= ((size_t)1 << 40))
     return fail(JGM_ERR_UNSUPPORTED, "mesh too large");
   return JGM_OK;
@@ -137,7 +147,7 @@ jgm_status jgm_destroy(jgm_context* ctx) {
                                &ctx->near.col_ptr, &ctx->near.col_sites, &ctx->near.col_ids,
                                &ctx->near.col_kmax, &ctx->near.xcen, &ctx->near.ycen,
                                &ctx->near.zcen, &ctx->near.adj_table, &ctx->near.corner_off,
-                               &ctx->near.cell_off, &ctx->near_h, &ctx->near_v};
+                               &ctx->near.cell_off, &ctx->near_h, &ctx->near_v, &ctx->vsites};
   for (auto* b : bufs) b->release();
   jgm::prof_clear(ctx);
   delete ctx;
@@ -298,6 +308,10 @@ size_t jgm_data_per_site(const jgm_context* ctx, jgm_kind kind) {
 
 size_t jgm_n_params(const jgm_context* ctx) { return ctx ? jgm::nparams(ctx) : 0; }
 
+size_t jgm_model_size(const jgm_context* ctx, jgm_kind kind) {
+  return ctx && kind_ok(kind) ? jgm::nparams(ctx, kind) : 0;
+}
+
 jgm_status jgm_forward(jgm_context* ctx, jgm_kind kind, jgm_precision prec, const double* model,
                        double* out) {
   jgm_status st = check_ready(ctx, kind, prec);
@@ -330,7 +344,7 @@ jgm_status jgm_adjoint(jgm_context* ctx, jgm_kind kind, jgm_precision prec, cons
   const size_t kraw = jgm::raw_per_site(kind);
   hipError_t e;
   if (S == 0) {
-    e = jgm::launch_fill_zero(ctx, grad, jgm::nparams(ctx));
+    e = jgm::launch_fill_zero(ctx, grad, jgm::nparams(ctx, kind));
     if (e != hipSuccess) return hip_fail(e, "adjoint");
     return JGM_OK;
   }
@@ -368,7 +382,7 @@ jgm_status jgm_forward_host(jgm_context* ctx, jgm_kind kind, jgm_precision prec,
   if (st != JGM_OK) return st;
   if (!model || !out) return fail(JGM_ERR_INVALID_ARGUMENT, "null model or output pointer");
   DeviceGuard g(ctx->device);
-  const size_t np = jgm::nparams(ctx);
+  const size_t np = jgm::nparams(ctx, kind);
   const size_t nout = ctx->S * data_per_site(ctx, kind);
   hipError_t e;
   if ((e = ctx->model_stage.ensure(sizeof(double) * (np ? np : 1))) != hipSuccess)
@@ -393,7 +407,7 @@ jgm_status jgm_adjoint_host(jgm_context* ctx, jgm_kind kind, jgm_precision prec,
   if (st != JGM_OK) return st;
   if (!misfit || !grad) return fail(JGM_ERR_INVALID_ARGUMENT, "null misfit or gradient pointer");
   DeviceGuard g(ctx->device);
-  const size_t np = jgm::nparams(ctx);
+  const size_t np = jgm::nparams(ctx, kind);
   const size_t nmis = ctx->S * data_per_site(ctx, kind);
   jgm::DeviceBuffer dmis;  // separate from the internal misfit_stage used by transforms
   hipError_t e;
@@ -431,6 +445,8 @@ jgm_status jgm_sensitivity_rows(jgm_context* ctx, jgm_kind kind, size_t site0, s
                                 double* rows) {
   jgm_status st = check_ready(ctx, kind, JGM_FP64);
   if (st != JGM_OK) return st;
+  if (fp64_only(kind))
+    return fail(JGM_ERR_UNSUPPORTED, "no sensitivity rows for kind %d", (int)kind);
   if (!rows && nsites) return fail(JGM_ERR_INVALID_ARGUMENT, "null rows pointer");
   if (site0 > ctx->S || nsites > ctx->S - site0)
     return fail(JGM_ERR_INVALID_ARGUMENT, "site range [%zu, %zu) outside [0, %zu)", site0,
@@ -447,6 +463,7 @@ jgm_status jgm_sensitivity_rows(jgm_context* ctx, jgm_kind kind, size_t site0, s
 jgm_status jgm_storedj_build(jgm_context* ctx, jgm_kind kind, jgm_precision prec) {
   jgm_status st = check_ready(ctx, kind, prec);
   if (st != JGM_OK) return st;
+  if (fp64_only(kind)) return fail(JGM_ERR_UNSUPPORTED, "no stored J for kind %d", (int)kind);
   DeviceGuard g(ctx->device);
   hipError_t e = jgm::launch_storedj_build(ctx, kind, prec);
   if (e == hipErrorNotSupported) return fail(JGM_ERR_UNSUPPORTED, "stored J not available");

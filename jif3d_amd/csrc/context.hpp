@@ -121,7 +121,7 @@ struct jgm_context {
 
   // scratch
   jgm::DeviceBuffer weights, partial, raw, coef, hpart, hsum, model_stage, out_stage, misfit_stage,
-      grad_stage, farpart, nearpart, coef32, near_h, near_v;
+      grad_stage, farpart, nearpart, coef32, near_h, near_v, vsites;
 
   jgm::Profiler prof;
 
@@ -168,6 +168,8 @@ __host__ __device__ inline size_t raw_per_site(int kind) {
     case JGM_KIND_TENSOR: return 9;
     case JGM_KIND_MAGNETIC: return 3;
     case JGM_KIND_SCALAR_TENSOR: return 10;
+    case JGM_KIND_MAGNETIC_GRADIENT: return 1;
+    case JGM_KIND_MAGNETIZATION: return 3;
     default: return 0;
   }
 }
@@ -189,5 +191,10 @@ struct ProfScope {
 inline size_t ncells(const jgm_context* c) { return c->nx * c->ny * c->nz; }
 inline size_t ncorners(const jgm_context* c) { return (c->nx + 1) * (c->ny + 1) * (c->nz + 1); }
 inline size_t nparams(const jgm_context* c) { return ncells(c) + c->bgthick.size(); }
+// model vector length of a kind: three values per cell and no background layers for
+// the magnetization kind (ThreeDMagnetizationModel), cells then layers otherwise
+inline size_t nparams(const jgm_context* c, int kind) {
+  return kind == JGM_KIND_MAGNETIZATION ? 3 * ncells(c) : nparams(c);
+}
 
 }  // namespace jgm

@@ -468,6 +468,329 @@ hipError_t adjoint_impl(jgm_context* ctx, int kind, const double* rm, double* gr
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Magnetic vertical gradient (OMPMagneticGradImp.cpp:13-106) and magnetization
+// (OMPMagnetizationImp.cpp:25-117).  The vertical gradient is the difference of the
+// susceptibility Bz at the site and 1 m below it, so both positions run through the
+// tensor corner kernels as 2S virtual sites.  The magnetization model has three values
+// per cell, B = 1e9 mu0/(4 pi G) T M, evaluated corner-first with three corner weight
+// sets in one pass.
+// ---------------------------------------------------------------------------
+
+// Virtual sites: (x, y, z) for s < S and (x, y, z + dz) for s >= S.
+__global__ __launch_bounds__(kBlock) void k_pair_sites(const double* __restrict__ sx,
+                                                       const double* __restrict__ sy,
+                                                       const double* __restrict__ sz, int S,
+                                                       double dz, double* __restrict__ vs) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= 2 * S) return;
+  const int s = v < S ? v : v - S;
+  vs[v] = sx[s];
+  vs[2 * S + v] = sy[s];
+  vs[4 * S + v] = sz[s] + (v < S ? 0.0 : dz);
+}
+
+// Bz projection of the tensor kernel's components (xx, yy, zz, xy, xz, yz):
+// (T b)_z = Uxz bx + Uyz by + Uzz bz, times F mu0/(4 pi) / G = F 1e-7 / G.
+__global__ __launch_bounds__(kBlock) void k_maggrad_finish(const double* __restrict__ partial,
+                                                           int nchunks, int S, FieldDir fd,
+                                                           double fs, double* __restrict__ out,
+                                                           double* __restrict__ raw_out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const int S2 = 2 * S;
+  double bz[2];
+  for (int h = 0; h < 2; ++h) {
+    const int v = s + h * S;
+    double zz = 0.0, xz = 0.0, yz = 0.0;
+    for (int c = 0; c < nchunks; ++c) {
+      zz += partial[((long)c * 6 + 2) * S2 + v];
+      xz += partial[((long)c * 6 + 4) * S2 + v];
+      yz += partial[((long)c * 6 + 5) * S2 + v];
+    }
+    // tensor signs: diagonal +G, off-diagonal -G (BasicGravElements.cpp:151, :201)
+    bz[h] = (-kGravConst * xz * fd.bx - kGravConst * yz * fd.by + kGravConst * zz * fd.bz) *
+            fs * (1e-7 / kGravConst);
+  }
+  const double v = bz[0] - bz[1];
+  out[s] = v;
+  if (raw_out) raw_out[s] = v;
+}
+
+// Adjoint coefficients of the 2S virtual sites: +r at the site, -r 1 m below.
+__global__ __launch_bounds__(kBlock) void k_maggrad_coef(const double* __restrict__ rm, int S,
+                                                         FieldDir fd, double fs,
+                                                         double* __restrict__ coef) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= 2 * S) return;
+  const double r = (v < S ? rm[v] : -rm[v - S]) * fs * (1e-7 / kGravConst);
+  double* c = coef + (long)v * 6;  // xx, yy, zz, xy, xz, yz with their sign * G
+  c[0] = 0.0;
+  c[1] = 0.0;
+  c[2] = r * fd.bz * kGravConst;
+  c[3] = 0.0;
+  c[4] = -r * fd.bx * kGravConst;
+  c[5] = -r * fd.by * kGravConst;
+}
+
+// B_i = sum_j T_ij w_j over corners, T_ij = sign_ij F_ij (G and the unit factor
+// applied in the finish); w = three corner weight sets [3][ncols * ncz].
+__global__ __launch_bounds__(kBlock) void k_forward_corners_mag3(
+    const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
+    int S, const double* __restrict__ Xc, const double* __restrict__ Yc,
+    const double* __restrict__ Zc, int ncy, int ncz, const double* __restrict__ w, long nq,
+    long ncols, long cols_per_chunk, double* __restrict__ partial) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sl = s < S ? s : S - 1;
+  const double px = sx[sl], py = sy[sl], pz = sz[sl];
+  const long col0 = (long)blockIdx.y * cols_per_chunk;
+  const long col1 = col0 + cols_per_chunk < ncols ? col0 + cols_per_chunk : ncols;
+  double bx = 0.0, by = 0.0, bz = 0.0;
+  for (long col = col0; col < col1; ++col) {
+    const int I = (int)(col / ncy);
+    const int J = (int)(col - (long)I * ncy);
+    const double x = px - Xc[I];
+    const double y = py - Yc[J];
+    const double* wc = w + col * ncz;
+    for (int K = 0; K < ncz; ++K) {
+      const double z = pz - Zc[K];
+      const double wx = wc[K], wy = wc[nq + K], wz = wc[2 * nq + K];
+      double t[6];  // xx, yy, zz, xy, xz, yz
+      corner_terms_f64<kMaskTensor>(x, y, z, t);
+      bx += t[0] * wx - t[3] * wy - t[4] * wz;
+      by += -t[3] * wx + t[1] * wy - t[5] * wz;
+      bz += -t[4] * wx - t[5] * wy + t[2] * wz;
+    }
+  }
+  if (s < S) {
+    partial[((long)blockIdx.y * 3 + 0) * S + s] = bx;
+    partial[((long)blockIdx.y * 3 + 1) * S + s] = by;
+    partial[((long)blockIdx.y * 3 + 2) * S + s] = bz;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_mag3_finish(
+    const double* __restrict__ partial, int nchunks, int S, double scale, int tf, double tfbx,
+    double tfby, double tfbz, double* __restrict__ out, double* __restrict__ raw_out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  double raw[3] = {0.0, 0.0, 0.0};
+  for (int c = 0; c < nchunks; ++c)
+    for (int i = 0; i < 3; ++i) raw[i] += partial[((long)c * 3 + i) * S + s];
+  for (int i = 0; i < 3; ++i) raw[i] *= scale;
+  store_site(raw, 3, s, tf, tfbx, tfby, tfbz, out, raw_out);
+}
+
+// Adjoint corner sums of the three model components: H_j = sum_s sum_i c_i(s) T_ij.
+__global__ __launch_bounds__(kBlock) void k_adjoint_corners_mag3(
+    const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
+    const double* __restrict__ coef, int S, int sites_per_chunk,
+    const double* __restrict__ Xc, const double* __restrict__ Yc,
+    const double* __restrict__ Zc, int ncy, int ncz, long ncorner, double* __restrict__ hpart) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long ql = q < ncorner ? q : ncorner - 1;
+  const int K = (int)(ql % ncz);
+  const long t = ql / ncz;
+  const int J = (int)(t % ncy);
+  const int I = (int)(t / ncy);
+  const double cx = Xc[I], cy = Yc[J], cz = Zc[K];
+  const int s0 = blockIdx.y * sites_per_chunk;
+  const int s1 = s0 + sites_per_chunk < S ? s0 + sites_per_chunk : S;
+  double hx = 0.0, hy = 0.0, hz = 0.0;
+  for (int s = s0; s < s1; ++s) {
+    double tv[6];
+    corner_terms_f64<kMaskTensor>(sx[s] - cx, sy[s] - cy, sz[s] - cz, tv);
+    const double* c = coef + (long)s * 3;
+    hx += c[0] * tv[0] - c[1] * tv[3] - c[2] * tv[4];
+    hy += -c[0] * tv[3] + c[1] * tv[1] - c[2] * tv[5];
+    hz += -c[0] * tv[4] - c[1] * tv[5] + c[2] * tv[2];
+  }
+  if (q < ncorner) {
+    double* o = hpart + (long)blockIdx.y * 3 * ncorner + q;
+    o[0] = hx;
+    o[ncorner] = hy;
+    o[2 * ncorner] = hz;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_mag3_coef(const double* __restrict__ rm, int S,
+                                                      double scale, double* __restrict__ coef) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 3L * S) coef[i] = rm[i] * scale;
+}
+
+// chunking of a lane-per-site corner walk (as forward_impl)
+void site_chunks(long S, long ncols, long* nchunks, long* cols_per_chunk) {
+  const long site_blocks = (S + kBlock - 1) / kBlock;
+  const long waves_per_chunk = site_blocks * (kBlock / 64);
+  long want = (kTargetWaves + waves_per_chunk - 1) / waves_per_chunk;
+  want = std::max<long>(1, std::min<long>(want, ncols));
+  *cols_per_chunk = (ncols + want - 1) / want;
+  *nchunks = (ncols + *cols_per_chunk - 1) / *cols_per_chunk;
+}
+
+// chunking of a lane-per-corner walk over sites (as adjoint_impl)
+void corner_chunks(long S, long nq, int* nchunks, int* per) {
+  const long corner_waves = (nq + 63) / 64;
+  int n = (int)std::min<long>(S, std::max<long>(1, (kTargetWaves + corner_waves - 1) / corner_waves));
+  *per = (int)((S + n - 1) / n);
+  *nchunks = (int)((S + *per - 1) / *per);
+}
+
+double magnetization_scale() {
+  const double mu0 = 4e-7 * kPi;  // OMPMagnetizationImp.cpp:35
+  return 1.0e9 * mu0 / (4 * kPi * kGravConst) * kGravConst;  // factor times the G of T
+}
+
+hipError_t maggrad_forward(jgm_context* ctx, const double* model, double* out,
+                           double* raw_out) {
+  hipStream_t st = ctx->stream;
+  const int S = (int)ctx->S;
+  const int S2 = 2 * S;
+  const long N = (long)ncells(ctx);
+  const int ncy = (int)ctx->ny + 1, ncz = (int)ctx->nz + 1;
+  const long ncols = (long)((int)ctx->nx + 1) * ncy;
+  long nchunks = 0, per = 1;
+  hipError_t e;
+  if ((e = ctx->vsites.ensure(sizeof(double) * 6 * (size_t)S)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_pair_sites, dim3(blocks_for(S2)), dim3(kBlock), 0, st,
+                     ctx->d_sx.as<double>(), ctx->d_sy.as<double>(), ctx->d_sz.as<double>(), S,
+                     1.0, ctx->vsites.as<double>());
+  if (N > 0) {
+    site_chunks(S2, ncols, &nchunks, &per);
+    if ((e = ctx->weights.ensure(sizeof(double) * (size_t)ncols * ncz)) != hipSuccess) return e;
+    if ((e = ctx->partial.ensure(sizeof(double) * (size_t)nchunks * 6 * S2)) != hipSuccess)
+      return e;
+    hipLaunchKernelGGL(k_corner_weights, dim3(blocks_for(ncols * ncz)), dim3(kBlock), 0, st,
+                       model, (int)ctx->nx, (int)ctx->ny, (int)ctx->nz,
+                       ctx->weights.as<double>());
+    const double* vs = ctx->vsites.as<const double>();
+    ProfScope ps(ctx, "forward_corners_fp64_maggrad");
+    hipLaunchKernelGGL((k_forward_corners<kMaskTensor>), dim3((S2 + kBlock - 1) / kBlock, nchunks),
+                       dim3(kBlock), 0, st, vs, vs + S2, vs + 2 * S2, S2, ctx->d_Xc.as<double>(),
+                       ctx->d_Yc.as<double>(), ctx->d_Zc.as<double>(), ncy, ncz,
+                       ctx->weights.as<const double>(), ncols, per, ctx->partial.as<double>());
+  }
+  hipLaunchKernelGGL(k_maggrad_finish, dim3(blocks_for(S)), dim3(kBlock), 0, st,
+                     ctx->partial.as<const double>(), (int)nchunks, S, field_dir(ctx),
+                     ctx->fieldstrength, out, raw_out);
+  return hipGetLastError();
+}
+
+hipError_t maggrad_adjoint(jgm_context* ctx, const double* rm, double* grad) {
+  hipStream_t st = ctx->stream;
+  const int S = (int)ctx->S;
+  const int S2 = 2 * S;
+  const long N = (long)ncells(ctx);
+  const int L = (int)ctx->bgthick.size();
+  const int ncy = (int)ctx->ny + 1, ncz = (int)ctx->nz + 1;
+  const long nq = (long)ncorners(ctx);
+  hipError_t e;
+  if (L > 0) {  // OMPMagneticGradImp::CalcBackground returns zeros
+    if ((e = launch_fill_zero(ctx, grad + N, (size_t)L)) != hipSuccess) return e;
+  }
+  if (N == 0) return hipSuccess;
+  if ((e = ctx->vsites.ensure(sizeof(double) * 6 * (size_t)S)) != hipSuccess) return e;
+  if ((e = ctx->coef.ensure(sizeof(double) * 6 * (size_t)S2)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_pair_sites, dim3(blocks_for(S2)), dim3(kBlock), 0, st,
+                     ctx->d_sx.as<double>(), ctx->d_sy.as<double>(), ctx->d_sz.as<double>(), S,
+                     1.0, ctx->vsites.as<double>());
+  hipLaunchKernelGGL(k_maggrad_coef, dim3(blocks_for(S2)), dim3(kBlock), 0, st, rm, S,
+                     field_dir(ctx), ctx->fieldstrength, ctx->coef.as<double>());
+  int nchunks, per;
+  corner_chunks(S2, nq, &nchunks, &per);
+  if ((e = ctx->hpart.ensure(sizeof(double) * (size_t)nchunks * nq)) != hipSuccess) return e;
+  const double* vs = ctx->vsites.as<const double>();
+  {
+    ProfScope ps(ctx, "adjoint_corners_fp64_maggrad");
+    hipLaunchKernelGGL((k_adjoint_corners<kMaskTensor>), dim3(blocks_for(nq), nchunks),
+                       dim3(kBlock), 0, st, vs, vs + S2, vs + 2 * S2, ctx->coef.as<const double>(),
+                       S2, per, ctx->d_Xc.as<double>(), ctx->d_Yc.as<double>(),
+                       ctx->d_Zc.as<double>(), ncy, ncz, nq, ctx->hpart.as<double>());
+  }
+  const double* H = ctx->hpart.as<const double>();
+  if (nchunks > 1) {
+    if ((e = ctx->hsum.ensure(sizeof(double) * (size_t)nq)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sum_chunks, dim3(blocks_for(nq)), dim3(kBlock), 0, st, H, nchunks, nq,
+                       ctx->hsum.as<double>());
+    H = ctx->hsum.as<const double>();
+  }
+  hipLaunchKernelGGL(k_corner_difference, dim3(blocks_for(N)), dim3(kBlock), 0, st, H,
+                     (int)ctx->nx, (int)ctx->ny, (int)ctx->nz, grad);
+  return hipGetLastError();
+}
+
+hipError_t magnetization_forward(jgm_context* ctx, const double* model, double* out,
+                                 double* raw_out) {
+  hipStream_t st = ctx->stream;
+  const int S = (int)ctx->S;
+  const long N = (long)ncells(ctx);
+  const int ncy = (int)ctx->ny + 1, ncz = (int)ctx->nz + 1;
+  const long ncols = (long)((int)ctx->nx + 1) * ncy;
+  const long nq = ncols * ncz;
+  long nchunks = 0, per = 1;
+  hipError_t e;
+  if (N > 0) {
+    site_chunks(S, ncols, &nchunks, &per);
+    if ((e = ctx->weights.ensure(sizeof(double) * 3 * (size_t)nq)) != hipSuccess) return e;
+    if ((e = ctx->partial.ensure(sizeof(double) * (size_t)nchunks * 3 * S)) != hipSuccess)
+      return e;
+    for (int j = 0; j < 3; ++j)
+      hipLaunchKernelGGL(k_corner_weights, dim3(blocks_for(nq)), dim3(kBlock), 0, st,
+                         model + j * N, (int)ctx->nx, (int)ctx->ny, (int)ctx->nz,
+                         ctx->weights.as<double>() + j * nq);
+    ProfScope ps(ctx, "forward_corners_fp64_magnetization");
+    hipLaunchKernelGGL(k_forward_corners_mag3, dim3((S + kBlock - 1) / kBlock, nchunks),
+                       dim3(kBlock), 0, st, ctx->d_sx.as<double>(), ctx->d_sy.as<double>(),
+                       ctx->d_sz.as<double>(), S, ctx->d_Xc.as<double>(),
+                       ctx->d_Yc.as<double>(), ctx->d_Zc.as<double>(), ncy, ncz,
+                       ctx->weights.as<const double>(), nq, ncols, per,
+                       ctx->partial.as<double>());
+  }
+  const double tf_bx = std::cos(ctx->tf_inc) * std::cos(ctx->tf_dec);
+  const double tf_by = std::cos(ctx->tf_inc) * std::sin(ctx->tf_dec);
+  const double tf_bz = std::sin(ctx->tf_inc);
+  hipLaunchKernelGGL(k_mag3_finish, dim3(blocks_for(S)), dim3(kBlock), 0, st,
+                     ctx->partial.as<const double>(), (int)nchunks, S, magnetization_scale(),
+                     ctx->transform, tf_bx, tf_by, tf_bz, out, raw_out);
+  return hipGetLastError();
+}
+
+hipError_t magnetization_adjoint(jgm_context* ctx, const double* rm, double* grad) {
+  hipStream_t st = ctx->stream;
+  const int S = (int)ctx->S;
+  const long N = (long)ncells(ctx);
+  const int ncy = (int)ctx->ny + 1, ncz = (int)ctx->nz + 1;
+  const long nq = (long)ncorners(ctx);
+  if (N == 0) return hipSuccess;
+  hipError_t e;
+  if ((e = ctx->coef.ensure(sizeof(double) * 3 * (size_t)S)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_mag3_coef, dim3(blocks_for(3L * S)), dim3(kBlock), 0, st, rm, S,
+                     magnetization_scale(), ctx->coef.as<double>());
+  int nchunks, per;
+  corner_chunks(S, nq, &nchunks, &per);
+  if ((e = ctx->hpart.ensure(sizeof(double) * (size_t)nchunks * 3 * nq)) != hipSuccess) return e;
+  {
+    ProfScope ps(ctx, "adjoint_corners_fp64_magnetization");
+    hipLaunchKernelGGL(k_adjoint_corners_mag3, dim3(blocks_for(nq), nchunks), dim3(kBlock), 0, st,
+                       ctx->d_sx.as<double>(), ctx->d_sy.as<double>(), ctx->d_sz.as<double>(),
+                       ctx->coef.as<const double>(), S, per, ctx->d_Xc.as<double>(),
+                       ctx->d_Yc.as<double>(), ctx->d_Zc.as<double>(), ncy, ncz, nq,
+                       ctx->hpart.as<double>());
+  }
+  const double* H = ctx->hpart.as<const double>();
+  if (nchunks > 1) {
+    if ((e = ctx->hsum.ensure(sizeof(double) * 3 * (size_t)nq)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sum_chunks, dim3(blocks_for(3 * nq)), dim3(kBlock), 0, st, H, nchunks,
+                       3 * nq, ctx->hsum.as<double>());
+    H = ctx->hsum.as<const double>();
+  }
+  for (int j = 0; j < 3; ++j)
+    hipLaunchKernelGGL(k_corner_difference, dim3(blocks_for(N)), dim3(kBlock), 0, st, H + j * nq,
+                       (int)ctx->nx, (int)ctx->ny, (int)ctx->nz, grad + j * N);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t launch_background_gradient(jgm_context* ctx, int kind, const double* rm,
@@ -510,6 +833,8 @@ hipError_t launch_background_table(jgm_context* ctx) {
 
 hipError_t launch_forward_fp64(jgm_context* ctx, int kind, const double* model, double* out,
                                double* raw_out) {
+  if (kind == JGM_KIND_MAGNETIC_GRADIENT) return maggrad_forward(ctx, model, out, raw_out);
+  if (kind == JGM_KIND_MAGNETIZATION) return magnetization_forward(ctx, model, out, raw_out);
   switch (mask_of(kind)) {
     case kMaskScalar: return forward_impl<kMaskScalar>(ctx, kind, model, out, raw_out);
     case kMaskAll: return forward_impl<kMaskAll>(ctx, kind, model, out, raw_out);
@@ -530,6 +855,8 @@ hipError_t launch_misfit_to_raw(jgm_context* ctx, int kind, const double* misfit
 
 hipError_t launch_adjoint_fp64(jgm_context* ctx, int kind, const double* /*model*/,
                                const double* raw_misfit, double* grad) {
+  if (kind == JGM_KIND_MAGNETIC_GRADIENT) return maggrad_adjoint(ctx, raw_misfit, grad);
+  if (kind == JGM_KIND_MAGNETIZATION) return magnetization_adjoint(ctx, raw_misfit, grad);
   switch (mask_of(kind)) {
     case kMaskScalar: return adjoint_impl<kMaskScalar>(ctx, kind, raw_misfit, grad);
     case kMaskAll: return adjoint_impl<kMaskAll>(ctx, kind, raw_misfit, grad);

@@ -149,6 +149,28 @@ class HipMagneticSusceptibilityImp(HipGravMagImplementation):
         return (self.Inclination, self.Declination, self.FieldStrength)
 
 
+class HipMagneticGradImp(HipGravMagImplementation):
+    """OMPMagneticGradImp(Inc, Dec, Fs) (jif3D/Magnetics/OMPMagneticGradImp.cpp:13-118):
+    one value per site, Bz at the site minus Bz 1 m below, no background; FP64."""
+    kind = nat.KIND_MAGNETIC_GRADIENT
+
+    def __init__(self, Inc=0.0, Dec=0.0, Fs=1.0, device=0, stream=None):
+        super().__init__(nat.FP64, device, stream)
+        self.Inclination, self.Declination, self.FieldStrength = float(Inc), float(Dec), float(Fs)
+
+    def _field(self):
+        return (self.Inclination, self.Declination, self.FieldStrength)
+
+
+class HipMagnetizationImp(HipGravMagImplementation):
+    """OMPMagnetizationImp (jif3D/Magnetics/OMPMagnetizationImp.cpp:25-125): Bx, By, Bz
+    of a ThreeDMagnetizationModel (model vector [Mx; My; Mz], no background); FP64."""
+    kind = nat.KIND_MAGNETIZATION
+
+    def __init__(self, device=0, stream=None):
+        super().__init__(nat.FP64, device, stream)
+
+
 class ThreeDGravMagCalculator:
     """Calculator base (jif3D/GravMag/ThreeDGravMagCalculator.h:48-173)."""
 

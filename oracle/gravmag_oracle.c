@@ -41,8 +41,11 @@
  * (test_ScalarCalculation.cpp:275-306), finite differences of the misfit, the
  * transform test values (jif3D/Gravity/test_transforms.cpp), a brute-force
  * Gauss-Legendre quadrature of the Newton kernel on small boxes, and the
- * long double restatement.  Parity unpinned: a11 and a12 values beyond
- * self-consistency (the reference's *_vector.simpeg fixtures are absent).
+ * long double restatement.  a11 and a12 have no fixtures of their own (the
+ * reference's *_vector.simpeg files are absent); they are pinned through the
+ * SimPEG-pinned susceptibility imp: the vertical gradient equals 4 pi 1e-7 times the
+ * difference of its Bz at z and z + 1 m, and the magnetization imp with
+ * M = chi F b_hat 1e-9/mu0 (one axis at a time, then oblique) equals it.
  */
 #include <float.h>
 #include <math.h>

@@ -90,3 +90,61 @@ def test_magnetic_lqderivative(orc):
     bh = np.array([np.cos(inc) * np.cos(dec), np.cos(inc) * np.sin(dec), np.sin(inc)])
     Jt = np.einsum("k,skc->sc", bh, J.reshape(200, 3, -1))
     np.testing.assert_allclose(g, 2 * Jt.T @ r, rtol=1e-10, atol=1e-12 * np.abs(g).max())
+
+
+def shifted(sites, dz):
+    return (sites[0], sites[1], sites[2] + dz)
+
+
+def test_maggrad_is_vertical_difference_of_pinned_bz(orc):
+    """a11 (OMPMagneticGradImp.cpp:13-106) against the SimPEG-pinned susceptibility imp:
+    Bz(z) - Bz(z + 1 m) with mu0/(4 pi) = 1e-7 in place of 1/(4 pi), i.e.
+    MAGGRAD = 4 pi 1e-7 (Bz_sus(z) - Bz_sus(z + 1)).  A wrong tensor row, sign or shift
+    direction fails this."""
+    m, sites = simpeg_model()
+    m.SetSusceptibilities(np.random.default_rng(8).random(1000) * 0.1)
+    inc, dec, fs = 0.4, 0.9, 48000.0
+    mv = m.GetModelParameters()
+    g = orc.calculate(orc.MAGGRAD, m, mv, sites, field=(inc, dec, fs))
+    b0 = orc.calculate(orc.MAGNETIC, m, mv, sites, field=(inc, dec, fs)).reshape(-1, 3)
+    b1 = orc.calculate(orc.MAGNETIC, m, mv, shifted(sites, 1.0),
+                       field=(inc, dec, fs)).reshape(-1, 3)
+    ref = 4 * np.pi * 1e-7 * (b0[:, 2] - b1[:, 2])
+    np.testing.assert_allclose(g, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+    wrong = 4 * np.pi * 1e-7 * (b0[:, 2] - orc.calculate(
+        orc.MAGNETIC, m, mv, shifted(sites, -1.0), field=(inc, dec, fs)).reshape(-1, 3)[:, 2])
+    assert not np.allclose(g, wrong, rtol=1e-3)
+
+
+def test_magnetization_reduces_to_induced(orc):
+    """a12 (OMPMagnetizationImp.cpp:25-117): B = 1e9 mu0/(4 pi G) T M.  With
+    M = chi F b_hat 1e-9/mu0 per cell it must equal the SimPEG-pinned susceptibility imp;
+    one axis at a time (b_hat = x, y, z) checks the [Mx; My; Mz] column order."""
+    m, sites = simpeg_model()
+    n = 1000
+    chi = np.random.default_rng(9).random(n) * 0.1
+    m.SetSusceptibilities(chi)
+    mu0 = 4e-7 * np.pi
+    fs = 50000.0
+    for inc, dec in ((0.0, 0.0), (0.0, np.pi / 2), (np.pi / 2, 0.0), (0.35, 1.2)):
+        bh = np.array([np.cos(inc) * np.cos(dec), np.cos(inc) * np.sin(dec), np.sin(inc)])
+        M = np.concatenate([chi * fs * 1e-9 / mu0 * bh[i] for i in range(3)])
+        mz = orc.calculate(orc.MAGNETIZATION, m, M, sites)
+        ref = orc.calculate(orc.MAGNETIC, m, m.GetModelParameters(), sites, field=(inc, dec, fs))
+        np.testing.assert_allclose(mz, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+
+
+def test_maggrad_and_magnetization_lqderivative(orc):
+    """LQDerivative = 2 J^T r for both imps (test_MagneticObjective.cpp:177-209 style)."""
+    m, sites = simpeg_model()
+    rng = np.random.default_rng(10)
+    m.SetSusceptibilities(rng.random(1000) * 0.1)
+    field = (0.4, 0.9, 48000.0)
+    for kind, k, npar in ((orc.MAGGRAD, 1, 1000), (orc.MAGNETIZATION, 3, 3000)):
+        mv = rng.random(npar)
+        r = rng.normal(size=200 * k)
+        g = orc.lqderivative(kind, m, mv, sites, r, field=field)
+        _, J = orc.calculate(kind, m, mv, sites, field=field, want_sens=True)
+        np.testing.assert_allclose(g, 2 * J.T @ r, rtol=1e-10, atol=1e-12 * np.abs(g).max())
+        d = orc.calculate(kind, m, mv, sites, field=field)
+        np.testing.assert_allclose(d, J @ mv, rtol=1e-10, atol=1e-12 * np.abs(d).max())
