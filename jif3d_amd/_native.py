@@ -25,7 +25,8 @@ RAW_PER_SITE = {KIND_SCALAR: 1, KIND_TENSOR: 9, KIND_MAGNETIC: 3, KIND_SCALAR_TE
 EXPORTS = [
     "jgm_last_error", "jgm_api_version", "jgm_raw_per_site", "jgm_create", "jgm_destroy",
     "jgm_set_stream", "jgm_synchronize", "jgm_set_mesh", "jgm_set_background", "jgm_set_sites",
-    "jgm_set_field", "jgm_set_transform", "jgm_set_mixed_threshold", "jgm_data_per_site",
+    "jgm_set_field", "jgm_set_transform", "jgm_set_mixed_threshold", "jgm_set_mixed_far_radius",
+    "jgm_data_per_site",
     "jgm_n_params", "jgm_forward", "jgm_forward_host", "jgm_adjoint", "jgm_adjoint_host",
     "jgm_sensitivity_rows", "jgm_model_size", "jgm_storedj_build", "jgm_storedj_rows",
     "jgm_storedj_apply",
@@ -72,6 +73,7 @@ def load():
         "jgm_set_field": [vp, d, d, d],
         "jgm_set_transform": [vp, i, d, d],
         "jgm_set_mixed_threshold": [vp, d],
+        "jgm_set_mixed_far_radius": [vp, d],
         "jgm_forward": [vp, i, i, vp, vp],
         "jgm_forward_host": [vp, i, i, pd, pd],
         "jgm_adjoint": [vp, i, i, vp, vp, vp],
@@ -242,6 +244,9 @@ class Context:
 
     def set_mixed_threshold(self, q0):
         check(self.lib.jgm_set_mixed_threshold(self.handle, float(q0)))
+
+    def set_mixed_far_radius(self, q1):
+        check(self.lib.jgm_set_mixed_far_radius(self.handle, float(q1)))
 
     def data_per_site(self, kind):
         return self.lib.jgm_data_per_site(self.handle, kind)

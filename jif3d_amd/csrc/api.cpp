@@ -147,7 +147,9 @@ jgm_status jgm_destroy(jgm_context* ctx) {
                                &ctx->near.col_ptr, &ctx->near.col_sites, &ctx->near.col_ids,
                                &ctx->near.col_kmax, &ctx->near.xcen, &ctx->near.ycen,
                                &ctx->near.zcen, &ctx->near.adj_table, &ctx->near.corner_off,
-                               &ctx->near.cell_off, &ctx->near_h, &ctx->near_v, &ctx->vsites};
+                               &ctx->near.cell_off, &ctx->near_h, &ctx->near_v, &ctx->vsites, &ctx->coef_mono,
+                               &ctx->near.site_order, &ctx->near.mid_box, &ctx->near.xoff,
+                               &ctx->near.yoff, &ctx->near.zoff};
   for (auto* b : bufs) b->release();
   jgm::prof_clear(ctx);
   delete ctx;
@@ -299,6 +301,14 @@ jgm_status jgm_set_mixed_threshold(jgm_context* ctx, double q0) {
   if (!(q0 >= 2.0 && q0 <= 1e6)) return fail(JGM_ERR_INVALID_ARGUMENT, "q0 must be in [2, 1e6]");
   ctx->mixed_q0 = q0;
   ctx->near.valid = false;
+  return JGM_OK;
+}
+
+jgm_status jgm_set_mixed_far_radius(jgm_context* ctx, double q1) {
+  if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
+  if (!(q1 == 0.0 || (q1 >= 2.0 && q1 <= 1e6)))
+    return fail(JGM_ERR_INVALID_ARGUMENT, "q1 must be 0 or in [2, 1e6]");
+  ctx->mixed_q1 = q1;
   return JGM_OK;
 }
 

@@ -55,9 +55,16 @@ struct NearBoxes {
   bool valid = false;
   // far-field geometry: cell centres (FP64)
   DeviceBuffer xcen, ycen, zcen;
+  // FP32 offset of each cell centre from the centre of the first cell of its 4-cell
+  // tile run along the axis (exact for uniform meshes)
+  DeviceBuffer xoff, yoff, zoff;
   int mesh_class = 2;  // 0: cubes of one size, 1: one cell shape, 2: general
   DeviceBuffer adj_table;  // FP32 per-cell table with unit density (G V, B...)
   bool adj_table_valid = false;
+  DeviceBuffer site_order;  // int32 [S]: sites in Morton order of (x, y), for compact waves
+  DeviceBuffer mid_box;     // int32 [S][6]: cells within the second radius (hexadecapole kept)
+  double half_diag = 0.0;   // largest cell half-diagonal (m)
+  double mid_q1 = -1.0;     // second radius the mid boxes were built for
 };
 
 // Optional per-kernel timing: HIP events recorded on the context's stream around
@@ -111,8 +118,11 @@ struct jgm_context {
   int transform = JGM_TF_NONE;
   double tf_inc = 0.0, tf_dec = 0.0;
 
-  // mixed precision near-field threshold in cell half-diagonals
+  // mixed precision near-field threshold in cell half-diagonals, and the radius (also
+  // in half-diagonals) beyond which the far field keeps only monopole (and quadrupole)
+  // terms; 0 keeps the hexadecapole everywhere
   double mixed_q0 = 8.0;
+  double mixed_q1 = 24.0;
 
   // caches
   jgm::DeviceBuffer bgT;  // [S][L][7] background box terms, valid when bg_valid
@@ -121,7 +131,7 @@ struct jgm_context {
 
   // scratch
   jgm::DeviceBuffer weights, partial, raw, coef, hpart, hsum, model_stage, out_stage, misfit_stage,
-      grad_stage, farpart, nearpart, coef32, near_h, near_v, vsites;
+      grad_stage, farpart, nearpart, coef32, coef_mono, near_h, near_v, vsites;
 
   jgm::Profiler prof;
 

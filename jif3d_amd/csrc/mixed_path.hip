@@ -14,11 +14,18 @@
 // U_ij = G rho V d2Phi/dx_i dx_j.  Odd orders vanish by symmetry, so the neglected
 // term is O((h/d)^6) relative.  Cubes have B = 0.  The formulas are evaluated in
 // direction cosines u = x/r so that no power of 1/r below r^-7 is formed (FP32
-// range).  Each site-centre difference is formed in FP64 and rounded once, so the
-// FP32 rounding errors of different pairs are independent.  Accumulation per site
-// (forward) and per cell (adjoint) runs in FP32 over short runs, flushed to
-// FP64 every 4 cells (forward) or 8 sites (adjoint): the rounding error of an FP32
-// run grows with the run length times its partial sum.
+// range).  Beyond a second radius (q1 half-diagonals, default 24; the per-site "mid
+// box" holds every cell closer than that) the hexadecapole, O((h/d)^4) of a cell's
+// field there, is dropped; for cubes the remaining monopole is evaluated in a
+// regrouped form (running sums of w s^3 and w s^5 along a column of cells in the
+// forward, the site's coefficients folded into one quadratic form in the adjoint).
+// Site offsets are formed in FP64 against the first cell of a tile and rounded once
+// per tile (forward) or once per site and lane (adjoint).  Accumulation per site
+// (forward) and per cell (adjoint) runs in FP32 over short runs, flushed to FP64 every
+// 4 cells or 4 columns (forward) or 8 sites (adjoint): the rounding error of an FP32
+// run grows with the run length times its partial sum.  Forward waves take their 64
+// sites in Morton order of (x, y) so that a wave's sites cluster and tiles outside all
+// their near and mid boxes take the cheap path for the whole wave.
 //
 // Layouts in HBM
 //   cell table   FP32: uniform mesh [c] = {w}; otherwise [c][12] =
@@ -30,7 +37,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
+#include <cstdint>
 #include <vector>
 
 #include "context.hpp"
@@ -85,8 +94,9 @@ __host__ __device__ inline MpConst mp_const(double dx, double dy, double dz) {
 }
 
 // Field of a prism of unit G*rho*V at offset (X, Y, Z) = site - centre, in the
-// kernel component order of MASK.  QUAD = false drops the quadrupole (B = 0).
-template <int MASK, bool QUAD>
+// kernel component order of MASK.  QUAD = false drops the quadrupole (B = 0 for
+// cubes); HEXA = false drops the hexadecapole (used beyond the second radius).
+template <int MASK, bool QUAD, bool HEXA = true>
 __device__ __forceinline__ void multipole_f32(float X, float Y, float Z, const MpConst& c,
                                               float* T) {
   const float r2 = fmaf(X, X, fmaf(Y, Y, Z * Z));
@@ -99,34 +109,42 @@ __device__ __forceinline__ void multipole_f32(float X, float Y, float Z, const M
   const float s7 = s5 * s2;
   const float ux = X * s, uy = Y * s, uz = Z * s;
   const float vx = ux * ux, vy = uy * uy, vz = uz * uz;
-  const float hh = fmaf(c.a1 * vx, vx,
-                        fmaf(c.a2 * vy, vy,
-                             fmaf(c.a3 * vz, vz,
-                                  fmaf(c.b12 * vx, vy, fmaf(c.b13 * vx, vz, c.b23 * vy * vz)))));
-  float P = fmaf(99.0f * hh, s7, 3.0f * s3);   // coefficient of u_i u_j
-  float base = fmaf(-9.0f * hh, s7, -s3);      // isotropic diagonal part
-  float q = 0.0f;
+  float P = 3.0f * s3;  // coefficient of u_i u_j
+  float base = -s3;     // isotropic diagonal part
+  float hx = 0.0f, hy = 0.0f, hz = 0.0f;
+  if constexpr (HEXA) {
+    const float hh = fmaf(c.a1 * vx, vx,
+                          fmaf(c.a2 * vy, vy,
+                               fmaf(c.a3 * vz, vz,
+                                    fmaf(c.b12 * vx, vy, fmaf(c.b13 * vx, vz, c.b23 * vy * vz)))));
+    P = fmaf(99.0f * hh, s7, P);
+    base = fmaf(-9.0f * hh, s7, base);
+    const float t2b = 2.0f * c.b12, t2c = 2.0f * c.b13, t2d = 2.0f * c.b23;
+    hx = fmaf(4.0f * c.a1, vx, fmaf(t2b, vy, t2c * vz));
+    hy = fmaf(4.0f * c.a2, vy, fmaf(t2b, vx, t2d * vz));
+    hz = fmaf(4.0f * c.a3, vz, fmaf(t2c, vx, t2d * vy));
+  }
   if constexpr (QUAD) {
-    q = fmaf(c.Bx, vx, fmaf(c.By, vy, c.Bz * vz));
+    const float q = fmaf(c.Bx, vx, fmaf(c.By, vy, c.Bz * vz));
     P = fmaf(35.0f * q, s5, P);
     base = fmaf(-5.0f * q, s5, base);
   }
-  const float t2b = 2.0f * c.b12, t2c = 2.0f * c.b13, t2d = 2.0f * c.b23;
-  const float hx = fmaf(4.0f * c.a1, vx, fmaf(t2b, vy, t2c * vz));
-  const float hy = fmaf(4.0f * c.a2, vy, fmaf(t2b, vx, t2d * vz));
-  const float hz = fmaf(4.0f * c.a3, vz, fmaf(t2c, vx, t2d * vy));
   int k = 0;
   if constexpr ((MASK & kGz) != 0) {
-    float g = fmaf(hz, s7, base);
+    float g = base;
+    if constexpr (HEXA) g = fmaf(hz, s7, g);
     if constexpr (QUAD) g = fmaf(2.0f * c.Bz, s5, g);
     T[k++] = Z * g;
   }
   if constexpr ((MASK & kDiag) != 0) {
-    const float m18 = -18.0f * s7;
-    const float hxx = fmaf(8.0f * c.a1, vx, hx), hyy = fmaf(8.0f * c.a2, vy, hy),
-                hzz = fmaf(8.0f * c.a3, vz, hz);
-    float px = fmaf(hx, m18, P), py = fmaf(hy, m18, P), pz = fmaf(hz, m18, P);
-    float ox = fmaf(hxx, s7, base), oy = fmaf(hyy, s7, base), oz = fmaf(hzz, s7, base);
+    float px = P, py = P, pz = P, ox = base, oy = base, oz = base;
+    if constexpr (HEXA) {
+      const float m18 = -18.0f * s7;
+      const float hxx = fmaf(8.0f * c.a1, vx, hx), hyy = fmaf(8.0f * c.a2, vy, hy),
+                  hzz = fmaf(8.0f * c.a3, vz, hz);
+      px = fmaf(hx, m18, px); py = fmaf(hy, m18, py); pz = fmaf(hz, m18, pz);
+      ox = fmaf(hxx, s7, ox); oy = fmaf(hyy, s7, oy); oz = fmaf(hzz, s7, oz);
+    }
     if constexpr (QUAD) {
       const float m20 = -20.0f * s5, t2 = 2.0f * s5;
       px = fmaf(c.Bx, m20, px); py = fmaf(c.By, m20, py); pz = fmaf(c.Bz, m20, pz);
@@ -137,10 +155,13 @@ __device__ __forceinline__ void multipole_f32(float X, float Y, float Z, const M
     T[k++] = fmaf(vz, pz, oz);
   }
   if constexpr ((MASK & kOff) != 0) {
-    const float m9 = -9.0f * s7, f4 = 4.0f * s7;
-    float cxy = fmaf(hx + hy, m9, fmaf(c.b12, f4, P));
-    float cxz = fmaf(hx + hz, m9, fmaf(c.b13, f4, P));
-    float cyz = fmaf(hy + hz, m9, fmaf(c.b23, f4, P));
+    float cxy = P, cxz = P, cyz = P;
+    if constexpr (HEXA) {
+      const float m9 = -9.0f * s7, f4 = 4.0f * s7;
+      cxy = fmaf(hx + hy, m9, fmaf(c.b12, f4, P));
+      cxz = fmaf(hx + hz, m9, fmaf(c.b13, f4, P));
+      cyz = fmaf(hy + hz, m9, fmaf(c.b23, f4, P));
+    }
     if constexpr (QUAD) {
       const float m10 = -10.0f * s5;
       cxy = fmaf(c.Bx + c.By, m10, cxy);
@@ -202,6 +223,9 @@ struct FarGeom {
   const double* xcen;
   const double* ycen;
   const double* zcen;
+  const float* xoff;  // centre minus the centre of the first cell of its 4-run (FP32)
+  const float* yoff;
+  const float* zoff;
   int nx, ny, nz;
   int ntx, nty, ntz;
 };
@@ -211,91 +235,170 @@ struct TableWidth {
   static constexpr int value = MESH == kMeshGeneral ? kTableWidth : 1;
 };
 
-// Forward far field: one lane per site; each wave walks a chunk of 4x4x4 tiles.
-// Cells inside the lane's near box contribute nothing here (FP64 near pass).
-template <int MASK, int MESH>
-__global__ __launch_bounds__(kBlock) void k_far_forward(
-    const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
-    int S, const int* __restrict__ box, FarGeom g, const float* __restrict__ table,
-    MpConst uc, long ntiles, long tiles_per_chunk, double* __restrict__ farpart) {
+// Sum over the cells of one 4x4x4 tile for one lane (site at Xt, Yt, Zt from the
+// centre of the tile's first cell), flushed to FP64 per column of cells.  MASKED
+// skips the lane's near-box cells (evaluated at a harmless far point, weight 0).
+template <int MASK, int MESH, bool HEXA, bool MASKED>
+__device__ __forceinline__ void far_tile(const FarGeom& g, const float* __restrict__ table,
+                                         const MpConst& uc, int i0, int j0, int k0, int ni,
+                                         int nj, int nk, float Xt, float Yt, float Zt,
+                                         const int (&bx)[6], double* acc64) {
   constexpr int NC = Ncomp<MASK>::value;
   constexpr int TW = TableWidth<MESH>::value;
   constexpr bool QUAD = MESH != kMeshCubes;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const int sl = s < S ? s : S - 1;
-  const double px = sx[sl], py = sy[sl], pz = sz[sl];
-  const int b_i0 = box[sl * 6 + 0], b_i1 = box[sl * 6 + 1];
-  const int b_j0 = box[sl * 6 + 2], b_j1 = box[sl * 6 + 3];
-  const int b_k0 = box[sl * 6 + 4], b_k1 = box[sl * 6 + 5];
+  for (int ii = 0; ii < ni; ++ii) {
+    const float X = Xt - g.xoff[i0 + ii];
+    const bool inx = MASKED && (i0 + ii >= bx[0]) && (i0 + ii <= bx[1]);
+    for (int jj = 0; jj < nj; ++jj) {
+      const float Y = Yt - g.yoff[j0 + jj];
+      const bool inxy = inx && (j0 + jj >= bx[2]) && (j0 + jj <= bx[3]);
+      const long cb = ((long)(i0 + ii) * g.ny + (j0 + jj)) * g.nz + k0;
+      float acc[NC];
+#pragma unroll
+      for (int m = 0; m < NC; ++m) acc[m] = 0.0f;
+      for (int kk = 0; kk < nk; ++kk) {
+        const bool near = inxy && (k0 + kk >= bx[4]) && (k0 + kk <= bx[5]);
+        const float Z = near ? 1.0e6f : Zt - g.zoff[k0 + kk];
+        const MpConst cc = cell_const<MESH>(table, cb + kk, uc);
+        float T[NC];
+        multipole_f32<MASK, QUAD, HEXA>(X, Y, Z, cc, T);
+        const float w = near ? 0.0f : table[(cb + kk) * TW];
+#pragma unroll
+        for (int m = 0; m < NC; ++m) acc[m] = fmaf(w, T[m], acc[m]);
+      }
+#pragma unroll
+      for (int m = 0; m < NC; ++m) acc64[m] += (double)acc[m];
+    }
+  }
+}
+
+// Monopole-only sum over a tile of cubes.  Along a column of cells X and Y are
+// fixed, so sum_k w T_m needs five running sums per column,
+//   S3 = sum w s^3, S5 = sum w s^5, SZ3 = sum w Z s^3, SZ5 = sum w Z s^5,
+//   SZZ5 = sum w Z^2 s^5            (s = 1/r),
+// and gz = -SZ3, Txx = 3X^2 S5 - S3, Tyy = 3Y^2 S5 - S3, Tzz = 3 SZZ5 - S3,
+// Txy = 3XY S5, Txz = 3X SZ5, Tyz = 3Y SZ5: the monopole terms of multipole_f32
+// regrouped, about half the operations per cell.
+template <int MASK>
+__device__ __forceinline__ void far_tile_mono(const FarGeom& g, const float* __restrict__ table,
+                                              int i0, int j0, int k0, int ni, int nj, int nk,
+                                              float Xt, float Yt, float Zt, double* acc64) {
+  constexpr int NC = Ncomp<MASK>::value;
+  constexpr bool GZ = (MASK & kGz) != 0, DIAG = (MASK & kDiag) != 0, OFF = (MASK & kOff) != 0;
+  // z offsets of the tile's cells; a cell past the mesh edge sits far away with weight 0
+  float Zc[kTile];
+  bool kv[kTile];
+#pragma unroll
+  for (int kk = 0; kk < kTile; ++kk) {
+    kv[kk] = kk < nk;
+    Zc[kk] = kv[kk] ? Zt - g.zoff[k0 + kk] : 1.0e6f;
+  }
+  for (int ii = 0; ii < ni; ++ii) {
+    const float X = Xt - g.xoff[i0 + ii];
+    const float X3 = 3.0f * X;
+    float row[NC];  // FP32 sums of this ii's columns, flushed to FP64 per ii
+#pragma unroll
+    for (int m = 0; m < NC; ++m) row[m] = 0.0f;
+    for (int jj = 0; jj < nj; ++jj) {
+      const float Y = Yt - g.yoff[j0 + jj];
+      const float XXYY = fmaf(X, X, Y * Y);
+      const float* wc = table + ((long)(i0 + ii) * g.ny + (j0 + jj)) * g.nz + k0;
+      float S3 = 0.0f, S5 = 0.0f, SZ3 = 0.0f, SZ5 = 0.0f, SZZ5 = 0.0f;
+#pragma unroll
+      for (int kk = 0; kk < kTile; ++kk) {
+        const float Z = Zc[kk];
+        const float ZZ = Z * Z;
+        const float r2 = XXYY + ZZ;
+        float sv = __builtin_amdgcn_rsqf(r2);
+        const float e = fmaf(-(r2 * sv), sv, 1.0f);  // one Newton step
+        sv = fmaf(0.5f * sv, e, sv);
+        const float s2 = sv * sv;
+        const float w = kv[kk] ? wc[kk] : 0.0f;
+        const float q3 = w * (s2 * sv);
+        if constexpr (GZ) SZ3 = fmaf(Z, q3, SZ3);
+        if constexpr (DIAG || OFF) {
+          const float q5 = q3 * s2;
+          S5 += q5;
+          if constexpr (DIAG) {
+            S3 += q3;
+            SZZ5 = fmaf(ZZ, q5, SZZ5);
+          }
+          if constexpr (OFF) SZ5 = fmaf(Z, q5, SZ5);
+        }
+      }
+      int k = 0;
+      const float Y3 = 3.0f * Y;
+      if constexpr (GZ) row[k++] -= SZ3;
+      if constexpr (DIAG) {
+        row[k] = fmaf(X3 * X, S5, row[k] - S3), ++k;
+        row[k] = fmaf(Y3 * Y, S5, row[k] - S3), ++k;
+        row[k] = fmaf(3.0f, SZZ5, row[k] - S3), ++k;
+      }
+      if constexpr (OFF) {
+        row[k] = fmaf(X3 * Y, S5, row[k]), ++k;
+        row[k] = fmaf(X3, SZ5, row[k]), ++k;
+        row[k] = fmaf(Y3, SZ5, row[k]), ++k;
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < NC; ++m) acc64[m] += (double)row[m];
+  }
+}
+
+// Forward far field: one lane per site; each wave walks a chunk of 4x4x4 tiles.
+// Lanes take sites in `order` (spatially compact waves).  Cells inside the lane's
+// near box contribute nothing here (FP64 near pass).  A tile outside the mid box of
+// every site of the wave (the box holding all cells within the second radius) keeps
+// only the monopole (and quadrupole) terms.  Site offsets are formed in FP64 against
+// the tile's first cell and rounded once per tile; cell offsets within a tile are exact
+// for uniform meshes.
+template <int MASK, int MESH>
+__global__ __launch_bounds__(kBlock) void k_far_forward(
+    const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
+    int S, const int* __restrict__ order, const int* __restrict__ box,
+    const int* __restrict__ mid, FarGeom g, const float* __restrict__ table, MpConst uc,
+    long ntiles, long tiles_per_chunk, double* __restrict__ farpart) {
+  constexpr int NC = Ncomp<MASK>::value;
+  const int li = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = order[li < S ? li : S - 1];
+  const double px = sx[s], py = sy[s], pz = sz[s];
+  int bx[6], mb[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    bx[q] = box[s * 6 + q];
+    mb[q] = mid[s * 6 + q];
+  }
   const long t0 = (long)blockIdx.y * tiles_per_chunk;
   const long t1 = t0 + tiles_per_chunk < ntiles ? t0 + tiles_per_chunk : ntiles;
   double acc64[NC];
 #pragma unroll
   for (int m = 0; m < NC; ++m) acc64[m] = 0.0;
-  const int ny = g.ny, nz = g.nz;
   for (long t = t0; t < t1; ++t) {
     const int tk = (int)(t % g.ntz);
     const long tt = t / g.ntz;
     const int tj = (int)(tt % g.nty);
     const int ti = (int)(tt / g.nty);
     const int i0 = ti * kTile, j0 = tj * kTile, k0 = tk * kTile;
-    const int ni = min(kTile, g.nx - i0), nj = min(kTile, ny - j0), nk = min(kTile, nz - k0);
-    const bool hit = !(b_i1 < i0 || b_i0 > i0 + ni - 1 || b_j1 < j0 || b_j0 > j0 + nj - 1 ||
-                       b_k1 < k0 || b_k0 > k0 + nk - 1);
-    float acc[NC];  // FP32 partial of one tile column (4 cells), flushed to FP64
-#pragma unroll
-    for (int m = 0; m < NC; ++m) acc[m] = 0.0f;
-    if (!__any(hit)) {
-      for (int ii = 0; ii < ni; ++ii) {
-        const float X = (float)(px - g.xcen[i0 + ii]);
-        for (int jj = 0; jj < nj; ++jj) {
-          const float Y = (float)(py - g.ycen[j0 + jj]);
-          const long cb = ((long)(i0 + ii) * ny + (j0 + jj)) * nz + k0;
-          for (int kk = 0; kk < nk; ++kk) {
-            const float Z = (float)(pz - g.zcen[k0 + kk]);
-            const MpConst cc = cell_const<MESH>(table, cb + kk, uc);
-            float T[NC];
-            multipole_f32<MASK, QUAD>(X, Y, Z, cc, T);
-            const float w = table[(cb + kk) * TW];
-#pragma unroll
-            for (int m = 0; m < NC; ++m) acc[m] = fmaf(w, T[m], acc[m]);
-          }
-#pragma unroll
-          for (int m = 0; m < NC; ++m) {
-            acc64[m] += (double)acc[m];
-            acc[m] = 0.0f;
-          }
-        }
-      }
-    } else {
-      for (int ii = 0; ii < ni; ++ii) {
-        const float X = (float)(px - g.xcen[i0 + ii]);
-        const bool inx = (i0 + ii >= b_i0) && (i0 + ii <= b_i1);
-        for (int jj = 0; jj < nj; ++jj) {
-          const float Y = (float)(py - g.ycen[j0 + jj]);
-          const bool inxy = inx && (j0 + jj >= b_j0) && (j0 + jj <= b_j1);
-          const long cb = ((long)(i0 + ii) * ny + (j0 + jj)) * nz + k0;
-          for (int kk = 0; kk < nk; ++kk) {
-            const bool near = inxy && (k0 + kk >= b_k0) && (k0 + kk <= b_k1);
-            // a near cell is evaluated at a harmless far point and weighted by zero
-            const float Z = near ? 1.0e6f : (float)(pz - g.zcen[k0 + kk]);
-            const MpConst cc = cell_const<MESH>(table, cb + kk, uc);
-            float T[NC];
-            multipole_f32<MASK, QUAD>(X, Y, Z, cc, T);
-            const float w = near ? 0.0f : table[(cb + kk) * TW];
-#pragma unroll
-            for (int m = 0; m < NC; ++m) acc[m] = fmaf(w, T[m], acc[m]);
-          }
-#pragma unroll
-          for (int m = 0; m < NC; ++m) {
-            acc64[m] += (double)acc[m];
-            acc[m] = 0.0f;
-          }
-        }
-      }
-    }
+    const int ni = min(kTile, g.nx - i0), nj = min(kTile, g.ny - j0), nk = min(kTile, g.nz - k0);
+    const float Xt = (float)(px - g.xcen[i0]), Yt = (float)(py - g.ycen[j0]),
+                Zt = (float)(pz - g.zcen[k0]);
+    const bool hit = !(bx[1] < i0 || bx[0] > i0 + ni - 1 || bx[3] < j0 || bx[2] > j0 + nj - 1 ||
+                       bx[5] < k0 || bx[4] > k0 + nk - 1);
+    const bool need = !(mb[1] < i0 || mb[0] > i0 + ni - 1 || mb[3] < j0 ||
+                        mb[2] > j0 + nj - 1 || mb[5] < k0 || mb[4] > k0 + nk - 1);
+    if (__any(hit))
+      far_tile<MASK, MESH, true, true>(g, table, uc, i0, j0, k0, ni, nj, nk, Xt, Yt, Zt, bx,
+                                       acc64);
+    else if (__any(need))
+      far_tile<MASK, MESH, true, false>(g, table, uc, i0, j0, k0, ni, nj, nk, Xt, Yt, Zt, bx,
+                                        acc64);
+    else if constexpr (MESH == kMeshCubes)
+      far_tile_mono<MASK>(g, table, i0, j0, k0, ni, nj, nk, Xt, Yt, Zt, acc64);
+    else
+      far_tile<MASK, MESH, false, false>(g, table, uc, i0, j0, k0, ni, nj, nk, Xt, Yt, Zt, bx,
+                                         acc64);
   }
-  if (s < S) {
+  if (li < S) {
 #pragma unroll
     for (int m = 0; m < NC; ++m) farpart[((long)blockIdx.y * NC + m) * S + s] = acc64[m];
   }
@@ -384,7 +487,8 @@ template <int MASK>
 __global__ __launch_bounds__(kBlock) void k_mixed_coef(const double* __restrict__ rm, int S,
                                                        int kind, FieldDir fd,
                                                        double* __restrict__ c64,
-                                                       float* __restrict__ c32) {
+                                                       float* __restrict__ c32,
+                                                       float* __restrict__ mono) {
   constexpr int NC = Ncomp<MASK>::value;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S) return;
@@ -396,66 +500,140 @@ __global__ __launch_bounds__(kBlock) void k_mixed_coef(const double* __restrict_
     c64[(long)s * NC + m] = c[m] * sc[m];
     c32[(long)s * NC + m] = (float)c[m];
   }
+  // monopole form of sum_m c_m T_m = s^5 Q(X, Y, Z) - s^3 (c_gz Z + c_xx + c_yy + c_zz),
+  // Q = 3 (c_xx X^2 + c_yy Y^2 + c_zz Z^2 + c_xy XY + c_xz XZ + c_yz YZ)
+  float* o = mono + (long)s * 8;
+  o[0] = (float)c7[0];
+  for (int m = 1; m < 7; ++m) o[m] = (float)(3.0 * c7[m]);
+  o[7] = (float)(c7[1] + c7[2] + c7[3]);
 }
 
-// Adjoint far field: one lane per cell of a 4x4x4 tile, the wave walks a chunk
-// of sites (wave-uniform); sites whose near box holds the cell are skipped.
+// Adjoint far field: one lane per kQ cells (k, k + 4, k + 8, ...) of a 4x4x(4 kQ) tile;
+// the wave walks a chunk of sites (wave-uniform) and accumulates
+// sum_m coef_m(s) T_m(s, c) in FP32, flushed to FP64 every 8 sites; cells in the site's
+// near box are skipped (FP64 near pass).  Site offsets are formed in FP64 against the
+// tile's first cell once per site and lane and shared by the lane's cells; a site whose
+// mid box misses the tile keeps only the monopole (and quadrupole) terms, a
+// wave-uniform branch.
+constexpr int kQ = 4;  // cells per lane of the far adjoint
 template <int MASK, int MESH>
 __global__ __launch_bounds__(kBlock) void k_far_adjoint(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
-    const float* __restrict__ coef, const int* __restrict__ box, int S, int sites_per_chunk,
-    FarGeom g, const float* __restrict__ table, MpConst uc, long ntiles,
-    double* __restrict__ gpart) {
+    const float* __restrict__ coef, const float* __restrict__ mono, const int* __restrict__ box,
+    const int* __restrict__ mid, int S, int sites_per_chunk, FarGeom g,
+    const float* __restrict__ table, MpConst uc, long ntiles, double* __restrict__ gpart) {
   constexpr int NC = Ncomp<MASK>::value;
   constexpr int TW = TableWidth<MESH>::value;
   constexpr bool QUAD = MESH != kMeshCubes;
+  constexpr int KT = kQ * kTile;  // tile depth in k
   const long t = ((long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
   const int lane = threadIdx.x & 63;
   if (t >= ntiles) return;
-  const int tk = (int)(t % g.ntz);
-  const long tt = t / g.ntz;
+  const int ntz = (g.nz + KT - 1) / KT;
+  const int tk = (int)(t % ntz);
+  const long tt = t / ntz;
   const int tj = (int)(tt % g.nty);
   const int ti = (int)(tt / g.nty);
-  const int i0 = ti * kTile, j0 = tj * kTile, k0 = tk * kTile;
-  const int ni = min(kTile, g.nx - i0), nj = min(kTile, g.ny - j0), nk = min(kTile, g.nz - k0);
+  const int i0 = ti * kTile, j0 = tj * kTile, k0 = tk * KT;
+  const int ni = min(kTile, g.nx - i0), nj = min(kTile, g.ny - j0), nk = min(KT, g.nz - k0);
   const int ii = lane >> 4, jj = (lane >> 2) & 3, kk = lane & 3;
-  const bool valid = ii < ni && jj < nj && kk < nk;
-  const int i = i0 + (valid ? ii : 0), j = j0 + (valid ? jj : 0), k = k0 + (valid ? kk : 0);
-  const long c = ((long)i * g.ny + j) * g.nz + k;
-  const double cx = g.xcen[i], cy = g.ycen[j], cz = g.zcen[k];
-  const MpConst cc = cell_const<MESH>(table, c, uc);
+  const bool vij = ii < ni && jj < nj;
+  const int i = i0 + (vij ? ii : 0), j = j0 + (vij ? jj : 0);
+  const double x0 = g.xcen[i0], y0 = g.ycen[j0], z0 = g.zcen[k0];
+  const float ox = (float)(g.xcen[i] - x0), oy = (float)(g.ycen[j] - y0);
+  bool vq[kQ];
+  int kq[kQ];
+  long cq[kQ];
+  float ozq[kQ];
+  MpConst ccq[kQ];
+#pragma unroll
+  for (int q = 0; q < kQ; ++q) {
+    vq[q] = vij && kk + q * kTile < nk;
+    kq[q] = k0 + (vq[q] ? kk + q * kTile : 0);
+    cq[q] = ((long)i * g.ny + j) * g.nz + kq[q];
+    ozq[q] = (float)(g.zcen[kq[q]] - z0);
+    ccq[q] = cell_const<MESH>(table, cq[q], uc);
+  }
   const int s0 = blockIdx.y * sites_per_chunk;
   const int s1 = s0 + sites_per_chunk < S ? s0 + sites_per_chunk : S;
-  double acc64 = 0.0;
-  float acc = 0.0f;
+  double a64[kQ];
+  float acc[kQ];
+#pragma unroll
+  for (int q = 0; q < kQ; ++q) {
+    a64[q] = 0.0;
+    acc[q] = 0.0f;
+  }
   int nacc = 0;
   for (int s = s0; s < s1; ++s) {
     const int* bx = box + s * 6;
+    const int* mb = mid + s * 6;
     const bool tile_hit = !(bx[1] < i0 || bx[0] > i0 + ni - 1 || bx[3] < j0 ||
                             bx[2] > j0 + nj - 1 || bx[5] < k0 || bx[4] > k0 + nk - 1);
-    bool near = false;
-    if (tile_hit)
-      near = i >= bx[0] && i <= bx[1] && j >= bx[2] && j <= bx[3] && k >= bx[4] && k <= bx[5];
-    // site - centre in FP64, rounded once per pair
-    const float X = (float)(sx[s] - cx), Y = (float)(sy[s] - cy);
-    const float Z = near ? 1.0e6f : (float)(sz[s] - cz);
-    float T[NC];
-    multipole_f32<MASK, QUAD>(X, Y, Z, cc, T);
-    const float* cs = coef + (long)s * NC;
-    float v = 0.0f;
+    const bool need = !(mb[1] < i0 || mb[0] > i0 + ni - 1 || mb[3] < j0 ||
+                        mb[2] > j0 + nj - 1 || mb[5] < k0 || mb[4] > k0 + nk - 1);
+    bool nq[kQ];
 #pragma unroll
-    for (int m = 0; m < NC; ++m) v = fmaf(cs[m], T[m], v);
-    acc += near ? 0.0f : v;
+    for (int q = 0; q < kQ; ++q) nq[q] = false;
+    if (tile_hit) {
+      const bool inxy = i >= bx[0] && i <= bx[1] && j >= bx[2] && j <= bx[3];
+#pragma unroll
+      for (int q = 0; q < kQ; ++q) nq[q] = inxy && kq[q] >= bx[4] && kq[q] <= bx[5];
+    }
+    const float Xt = (float)(sx[s] - x0), Yt = (float)(sy[s] - y0), Zt = (float)(sz[s] - z0);
+    const float X = Xt - ox, Y = Yt - oy;
+    float Zq[kQ], pq[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) Zq[q] = nq[q] ? 1.0e6f : Zt - ozq[q];
+    if (MESH == kMeshCubes && !need) {
+      // monopole: s^5 Q - s^3 (c_gz Z + C) with the site's regrouped coefficients
+      const float* mc = mono + (long)s * 8;
+      const float XX = X * X, YY = Y * Y;
+      const float XXYY = XX + YY;
+      const float Qxy = fmaf(mc[1], XX, fmaf(mc[2], YY, mc[4] * (X * Y)));
+      const float Lz = fmaf(mc[5], X, mc[6] * Y);
+#pragma unroll
+      for (int q = 0; q < kQ; ++q) {
+        const float Z = Zq[q];
+        const float r2 = fmaf(Z, Z, XXYY);
+        float sv = __builtin_amdgcn_rsqf(r2);
+        const float e = fmaf(-(r2 * sv), sv, 1.0f);  // one Newton step
+        sv = fmaf(0.5f * sv, e, sv);
+        const float s2 = sv * sv, s3 = s2 * sv;
+        const float Q = fmaf(Z, fmaf(mc[3], Z, Lz), Qxy);
+        pq[q] = fmaf(Q, s3 * s2, -s3 * fmaf(mc[0], Z, mc[7]));
+      }
+    } else {
+      const float* cs = coef + (long)s * NC;
+#pragma unroll
+      for (int q = 0; q < kQ; ++q) {
+        float T[NC];
+        if (need)
+          multipole_f32<MASK, QUAD, true>(X, Y, Zq[q], ccq[q], T);
+        else
+          multipole_f32<MASK, QUAD, false>(X, Y, Zq[q], ccq[q], T);
+        float p = 0.0f;
+#pragma unroll
+        for (int m = 0; m < NC; ++m) p = fmaf(cs[m], T[m], p);
+        pq[q] = p;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) acc[q] += nq[q] ? 0.0f : pq[q];
     if (++nacc == 8) {
-      acc64 += (double)acc;
-      acc = 0.0f;
+#pragma unroll
+      for (int q = 0; q < kQ; ++q) {
+        a64[q] += (double)acc[q];
+        acc[q] = 0.0f;
+      }
       nacc = 0;
     }
   }
-  acc64 += (double)acc;
-  if (valid) {
-    const double w = (double)table[c * TW];  // G * V
-    gpart[(long)blockIdx.y * ((long)g.nx * g.ny * g.nz) + c] = 2.0 * w * acc64;
+  const long ncell = (long)g.nx * g.ny * g.nz;
+#pragma unroll
+  for (int q = 0; q < kQ; ++q) {
+    a64[q] += (double)acc[q];
+    if (vq[q])  // G V
+      gpart[(long)blockIdx.y * ncell + cq[q]] = 2.0 * (double)table[cq[q] * TW] * a64[q];
   }
 }
 
@@ -564,6 +742,9 @@ FarGeom far_geom(const jgm_context* ctx) {
   g.xcen = ctx->near.xcen.as<const double>();
   g.ycen = ctx->near.ycen.as<const double>();
   g.zcen = ctx->near.zcen.as<const double>();
+  g.xoff = ctx->near.xoff.as<const float>();
+  g.yoff = ctx->near.yoff.as<const float>();
+  g.zoff = ctx->near.zoff.as<const float>();
   g.nx = (int)ctx->nx;
   g.ny = (int)ctx->ny;
   g.nz = (int)ctx->nz;
@@ -576,6 +757,37 @@ FarGeom far_geom(const jgm_context* ctx) {
 MpConst uniform_const(const jgm_context* ctx) {
   if (ctx->nx == 0 || ctx->ny == 0 || ctx->nz == 0) return MpConst{};
   return mp_const(ctx->xs[0], ctx->ys[0], ctx->zs[0]);
+}
+
+// Sites sorted by the Morton code of their (x, y) position in the survey's bounding
+// box, so that the 64 lanes of a far-field wave cover a compact patch of sites.
+std::vector<int> morton_order(const jgm_context* ctx) {
+  const size_t S = ctx->S;
+  std::vector<int> order(std::max<size_t>(1, S), 0);
+  if (S == 0) return order;
+  double x0 = ctx->hsx[0], x1 = x0, y0 = ctx->hsy[0], y1 = y0;
+  for (size_t s = 0; s < S; ++s) {
+    x0 = std::min(x0, ctx->hsx[s]); x1 = std::max(x1, ctx->hsx[s]);
+    y0 = std::min(y0, ctx->hsy[s]); y1 = std::max(y1, ctx->hsy[s]);
+  }
+  const double sx = x1 > x0 ? 65535.0 / (x1 - x0) : 0.0, sy = y1 > y0 ? 65535.0 / (y1 - y0) : 0.0;
+  auto spread = [](uint32_t v) {  // 16 bits -> even bit positions
+    v &= 0xFFFF;
+    v = (v | (v << 8)) & 0x00FF00FF;
+    v = (v | (v << 4)) & 0x0F0F0F0F;
+    v = (v | (v << 2)) & 0x33333333;
+    v = (v | (v << 1)) & 0x55555555;
+    return v;
+  };
+  std::vector<uint32_t> code(S);
+  for (size_t s = 0; s < S; ++s) {
+    const uint32_t qx = (uint32_t)((ctx->hsx[s] - x0) * sx), qy = (uint32_t)((ctx->hsy[s] - y0) * sy);
+    code[s] = spread(qx) | (spread(qy) << 1);
+    order[s] = (int)s;
+  }
+  std::stable_sort(order.begin(), order.begin() + S,
+                   [&](int a, int b) { return code[a] < code[b]; });
+  return order;
 }
 
 template <class T>
@@ -635,7 +847,8 @@ hipError_t forward_mixed_impl(jgm_context* ctx, int kind, const double* model, d
       ProfScope ps(ctx, far_name(MASK, false));
       hipLaunchKernelGGL((k_far_forward<MASK, MESH>), dim3(site_blocks, nchunks), dim3(kBlock), 0,
                          st, ctx->d_sx.as<double>(), ctx->d_sy.as<double>(),
-                         ctx->d_sz.as<double>(), S, ctx->near.box.as<const int>(), g,
+                         ctx->d_sz.as<double>(), S, ctx->near.site_order.as<const int>(),
+                         ctx->near.box.as<const int>(), ctx->near.mid_box.as<const int>(), g,
                          ctx->weights.as<const float>(), uniform_const(ctx), ntiles, per,
                          ctx->farpart.as<double>());
     }
@@ -674,8 +887,10 @@ hipError_t adjoint_mixed_impl(jgm_context* ctx, int kind, const double* rm, doub
   if (N > 0) {
     if ((e = ctx->coef.ensure(sizeof(double) * (size_t)S * NC)) != hipSuccess) return e;
     if ((e = ctx->coef32.ensure(sizeof(float) * (size_t)S * NC)) != hipSuccess) return e;
+    if ((e = ctx->coef_mono.ensure(sizeof(float) * (size_t)S * 8)) != hipSuccess) return e;
     hipLaunchKernelGGL((k_mixed_coef<MASK>), dim3(blocks_for(S)), dim3(kBlock), 0, st, rm, S,
-                       kind, field_dir(ctx), ctx->coef.as<double>(), ctx->coef32.as<float>());
+                       kind, field_dir(ctx), ctx->coef.as<double>(), ctx->coef32.as<float>(),
+                       ctx->coef_mono.as<float>());
     if (!nb.adj_table_valid) {
       if ((e = nb.adj_table.ensure(sizeof(float) * (size_t)N * TableWidth<MESH>::value)) !=
           hipSuccess)
@@ -686,19 +901,23 @@ hipError_t adjoint_mixed_impl(jgm_context* ctx, int kind, const double* rm, doub
                          g.nz, nb.adj_table.as<float>());
       nb.adj_table_valid = true;
     }
+    // the adjoint walks 4x4x(4 kQ) tiles, kQ cells per lane
+    const long ntiles8 = (long)g.ntx * g.nty * ((g.nz + kQ * kTile - 1) / (kQ * kTile));
     int nchunks = (int)std::min<long>(
-        S, std::max<long>(1, (kTargetWaves + ntiles - 1) / ntiles));
+        S, std::max<long>(1, (kTargetWaves + ntiles8 - 1) / ntiles8));
     const int per = (S + nchunks - 1) / nchunks;
     nchunks = (S + per - 1) / per;
     if ((e = ctx->hpart.ensure(sizeof(double) * (size_t)nchunks * N)) != hipSuccess) return e;
     if ((e = ctx->hsum.ensure(sizeof(double) * (size_t)N)) != hipSuccess) return e;
     {
       ProfScope ps(ctx, far_name(MASK, true));
-      hipLaunchKernelGGL((k_far_adjoint<MASK, MESH>), dim3(blocks_for(ntiles * 64), nchunks),
+      hipLaunchKernelGGL((k_far_adjoint<MASK, MESH>), dim3(blocks_for(ntiles8 * 64), nchunks),
                          dim3(kBlock), 0, st, ctx->d_sx.as<double>(), ctx->d_sy.as<double>(),
                          ctx->d_sz.as<double>(), ctx->coef32.as<const float>(),
-                         nb.box.as<const int>(), S, per, g, nb.adj_table.as<const float>(),
-                         uniform_const(ctx), ntiles, ctx->hpart.as<double>());
+                         ctx->coef_mono.as<const float>(), nb.box.as<const int>(),
+                         nb.mid_box.as<const int>(), S, per, g,
+                         nb.adj_table.as<const float>(), uniform_const(ctx), ntiles8,
+                         ctx->hpart.as<double>());
     }
     hipLaunchKernelGGL(k_fill_zero, dim3(blocks_for(N)), dim3(kBlock), 0, st,
                        ctx->hsum.as<double>(), (size_t)N);
@@ -780,6 +999,7 @@ hipError_t build_near_boxes(jgm_context* ctx) {
   // any cell outside the box is >= R away along some axis, and R >= q0 * its half-diagonal
   const double R = ctx->mixed_q0 * 0.5 * std::sqrt(mx * mx + my * my + mz * mz);
   nb.radius = R;
+  nb.half_diag = 0.5 * std::sqrt(mx * mx + my * my + mz * mz);
   bool uniform = nx > 0 && ny > 0 && nz > 0;
   for (double v : ctx->xs) uniform = uniform && v == ctx->xs[0];
   for (double v : ctx->ys) uniform = uniform && v == ctx->ys[0];
@@ -856,6 +1076,33 @@ hipError_t build_near_boxes(jgm_context* ctx) {
   if ((e = upload_vec(nb.xcen, xcen, st)) != hipSuccess) return e;
   if ((e = upload_vec(nb.ycen, ycen, st)) != hipSuccess) return e;
   if ((e = upload_vec(nb.zcen, zcen, st)) != hipSuccess) return e;
+  auto run_offsets = [](const std::vector<double>& cen) {
+    std::vector<float> off(cen.size());
+    for (size_t i = 0; i < cen.size(); ++i) off[i] = (float)(cen[i] - cen[i / kTile * kTile]);
+    return off;
+  };
+  const std::vector<float> xoff = run_offsets(xcen), yoff = run_offsets(ycen),
+                           zoff = run_offsets(zcen);
+  if ((e = upload_vec(nb.xoff, xoff, st)) != hipSuccess) return e;
+  if ((e = upload_vec(nb.yoff, yoff, st)) != hipSuccess) return e;
+  if ((e = upload_vec(nb.zoff, zoff, st)) != hipSuccess) return e;
+  const std::vector<int> order = morton_order(ctx);
+  if ((e = upload_vec(nb.site_order, order, st)) != hipSuccess) return e;
+  // mid boxes: every cell within the second radius q1 of the site (all cells for q1 = 0)
+  std::vector<int> mid(std::max<size_t>(1, S) * 6);
+  const double R1 = ctx->mixed_q1 * nb.half_diag;
+  for (size_t s = 0; s < S; ++s) {
+    int* o = &mid[s * 6];
+    if (!(ctx->mixed_q1 > 0.0)) {
+      o[0] = 0; o[1] = nx - 1; o[2] = 0; o[3] = ny - 1; o[4] = 0; o[5] = nz - 1;
+      continue;
+    }
+    axis_range(xcen, ctx->hsx[s], R1, &o[0], &o[1]);
+    axis_range(ycen, ctx->hsy[s], R1, &o[2], &o[3]);
+    axis_range(zcen, ctx->hsz[s], R1, &o[4], &o[5]);
+  }
+  if ((e = upload_vec(nb.mid_box, mid, st)) != hipSuccess) return e;
+  nb.mid_q1 = ctx->mixed_q1;
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
   nb.adj_table_valid = false;
   nb.valid = true;
@@ -864,7 +1111,7 @@ hipError_t build_near_boxes(jgm_context* ctx) {
 
 hipError_t launch_forward_mixed(jgm_context* ctx, int kind, const double* model, double* out,
                                 double* raw_out) {
-  if (!ctx->near.valid) {
+  if (!ctx->near.valid || ctx->near.mid_q1 != ctx->mixed_q1) {
     hipError_t e = build_near_boxes(ctx);
     if (e != hipSuccess) return e;
   }
@@ -877,7 +1124,7 @@ hipError_t launch_forward_mixed(jgm_context* ctx, int kind, const double* model,
 
 hipError_t launch_adjoint_mixed(jgm_context* ctx, int kind, const double* /*model*/,
                                 const double* raw_misfit, double* grad) {
-  if (!ctx->near.valid) {
+  if (!ctx->near.valid || ctx->near.mid_q1 != ctx->mixed_q1) {
     hipError_t e = build_near_boxes(ctx);
     if (e != hipSuccess) return e;
   }
