@@ -52,6 +52,9 @@ KERNEL_SYMBOL = {
     "far_adjoint_fp32_gz_ftg": "k_far_adjoint<7, 0>",
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gz_ftg_128x128x64_hbm_traffic.md")
+VALU_FILE = os.path.join(ROOT, "profiles", "gz_ftg_128x128x64_valu_counters.md")
+# VALU issue peak: 256 CUs x 4 SIMDs x 16 lanes per clock at 2.4 GHz (lane operations/s)
+VALU_ISSUE_PEAK = 256 * 4 * 16 * 2.4e9
 
 
 def profiled_traffic(kname):
@@ -63,6 +66,19 @@ def profiled_traffic(kname):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
         if len(cells) == 4 and cells[0] == f"`{sym}`":
             return float(cells[3]) * 1e6
+    return None
+
+
+def profiled_valu(kname):
+    """(VALU lane instructions, FLOPs) per dispatch of a kernel from the committed
+    counter summary (wave-level counts times 64), or None."""
+    sym = KERNEL_SYMBOL.get(kname)
+    if sym is None or not os.path.exists(VALU_FILE):
+        return None
+    for line in open(VALU_FILE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[0] == f"`{sym}`":
+            return float(cells[1]) * 64, (float(cells[2]) + float(cells[3])) * 64
     return None
 
 
@@ -275,8 +291,16 @@ def main():
         "roofline": {"bound": "valu", "kernel": kname, "achieved": achieved / 1e12,
                      "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": profiled_traffic(kname),
-                     "work": f"{fe_per_pair(rho_c):.1f} flop-equivalents per pair x {S * N} pairs"},
+                     "work": f"{fe_per_pair(rho_c):.1f} flop-equivalents per pair (closed-form "
+                             f"counting contract) x {S * N} pairs"},
     }
+    hw = profiled_valu(kname)
+    if hw is not None:
+        # what the kernel executes, from hardware counters: the multipole far field needs
+        # far fewer operations per pair than the closed form the contract counts
+        line["roofline"]["hw_flops_achieved"] = hw[1] / avg_s / 1e12
+        line["roofline"]["hw_flops_frac"] = hw[1] / avg_s / peak
+        line["roofline"]["valu_issue_frac"] = hw[0] / avg_s / VALU_ISSUE_PEAK
     if "fp64" in res and main_p != "fp64":
         line["fp64_pairs_per_s"] = res["fp64"]["pairs_per_s"]
         line["speedup_vs_fp64"] = res["fp64"]["ms_per_step"] / r["ms_per_step"]
