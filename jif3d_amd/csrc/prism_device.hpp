@@ -46,6 +46,31 @@ struct Ncomp {
 // with the tensor terms present gz reuses the guarded logs of the off-diagonal
 // terms (they coincide unless x = z = 0 and y < 0, where the reference's gz term
 // is 0 * log(0) = NaN).
+//
+// Two identities of the atan2 terms save two of the four atan2 evaluations when the
+// diagonal terms are present and no coordinate is zero:
+//  atan2(yz, xr) + atan2(xz, yr) + atan2(xy, zr) = (pi/2) * {1, 1, -3, 5}[number of
+//    negative coordinates]  (the solid-angle identity of a rectangular corner, with the
+//    quadrant offsets of atan2), and
+//  atan2(zr, xy) + atan2(xy, zr) = -3pi/2 if xy < 0 and z < 0, pi/2 otherwise.
+// The term of the coordinate smallest in magnitude (the largest angle) is the one
+// derived, so the subtraction does not cancel.
+template <int MASK>
+__device__ __forceinline__ void diag_terms_f64(double x, double y, double z, double r,
+                                               double* A, double* B, double* C) {
+  const double ax = fabs(x), ay = fabs(y), az = fabs(z);
+  const int which = (ax <= ay && ax <= az) ? 0 : (ay <= az ? 1 : 2);  // derived term
+  const double p1 = which == 0 ? x * z : y * z, q1 = which == 0 ? y * r : x * r;
+  const double p2 = which == 2 ? x * z : x * y, q2 = which == 2 ? y * r : z * r;
+  const double t1 = atan2(p1, q1), t2 = atan2(p2, q2);
+  const int neg = (x < 0.0) + (y < 0.0) + (z < 0.0);
+  const double K = neg <= 1 ? 0.5 * kPi : (neg == 2 ? -1.5 * kPi : 2.5 * kPi);
+  const double t3 = (K - t1) - t2;
+  *A = which == 0 ? t3 : t1;
+  *B = which == 0 ? t1 : (which == 1 ? t3 : t2);
+  *C = which == 2 ? t3 : t2;
+}
+
 template <int MASK>
 __device__ __forceinline__ void corner_terms_f64(double x, double y, double z, double* t) {
   const double r = sqrt(x * x + y * y + z * z);
@@ -54,16 +79,26 @@ __device__ __forceinline__ void corner_terms_f64(double x, double y, double z, d
     const double lz = log(fmax(z + r, DBL_EPSILON));
     const double ly = log(fmax(y + r, DBL_EPSILON));
     const double lx = log(fmax(x + r, DBL_EPSILON));
-    if constexpr ((MASK & kGz) != 0) {
+    if constexpr ((MASK & kDiag) != 0) {
+      double A, B, C, G;
+      if (x != 0.0 && y != 0.0 && z != 0.0 && r >= DBL_EPSILON) {
+        diag_terms_f64<MASK>(x, y, z, r, &A, &B, &C);
+        G = ((x * y < 0.0 && z < 0.0) ? -1.5 * kPi : 0.5 * kPi) - C;
+      } else {
+        A = atan2(y * z, x * r);
+        B = atan2(x * z, y * r);
+        C = atan2(x * y, z * r);
+        G = atan2(z * fmax(r, DBL_EPSILON), x * y);
+      }
+      if constexpr ((MASK & kGz) != 0) t[k++] = x * ly + y * lx + z * G;
+      t[k++] = A;
+      t[k++] = B;
+      t[k++] = C;
+    } else if constexpr ((MASK & kGz) != 0) {
       const double rg = fmax(r, DBL_EPSILON);
       double v = x * ly + y * lx;
       v += z * atan2(z * rg, x * y);
       t[k++] = v;
-    }
-    if constexpr ((MASK & kDiag) != 0) {
-      t[k++] = atan2(y * z, x * r);
-      t[k++] = atan2(x * z, y * r);
-      t[k++] = atan2(x * y, z * r);
     }
     t[k++] = lz;
     t[k++] = ly;
