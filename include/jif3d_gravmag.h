@@ -149,6 +149,32 @@ jgm_status jgm_adjoint(jgm_context *ctx, jgm_kind kind, jgm_precision prec, cons
 jgm_status jgm_adjoint_host(jgm_context *ctx, jgm_kind kind, jgm_precision prec,
                             const double *model, const double *misfit, double *grad);
 
+/* Batched forms of jgm_forward / jgm_adjoint for several models at once, e.g. the
+ * trial steps of a line search, which the reference evaluates one Calculate /
+ * LQDerivative pair at a time (jif3D/Inversion/mcsrch.cpp:247-248 through
+ * ObjectiveFunction::CalcMisfit/CalcGradient, ObjectiveFunction.h:165-229).
+ * Model b is models[b * model_ld ...] (model_ld >= jgm_model_size), its data
+ * out[b * out_ld ...] (out_ld >= nsites * data_per_site), its misfit
+ * misfits[b * misfit_ld ...] and its gradient grads[b * grad_ld ...] (grad_ld >=
+ * jgm_model_size); all device memory.  With JGM_FP64 and the gravity and
+ * susceptibility kinds up to 8 models share each corner-function evaluation, so a
+ * batch of 8 costs little more than one model; each result is bit-identical to the
+ * single-model call.  JGM_MIXED and the vertical-gradient and magnetization kinds run
+ * the models one after another.  nmodels == 0 is a no-op. */
+jgm_status jgm_forward_batch(jgm_context *ctx, jgm_kind kind, jgm_precision prec, size_t nmodels,
+                             const double *models, size_t model_ld, double *out, size_t out_ld);
+jgm_status jgm_adjoint_batch(jgm_context *ctx, jgm_kind kind, jgm_precision prec, size_t nmodels,
+                             const double *models, size_t model_ld, const double *misfits,
+                             size_t misfit_ld, double *grads, size_t grad_ld);
+/* The same with host memory and packed batches (model_ld = jgm_model_size, out_ld =
+ * misfit_ld = nsites * data_per_site, grad_ld = jgm_model_size); models may be NULL
+ * for jgm_adjoint_batch_host when no transform is set. */
+jgm_status jgm_forward_batch_host(jgm_context *ctx, jgm_kind kind, jgm_precision prec,
+                                  size_t nmodels, const double *models, double *out);
+jgm_status jgm_adjoint_batch_host(jgm_context *ctx, jgm_kind kind, jgm_precision prec,
+                                  size_t nmodels, const double *models, const double *misfits,
+                                  double *grads);
+
 /* Raw (untransformed) FP64 sensitivity rows of sites [site0, site0+nsites) in the
  * layout FullSensitivityGravMagCalculator::HandleSensitivities builds
  * (jif3D/GravMag/FullSensitivityGravMagCalculator.h:99-114): nsites*raw_per_site rows
@@ -194,6 +220,25 @@ jgm_status jgm_storedj_forward_host(jgm_context *ctx, jgm_gemm_precision prec,
                                     const double *model, double *out);
 jgm_status jgm_storedj_gradient_host(jgm_context *ctx, jgm_gemm_precision prec,
                                      const double *model, const double *misfit, double *grad);
+/* Batched cached forward and gradient for several models (line-search trials, see
+ * jgm_forward_batch): up to 64 models share one pass over J, because the products'
+ * right-hand-side tile is 64 wide, so a batch of 64 costs what one model costs (the
+ * FullSensitivityGravMagCalculator of the reference does one prec_prod per model,
+ * FullSensitivityGravMagCalculator.h:151, :220).  Leading dimensions and the host forms
+ * as for jgm_forward_batch / jgm_adjoint_batch; results equal the single-model calls
+ * bit for bit. */
+jgm_status jgm_storedj_forward_batch(jgm_context *ctx, jgm_gemm_precision prec, size_t nmodels,
+                                     const double *models, size_t model_ld, double *out,
+                                     size_t out_ld);
+jgm_status jgm_storedj_gradient_batch(jgm_context *ctx, jgm_gemm_precision prec, size_t nmodels,
+                                      const double *models, size_t model_ld,
+                                      const double *misfits, size_t misfit_ld, double *grads,
+                                      size_t grad_ld);
+jgm_status jgm_storedj_forward_batch_host(jgm_context *ctx, jgm_gemm_precision prec,
+                                          size_t nmodels, const double *models, double *out);
+jgm_status jgm_storedj_gradient_batch_host(jgm_context *ctx, jgm_gemm_precision prec,
+                                           size_t nmodels, const double *models,
+                                           const double *misfits, double *grads);
 jgm_status jgm_storedj_release(jgm_context *ctx);
 
 /* Device memory on the context's device, for callers without their own HIP

@@ -28,6 +28,9 @@ EXPORTS = [
     "jgm_set_field", "jgm_set_transform", "jgm_set_mixed_threshold", "jgm_set_mixed_far_radius",
     "jgm_data_per_site",
     "jgm_n_params", "jgm_forward", "jgm_forward_host", "jgm_adjoint", "jgm_adjoint_host",
+    "jgm_forward_batch", "jgm_adjoint_batch", "jgm_forward_batch_host", "jgm_adjoint_batch_host",
+    "jgm_storedj_forward_batch", "jgm_storedj_gradient_batch", "jgm_storedj_forward_batch_host",
+    "jgm_storedj_gradient_batch_host",
     "jgm_sensitivity_rows", "jgm_model_size", "jgm_storedj_build", "jgm_storedj_rows",
     "jgm_storedj_apply",
     "jgm_storedj_apply_t", "jgm_storedj_forward", "jgm_storedj_gradient",
@@ -78,6 +81,10 @@ def load():
         "jgm_forward_host": [vp, i, i, pd, pd],
         "jgm_adjoint": [vp, i, i, vp, vp, vp],
         "jgm_adjoint_host": [vp, i, i, pd, pd, pd],
+        "jgm_forward_batch": [vp, i, i, sz, vp, sz, vp, sz],
+        "jgm_adjoint_batch": [vp, i, i, sz, vp, sz, vp, sz, vp, sz],
+        "jgm_forward_batch_host": [vp, i, i, sz, pd, pd],
+        "jgm_adjoint_batch_host": [vp, i, i, sz, pd, pd, pd],
         "jgm_sensitivity_rows": [vp, i, sz, sz, vp],
         "jgm_storedj_build": [vp, i, i],
         "jgm_storedj_apply": [vp, i, vp, sz, vp],
@@ -86,6 +93,10 @@ def load():
         "jgm_storedj_gradient": [vp, i, vp, vp, vp],
         "jgm_storedj_forward_host": [vp, i, pd, pd],
         "jgm_storedj_gradient_host": [vp, i, pd, pd, pd],
+        "jgm_storedj_forward_batch": [vp, i, sz, vp, sz, vp, sz],
+        "jgm_storedj_gradient_batch": [vp, i, sz, vp, sz, vp, sz, vp, sz],
+        "jgm_storedj_forward_batch_host": [vp, i, sz, pd, pd],
+        "jgm_storedj_gradient_batch_host": [vp, i, sz, pd, pd, pd],
         "jgm_storedj_release": [vp],
         "jgm_device_alloc": [vp, sz, ctypes.POINTER(vp)],
         "jgm_device_free": [vp, vp],
@@ -283,6 +294,48 @@ class Context:
         check(self.lib.jgm_adjoint_host(self.handle, kind, prec, _pd(m), _pd(r), _pd(g)))
         return g
 
+    def forward_batch(self, kind, prec, models, out=None, nmodels=None):
+        """Forward of several models.  Host: models (B, model_size) -> (B, nout).
+        Device arrays: packed (B * model_size) in, (B * nout) out, nmodels = B."""
+        nout = self.nsites * self.data_per_site(kind)
+        if _is_device(models):
+            if out is None or nmodels is None:
+                raise ValueError("device forward_batch needs an output array and nmodels")
+            check(self.lib.jgm_forward_batch(self.handle, kind, prec, int(nmodels),
+                                             _devptr(models), self.model_size(kind),
+                                             _devptr(out), nout))
+            return out
+        m = np.ascontiguousarray(np.asarray(models, dtype=np.float64).reshape(
+            -1, self.model_size(kind)))
+        res = np.zeros((m.shape[0], nout))
+        check(self.lib.jgm_forward_batch_host(self.handle, kind, prec, m.shape[0], _pd(m),
+                                              _pd(res)))
+        return res
+
+    def adjoint_batch(self, kind, prec, models, misfits, out=None, nmodels=None):
+        """2 J^T D^T r for several (model, misfit) pairs.  Host: models (B, model_size) or
+        None (no transform), misfits (B, nout) -> (B, model_size).  Device: packed."""
+        np_ = self.model_size(kind)
+        nout = self.nsites * self.data_per_site(kind)
+        if _is_device(misfits):
+            if out is None or nmodels is None:
+                raise ValueError("device adjoint_batch needs an output array and nmodels")
+            mp = _devptr(models) if models is not None else None
+            check(self.lib.jgm_adjoint_batch(self.handle, kind, prec, int(nmodels), mp, np_,
+                                             _devptr(misfits), nout, _devptr(out), np_))
+            return out
+        r = np.ascontiguousarray(np.asarray(misfits, dtype=np.float64).reshape(-1, nout))
+        m = None
+        if models is not None:
+            m = np.ascontiguousarray(np.asarray(models, dtype=np.float64).reshape(-1, np_))
+            if m.shape[0] != r.shape[0]:
+                raise ValueError("models and misfits differ in batch size")
+        g = np.zeros((r.shape[0], np_))
+        check(self.lib.jgm_adjoint_batch_host(self.handle, kind, prec, r.shape[0],
+                                              _pd(m) if m is not None else None, _pd(r),
+                                              _pd(g)))
+        return g
+
     def sensitivity_rows(self, kind, site0, nsites, out):
         check(self.lib.jgm_sensitivity_rows(self.handle, kind, site0, nsites, _devptr(out)))
         return out
@@ -367,6 +420,30 @@ class Context:
         r = _f64(misfit)
         g = np.zeros(self.n_params())
         check(self.lib.jgm_storedj_gradient_host(self.handle, prec, _pd(m), _pd(r), _pd(g)))
+        return g
+
+    def storedj_forward_batch(self, prec, models):
+        """Cached forward of several models (host (B, n_params) -> (B, nout)); up to 64
+        models share one pass over J."""
+        m = np.ascontiguousarray(np.asarray(models, dtype=np.float64).reshape(-1, self.n_params()))
+        res = np.zeros((m.shape[0], self.nsites * self.data_per_site(self.storedj_kind)))
+        check(self.lib.jgm_storedj_forward_batch_host(self.handle, prec, m.shape[0], _pd(m),
+                                                      _pd(res)))
+        return res
+
+    def storedj_gradient_batch(self, prec, models, misfits):
+        """Cached gradients of several (model, misfit) pairs (models may be None without
+        a transform)."""
+        nout = self.nsites * self.data_per_site(self.storedj_kind)
+        r = np.ascontiguousarray(np.asarray(misfits, dtype=np.float64).reshape(-1, nout))
+        m = None
+        if models is not None:
+            m = np.ascontiguousarray(np.asarray(models, dtype=np.float64).reshape(
+                -1, self.n_params()))
+        g = np.zeros((r.shape[0], self.n_params()))
+        check(self.lib.jgm_storedj_gradient_batch_host(self.handle, prec, r.shape[0],
+                                                       _pd(m) if m is not None else None,
+                                                       _pd(r), _pd(g)))
         return g
 
     def storedj_release(self):

@@ -101,6 +101,46 @@ size_t data_per_site(const jgm_context* ctx, int kind) {
   return ctx->transform == JGM_TF_NONE ? jgm::raw_per_site(kind) : 1;
 }
 
+// Host-memory form of a device entry point: copies the inputs into temporary device
+// buffers, runs `call` on their device pointers, copies the outputs back and waits.
+struct HostArg {
+  const double* in;  // host input (or nullptr)
+  double* out;       // host output (or nullptr)
+  size_t n;          // doubles
+};
+template <size_t NA, class F>
+jgm_status run_staged(jgm_context* ctx, const char* where, const HostArg (&args)[NA], F call) {
+  jgm::DeviceBuffer buf[NA];
+  const double* dev_in[NA];
+  double* dev_out[NA];
+  hipError_t e = hipSuccess;
+  for (size_t a = 0; a < NA && e == hipSuccess; ++a) {
+    dev_in[a] = nullptr;
+    dev_out[a] = nullptr;
+    if (!args[a].in && !args[a].out) continue;
+    e = buf[a].ensure(sizeof(double) * (args[a].n ? args[a].n : 1));
+    if (e == hipSuccess && args[a].in && args[a].n)
+      e = hipMemcpyAsync(buf[a].ptr, args[a].in, sizeof(double) * args[a].n,
+                         hipMemcpyHostToDevice, ctx->stream);
+    dev_in[a] = static_cast<const double*>(buf[a].ptr);
+    dev_out[a] = static_cast<double*>(buf[a].ptr);
+  }
+  jgm_status st = JGM_OK;
+  if (e == hipSuccess) {
+    st = call(dev_in, dev_out);
+    for (size_t a = 0; a < NA && st == JGM_OK && e == hipSuccess; ++a)
+      if (args[a].out && args[a].n)
+        e = hipMemcpyAsync(args[a].out, buf[a].ptr, sizeof(double) * args[a].n,
+                           hipMemcpyDeviceToHost, ctx->stream);
+  }
+  hipError_t e2 = hipStreamSynchronize(ctx->stream);
+  for (size_t a = 0; a < NA; ++a) buf[a].release();
+  if (st != JGM_OK) return st;
+  if (e != hipSuccess) return hip_fail(e, where);
+  if (e2 != hipSuccess) return hip_fail(e2, where);
+  return JGM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -451,6 +491,127 @@ jgm_status jgm_adjoint_host(jgm_context* ctx, jgm_kind kind, jgm_precision prec,
   return JGM_OK;
 }
 
+jgm_status jgm_forward_batch(jgm_context* ctx, jgm_kind kind, jgm_precision prec, size_t nmodels,
+                             const double* models, size_t model_ld, double* out, size_t out_ld) {
+  jgm_status st = check_ready(ctx, kind, prec);
+  if (st != JGM_OK) return st;
+  if (nmodels == 0) return JGM_OK;
+  if (!models || !out) return fail(JGM_ERR_INVALID_ARGUMENT, "null models or output pointer");
+  const size_t np = jgm::nparams(ctx, kind), nout = ctx->S * data_per_site(ctx, kind);
+  if (nmodels > 1 && (model_ld < np || out_ld < nout))
+    return fail(JGM_ERR_INVALID_ARGUMENT,
+                "leading dimensions too small (model_ld %zu < %zu or out_ld %zu < %zu)", model_ld,
+                np, out_ld, nout);
+  DeviceGuard g(ctx->device);
+  if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+  if (ctx->S == 0) return JGM_OK;
+  if ((st = ensure_background(ctx)) != JGM_OK) return st;
+  hipError_t e = hipSuccess;
+  if (prec == JGM_MIXED) {
+    for (size_t b = 0; b < nmodels && e == hipSuccess; ++b)
+      e = jgm::launch_forward_mixed(ctx, kind, models + b * model_ld, out + b * out_ld, nullptr);
+  } else {
+    e = jgm::launch_forward_fp64_batch(ctx, kind, nmodels, models, model_ld, out, out_ld, nullptr,
+                                       0);
+  }
+  if (e != hipSuccess) return hip_fail(e, "forward_batch");
+  return JGM_OK;
+}
+
+jgm_status jgm_adjoint_batch(jgm_context* ctx, jgm_kind kind, jgm_precision prec, size_t nmodels,
+                             const double* models, size_t model_ld, const double* misfits,
+                             size_t misfit_ld, double* grads, size_t grad_ld) {
+  jgm_status st = check_ready(ctx, kind, prec);
+  if (st != JGM_OK) return st;
+  if (nmodels == 0) return JGM_OK;
+  if (!misfits || !grads) return fail(JGM_ERR_INVALID_ARGUMENT, "null misfit or gradient pointer");
+  if (ctx->transform != JGM_TF_NONE && !models)
+    return fail(JGM_ERR_INVALID_ARGUMENT, "a transform needs the models for its Jacobian");
+  const size_t S = ctx->S, kraw = jgm::raw_per_site(kind);
+  const size_t np = jgm::nparams(ctx, kind), nmis = S * data_per_site(ctx, kind);
+  if (nmodels > 1 && (misfit_ld < nmis || grad_ld < np ||
+                      (ctx->transform != JGM_TF_NONE && model_ld < np)))
+    return fail(JGM_ERR_INVALID_ARGUMENT, "leading dimensions too small");
+  DeviceGuard g(ctx->device);
+  if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+  if ((st = ensure_background(ctx)) != JGM_OK) return st;
+  hipError_t e = hipSuccess;
+  if (S == 0) {
+    for (size_t b = 0; b < nmodels && e == hipSuccess; ++b)
+      e = jgm::launch_fill_zero(ctx, grads + b * grad_ld, np);
+    if (e != hipSuccess) return hip_fail(e, "adjoint_batch");
+    return JGM_OK;
+  }
+  const double* rm = misfits;
+  size_t rm_ld = misfit_ld;
+  if (ctx->transform != JGM_TF_NONE) {
+    // raw data of every model for the transform Jacobians, then D^T r per model
+    const size_t nraw = S * kraw;
+    if ((e = ctx->raw.ensure(sizeof(double) * nmodels * nraw)) != hipSuccess ||
+        (e = ctx->out_stage.ensure(sizeof(double) * nmodels * S)) != hipSuccess ||
+        (e = ctx->misfit_stage.ensure(sizeof(double) * nmodels * nraw)) != hipSuccess)
+      return hip_fail(e, "adjoint_batch");
+    double* raw = ctx->raw.as<double>();
+    double* outs = ctx->out_stage.as<double>();
+    if (prec == JGM_MIXED) {
+      for (size_t b = 0; b < nmodels && e == hipSuccess; ++b)
+        e = jgm::launch_forward_mixed(ctx, kind, models + b * model_ld, outs + b * S,
+                                      raw + b * nraw);
+    } else {
+      e = jgm::launch_forward_fp64_batch(ctx, kind, nmodels, models, model_ld, outs, S, raw,
+                                         nraw);
+    }
+    if (e != hipSuccess) return hip_fail(e, "adjoint_batch (raw forward)");
+    for (size_t b = 0; b < nmodels && e == hipSuccess; ++b)
+      e = jgm::launch_misfit_to_raw(ctx, kind, misfits + b * misfit_ld, raw + b * nraw,
+                                    ctx->misfit_stage.as<double>() + b * nraw);
+    if (e != hipSuccess) return hip_fail(e, "adjoint_batch (transform)");
+    rm = ctx->misfit_stage.as<const double>();
+    rm_ld = nraw;
+  }
+  if (prec == JGM_MIXED) {
+    for (size_t b = 0; b < nmodels && e == hipSuccess; ++b)
+      e = jgm::launch_adjoint_mixed(ctx, kind, models ? models + b * model_ld : nullptr,
+                                    rm + b * rm_ld, grads + b * grad_ld);
+  } else {
+    e = jgm::launch_adjoint_fp64_batch(ctx, kind, nmodels, rm, rm_ld, grads, grad_ld);
+  }
+  if (e != hipSuccess) return hip_fail(e, "adjoint_batch");
+  return JGM_OK;
+}
+
+jgm_status jgm_forward_batch_host(jgm_context* ctx, jgm_kind kind, jgm_precision prec,
+                                  size_t nmodels, const double* models, double* out) {
+  jgm_status st = check_ready(ctx, kind, prec);
+  if (st != JGM_OK) return st;
+  if (nmodels == 0) return JGM_OK;
+  if (!models || !out) return fail(JGM_ERR_INVALID_ARGUMENT, "null models or output pointer");
+  DeviceGuard g(ctx->device);
+  const size_t np = jgm::nparams(ctx, kind), nout = ctx->S * data_per_site(ctx, kind);
+  const HostArg args[2] = {{models, nullptr, nmodels * np}, {nullptr, out, nmodels * nout}};
+  return run_staged(ctx, "forward_batch_host", args, [&](const double* const* in, double* const* o) {
+    return jgm_forward_batch(ctx, kind, prec, nmodels, in[0], np, o[1], nout);
+  });
+}
+
+jgm_status jgm_adjoint_batch_host(jgm_context* ctx, jgm_kind kind, jgm_precision prec,
+                                  size_t nmodels, const double* models, const double* misfits,
+                                  double* grads) {
+  jgm_status st = check_ready(ctx, kind, prec);
+  if (st != JGM_OK) return st;
+  if (nmodels == 0) return JGM_OK;
+  if (!misfits || !grads) return fail(JGM_ERR_INVALID_ARGUMENT, "null misfit or gradient pointer");
+  DeviceGuard g(ctx->device);
+  const size_t np = jgm::nparams(ctx, kind), nmis = ctx->S * data_per_site(ctx, kind);
+  const HostArg args[3] = {{models, nullptr, models ? nmodels * np : 0},
+                           {misfits, nullptr, nmodels * nmis},
+                           {nullptr, grads, nmodels * np}};
+  return run_staged(ctx, "adjoint_batch_host", args, [&](const double* const* in, double* const* o) {
+    return jgm_adjoint_batch(ctx, kind, prec, nmodels, models ? in[0] : nullptr, np, in[1], nmis,
+                             o[2], np);
+  });
+}
+
 jgm_status jgm_sensitivity_rows(jgm_context* ctx, jgm_kind kind, size_t site0, size_t nsites,
                                 double* rows) {
   jgm_status st = check_ready(ctx, kind, JGM_FP64);
@@ -567,6 +728,105 @@ jgm_status jgm_storedj_gradient(jgm_context* ctx, jgm_gemm_precision prec, const
   e = jgm::launch_storedj_gradient(ctx, prec, rm, grad);
   if (e != hipSuccess) return hip_fail(e, "stored-J gradient");
   return JGM_OK;
+}
+
+jgm_status jgm_storedj_forward_batch(jgm_context* ctx, jgm_gemm_precision prec, size_t nmodels,
+                                     const double* models, size_t model_ld, double* out,
+                                     size_t out_ld) {
+  jgm_status st = storedj_ready(ctx, prec);
+  if (st != JGM_OK) return st;
+  if (nmodels == 0) return JGM_OK;
+  if (!models || !out) return fail(JGM_ERR_INVALID_ARGUMENT, "null models or output pointer");
+  const size_t np = jgm::nparams(ctx), nout = ctx->S * data_per_site(ctx, ctx->storedJ_kind);
+  if (nmodels > 1 && (model_ld < np || out_ld < nout))
+    return fail(JGM_ERR_INVALID_ARGUMENT, "leading dimensions too small");
+  DeviceGuard g(ctx->device);
+  if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+  if ((st = ensure_background(ctx)) != JGM_OK) return st;
+  hipError_t e = jgm::launch_storedj_forward_batch(ctx, prec, nmodels, models, model_ld, out,
+                                                   out_ld, nullptr, 0);
+  if (e != hipSuccess) return hip_fail(e, "stored-J forward batch");
+  return JGM_OK;
+}
+
+jgm_status jgm_storedj_gradient_batch(jgm_context* ctx, jgm_gemm_precision prec, size_t nmodels,
+                                      const double* models, size_t model_ld,
+                                      const double* misfits, size_t misfit_ld, double* grads,
+                                      size_t grad_ld) {
+  jgm_status st = storedj_ready(ctx, prec);
+  if (st != JGM_OK) return st;
+  if (nmodels == 0) return JGM_OK;
+  if (!misfits || !grads) return fail(JGM_ERR_INVALID_ARGUMENT, "null misfit or gradient pointer");
+  if (ctx->transform != JGM_TF_NONE && !models)
+    return fail(JGM_ERR_INVALID_ARGUMENT, "a transform needs the models for its Jacobian");
+  const int kind = ctx->storedJ_kind;
+  const size_t S = ctx->S, kraw = jgm::raw_per_site(kind);
+  const size_t np = jgm::nparams(ctx), nmis = S * data_per_site(ctx, kind);
+  if (nmodels > 1 && (misfit_ld < nmis || grad_ld < np ||
+                      (ctx->transform != JGM_TF_NONE && model_ld < np)))
+    return fail(JGM_ERR_INVALID_ARGUMENT, "leading dimensions too small");
+  DeviceGuard g(ctx->device);
+  if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+  if ((st = ensure_background(ctx)) != JGM_OK) return st;
+  hipError_t e;
+  const double* rm = misfits;
+  size_t rm_ld = misfit_ld;
+  if (ctx->transform != JGM_TF_NONE) {
+    const size_t nraw = S * kraw;
+    if ((e = ctx->raw.ensure(sizeof(double) * nmodels * nraw)) != hipSuccess ||
+        (e = ctx->out_stage.ensure(sizeof(double) * nmodels * S)) != hipSuccess ||
+        (e = ctx->misfit_stage.ensure(sizeof(double) * nmodels * nraw)) != hipSuccess)
+      return hip_fail(e, "stored-J gradient batch");
+    e = jgm::launch_storedj_forward_batch(ctx, prec, nmodels, models, model_ld,
+                                          ctx->out_stage.as<double>(), S, ctx->raw.as<double>(),
+                                          nraw);
+    if (e != hipSuccess) return hip_fail(e, "stored-J gradient batch (raw forward)");
+    for (size_t b = 0; b < nmodels && e == hipSuccess; ++b)
+      e = jgm::launch_misfit_to_raw(ctx, kind, misfits + b * misfit_ld,
+                                    ctx->raw.as<const double>() + b * nraw,
+                                    ctx->misfit_stage.as<double>() + b * nraw);
+    if (e != hipSuccess) return hip_fail(e, "stored-J gradient batch (transform)");
+    rm = ctx->misfit_stage.as<const double>();
+    rm_ld = nraw;
+  }
+  e = jgm::launch_storedj_gradient_batch(ctx, prec, nmodels, rm, rm_ld, grads, grad_ld);
+  if (e != hipSuccess) return hip_fail(e, "stored-J gradient batch");
+  return JGM_OK;
+}
+
+jgm_status jgm_storedj_forward_batch_host(jgm_context* ctx, jgm_gemm_precision prec,
+                                          size_t nmodels, const double* models, double* out) {
+  jgm_status st = storedj_ready(ctx, prec);
+  if (st != JGM_OK) return st;
+  if (nmodels == 0) return JGM_OK;
+  if (!models || !out) return fail(JGM_ERR_INVALID_ARGUMENT, "null models or output pointer");
+  DeviceGuard g(ctx->device);
+  const size_t np = jgm::nparams(ctx), nout = ctx->S * data_per_site(ctx, ctx->storedJ_kind);
+  const HostArg args[2] = {{models, nullptr, nmodels * np}, {nullptr, out, nmodels * nout}};
+  return run_staged(ctx, "storedj_forward_batch_host", args,
+                    [&](const double* const* in, double* const* o) {
+                      return jgm_storedj_forward_batch(ctx, prec, nmodels, in[0], np, o[1], nout);
+                    });
+}
+
+jgm_status jgm_storedj_gradient_batch_host(jgm_context* ctx, jgm_gemm_precision prec,
+                                           size_t nmodels, const double* models,
+                                           const double* misfits, double* grads) {
+  jgm_status st = storedj_ready(ctx, prec);
+  if (st != JGM_OK) return st;
+  if (nmodels == 0) return JGM_OK;
+  if (!misfits || !grads) return fail(JGM_ERR_INVALID_ARGUMENT, "null misfit or gradient pointer");
+  DeviceGuard g(ctx->device);
+  const size_t np = jgm::nparams(ctx), nmis = ctx->S * data_per_site(ctx, ctx->storedJ_kind);
+  const HostArg args[3] = {{models, nullptr, models ? nmodels * np : 0},
+                           {misfits, nullptr, nmodels * nmis},
+                           {nullptr, grads, nmodels * np}};
+  return run_staged(ctx, "storedj_gradient_batch_host", args,
+                    [&](const double* const* in, double* const* o) {
+                      return jgm_storedj_gradient_batch(ctx, prec, nmodels,
+                                                        models ? in[0] : nullptr, np, in[1], nmis,
+                                                        o[2], np);
+                    });
 }
 
 jgm_status jgm_storedj_forward_host(jgm_context* ctx, jgm_gemm_precision prec,

@@ -152,6 +152,13 @@ hipError_t launch_forward_fp64(jgm_context* ctx, int kind, const double* model, 
                                double* raw_out);
 hipError_t launch_adjoint_fp64(jgm_context* ctx, int kind, const double* model,
                                const double* raw_misfit, double* grad);
+// batches: nmodels models / residual sets, packed with the given leading dimensions
+hipError_t launch_forward_fp64_batch(jgm_context* ctx, int kind, size_t nmodels,
+                                     const double* models, size_t model_ld, double* out,
+                                     size_t out_ld, double* raw_out, size_t raw_ld);
+hipError_t launch_adjoint_fp64_batch(jgm_context* ctx, int kind, size_t nsets,
+                                     const double* raw_misfits, size_t misfit_ld, double* grads,
+                                     size_t grad_ld);
 hipError_t launch_misfit_to_raw(jgm_context* ctx, int kind, const double* misfit,
                                 const double* raw, double* raw_misfit);
 hipError_t launch_sensitivity_rows(jgm_context* ctx, int kind, size_t site0, size_t nsites,
@@ -171,6 +178,12 @@ hipError_t launch_storedj_forward(jgm_context* ctx, int gemm_precision, const do
                                   double* out, double* raw_out);
 hipError_t launch_storedj_gradient(jgm_context* ctx, int gemm_precision,
                                    const double* raw_misfit, double* grad);
+hipError_t launch_storedj_forward_batch(jgm_context* ctx, int gemm_precision, size_t nmodels,
+                                        const double* models, size_t model_ld, double* out,
+                                        size_t out_ld, double* raw_out, size_t raw_ld);
+hipError_t launch_storedj_gradient_batch(jgm_context* ctx, int gemm_precision, size_t nsets,
+                                         const double* raw_misfits, size_t misfit_ld,
+                                         double* grads, size_t grad_ld);
 
 __host__ __device__ inline size_t raw_per_site(int kind) {
   switch (kind) {

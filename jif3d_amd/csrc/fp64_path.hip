@@ -25,9 +25,11 @@ constexpr int kBlock = 256;
 constexpr int kTargetWaves = 16384;  // ~16 waves per SIMD over 256 CUs x 4 SIMDs
 
 // w(I,J,K) = sum_{a,b,c} sigma(a,b,c) rho(I-a, J-b, K-c), rho zero outside the mesh.
+// For a batch of nb models (model m at rho + m * model_ld) the weights of one corner
+// are stored together: w[q * nb + m].
 __global__ __launch_bounds__(kBlock) void k_corner_weights(const double* __restrict__ rho,
-                                                           int nx, int ny, int nz,
-                                                           double* __restrict__ w) {
+                                                           long model_ld, int nb, int nx, int ny,
+                                                           int nz, double* __restrict__ w) {
   const long ncx = nx + 1, ncy = ny + 1, ncz = nz + 1;
   const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= ncx * ncy * ncz) return;
@@ -35,24 +37,29 @@ __global__ __launch_bounds__(kBlock) void k_corner_weights(const double* __restr
   const long t = q / ncz;
   const int J = (int)(t % ncy);
   const int I = (int)(t / ncy);
-  double acc = 0.0;
+  for (int m = 0; m < nb; ++m) {
+    const double* r = rho + m * model_ld;
+    double acc = 0.0;
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+      for (int b = 0; b < 2; ++b)
 #pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        const int i = I - a, j = J - b, k = K - c;
-        if (i < 0 || j < 0 || k < 0 || i >= nx || j >= ny || k >= nz) continue;
-        const double v = rho[((long)i * ny + j) * nz + k];
-        acc += ((a + b + c) & 1) ? v : -v;
-      }
-  w[q] = acc;
+        for (int c = 0; c < 2; ++c) {
+          const int i = I - a, j = J - b, k = K - c;
+          if (i < 0 || j < 0 || k < 0 || i >= nx || j >= ny || k >= nz) continue;
+          const double v = r[((long)i * ny + j) * nz + k];
+          acc += ((a + b + c) & 1) ? v : -v;
+        }
+    w[q * nb + m] = acc;
+  }
 }
 
 // Forward: one lane per site; the wave walks a chunk of corner columns (I, J)
-// and all K.  Corner coordinates and weights are wave-uniform (scalar loads).
-template <int MASK>
+// and all K.  Corner coordinates and weights are wave-uniform (scalar loads).  NB
+// models share every corner evaluation (weights w[q * NB + b]); the partial sums of
+// model b go to partial[((b * nchunks + chunk) * NC + m) * S + s].
+template <int MASK, int NB>
 __global__ __launch_bounds__(kBlock) void k_forward_corners(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     int S, const double* __restrict__ Xc, const double* __restrict__ Yc,
@@ -64,27 +71,35 @@ __global__ __launch_bounds__(kBlock) void k_forward_corners(
   const double px = sx[sl], py = sy[sl], pz = sz[sl];
   const long col0 = (long)blockIdx.y * cols_per_chunk;
   const long col1 = col0 + cols_per_chunk < ncols ? col0 + cols_per_chunk : ncols;
-  double acc[NC];
+  double acc[NB][NC];
 #pragma unroll
-  for (int m = 0; m < NC; ++m) acc[m] = 0.0;
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int m = 0; m < NC; ++m) acc[b][m] = 0.0;
   for (long col = col0; col < col1; ++col) {
     const int I = (int)(col / ncy);
     const int J = (int)(col - (long)I * ncy);
     const double x = px - Xc[I];
     const double y = py - Yc[J];
-    const double* wc = w + col * ncz;
+    const double* wc = w + col * ncz * NB;
     for (int K = 0; K < ncz; ++K) {
       const double z = pz - Zc[K];
-      const double wk = wc[K];
       double t[NC];
       corner_terms_f64<MASK>(x, y, z, t);
 #pragma unroll
-      for (int m = 0; m < NC; ++m) acc[m] = fma(wk, t[m], acc[m]);
+      for (int b = 0; b < NB; ++b) {
+        const double wk = wc[K * NB + b];
+#pragma unroll
+        for (int m = 0; m < NC; ++m) acc[b][m] = fma(wk, t[m], acc[b][m]);
+      }
     }
   }
   if (s < S) {
 #pragma unroll
-    for (int m = 0; m < NC; ++m) partial[((long)blockIdx.y * NC + m) * S + s] = acc[m];
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int m = 0; m < NC; ++m)
+        partial[(((long)b * gridDim.y + blockIdx.y) * NC + m) * S + s] = acc[b][m];
   }
 }
 
@@ -188,11 +203,12 @@ __global__ __launch_bounds__(kBlock) void k_misfit_to_raw(const double* __restri
 
 // Per-site coefficients of the 7 corner sums for the adjoint:
 // coef[s][m] = sum_k rraw[s][k] * dJ_k/dS_m where S_m is the corner sum of component m
-// (including its sign and G).  Stored in the kernel's component order for MASK.
+// (including its sign and G).  Stored in the kernel's component order for MASK at
+// coef[s * ld + m] (ld = NC for one model; NB * NC with model b at coef + b * NC).
 template <int MASK>
 __global__ __launch_bounds__(kBlock) void k_adjoint_coef(const double* __restrict__ rm, int S,
                                                          int kind, FieldDir fd,
-                                                         double* __restrict__ coef) {
+                                                         double* __restrict__ coef, int ld) {
   constexpr int NC = Ncomp<MASK>::value;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S) return;
@@ -201,15 +217,17 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_coef(const double* __restric
   double sc[NC];
   comp_scales<MASK>(sc);
   int k = 0;
-  if (MASK & kGz) coef[(long)s * NC + k] = c[0] * sc[k], ++k;
+  if (MASK & kGz) coef[(long)s * ld + k] = c[0] * sc[k], ++k;
   if (MASK & kDiag)
-    for (int m = 1; m <= 3; ++m) coef[(long)s * NC + k] = c[m] * sc[k], ++k;
+    for (int m = 1; m <= 3; ++m) coef[(long)s * ld + k] = c[m] * sc[k], ++k;
   if (MASK & kOff)
-    for (int m = 4; m <= 6; ++m) coef[(long)s * NC + k] = c[m] * sc[k], ++k;
+    for (int m = 4; m <= 6; ++m) coef[(long)s * ld + k] = c[m] * sc[k], ++k;
 }
 
-// Adjoint: one lane per corner; the wave walks a chunk of sites (wave-uniform).
-template <int MASK>
+// Adjoint: one lane per corner; the wave walks a chunk of sites (wave-uniform).  NB
+// residual sets share every corner evaluation (coefficients coef[(s * NB + b) * NC + m]);
+// the corner sums of set b go to hpart[(b * nchunks + chunk) * ncorner + q].
+template <int MASK, int NB>
 __global__ __launch_bounds__(kBlock) void k_adjoint_corners(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     const double* __restrict__ coef, int S, int sites_per_chunk,
@@ -225,16 +243,23 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_corners(
   const double cx = Xc[I], cy = Yc[J], cz = Zc[K];
   const int s0 = blockIdx.y * sites_per_chunk;
   const int s1 = s0 + sites_per_chunk < S ? s0 + sites_per_chunk : S;
-  double h = 0.0;
+  double h[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) h[b] = 0.0;
   for (int s = s0; s < s1; ++s) {
     const double x = sx[s] - cx, y = sy[s] - cy, z = sz[s] - cz;
     double tv[NC];
     corner_terms_f64<MASK>(x, y, z, tv);
-    const double* cs = coef + (long)s * NC;
+    const double* cs = coef + (long)s * NB * NC;
 #pragma unroll
-    for (int m = 0; m < NC; ++m) h = fma(cs[m], tv[m], h);
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int m = 0; m < NC; ++m) h[b] = fma(cs[b * NC + m], tv[m], h[b]);
   }
-  if (q < ncorner) hpart[(long)blockIdx.y * ncorner + q] = h;
+  if (q < ncorner) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) hpart[((long)b * gridDim.y + blockIdx.y) * ncorner + q] = h[b];
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_sum_chunks(const double* __restrict__ part,
@@ -370,9 +395,31 @@ int ncomp_of(int mask) {
   return ((mask & kGz) ? 1 : 0) + ((mask & kDiag) ? 3 : 0) + ((mask & kOff) ? 3 : 0);
 }
 
+// Launch k_forward_corners / k_adjoint_corners for a batch size known at run time.
+template <int MASK, class... Args>
+void launch_forward_corners(int nb, dim3 grid, hipStream_t st, Args... args) {
+  switch (nb) {
+    case 8: hipLaunchKernelGGL((k_forward_corners<MASK, 8>), grid, dim3(kBlock), 0, st, args...); break;
+    case 4: hipLaunchKernelGGL((k_forward_corners<MASK, 4>), grid, dim3(kBlock), 0, st, args...); break;
+    case 2: hipLaunchKernelGGL((k_forward_corners<MASK, 2>), grid, dim3(kBlock), 0, st, args...); break;
+    default: hipLaunchKernelGGL((k_forward_corners<MASK, 1>), grid, dim3(kBlock), 0, st, args...); break;
+  }
+}
+template <int MASK, class... Args>
+void launch_adjoint_corners(int nb, dim3 grid, hipStream_t st, Args... args) {
+  switch (nb) {
+    case 8: hipLaunchKernelGGL((k_adjoint_corners<MASK, 8>), grid, dim3(kBlock), 0, st, args...); break;
+    case 4: hipLaunchKernelGGL((k_adjoint_corners<MASK, 4>), grid, dim3(kBlock), 0, st, args...); break;
+    case 2: hipLaunchKernelGGL((k_adjoint_corners<MASK, 2>), grid, dim3(kBlock), 0, st, args...); break;
+    default: hipLaunchKernelGGL((k_adjoint_corners<MASK, 1>), grid, dim3(kBlock), 0, st, args...); break;
+  }
+}
+
+// Forward of nb in {1, 2, 4, 8} models (model b at models + b * model_ld, its data at
+// out + b * out_ld, raw data at raw_out + b * raw_ld when raw_out is given).
 template <int MASK>
-hipError_t forward_impl(jgm_context* ctx, int kind, const double* model, double* out,
-                        double* raw_out) {
+hipError_t forward_impl(jgm_context* ctx, int kind, int nb, const double* models, size_t model_ld,
+                        double* out, size_t out_ld, double* raw_out, size_t raw_ld) {
   constexpr int NC = Ncomp<MASK>::value;
   hipStream_t st = ctx->stream;
   const int S = (int)ctx->S;
@@ -392,36 +439,42 @@ hipError_t forward_impl(jgm_context* ctx, int kind, const double* model, double*
   nchunks = (int)((ncols + cols_per_chunk - 1) / cols_per_chunk);
   hipError_t e;
   if (N > 0) {
-    e = ctx->weights.ensure(sizeof(double) * (size_t)ncols * ncz);
+    e = ctx->weights.ensure(sizeof(double) * (size_t)nb * ncols * ncz);
     if (e != hipSuccess) return e;
-    e = ctx->partial.ensure(sizeof(double) * (size_t)nchunks * NC * S);
+    e = ctx->partial.ensure(sizeof(double) * (size_t)nb * nchunks * NC * S);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_corner_weights, dim3(blocks_for(ncols * ncz)), dim3(kBlock), 0, st,
-                       model, (int)ctx->nx, (int)ctx->ny, (int)ctx->nz,
+                       models, (long)model_ld, nb, (int)ctx->nx, (int)ctx->ny, (int)ctx->nz,
                        ctx->weights.as<double>());
     ProfScope ps(ctx, MASK == kMaskScalar ? "forward_corners_fp64_gz"
                       : MASK == kMaskAll  ? "forward_corners_fp64_gz_ftg"
                                           : "forward_corners_fp64_ftg");
-    hipLaunchKernelGGL((k_forward_corners<MASK>), dim3(site_blocks, nchunks), dim3(kBlock), 0, st,
-                       ctx->d_sx.as<double>(), ctx->d_sy.as<double>(), ctx->d_sz.as<double>(), S,
-                       ctx->d_Xc.as<double>(), ctx->d_Yc.as<double>(), ctx->d_Zc.as<double>(),
-                       ncy, ncz, ctx->weights.as<const double>(), ncols, cols_per_chunk,
-                       ctx->partial.as<double>());
+    launch_forward_corners<MASK>(nb, dim3(site_blocks, nchunks), st,
+                                 ctx->d_sx.as<const double>(), ctx->d_sy.as<const double>(),
+                                 ctx->d_sz.as<const double>(), S, ctx->d_Xc.as<const double>(),
+                                 ctx->d_Yc.as<const double>(), ctx->d_Zc.as<const double>(), ncy,
+                                 ncz, ctx->weights.as<const double>(), ncols, cols_per_chunk,
+                                 ctx->partial.as<double>());
   } else {
     nchunks = 0;
   }
   const double tf_bx = std::cos(ctx->tf_inc) * std::cos(ctx->tf_dec);
   const double tf_by = std::cos(ctx->tf_inc) * std::sin(ctx->tf_dec);
   const double tf_bz = std::sin(ctx->tf_inc);
-  hipLaunchKernelGGL((k_forward_finish<MASK>), dim3(blocks_for(S)), dim3(kBlock), 0, st,
-                     ctx->partial.as<const double>(), nchunks, S, ctx->bgT.as<const double>(),
-                     model + N, L, kind, field_dir(ctx), ctx->transform, tf_bx, tf_by, tf_bz,
-                     out, raw_out);
+  for (int b = 0; b < nb; ++b)
+    hipLaunchKernelGGL((k_forward_finish<MASK>), dim3(blocks_for(S)), dim3(kBlock), 0, st,
+                       ctx->partial.as<const double>() + (size_t)b * nchunks * NC * S, nchunks, S,
+                       ctx->bgT.as<const double>(), models + b * model_ld + N, L, kind,
+                       field_dir(ctx), ctx->transform, tf_bx, tf_by, tf_bz, out + b * out_ld,
+                       raw_out ? raw_out + b * raw_ld : nullptr);
   return hipGetLastError();
 }
 
+// Adjoint of nb in {1, 2, 4, 8} raw residual sets (set b at rm + b * rm_ld, its
+// gradient at grad + b * grad_ld).
 template <int MASK>
-hipError_t adjoint_impl(jgm_context* ctx, int kind, const double* rm, double* grad) {
+hipError_t adjoint_impl(jgm_context* ctx, int kind, int nb, const double* rm, size_t rm_ld,
+                        double* grad, size_t grad_ld) {
   constexpr int NC = Ncomp<MASK>::value;
   hipStream_t st = ctx->stream;
   const int S = (int)ctx->S;
@@ -429,41 +482,50 @@ hipError_t adjoint_impl(jgm_context* ctx, int kind, const double* rm, double* gr
   const int L = (int)ctx->bgthick.size();
   const int ncy = (int)ctx->ny + 1, ncz = (int)ctx->nz + 1;
   const long nq = (long)ncorners(ctx);
-  hipError_t e = ctx->coef.ensure(sizeof(double) * (size_t)S * NC);
+  hipError_t e = ctx->coef.ensure(sizeof(double) * (size_t)nb * S * NC);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_adjoint_coef<MASK>), dim3(blocks_for(S)), dim3(kBlock), 0, st, rm, S,
-                     kind, field_dir(ctx), ctx->coef.as<double>());
+  for (int b = 0; b < nb; ++b)
+    hipLaunchKernelGGL((k_adjoint_coef<MASK>), dim3(blocks_for(S)), dim3(kBlock), 0, st,
+                       rm + b * rm_ld, S, kind, field_dir(ctx), ctx->coef.as<double>() + b * NC,
+                       nb * NC);
   if (N > 0) {
     const long corner_waves = (nq + 63) / 64;
     int nchunks = (int)std::min<long>(S, std::max<long>(1, (kTargetWaves + corner_waves - 1) / corner_waves));
     const int per = (S + nchunks - 1) / nchunks;
     nchunks = (S + per - 1) / per;
-    e = ctx->hpart.ensure(sizeof(double) * (size_t)nchunks * nq);
+    e = ctx->hpart.ensure(sizeof(double) * (size_t)nb * nchunks * nq);
     if (e != hipSuccess) return e;
     {
     ProfScope ps(ctx, MASK == kMaskScalar ? "adjoint_corners_fp64_gz"
                       : MASK == kMaskAll  ? "adjoint_corners_fp64_gz_ftg"
                                           : "adjoint_corners_fp64_ftg");
-    hipLaunchKernelGGL((k_adjoint_corners<MASK>), dim3(blocks_for(nq), nchunks), dim3(kBlock), 0,
-                       st, ctx->d_sx.as<double>(), ctx->d_sy.as<double>(), ctx->d_sz.as<double>(),
-                       ctx->coef.as<const double>(), S, per, ctx->d_Xc.as<double>(),
-                       ctx->d_Yc.as<double>(), ctx->d_Zc.as<double>(), ncy, ncz, nq,
-                       ctx->hpart.as<double>());
+    launch_adjoint_corners<MASK>(nb, dim3(blocks_for(nq), nchunks), st,
+                                 ctx->d_sx.as<const double>(), ctx->d_sy.as<const double>(),
+                                 ctx->d_sz.as<const double>(), ctx->coef.as<const double>(), S,
+                                 per, ctx->d_Xc.as<const double>(), ctx->d_Yc.as<const double>(),
+                                 ctx->d_Zc.as<const double>(), ncy, ncz, nq,
+                                 ctx->hpart.as<double>());
     }
-    const double* H = ctx->hpart.as<const double>();
     if (nchunks > 1) {
       e = ctx->hsum.ensure(sizeof(double) * (size_t)nq);
       if (e != hipSuccess) return e;
-      hipLaunchKernelGGL(k_sum_chunks, dim3(blocks_for(nq)), dim3(kBlock), 0, st, H, nchunks, nq,
-                         ctx->hsum.as<double>());
-      H = ctx->hsum.as<const double>();
     }
-    hipLaunchKernelGGL(k_corner_difference, dim3(blocks_for(N)), dim3(kBlock), 0, st, H,
-                       (int)ctx->nx, (int)ctx->ny, (int)ctx->nz, grad);
+    for (int b = 0; b < nb; ++b) {
+      const double* H = ctx->hpart.as<const double>() + (size_t)b * nchunks * nq;
+      if (nchunks > 1) {
+        hipLaunchKernelGGL(k_sum_chunks, dim3(blocks_for(nq)), dim3(kBlock), 0, st, H, nchunks,
+                           nq, ctx->hsum.as<double>());
+        H = ctx->hsum.as<const double>();
+      }
+      hipLaunchKernelGGL(k_corner_difference, dim3(blocks_for(N)), dim3(kBlock), 0, st, H,
+                         (int)ctx->nx, (int)ctx->ny, (int)ctx->nz, grad + b * grad_ld);
+    }
   }
   if (L > 0) {
-    hipError_t eb = launch_background_gradient(ctx, kind, rm, grad + N);
-    if (eb != hipSuccess) return eb;
+    for (int b = 0; b < nb; ++b) {
+      hipError_t eb = launch_background_gradient(ctx, kind, rm + b * rm_ld, grad + b * grad_ld + N);
+      if (eb != hipSuccess) return eb;
+    }
   }
   return hipGetLastError();
 }
@@ -662,11 +724,11 @@ hipError_t maggrad_forward(jgm_context* ctx, const double* model, double* out,
     if ((e = ctx->partial.ensure(sizeof(double) * (size_t)nchunks * 6 * S2)) != hipSuccess)
       return e;
     hipLaunchKernelGGL(k_corner_weights, dim3(blocks_for(ncols * ncz)), dim3(kBlock), 0, st,
-                       model, (int)ctx->nx, (int)ctx->ny, (int)ctx->nz,
+                       model, 0L, 1, (int)ctx->nx, (int)ctx->ny, (int)ctx->nz,
                        ctx->weights.as<double>());
     const double* vs = ctx->vsites.as<const double>();
     ProfScope ps(ctx, "forward_corners_fp64_maggrad");
-    hipLaunchKernelGGL((k_forward_corners<kMaskTensor>), dim3((S2 + kBlock - 1) / kBlock, nchunks),
+    hipLaunchKernelGGL((k_forward_corners<kMaskTensor, 1>), dim3((S2 + kBlock - 1) / kBlock, nchunks),
                        dim3(kBlock), 0, st, vs, vs + S2, vs + 2 * S2, S2, ctx->d_Xc.as<double>(),
                        ctx->d_Yc.as<double>(), ctx->d_Zc.as<double>(), ncy, ncz,
                        ctx->weights.as<const double>(), ncols, per, ctx->partial.as<double>());
@@ -703,7 +765,7 @@ hipError_t maggrad_adjoint(jgm_context* ctx, const double* rm, double* grad) {
   const double* vs = ctx->vsites.as<const double>();
   {
     ProfScope ps(ctx, "adjoint_corners_fp64_maggrad");
-    hipLaunchKernelGGL((k_adjoint_corners<kMaskTensor>), dim3(blocks_for(nq), nchunks),
+    hipLaunchKernelGGL((k_adjoint_corners<kMaskTensor, 1>), dim3(blocks_for(nq), nchunks),
                        dim3(kBlock), 0, st, vs, vs + S2, vs + 2 * S2, ctx->coef.as<const double>(),
                        S2, per, ctx->d_Xc.as<double>(), ctx->d_Yc.as<double>(),
                        ctx->d_Zc.as<double>(), ncy, ncz, nq, ctx->hpart.as<double>());
@@ -737,7 +799,7 @@ hipError_t magnetization_forward(jgm_context* ctx, const double* model, double* 
       return e;
     for (int j = 0; j < 3; ++j)
       hipLaunchKernelGGL(k_corner_weights, dim3(blocks_for(nq)), dim3(kBlock), 0, st,
-                         model + j * N, (int)ctx->nx, (int)ctx->ny, (int)ctx->nz,
+                         model + j * N, 0L, 1, (int)ctx->nx, (int)ctx->ny, (int)ctx->nz,
                          ctx->weights.as<double>() + j * nq);
     ProfScope ps(ctx, "forward_corners_fp64_magnetization");
     hipLaunchKernelGGL(k_forward_corners_mag3, dim3((S + kBlock - 1) / kBlock, nchunks),
@@ -835,11 +897,40 @@ hipError_t launch_forward_fp64(jgm_context* ctx, int kind, const double* model, 
                                double* raw_out) {
   if (kind == JGM_KIND_MAGNETIC_GRADIENT) return maggrad_forward(ctx, model, out, raw_out);
   if (kind == JGM_KIND_MAGNETIZATION) return magnetization_forward(ctx, model, out, raw_out);
-  switch (mask_of(kind)) {
-    case kMaskScalar: return forward_impl<kMaskScalar>(ctx, kind, model, out, raw_out);
-    case kMaskAll: return forward_impl<kMaskAll>(ctx, kind, model, out, raw_out);
-    default: return forward_impl<kMaskTensor>(ctx, kind, model, out, raw_out);
+  return launch_forward_fp64_batch(ctx, kind, 1, model, 0, out, 0, raw_out, 0);
+}
+
+hipError_t launch_forward_fp64_batch(jgm_context* ctx, int kind, size_t nmodels,
+                                     const double* models, size_t model_ld, double* out,
+                                     size_t out_ld, double* raw_out, size_t raw_ld) {
+  for (size_t b0 = 0; b0 < nmodels;) {
+    const size_t left = nmodels - b0;
+    const int nb = left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
+    const double* m = models + b0 * model_ld;
+    double* o = out + b0 * out_ld;
+    double* r = raw_out ? raw_out + b0 * raw_ld : nullptr;
+    hipError_t e;
+    if (kind == JGM_KIND_MAGNETIC_GRADIENT || kind == JGM_KIND_MAGNETIZATION) {
+      e = kind == JGM_KIND_MAGNETIC_GRADIENT ? maggrad_forward(ctx, m, o, r)
+                                             : magnetization_forward(ctx, m, o, r);
+      b0 += 1;
+    } else {
+      switch (mask_of(kind)) {
+        case kMaskScalar:
+          e = forward_impl<kMaskScalar>(ctx, kind, nb, m, model_ld, o, out_ld, r, raw_ld);
+          break;
+        case kMaskAll:
+          e = forward_impl<kMaskAll>(ctx, kind, nb, m, model_ld, o, out_ld, r, raw_ld);
+          break;
+        default:
+          e = forward_impl<kMaskTensor>(ctx, kind, nb, m, model_ld, o, out_ld, r, raw_ld);
+          break;
+      }
+      b0 += nb;
+    }
+    if (e != hipSuccess) return e;
   }
+  return hipSuccess;
 }
 
 hipError_t launch_misfit_to_raw(jgm_context* ctx, int kind, const double* misfit,
@@ -855,13 +946,39 @@ hipError_t launch_misfit_to_raw(jgm_context* ctx, int kind, const double* misfit
 
 hipError_t launch_adjoint_fp64(jgm_context* ctx, int kind, const double* /*model*/,
                                const double* raw_misfit, double* grad) {
-  if (kind == JGM_KIND_MAGNETIC_GRADIENT) return maggrad_adjoint(ctx, raw_misfit, grad);
-  if (kind == JGM_KIND_MAGNETIZATION) return magnetization_adjoint(ctx, raw_misfit, grad);
-  switch (mask_of(kind)) {
-    case kMaskScalar: return adjoint_impl<kMaskScalar>(ctx, kind, raw_misfit, grad);
-    case kMaskAll: return adjoint_impl<kMaskAll>(ctx, kind, raw_misfit, grad);
-    default: return adjoint_impl<kMaskTensor>(ctx, kind, raw_misfit, grad);
+  return launch_adjoint_fp64_batch(ctx, kind, 1, raw_misfit, 0, grad, 0);
+}
+
+hipError_t launch_adjoint_fp64_batch(jgm_context* ctx, int kind, size_t nsets,
+                                     const double* raw_misfits, size_t misfit_ld, double* grads,
+                                     size_t grad_ld) {
+  for (size_t b0 = 0; b0 < nsets;) {
+    const size_t left = nsets - b0;
+    const int nb = left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
+    const double* r = raw_misfits + b0 * misfit_ld;
+    double* g = grads + b0 * grad_ld;
+    hipError_t e;
+    if (kind == JGM_KIND_MAGNETIC_GRADIENT || kind == JGM_KIND_MAGNETIZATION) {
+      e = kind == JGM_KIND_MAGNETIC_GRADIENT ? maggrad_adjoint(ctx, r, g)
+                                             : magnetization_adjoint(ctx, r, g);
+      b0 += 1;
+    } else {
+      switch (mask_of(kind)) {
+        case kMaskScalar:
+          e = adjoint_impl<kMaskScalar>(ctx, kind, nb, r, misfit_ld, g, grad_ld);
+          break;
+        case kMaskAll:
+          e = adjoint_impl<kMaskAll>(ctx, kind, nb, r, misfit_ld, g, grad_ld);
+          break;
+        default:
+          e = adjoint_impl<kMaskTensor>(ctx, kind, nb, r, misfit_ld, g, grad_ld);
+          break;
+      }
+      b0 += nb;
+    }
+    if (e != hipSuccess) return e;
   }
+  return hipSuccess;
 }
 
 hipError_t launch_sensitivity_rows(jgm_context* ctx, int kind, size_t site0, size_t nsites,

@@ -392,26 +392,33 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jtr(const float* __restrict__ J
 
 // One model (or gradient) vector as the first right-hand side of a padded FP32
 // operand Xt[npad][64]; the other 63 columns and the padding rows are zero.
-__global__ __launch_bounds__(kBlock) void k_vec_to_rhs(const double* __restrict__ v, long n,
-                                                       long npad, float* __restrict__ Xt) {
+// FP64 model vectors (model b at v + b * v_ld, b < nb) -> right-hand sides of
+// Xt[npad][P] (FP32; columns nb.. and rows n.. zero).
+__global__ __launch_bounds__(kBlock) void k_vec_to_rhs(const double* __restrict__ v, long v_ld,
+                                                       int nb, long n, long npad, int P,
+                                                       float* __restrict__ Xt) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= npad * 64) return;
-  const long k = idx >> 6;
-  Xt[idx] = ((idx & 63) == 0 && k < n) ? (float)v[k] : 0.0f;
+  if (idx >= npad * P) return;
+  const long k = idx / P;
+  const int b = (int)(idx - k * P);
+  Xt[idx] = (b < nb && k < n) ? (float)v[b * v_ld + k] : 0.0f;
 }
 
-// Raw misfit (raw layout per site) -> compact residual rows as the first right-hand
-// side of Xt[ld][64]: the symmetric tensor entries appear twice in the raw layout, so
-// their compact rows receive the sum of both residuals.
-__global__ __launch_bounds__(kBlock) void k_rm_to_compact(const double* __restrict__ rm, int S,
-                                                          int kind, long ld,
+// Raw misfits (raw layout per site; set b at rm + b * rm_ld, b < nb) -> compact
+// residual rows as right-hand sides of Xt[ld][P]: the symmetric tensor entries appear
+// twice in the raw layout, so their compact rows receive the sum of both residuals.
+__global__ __launch_bounds__(kBlock) void k_rm_to_compact(const double* __restrict__ rm0,
+                                                          long rm_ld, int nb, int S, int kind,
+                                                          long ld, int P,
                                                           float* __restrict__ Xt) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= ld * 64) return;
-  const long row = idx >> 6;
+  if (idx >= ld * P) return;
+  const long row = idx / P;
+  const int b = (int)(idx - row * P);
   const int kJ = compact_rows(kind);
   float v = 0.0f;
-  if ((idx & 63) == 0 && row < (long)S * kJ) {
+  if (b < nb && row < (long)S * kJ) {
+    const double* rm = rm0 + b * rm_ld;
     const long s = row / kJ;
     int q = (int)(row % kJ);
     if (kind == JGM_KIND_SCALAR) {
@@ -614,46 +621,84 @@ hipError_t launch_storedj_apply(jgm_context* ctx, int gemm_precision, int transp
 
 hipError_t launch_storedj_forward(jgm_context* ctx, int gemm_precision, const double* model,
                                   double* out, double* raw_out) {
+  return launch_storedj_forward_batch(ctx, gemm_precision, 1, model, 0, out, 0, raw_out, 0);
+}
+
+// Up to 64 models per J M product (the product's right-hand-side tile is 64 wide, so
+// a batch costs what one model costs); larger batches take several products.
+hipError_t launch_storedj_forward_batch(jgm_context* ctx, int gemm_precision, size_t nmodels,
+                                        const double* models, size_t model_ld, double* out,
+                                        size_t out_ld, double* raw_out, size_t raw_ld) {
   const long R = (long)ctx->storedJ_rows, N = (long)ctx->storedJ_cols;
   const long colpad = (long)ctx->storedJ_colpad;
   const int kind = ctx->storedJ_kind;
   const int S = (int)ctx->S;
   const int L = (int)ctx->bgthick.size();
-  hipError_t e = ctx->coef32.ensure(sizeof(float) * (size_t)colpad * 64);
+  const int P = 64;
+  hipError_t e = ctx->coef32.ensure(sizeof(float) * (size_t)colpad * P);
   if (e != hipSuccess) return e;
-  if ((e = ctx->storedJ_vec.ensure(sizeof(float) * (size_t)std::max(R, N))) != hipSuccess) return e;
-  float* Xt = ctx->coef32.as<float>();
-  hipLaunchKernelGGL(k_vec_to_rhs, dim3(blocks_for(colpad * 64)), dim3(kBlock), 0, ctx->stream,
-                     model, N, colpad, Xt);
-  if ((e = run_jm(ctx, gemm_precision, Xt, 64, 1, ctx->storedJ_vec.as<float>())) != hipSuccess)
+  const size_t nb_max = std::min<size_t>(nmodels, (size_t)P);
+  if ((e = ctx->storedJ_vec.ensure(sizeof(float) * (size_t)std::max(R * (long)nb_max, N))) !=
+      hipSuccess)
     return e;
+  float* Xt = ctx->coef32.as<float>();
   const double tf_bx = std::cos(ctx->tf_inc) * std::cos(ctx->tf_dec);
   const double tf_by = std::cos(ctx->tf_inc) * std::sin(ctx->tf_dec);
   const double tf_bz = std::sin(ctx->tf_inc);
-  hipLaunchKernelGGL(k_storedj_finish, dim3(blocks_for(S)), dim3(kBlock), 0, ctx->stream,
-                     ctx->storedJ_vec.as<const float>(), S, kind, ctx->bgT.as<const double>(),
-                     model + N, L, field_dir(ctx), ctx->transform, tf_bx, tf_by, tf_bz, out,
-                     raw_out);
-  return hipGetLastError();
+  for (size_t b0 = 0; b0 < nmodels; b0 += P) {
+    const int nb = (int)std::min<size_t>(nmodels - b0, (size_t)P);
+    const double* m = models + b0 * model_ld;
+    hipLaunchKernelGGL(k_vec_to_rhs, dim3(blocks_for(colpad * P)), dim3(kBlock), 0, ctx->stream,
+                       m, (long)model_ld, nb, N, colpad, P, Xt);
+    float* Y = ctx->storedJ_vec.as<float>();
+    if ((e = run_jm(ctx, gemm_precision, Xt, P, nb, Y)) != hipSuccess) return e;
+    for (int b = 0; b < nb; ++b)
+      hipLaunchKernelGGL(k_storedj_finish, dim3(blocks_for(S)), dim3(kBlock), 0, ctx->stream,
+                         Y + (size_t)b * R, S, kind, ctx->bgT.as<const double>(),
+                         m + b * model_ld + N, L, field_dir(ctx), ctx->transform, tf_bx, tf_by,
+                         tf_bz, out + (b0 + b) * out_ld,
+                         raw_out ? raw_out + (b0 + b) * raw_ld : nullptr);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_storedj_gradient(jgm_context* ctx, int gemm_precision,
                                    const double* raw_misfit, double* grad) {
+  return launch_storedj_gradient_batch(ctx, gemm_precision, 1, raw_misfit, 0, grad, 0);
+}
+
+hipError_t launch_storedj_gradient_batch(jgm_context* ctx, int gemm_precision, size_t nsets,
+                                         const double* raw_misfits, size_t misfit_ld,
+                                         double* grads, size_t grad_ld) {
   const long R = (long)ctx->storedJ_rows, N = (long)ctx->storedJ_cols;
   const long ld = (long)ctx->storedJ_ld;
   const int kind = ctx->storedJ_kind;
-  hipError_t e = ctx->coef32.ensure(sizeof(float) * (size_t)ld * 64);
+  const int P = 64;
+  hipError_t e = ctx->coef32.ensure(sizeof(float) * (size_t)ld * P);
   if (e != hipSuccess) return e;
-  if ((e = ctx->storedJ_vec.ensure(sizeof(float) * (size_t)std::max(R, N))) != hipSuccess) return e;
-  float* Xt = ctx->coef32.as<float>();
-  hipLaunchKernelGGL(k_rm_to_compact, dim3(blocks_for(ld * 64)), dim3(kBlock), 0, ctx->stream,
-                     raw_misfit, (int)ctx->S, kind, ld, Xt);
-  if ((e = run_jtr(ctx, gemm_precision, Xt, 64, 1, ctx->storedJ_vec.as<float>())) != hipSuccess)
+  const size_t nb_max = std::min<size_t>(nsets, (size_t)P);
+  if ((e = ctx->storedJ_vec.ensure(sizeof(float) * (size_t)std::max(R, N * (long)nb_max))) !=
+      hipSuccess)
     return e;
-  hipLaunchKernelGGL(k_scale_to_f64, dim3(blocks_for(N)), dim3(kBlock), 0, ctx->stream,
-                     ctx->storedJ_vec.as<const float>(), N, 2.0, grad);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  return launch_background_gradient(ctx, kind, raw_misfit, grad + N);
+  float* Xt = ctx->coef32.as<float>();
+  for (size_t b0 = 0; b0 < nsets; b0 += P) {
+    const int nb = (int)std::min<size_t>(nsets - b0, (size_t)P);
+    const double* rm = raw_misfits + b0 * misfit_ld;
+    hipLaunchKernelGGL(k_rm_to_compact, dim3(blocks_for(ld * P)), dim3(kBlock), 0, ctx->stream,
+                       rm, (long)misfit_ld, nb, (int)ctx->S, kind, ld, P, Xt);
+    float* G = ctx->storedJ_vec.as<float>();
+    if ((e = run_jtr(ctx, gemm_precision, Xt, P, nb, G)) != hipSuccess) return e;
+    for (int b = 0; b < nb; ++b) {
+      double* g = grads + (b0 + b) * grad_ld;
+      hipLaunchKernelGGL(k_scale_to_f64, dim3(blocks_for(N)), dim3(kBlock), 0, ctx->stream,
+                         G + (size_t)b * N, N, 2.0, g);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+      if ((e = launch_background_gradient(ctx, kind, rm + b * misfit_ld, g + N)) != hipSuccess)
+        return e;
+    }
+  }
+  return hipSuccess;
 }
 
 }  // namespace jgm

@@ -111,6 +111,21 @@ class HipGravMagImplementation:
             raise ValueError(f"misfit has {misfit.size} entries, expected {need}")
         return ctx.adjoint(self.kind, self.precision, model.GetModelParameters(), misfit)
 
+    def CalculateBatch(self, model, data, models):
+        """Calculate for several model vectors on the geometry of `model` (rows of
+        `models`); FP64 batches share the corner-function evaluations."""
+        ctx = self.context(model, data)
+        return ctx.forward_batch(self.kind, self.precision, models)
+
+    def LQDerivativeBatch(self, model, data, models, misfits):
+        """LQDerivative for several (model vector, misfit) pairs, one per row."""
+        ctx = self.context(model, data)
+        misfits = np.asarray(misfits, dtype=np.float64)
+        need = data.nsites * self.GetDataPerMeasurement()
+        if misfits.ndim != 2 or misfits.shape[1] != need:
+            raise ValueError(f"misfits must have shape (B, {need})")
+        return ctx.adjoint_batch(self.kind, self.precision, models, misfits)
+
     def SensitivityRows(self, model, data, site0=0, nsites=None):
         """Raw FP64 J rows (reference layout) of a site range, copied to the host."""
         ctx = self.context(model, data)
@@ -196,6 +211,16 @@ class ThreeDGravMagCalculator:
         self.CheckModelConsistency(model)
         return self.Imp.LQDerivative(model, data, misfit)
 
+    def CalculateBatch(self, model, data, models):
+        """Forward responses of several model vectors (rows) on the geometry of `model`:
+        the batched form of Calculate for line-search trials."""
+        self.CheckModelConsistency(model)
+        return self.Imp.CalculateBatch(model, data, models)
+
+    def LQDerivativeBatch(self, model, data, models, misfits):
+        self.CheckModelConsistency(model)
+        return self.Imp.LQDerivativeBatch(model, data, models, misfits)
+
 
 class MinMemGravMagCalculator(ThreeDGravMagCalculator):
     """No stored sensitivities: forward and gradient are recomputed matrix-free."""
@@ -232,6 +257,32 @@ class CachedGravMagCalculator(ThreeDGravMagCalculator):
             return self.CachedLQDerivative(model, data, misfit)
         return self.Imp.LQDerivative(model, data, misfit)
 
+    def CalculateBatch(self, model, data, models):
+        """With a valid cache every model goes through the cached path in one batch;
+        otherwise the first model is calculated as new (filling the cache, as
+        Calculate would) and the others through the cache."""
+        self.CheckModelConsistency(model)
+        models = np.atleast_2d(np.asarray(models, dtype=np.float64))
+        key = self._key(model, data)
+        if self._cache_key == key:
+            return self.CalculateCachedResultBatch(model, data, models)
+        saved = model.GetModelParameters().copy()
+        model.SetModelParameters(models[0])
+        try:
+            first = self.CalculateNewModel(model, data)
+        finally:
+            model.SetModelParameters(saved)
+        self._cache_key = key
+        rest = (self.CalculateCachedResultBatch(model, data, models[1:]) if len(models) > 1
+                else np.zeros((0, first.size)))
+        return np.vstack([first[None, :], rest])
+
+    def LQDerivativeBatch(self, model, data, models, misfits):
+        self.CheckModelConsistency(model)
+        if self._cache_key == self._key(model, data):
+            return self.CachedLQDerivativeBatch(model, data, models, misfits)
+        return self.Imp.LQDerivativeBatch(model, data, models, misfits)
+
 
 class FullSensitivityGravMagCalculator(CachedGravMagCalculator):
     """Stored sensitivities (jif3D/GravMag/FullSensitivityGravMagCalculator.h:99-221),
@@ -259,3 +310,15 @@ class FullSensitivityGravMagCalculator(CachedGravMagCalculator):
         if misfit.size != need:
             raise ValueError(f"misfit has {misfit.size} entries, expected {need}")
         return ctx.storedj_gradient(self.gemm_precision, model.GetModelParameters(), misfit)
+
+    def CalculateCachedResultBatch(self, model, data, models):
+        ctx = self.Imp.context(model, data)
+        return ctx.storedj_forward_batch(self.gemm_precision, models)
+
+    def CachedLQDerivativeBatch(self, model, data, models, misfits):
+        ctx = self.Imp.context(model, data)
+        misfits = np.asarray(misfits, dtype=np.float64)
+        need = data.nsites * self.GetDataPerMeasurement()
+        if misfits.ndim != 2 or misfits.shape[1] != need:
+            raise ValueError(f"misfits must have shape (B, {need})")
+        return ctx.storedj_gradient_batch(self.gemm_precision, models, misfits)

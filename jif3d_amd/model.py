@@ -214,12 +214,38 @@ class ThreeDMagnetizationModel(ThreeDModelBase):
     def GetNModelParm(self):
         return 3 * self.ncells
 
+    def SetModelParameters(self, m):
+        m = np.asarray(m, dtype=np.float64)
+        n = self.ncells
+        self.SetMagnetization(m[:n], m[n:2 * n], m[2 * n:3 * n])
+
 
 class GeneralData:
-    """Measurement positions (and optionally data/errors)."""
+    """Measurement positions and optionally data with errors
+    (jif3D/DataBase/GeneralData.h:19-142)."""
 
     def __init__(self):
         self._x, self._y, self._z = [], [], []
+        self._data = np.zeros(0)
+        self._errors = np.zeros(0)
+
+    def GetData(self):
+        return self._data
+
+    def GetErrors(self):
+        return self._errors
+
+    def SetDataAndErrors(self, d, e):
+        d, e = np.asarray(d, dtype=np.float64), np.asarray(e, dtype=np.float64)
+        if d.shape != e.shape:
+            raise ValueError("Data and Errors do not have the same size !")
+        self._data, self._errors = d.copy(), e.copy()
+
+    def SetErrors(self, e):
+        e = np.asarray(e, dtype=np.float64)
+        if e.shape != self._data.shape:
+            raise ValueError("Data and Errors do not have the same size !")
+        self._errors = e.copy()
 
     def AddMeasurementPoint(self, x, y, z):
         self._x.append(float(x))

@@ -1,0 +1,206 @@
+"""Batched forward and gradient (several models per call) on the GPU.
+
+Each batched result must equal the single-model call bit for bit: the batch shares the
+corner-function evaluations (matrix-free FP64) or the pass over the stored J, and every
+model's sums run in the same order as alone.  Batch sizes cover the 8/4/2/1 split of the
+FP64 kernels and more than the 64 right-hand sides of one stored-J product.  The
+objective layer is checked against the oracle and by central differences like
+jif3D/Gravity/test_GravityObjective.cpp:31-110."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from jif3d_amd import _native as nat
+from jif3d_amd.imp import (ScalarHipGravityImp, TensorHipGravityImp, ScalarTensorHipGravityImp,
+                           HipMagneticSusceptibilityImp, HipMagneticGradImp,
+                           HipMagnetizationImp, MinMemGravMagCalculator,
+                           FullSensitivityGravMagCalculator, FTGInvariant, TotalFieldAnomaly)
+from jif3d_amd.model import (ThreeDGravityModel, ThreeDSusceptibilityModel,
+                             ThreeDMagnetizationModel, TensorGravityData)
+from jif3d_amd.objective import ThreeDModelObjective
+
+pytestmark = pytest.mark.gpu
+FIELD = (1.1, 0.2, 50000.0)
+
+
+def geometry(model, seed, nx=9, ny=8, nz=6, nmeas=23, bg=True):
+    rng = np.random.default_rng(seed)
+    model.SetMeshSize(nx, ny, nz)
+    model.SetXCellSizes(rng.integers(10, 30, nx) + 0.0)
+    model.SetYCellSizes(rng.integers(10, 30, ny) + 0.0)
+    model.SetZCellSizes(rng.integers(5, 20, nz) + 0.0)
+    if bg and hasattr(model, "SetBackgroundThicknesses"):
+        model.SetBackgroundThicknesses([15.0, 200.0, 1e4])
+    d = TensorGravityData()
+    xl, yl = sum(model.sizes[0]), sum(model.sizes[1])
+    for _ in range(nmeas):
+        d.AddMeasurementPoint(rng.uniform(0, xl), rng.uniform(0, yl), -rng.uniform(1, 60))
+    return d, rng
+
+
+def models_for(ctx, kind, rng, nb):
+    n = ctx.model_size(kind)
+    return 1.0 + rng.random((nb, n)) * (0.05 if kind in (nat.KIND_MAGNETIC,
+                                                           nat.KIND_MAGNETIC_GRADIENT) else 10.0)
+
+
+KINDS = [nat.KIND_SCALAR, nat.KIND_TENSOR, nat.KIND_MAGNETIC, nat.KIND_SCALAR_TENSOR,
+         nat.KIND_MAGNETIC_GRADIENT, nat.KIND_MAGNETIZATION]
+
+
+def context(kind, seed=1):
+    m = ThreeDMagnetizationModel() if kind == nat.KIND_MAGNETIZATION else ThreeDGravityModel()
+    d, rng = geometry(m, seed)
+    ctx = nat.Context(0)
+    ctx.configure(m, d, FIELD)
+    return ctx, rng
+
+
+@pytest.mark.parametrize("prec", [nat.FP64, nat.MIXED])
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("nb", [3, 8, 13])
+def test_batch_equals_single(kind, prec, nb):
+    if prec == nat.MIXED and kind in (nat.KIND_MAGNETIC_GRADIENT, nat.KIND_MAGNETIZATION):
+        pytest.skip("FP64-only kind")
+    ctx, rng = context(kind, seed=nb)
+    M = models_for(ctx, kind, rng, nb)
+    F = ctx.forward_batch(kind, prec, M)
+    nout = ctx.nsites * ctx.data_per_site(kind)
+    R = rng.normal(size=(nb, nout))
+    G = ctx.adjoint_batch(kind, prec, M, R)
+    assert F.shape == (nb, nout) and G.shape == (nb, ctx.model_size(kind))
+    for b in range(nb):
+        assert np.array_equal(F[b], ctx.forward(kind, prec, M[b])), b
+        assert np.array_equal(G[b], ctx.adjoint(kind, prec, M[b], R[b])), b
+
+
+@pytest.mark.parametrize("kind,tf", [(nat.KIND_TENSOR, nat.TF_FTG_INVARIANT),
+                                     (nat.KIND_MAGNETIC, nat.TF_TOTAL_FIELD_ANOMALY),
+                                     (nat.KIND_MAGNETIZATION, nat.TF_TOTAL_FIELD)])
+def test_batch_with_transform(kind, tf):
+    """The transform Jacobian is taken at each model of the batch."""
+    ctx, rng = context(kind, seed=5)
+    ctx.set_transform(tf, 0.9, 0.3)
+    M = models_for(ctx, kind, rng, 6)
+    F = ctx.forward_batch(kind, nat.FP64, M)
+    R = rng.normal(size=F.shape)
+    G = ctx.adjoint_batch(kind, nat.FP64, M, R)
+    for b in range(6):
+        assert np.array_equal(F[b], ctx.forward(kind, nat.FP64, M[b]))
+        assert np.array_equal(G[b], ctx.adjoint(kind, nat.FP64, M[b], R[b]))
+
+
+def test_batch_device_arrays_with_leading_dimensions():
+    """Device entry points with padded leading dimensions leave the padding alone."""
+    kind = nat.KIND_SCALAR_TENSOR
+    ctx, rng = context(kind, seed=7)
+    nb, n = 5, ctx.model_size(kind)
+    nout = ctx.nsites * ctx.data_per_site(kind)
+    mld, old, gld = n + 3, nout + 5, n + 7
+    M = models_for(ctx, kind, rng, nb)
+    Mp = np.zeros((nb, mld))
+    Mp[:, :n] = M
+    R = rng.normal(size=(nb, nout))
+    Rp = np.zeros((nb, old))
+    Rp[:, :nout] = R
+    dM, dR = ctx.to_device(Mp.ravel()), ctx.to_device(Rp.ravel())
+    dF = ctx.to_device(np.full(nb * old, 7.0))
+    dG = ctx.to_device(np.full(nb * gld, 7.0))
+    lib, h = ctx.lib, ctx.handle
+    nat.check(lib.jgm_forward_batch(h, kind, nat.FP64, nb, ctypes.c_void_p(dM.ptr), mld,
+                                    ctypes.c_void_p(dF.ptr), old))
+    nat.check(lib.jgm_adjoint_batch(h, kind, nat.FP64, nb, ctypes.c_void_p(dM.ptr), mld,
+                                    ctypes.c_void_p(dR.ptr), old, ctypes.c_void_p(dG.ptr), gld))
+    ctx.synchronize()
+    F = dF.to_host().reshape(nb, old)
+    G = dG.to_host().reshape(nb, gld)
+    assert np.all(F[:, nout:] == 7.0) and np.all(G[:, n:] == 7.0)
+    for b in range(nb):
+        assert np.array_equal(F[b, :nout], ctx.forward(kind, nat.FP64, M[b]))
+        assert np.array_equal(G[b, :n], ctx.adjoint(kind, nat.FP64, M[b], R[b]))
+    # leading dimensions smaller than a model or a data vector are refused
+    st = lib.jgm_forward_batch(h, kind, nat.FP64, nb, ctypes.c_void_p(dM.ptr), n - 1,
+                               ctypes.c_void_p(dF.ptr), old)
+    assert st == nat.JGM_ERR_INVALID_ARGUMENT
+    assert lib.jgm_forward_batch(h, kind, nat.FP64, 0, None, 0, None, 0) == nat.JGM_OK
+
+
+@pytest.mark.parametrize("gemm", [nat.GEMM_FP32, nat.GEMM_BF16_REFINED])
+@pytest.mark.parametrize("nb", [3, 70])
+def test_storedj_batch_equals_single(gemm, nb):
+    kind = nat.KIND_SCALAR_TENSOR
+    ctx, rng = context(kind, seed=9)
+    ctx.storedj_build(kind, nat.FP64)
+    M = models_for(ctx, kind, rng, nb)
+    F = ctx.storedj_forward_batch(gemm, M)
+    R = rng.normal(size=F.shape)
+    G = ctx.storedj_gradient_batch(gemm, None, R)
+    for b in sorted({0, 1, nb // 2, nb - 1}):
+        assert np.array_equal(F[b], ctx.storedj_forward(gemm, M[b])), b
+        assert np.array_equal(G[b], ctx.storedj_gradient(gemm, None, R[b])), b
+    # and the cached batch agrees with the direct FP64 batch to the FP32 J's accuracy
+    D = ctx.forward_batch(kind, nat.FP64, M)
+    assert np.max(np.abs(F - D)) <= 2e-6 * np.max(np.abs(D))
+
+
+def objective_problem(calc_cls, seed=11, **kw):
+    m = ThreeDGravityModel()
+    d, rng = geometry(m, seed, nx=6, ny=5, nz=4, nmeas=9)
+    m.SetDensities(1.0 + 10.0 * rng.random(m.ncells))
+    m.SetBackgroundDensities([2.0, 3.0, 1.5])
+    calc = calc_cls(TensorHipGravityImp(), **kw)
+    obs = calc.Calculate(m, d) * 1.1
+    d.SetDataAndErrors(obs, np.abs(obs) * 0.02 + 1e-12)
+    obj = ThreeDModelObjective(calc)
+    obj.SetObservedData(d)
+    obj.SetCoarseModelGeometry(m)
+    return obj, m, d, rng
+
+
+@pytest.mark.parametrize("calc_cls", [MinMemGravMagCalculator, FullSensitivityGravMagCalculator])
+def test_objective_batch_matches_single(calc_cls):
+    obj, m, d, rng = objective_problem(calc_cls)
+    m0 = m.GetModelParameters().copy()
+    trials = m0[None, :] * (1.0 + 0.05 * rng.uniform(-1, 1, (5, m0.size)))
+    chis = obj.CalcMisfitBatch(trials)
+    grads = obj.CalcGradientBatch(trials)
+    for b in range(5):
+        chi = obj.CalcMisfit(trials[b])
+        g = obj.CalcGradient(trials[b])
+        assert abs(chis[b] - chi) <= 1e-12 * chi
+        assert np.array_equal(grads[b], g)
+    with pytest.raises(ValueError):
+        obj.CalcGradientBatch(trials[:2])
+
+
+def test_objective_gradient_fd_and_oracle(orc):
+    """CheckGradient of test_GravityObjective.cpp:31-51 (delta = 1 % of each parameter,
+    agreement to 0.001 %), on the batched path; and the gradient against the oracle's
+    LQDerivative of the weighted difference."""
+    obj, m, d, rng = objective_problem(MinMemGravMagCalculator)
+    m0 = m.GetModelParameters().copy()
+    chi = obj.CalcMisfitBatch(m0[None, :])
+    g = obj.CalcGradientBatch(m0[None, :])[0]
+    n = m0.size
+    delta = 0.01 * m0
+    trials = np.vstack([m0 + np.diag(delta), m0 - np.diag(delta)])
+    f = obj.CalcMisfitBatch(trials)
+    fd = (f[:n] - f[n:]) / (2 * delta)
+    assert np.max(np.abs(fd - g) / np.maximum(np.abs(g), 1e-6 * np.max(np.abs(g)))) <= 1e-5
+    w = (obj.CalcMisfit(m0), obj.DataDifference)[1]
+    gref = orc.lqderivative(orc.TENSOR, m, m0, d.positions, w)
+    assert np.max(np.abs(g - gref)) <= 1e-10 * np.max(np.abs(gref))
+    assert chi[0] > 0
+
+
+def test_evaluate_along_direction():
+    obj, m, d, rng = objective_problem(MinMemGravMagCalculator, seed=12)
+    m0 = m.GetModelParameters().copy()
+    obj.CalcMisfit(m0)
+    p = -obj.CalcGradient(m0)
+    steps = np.array([0.0, 1e-3, 1e-2, 1e-1]) * np.max(np.abs(m0)) / np.max(np.abs(p))
+    chis, grads = obj.EvaluateAlongDirection(m0, p, steps)
+    assert chis.shape == (4,) and grads.shape == (4, m0.size)
+    assert abs(chis[0] - obj.CalcMisfit(m0)) <= 1e-12 * chis[0]
+    assert chis[1] < chis[0]  # a small step along the negative gradient decreases chi^2
