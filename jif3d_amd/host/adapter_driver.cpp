@@ -3,18 +3,27 @@
 // through HipFullSensitivityCalculator: a direct Calculate that stores J, then the
 // cached Calculate and LQDerivative) and writes the results.
 //
+// --batch B: B models at once, model b = model * (1 + 0.05 b) and misfit b =
+//            misfit * (b + 1), through CalculateBatch / LQDerivativeBatch.
+// --objective: with --batch B, GravMagObjective over the same calculator with the
+//            misfit vector as observed data (errors 1): writes B chi^2 values, then the
+//            B gradients.
+//
 // input  (little-endian): int32 kind, int32 precision, int32 nx, ny, nz, nbg, nsites;
 //        then doubles: xcoords[nx+1], ycoords[ny+1], zcoords[nz+1], xsizes[nx],
 //        ysizes[ny], zsizes[nz], bg[nbg], sx[ns], sy[ns], sz[ns], field[3],
 //        model[nx*ny*nz + nbg], misfit[ns*raw_per_site]
-// output: doubles forward[ns*raw_per_site], gradient[nx*ny*nz + nbg]
+// output: doubles forward[ns*raw_per_site], gradient[nx*ny*nz + nbg] (per model for
+//         --batch, all forwards first)
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <iostream>
 #include <string>
 #include <vector>
 
+#include "gravmag_objective.hpp"
 #include "hip_gravmag_imp.hpp"
 
 namespace {
@@ -26,10 +35,51 @@ std::vector<double> read_doubles(std::ifstream& in, size_t n) {
 }
 }  // namespace
 
+template <class Calc>
+void run(Calc& calc, const jif3d_amd::Mesh& mesh, const jif3d_amd::Sites& sites,
+         const std::vector<double>& model, const std::vector<double>& misfit, size_t nbatch,
+         bool objective, bool prime, std::vector<double>* first, std::vector<double>* second) {
+  if (prime) calc.Calculate(mesh, sites, model);  // direct; stores J
+  if (nbatch == 0) {
+    *first = calc.Calculate(mesh, sites, model);
+    *second = calc.LQDerivative(mesh, sites, model, misfit);
+    return;
+  }
+  std::vector<double> models, misfits;
+  for (size_t b = 0; b < nbatch; ++b) {
+    for (double v : model) models.push_back(v * (1.0 + 0.05 * b));
+    for (double v : misfit) misfits.push_back(v * (b + 1.0));
+  }
+  if (objective) {
+    jif3d_amd::GravMagObjective<Calc> obj(calc, mesh, sites);
+    obj.SetObservedData(misfit, std::vector<double>(misfit.size(), 1.0));
+    *first = obj.CalcMisfitBatch(models, nbatch);
+    *second = obj.CalcGradientBatch(models, nbatch);
+    return;
+  }
+  *first = calc.CalculateBatch(mesh, sites, models, nbatch);
+  *second = calc.LQDerivativeBatch(mesh, sites, models, misfits, nbatch);
+}
+
 int main(int argc, char** argv) {
-  const bool full = argc == 4 && std::string(argv[3]) == "--full";
-  if (argc != 3 && !full) {
-    std::cerr << "usage: adapter_driver <input> <output> [--full]\n";
+  bool full = false, objective = false, ok = argc >= 3;
+  size_t nbatch = 0;
+  for (int a = 3; a < argc && ok; ++a) {
+    const std::string opt(argv[a]);
+    if (opt == "--full") {
+      full = true;
+    } else if (opt == "--objective") {
+      objective = true;
+    } else if (opt == "--batch" && a + 1 < argc) {
+      char* end = nullptr;
+      nbatch = std::strtoul(argv[++a], &end, 10);
+      ok = nbatch > 0 && end && *end == '\0';
+    } else {
+      ok = false;
+    }
+  }
+  if (!ok || (objective && nbatch == 0)) {
+    std::cerr << "usage: adapter_driver <input> <output> [--full] [--batch B [--objective]]\n";
     return 2;
   }
   try {
@@ -60,12 +110,9 @@ int main(int argc, char** argv) {
     std::vector<double> fwd, grad;
     if (full) {
       jif3d_amd::HipFullSensitivityCalculator calc(imp);
-      calc.Calculate(mesh, sites, model);         // direct; stores J
-      fwd = calc.Calculate(mesh, sites, model);   // J m
-      grad = calc.LQDerivative(mesh, sites, model, misfit);
+      run(calc, mesh, sites, model, misfit, nbatch, objective, true, &fwd, &grad);
     } else {
-      fwd = imp.Calculate(mesh, sites, model);
-      grad = imp.LQDerivative(mesh, sites, model, misfit);
+      run(imp, mesh, sites, model, misfit, nbatch, objective, false, &fwd, &grad);
     }
     std::ofstream out(argv[2], std::ios::binary);
     out.write(reinterpret_cast<const char*>(fwd.data()),

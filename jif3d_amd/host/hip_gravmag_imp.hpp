@@ -10,6 +10,7 @@
 // like the reference's CUDA imps (jif3D/Gravity/ScalarCudaGravityImp.h:75-76).
 #pragma once
 
+#include <algorithm>
 #include <cstddef>
 #include <stdexcept>
 #include <string>
@@ -90,7 +91,7 @@ class HipGravMagImplementation {
   std::vector<double> Calculate(const Mesh& mesh, const Sites& sites,
                                 const std::vector<double>& model) {
     Configure(mesh, sites);
-    if (model.size() != jgm_n_params(ctx_))
+    if (model.size() != ModelSize())
       throw FatalException("model vector length does not match the mesh and background");
     std::vector<double> out(sites.size() * GetDataPerMeasurement());
     check(jgm_forward_host(ctx_, kind_, prec_, model.data(), out.data()), "jgm_forward_host");
@@ -104,11 +105,45 @@ class HipGravMagImplementation {
     Configure(mesh, sites);
     if (misfit.size() != sites.size() * GetDataPerMeasurement())
       throw FatalException("misfit length does not match the number of data");
-    std::vector<double> grad(jgm_n_params(ctx_));
+    std::vector<double> grad(ModelSize());
     check(jgm_adjoint_host(ctx_, kind_, prec_, model.data(), misfit.data(), grad.data()),
           "jgm_adjoint_host");
     return grad;
   }
+
+  // Calculate for nmodels model vectors at once (models: nmodels * ModelSize values,
+  // model-major), e.g. the trial steps of a line search; returns nmodels * nsites *
+  // GetDataPerMeasurement values.  FP64 batches share the corner-function evaluations.
+  std::vector<double> CalculateBatch(const Mesh& mesh, const Sites& sites,
+                                     const std::vector<double>& models, size_t nmodels) {
+    Configure(mesh, sites);
+    if (models.size() != nmodels * ModelSize())
+      throw FatalException("models do not hold nmodels model vectors");
+    std::vector<double> out(nmodels * sites.size() * GetDataPerMeasurement());
+    check(jgm_forward_batch_host(ctx_, kind_, prec_, nmodels, models.data(), out.data()),
+          "jgm_forward_batch_host");
+    return out;
+  }
+
+  // LQDerivative for nmodels (model, misfit) pairs; models may be empty when no
+  // transform is set.  Returns nmodels * ModelSize values.
+  std::vector<double> LQDerivativeBatch(const Mesh& mesh, const Sites& sites,
+                                        const std::vector<double>& models,
+                                        const std::vector<double>& misfits, size_t nmodels) {
+    Configure(mesh, sites);
+    if (misfits.size() != nmodels * sites.size() * GetDataPerMeasurement())
+      throw FatalException("misfits do not hold nmodels misfit vectors");
+    if (!models.empty() && models.size() != nmodels * ModelSize())
+      throw FatalException("models do not hold nmodels model vectors");
+    std::vector<double> grads(nmodels * ModelSize());
+    check(jgm_adjoint_batch_host(ctx_, kind_, prec_, nmodels,
+                                 models.empty() ? nullptr : models.data(), misfits.data(),
+                                 grads.data()),
+          "jgm_adjoint_batch_host");
+    return grads;
+  }
+
+  size_t ModelSize() const { return jgm_model_size(ctx_, kind_); }
 
   jgm_context* context() { return ctx_; }
   jgm_kind kind() const { return kind_; }
@@ -178,6 +213,43 @@ class HipFullSensitivityCalculator {
                                     grad.data()),
           "jgm_storedj_gradient_host");
     return grad;
+  }
+
+  // Batched forms: with a valid stored J up to 64 models share each pass over J;
+  // otherwise the first model is calculated directly (building J) and the rest from J.
+  std::vector<double> CalculateBatch(const Mesh& mesh, const Sites& sites,
+                                     const std::vector<double>& models, size_t nmodels) {
+    const size_t np = imp_.ModelSize(), nd = sites.size() * imp_.GetDataPerMeasurement();
+    if (models.size() != nmodels * np)
+      throw FatalException("models do not hold nmodels model vectors");
+    std::vector<double> out(nmodels * nd);
+    size_t first = 0;
+    if (nmodels && !Cached(mesh, sites)) {
+      std::vector<double> m0(models.begin(), models.begin() + np);
+      std::vector<double> d0 = Calculate(mesh, sites, m0);
+      std::copy(d0.begin(), d0.end(), out.begin());
+      first = 1;
+    }
+    if (nmodels > first)
+      check(jgm_storedj_forward_batch_host(imp_.context(), prec_, nmodels - first,
+                                           models.data() + first * np, out.data() + first * nd),
+            "jgm_storedj_forward_batch_host");
+    return out;
+  }
+
+  std::vector<double> LQDerivativeBatch(const Mesh& mesh, const Sites& sites,
+                                        const std::vector<double>& models,
+                                        const std::vector<double>& misfits, size_t nmodels) {
+    if (!Cached(mesh, sites)) return imp_.LQDerivativeBatch(mesh, sites, models, misfits, nmodels);
+    const size_t np = imp_.ModelSize();
+    if (misfits.size() != nmodels * sites.size() * imp_.GetDataPerMeasurement())
+      throw FatalException("misfits do not hold nmodels misfit vectors");
+    std::vector<double> grads(nmodels * np);
+    check(jgm_storedj_gradient_batch_host(imp_.context(), prec_, nmodels,
+                                          models.empty() ? nullptr : models.data(),
+                                          misfits.data(), grads.data()),
+          "jgm_storedj_gradient_batch_host");
+    return grads;
   }
 
  private:

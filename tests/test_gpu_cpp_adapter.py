@@ -61,6 +61,68 @@ def test_cpp_adapter_matches_oracle(orc, kind, okind, k, full):
     assert np.max(np.abs(grad - gref)) <= tg * np.max(np.abs(gref))
 
 
+def small_problem(seed):
+    rng = np.random.default_rng(seed)
+    m = ThreeDGravityModel()
+    m.SetMeshSize(6, 5, 4)
+    m.SetXCellSizes(rng.integers(10, 35, 6).astype(float))
+    m.SetYCellSizes(rng.integers(10, 35, 5).astype(float))
+    m.SetZCellSizes(rng.integers(10, 35, 4).astype(float))
+    m.SetDensities(rng.random(120) * 10 + 1)
+    m.SetBackgroundThicknesses([30.0, 500.0])
+    m.SetBackgroundDensities([2.0, 3.0])
+    ns = 7
+    sites = (rng.random(ns) * 150, rng.random(ns) * 150, -rng.random(ns) * 50 - 1)
+    return m, sites, rng
+
+
+def run_driver(m, sites, kind, misfit, *opts):
+    with tempfile.TemporaryDirectory() as td:
+        inp, out = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        write_problem(inp, kind, 0, m, sites, (0.6, 0.2, 48000.0), misfit)
+        res = subprocess.run([DRIVER, inp, out] + list(opts), capture_output=True, text=True,
+                             timeout=120)
+        assert res.returncode == 0, res.stderr
+        return np.fromfile(out, dtype="<f8")
+
+
+@pytest.mark.parametrize("full", [False, True])
+def test_cpp_adapter_batch(orc, full):
+    """CalculateBatch / LQDerivativeBatch of 5 models (model b = m (1 + 0.05 b), misfit
+    b = r (b + 1)) against the oracle, direct FP64 or through the stored J."""
+    m, sites, rng = small_problem(50)
+    ns, nb = len(sites[0]), 5
+    misfit = rng.normal(size=ns * 9)
+    opts = ["--batch", str(nb)] + (["--full"] if full else [])
+    data = run_driver(m, sites, 2, misfit, *opts)
+    n = m.ncells + 2
+    fwd, grad = data[: nb * ns * 9].reshape(nb, -1), data[nb * ns * 9:].reshape(nb, n)
+    tf, tg = (2e-6, 2e-6) if full else (1e-11, 1e-10)
+    for b in range(nb):
+        mb = m.GetModelParameters() * (1 + 0.05 * b)
+        ref = orc.calculate(2, m, mb, sites)
+        gref = orc.lqderivative(2, m, mb, sites, misfit * (b + 1))
+        assert np.max(np.abs(fwd[b] - ref)) <= tf * np.max(np.abs(ref))
+        assert np.max(np.abs(grad[b] - gref)) <= tg * np.max(np.abs(gref))
+
+
+def test_cpp_objective_batch(orc):
+    """GravMagObjective over the imp: chi^2 = sum (d_b - obs)^2 and its gradient
+    2 J^T (d_b - obs) for each model of the batch (errors 1)."""
+    m, sites, rng = small_problem(51)
+    ns, nb = len(sites[0]), 4
+    obs = orc.calculate(1, m, m.GetModelParameters() * 1.3, sites)
+    data = run_driver(m, sites, 1, obs, "--batch", str(nb), "--objective")
+    n = m.ncells + 2
+    chi, grad = data[:nb], data[nb:].reshape(nb, n)
+    for b in range(nb):
+        mb = m.GetModelParameters() * (1 + 0.05 * b)
+        diff = orc.calculate(1, m, mb, sites) - obs
+        assert abs(chi[b] - np.sum(diff ** 2)) <= 1e-10 * np.sum(diff ** 2)
+        gref = orc.lqderivative(1, m, mb, sites, diff)
+        assert np.max(np.abs(grad[b] - gref)) <= 1e-10 * np.max(np.abs(gref))
+
+
 def test_cpp_adapter_reports_errors():
     with tempfile.TemporaryDirectory() as td:
         inp = os.path.join(td, "bad.bin")
@@ -68,3 +130,6 @@ def test_cpp_adapter_reports_errors():
         res = subprocess.run([DRIVER, inp, os.path.join(td, "o")], capture_output=True,
                              text=True, timeout=60)
         assert res.returncode == 1 and "cannot read header" in res.stderr
+        res = subprocess.run([DRIVER, inp, os.path.join(td, "o"), "--batch", "x"],
+                             capture_output=True, text=True, timeout=60)
+        assert res.returncode == 2 and "usage" in res.stderr
