@@ -8,6 +8,8 @@
 // --objective: with --batch B, GravMagObjective over the same calculator with the
 //            misfit vector as observed data (errors 1): writes B chi^2 values, then the
 //            B gradients.
+// --middle-sens: writes the depth profile of CalculateMiddleSens (nz values), then the
+//            fitted z0 for the exponent -2 and the nz depth weights.
 //
 // input  (little-endian): int32 kind, int32 precision, int32 nx, ny, nz, nbg, nsites;
 //        then doubles: xcoords[nx+1], ycoords[ny+1], zcoords[nz+1], xsizes[nx],
@@ -23,6 +25,7 @@
 #include <string>
 #include <vector>
 
+#include "depth_weighting.hpp"
 #include "gravmag_objective.hpp"
 #include "hip_gravmag_imp.hpp"
 
@@ -62,7 +65,7 @@ void run(Calc& calc, const jif3d_amd::Mesh& mesh, const jif3d_amd::Sites& sites,
 }
 
 int main(int argc, char** argv) {
-  bool full = false, objective = false, ok = argc >= 3;
+  bool full = false, objective = false, middle = false, ok = argc >= 3;
   size_t nbatch = 0;
   for (int a = 3; a < argc && ok; ++a) {
     const std::string opt(argv[a]);
@@ -70,6 +73,8 @@ int main(int argc, char** argv) {
       full = true;
     } else if (opt == "--objective") {
       objective = true;
+    } else if (opt == "--middle-sens") {
+      middle = true;
     } else if (opt == "--batch" && a + 1 < argc) {
       char* end = nullptr;
       nbatch = std::strtoul(argv[++a], &end, 10);
@@ -79,7 +84,8 @@ int main(int argc, char** argv) {
     }
   }
   if (!ok || (objective && nbatch == 0)) {
-    std::cerr << "usage: adapter_driver <input> <output> [--full] [--batch B [--objective]]\n";
+    std::cerr << "usage: adapter_driver <input> <output> [--full] [--batch B [--objective]] "
+                 "[--middle-sens]\n";
     return 2;
   }
   try {
@@ -108,7 +114,13 @@ int main(int argc, char** argv) {
     if (kind == JGM_KIND_MAGNETIC) imp.SetField(field[0], field[1], field[2]);
     const std::vector<double> misfit = read_doubles(in, ns * imp.RawDataPerMeasurement());
     std::vector<double> fwd, grad;
-    if (full) {
+    if (middle) {
+      fwd = jif3d_amd::CalculateMiddleSens(imp, mesh, model);
+      const jif3d_amd::WeightingTerm f(-2.0);
+      const double z0 = jif3d_amd::FitZ0(fwd, mesh.zsizes, f);
+      grad = jif3d_amd::ConstructDepthWeighting(mesh.zsizes, z0, f);
+      grad.insert(grad.begin(), z0);
+    } else if (full) {
       jif3d_amd::HipFullSensitivityCalculator calc(imp);
       run(calc, mesh, sites, model, misfit, nbatch, objective, true, &fwd, &grad);
     } else {
