@@ -241,6 +241,23 @@ jgm_status jgm_storedj_gradient_batch_host(jgm_context *ctx, jgm_gemm_precision 
                                            const double *misfits, double *grads);
 jgm_status jgm_storedj_release(jgm_context *ctx);
 
+/* Stored J for problems whose J does not fit in device memory: the role of the
+ * reference's DiskGravMagCalculator (jif3D/GravMag/DiskGravMagCalculator.h:100-200),
+ * which writes J to a file and reads it back for every cached product.  Here J (same
+ * FP32 compact layout) is built in column blocks of block_cells cells (rounded up to a
+ * multiple of 256) and kept in page-locked host memory; every product afterwards
+ * streams the blocks over PCIe into two HBM staging buffers, the copy of one block
+ * overlapping the matrix-core work on the previous one.  All stored-J entry points
+ * (apply, apply_t, forward, gradient and their batch forms) work unchanged; J^T R
+ * results equal the HBM-resident ones bit for bit, J M results differ only by the
+ * grouping of the FP64 sums over blocks.  Device memory used: 2 * block_cells * rows
+ * * 4 bytes plus the operands.  The reference's cached Disk gradient omits the factor 2
+ * of LQDerivative (DiskGravMagCalculator.h:164-199); this one keeps it.
+ * jgm_storedj_host_resident returns 1 when the current stored J lives in host memory. */
+jgm_status jgm_storedj_build_host(jgm_context *ctx, jgm_kind kind, jgm_precision prec,
+                                  size_t block_cells);
+int jgm_storedj_host_resident(const jgm_context *ctx);
+
 /* Device memory on the context's device, for callers without their own HIP
  * allocator (the CUDA host wrappers did this inside PrepareData/FreeData,
  * jif3D/Gravity/gravcuda.cu:8-107).  Copies are ordered on the context's stream

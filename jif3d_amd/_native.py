@@ -30,7 +30,7 @@ EXPORTS = [
     "jgm_n_params", "jgm_forward", "jgm_forward_host", "jgm_adjoint", "jgm_adjoint_host",
     "jgm_forward_batch", "jgm_adjoint_batch", "jgm_forward_batch_host", "jgm_adjoint_batch_host",
     "jgm_storedj_forward_batch", "jgm_storedj_gradient_batch", "jgm_storedj_forward_batch_host",
-    "jgm_storedj_gradient_batch_host",
+    "jgm_storedj_gradient_batch_host", "jgm_storedj_build_host", "jgm_storedj_host_resident",
     "jgm_sensitivity_rows", "jgm_model_size", "jgm_storedj_build", "jgm_storedj_rows",
     "jgm_storedj_apply",
     "jgm_storedj_apply_t", "jgm_storedj_forward", "jgm_storedj_gradient",
@@ -87,6 +87,7 @@ def load():
         "jgm_adjoint_batch_host": [vp, i, i, sz, pd, pd, pd],
         "jgm_sensitivity_rows": [vp, i, sz, sz, vp],
         "jgm_storedj_build": [vp, i, i],
+        "jgm_storedj_build_host": [vp, i, i, sz],
         "jgm_storedj_apply": [vp, i, vp, sz, vp],
         "jgm_storedj_apply_t": [vp, i, vp, sz, vp],
         "jgm_storedj_forward": [vp, i, vp, vp],
@@ -118,6 +119,8 @@ def load():
     lib.jgm_n_params.argtypes = [vp]
     lib.jgm_model_size.restype = sz
     lib.jgm_model_size.argtypes = [vp, i]
+    lib.jgm_storedj_host_resident.restype = i
+    lib.jgm_storedj_host_resident.argtypes = [vp]
     lib.jgm_storedj_rows.restype = sz
     lib.jgm_storedj_rows.argtypes = [vp]
     lib.jgm_profile_count.restype = sz
@@ -340,9 +343,17 @@ class Context:
         check(self.lib.jgm_sensitivity_rows(self.handle, kind, site0, nsites, _devptr(out)))
         return out
 
-    def storedj_build(self, kind, prec):
-        check(self.lib.jgm_storedj_build(self.handle, kind, prec))
+    def storedj_build(self, kind, prec, host_block_cells=None):
+        """Build the stored J in HBM, or with host_block_cells in page-locked host
+        memory, streamed to the device in blocks of that many cells."""
+        if host_block_cells:
+            check(self.lib.jgm_storedj_build_host(self.handle, kind, prec, int(host_block_cells)))
+        else:
+            check(self.lib.jgm_storedj_build(self.handle, kind, prec))
         self.storedj_kind = kind
+
+    def storedj_host_resident(self):
+        return bool(self.lib.jgm_storedj_host_resident(self.handle))
 
     def storedj_rows(self):
         return self.lib.jgm_storedj_rows(self.handle)

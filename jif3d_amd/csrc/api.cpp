@@ -189,8 +189,13 @@ jgm_status jgm_destroy(jgm_context* ctx) {
                                &ctx->near.zcen, &ctx->near.adj_table, &ctx->near.corner_off,
                                &ctx->near.cell_off, &ctx->near_h, &ctx->near_v, &ctx->vsites, &ctx->coef_mono,
                                &ctx->near.site_order, &ctx->near.mid_box, &ctx->near.xoff,
-                               &ctx->near.yoff, &ctx->near.zoff};
+                               &ctx->near.yoff, &ctx->near.zoff, &ctx->acc64};
   for (auto* b : bufs) b->release();
+  jgm::release_host_storedj(ctx);
+  if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
+  for (hipEvent_t ev : {ctx->ev_copied[0], ctx->ev_copied[1], ctx->ev_used[0], ctx->ev_used[1],
+                        ctx->ev_start})
+    if (ev) (void)hipEventDestroy(ev);
   jgm::prof_clear(ctx);
   delete ctx;
   return JGM_OK;
@@ -642,6 +647,22 @@ jgm_status jgm_storedj_build(jgm_context* ctx, jgm_kind kind, jgm_precision prec
   return JGM_OK;
 }
 
+jgm_status jgm_storedj_build_host(jgm_context* ctx, jgm_kind kind, jgm_precision prec,
+                                  size_t block_cells) {
+  jgm_status st = check_ready(ctx, kind, prec);
+  if (st != JGM_OK) return st;
+  if (fp64_only(kind)) return fail(JGM_ERR_UNSUPPORTED, "no stored J for kind %d", (int)kind);
+  if (block_cells == 0) return fail(JGM_ERR_INVALID_ARGUMENT, "block_cells must be positive");
+  DeviceGuard g(ctx->device);
+  hipError_t e = jgm::launch_storedj_build_resident(ctx, kind, prec, block_cells);
+  if (e != hipSuccess) return hip_fail(e, "host-resident stored J build");
+  return JGM_OK;
+}
+
+int jgm_storedj_host_resident(const jgm_context* ctx) {
+  return ctx && ctx->storedJ_kind && ctx->storedJ_host ? 1 : 0;
+}
+
 size_t jgm_storedj_rows(const jgm_context* ctx) {
   return ctx && ctx->storedJ_kind ? ctx->storedJ_rows : 0;
 }
@@ -899,6 +920,8 @@ jgm_status jgm_storedj_release(jgm_context* ctx) {
   DeviceGuard g(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   ctx->storedJ.release();
+  jgm::release_host_storedj(ctx);
+  ctx->acc64.release();
   ctx->storedJ_kind = 0;
   ctx->storedJ_rows = ctx->storedJ_cols = 0;
   return JGM_OK;

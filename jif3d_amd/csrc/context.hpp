@@ -141,6 +141,15 @@ struct jgm_context {
   jgm::DeviceBuffer storedJ, storedJ_vec;
   int storedJ_kind = 0;
   size_t storedJ_rows = 0, storedJ_cols = 0, storedJ_ld = 0, storedJ_colpad = 0;
+  // host-resident stored J: the whole J in page-locked host memory, streamed in column
+  // blocks of storedJ_block cells through two HBM staging buffers (storedJ then holds
+  // the staging buffers); null when J is in HBM
+  float* storedJ_host = nullptr;
+  size_t storedJ_block = 0;
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t ev_copied[2] = {nullptr, nullptr}, ev_used[2] = {nullptr, nullptr};
+  hipEvent_t ev_start = nullptr;
+  jgm::DeviceBuffer acc64;  // FP64 totals of J M over the streamed blocks
 };
 
 namespace jgm {
@@ -172,6 +181,9 @@ hipError_t launch_fill_zero(jgm_context* ctx, double* p, size_t n);
 hipError_t launch_background_gradient(jgm_context* ctx, int kind, const double* raw_misfit,
                                       double* grad_bg);
 hipError_t launch_storedj_build(jgm_context* ctx, int kind, int precision);
+hipError_t launch_storedj_build_resident(jgm_context* ctx, int kind, int precision,
+                                         size_t block_cells);
+void release_host_storedj(jgm_context* ctx);
 hipError_t launch_storedj_apply(jgm_context* ctx, int gemm_precision, int transpose,
                                 const float* in, size_t nrhs, float* out);
 hipError_t launch_storedj_forward(jgm_context* ctx, int gemm_precision, const double* model,

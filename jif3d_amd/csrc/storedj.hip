@@ -81,15 +81,16 @@ __global__ __launch_bounds__(kBlock) void k_storedj_build(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     int S, const double* __restrict__ xlo, const double* __restrict__ ylo,
     const double* __restrict__ zlo, const double* __restrict__ xsz,
-    const double* __restrict__ ysz, const double* __restrict__ zsz, int ny, int nz, long ncell,
-    long cells_per_chunk, int kind, FieldDir fd, long ld, float* __restrict__ J) {
+    const double* __restrict__ ysz, const double* __restrict__ zsz, int ny, int nz, long cbase,
+    long cend, long cells_per_chunk, int kind, FieldDir fd, long ld, float* __restrict__ J) {
   constexpr int NC = Ncomp<MASK>::value;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int sl = s < S ? s : S - 1;
   const double px = sx[sl], py = sy[sl], pz = sz[sl];
   const int kJ = compact_rows(kind);
-  const long c0 = (long)blockIdx.y * cells_per_chunk;
-  const long c1 = c0 + cells_per_chunk < ncell ? c0 + cells_per_chunk : ncell;
+  // cells [cbase, cend) of the mesh; column c - cbase of J
+  const long c0 = cbase + (long)blockIdx.y * cells_per_chunk;
+  const long c1 = c0 + cells_per_chunk < cend ? c0 + cells_per_chunk : cend;
   for (long c = c0; c < c1; ++c) {
     const int k = (int)(c % nz);
     const long t = c / nz;
@@ -99,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void k_storedj_build(
     box_terms_f64<MASK>(px, py, pz, xlo[i], ylo[j], zlo[k], xsz[i], ysz[j], zsz[k], v);
     spread_components<MASK>(v, u);  // gz, Uxx, Uyy, Uzz, Uxy, Uxz, Uyz
     if (s >= S) continue;
-    float* o = J + c * ld + (long)s * kJ;
+    float* o = J + (c - cbase) * ld + (long)s * kJ;
     if (kind == JGM_KIND_SCALAR) {
       o[0] = (float)u[0];
     } else if (kind == JGM_KIND_MAGNETIC) {
@@ -311,7 +312,8 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm_reduce(const float* __restri
 template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_gemm_jtr(const float* __restrict__ J, long ld, long N,
                                                      long nblk, const float* __restrict__ Rt,
-                                                     int P, float* __restrict__ G, int nrhs) {
+                                                     int P, float* __restrict__ G, long ldg,
+                                                     int nrhs) {
   __shared__ float jt[kStage * kPitch];                           // [row][cell]
   __shared__ __attribute__((aligned(16))) float rt[kStage * 64];  // [row][rhs]
   const int b0 = blockIdx.y * 64;
@@ -383,7 +385,7 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jtr(const float* __restrict__ J
 #pragma unroll
           for (int r = 0; r < 16; ++r) {
             const int b = b0 + 32 * ta + 8 * (r >> 2) + 4 * lk + (r & 3);
-            if (b < nrhs) G[c + (long)b * N] = tot[ta][tb][r];
+            if (b < nrhs) G[c + (long)b * ldg] = tot[ta][tb][r];
           }
         }
       }
@@ -492,7 +494,124 @@ __global__ __launch_bounds__(kBlock) void k_scale_to_f64(const float* __restrict
 
 }  // namespace
 
+namespace {
+
+// Y += sum over splits of Yp[split][row][b] (FP64 totals of a J streamed in blocks)
+__global__ __launch_bounds__(kBlock) void k_gemm_jm_accumulate(const float* __restrict__ Yp,
+                                                               int nsplit, long R, int P,
+                                                               int nrhs, double* __restrict__ Y) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= R * nrhs) return;
+  const long row = idx / nrhs;
+  const int b = (int)(idx % nrhs);
+  double a = 0.0;
+  for (int s = 0; s < nsplit; ++s) a += Yp[(long)s * R * P + row * P + b];
+  Y[row + (long)b * R] += a;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_zero_f64(double* __restrict__ p, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_f64_to_f32(const double* __restrict__ a, long n,
+                                                       float* __restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)a[i];
+}
+
+// Build the J columns of cells [c0, c1) into J (column c - c0 at J + (c - c0) * ld).
+hipError_t build_columns(jgm_context* ctx, int kind, long c0, long c1, long ld, float* J) {
+  const int S = (int)ctx->S;
+  const int site_blocks = (S + kBlock - 1) / kBlock;
+  const long n = c1 - c0;
+  const long waves_per_chunk = (long)site_blocks * (kBlock / 64);
+  long want = std::max<long>(1, std::min<long>(n, (16384 + waves_per_chunk - 1) / waves_per_chunk));
+  const long per = (n + want - 1) / want;
+  const int nchunks = (int)((n + per - 1) / per);
+  const int ny = (int)ctx->ny, nz = (int)ctx->nz;
+  FieldDir fd = field_dir(ctx);
+  ProfScope ps(ctx, "storedj_build_fp64");
+#define JGM_BUILD(M)                                                                            \
+  hipLaunchKernelGGL((k_storedj_build<M>), dim3(site_blocks, nchunks), dim3(kBlock), 0,        \
+                     ctx->stream, ctx->d_sx.as<double>(), ctx->d_sy.as<double>(),               \
+                     ctx->d_sz.as<double>(), S, ctx->d_xlo.as<double>(), ctx->d_ylo.as<double>(), \
+                     ctx->d_zlo.as<double>(), ctx->d_xsz.as<double>(), ctx->d_ysz.as<double>(), \
+                     ctx->d_zsz.as<double>(), ny, nz, c0, c1, per, kind, fd, ld, J)
+  switch (mask_of(kind)) {
+    case kMaskScalar: JGM_BUILD(kMaskScalar); break;
+    case kMaskAll: JGM_BUILD(kMaskAll); break;
+    default: JGM_BUILD(kMaskTensor); break;
+  }
+#undef JGM_BUILD
+  return hipGetLastError();
+}
+
+hipError_t ensure_streaming(jgm_context* ctx) {
+  hipError_t e = hipSuccess;
+  if (!ctx->copy_stream) e = hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking);
+  for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+    if (!ctx->ev_copied[i]) e = hipEventCreateWithFlags(&ctx->ev_copied[i], hipEventDisableTiming);
+    if (e == hipSuccess && !ctx->ev_used[i])
+      e = hipEventCreateWithFlags(&ctx->ev_used[i], hipEventDisableTiming);
+  }
+  if (e == hipSuccess && !ctx->ev_start)
+    e = hipEventCreateWithFlags(&ctx->ev_start, hipEventDisableTiming);
+  return e;
+}
+
+// Calls fn(J_block, first cell, cells) for every column block of the stored J: once for
+// a J in HBM; for a J in host memory, per block after its copy into one of two HBM
+// staging buffers (copy stream), so that the copy of block b + 1 overlaps the products
+// on block b (context stream).  A staging buffer is refilled only after the products
+// that read it have finished.
+template <class F>
+hipError_t for_each_block(jgm_context* ctx, F&& fn) {
+  const long ld = (long)ctx->storedJ_ld, colpad = (long)ctx->storedJ_colpad;
+  if (!ctx->storedJ_host) return fn(ctx->storedJ.as<const float>(), 0L, colpad);
+  hipError_t e = ensure_streaming(ctx);
+  if (e != hipSuccess) return e;
+  const long blk = (long)ctx->storedJ_block;
+  float* stage[2] = {ctx->storedJ.as<float>(), ctx->storedJ.as<float>() + blk * ld};
+  // earlier work on the context stream may still read the staging buffers
+  if ((e = hipEventRecord(ctx->ev_start, ctx->stream)) != hipSuccess) return e;
+  if ((e = hipStreamWaitEvent(ctx->copy_stream, ctx->ev_start, 0)) != hipSuccess) return e;
+  for (long c0 = 0, b = 0; c0 < colpad; c0 += blk, ++b) {
+    const int i = (int)(b & 1);
+    const long nc = std::min(blk, colpad - c0);
+    if (b >= 2 && (e = hipStreamWaitEvent(ctx->copy_stream, ctx->ev_used[i], 0)) != hipSuccess)
+      return e;
+    if ((e = hipMemcpyAsync(stage[i], ctx->storedJ_host + c0 * ld, sizeof(float) * nc * ld,
+                            hipMemcpyHostToDevice, ctx->copy_stream)) != hipSuccess)
+      return e;
+    if ((e = hipEventRecord(ctx->ev_copied[i], ctx->copy_stream)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(ctx->stream, ctx->ev_copied[i], 0)) != hipSuccess) return e;
+    if ((e = fn(static_cast<const float*>(stage[i]), c0, nc)) != hipSuccess) return e;
+    if ((e = hipEventRecord(ctx->ev_used[i], ctx->stream)) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+}  // namespace
+
+void release_host_storedj(jgm_context* ctx) {
+  if (ctx->storedJ_host) {
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
+    (void)hipHostFree(ctx->storedJ_host);
+  }
+  ctx->storedJ_host = nullptr;
+  ctx->storedJ_block = 0;
+}
+
 hipError_t launch_storedj_build(jgm_context* ctx, int kind, int precision) {
+  return launch_storedj_build_resident(ctx, kind, precision, 0);
+}
+
+// block_cells == 0: J in HBM.  Otherwise J in page-locked host memory, built in blocks
+// of block_cells cells (a multiple of 256) through an HBM staging buffer.
+hipError_t launch_storedj_build_resident(jgm_context* ctx, int kind, int precision,
+                                         size_t block_cells) {
   (void)precision;  // the build evaluates the FP64 closed form; J is stored in FP32
   const int S = (int)ctx->S;
   const long N = (long)ncells(ctx);
@@ -501,32 +620,42 @@ hipError_t launch_storedj_build(jgm_context* ctx, int kind, int precision) {
   if (S == 0 || N == 0) return hipErrorInvalidValue;
   const long ld = round_up(R, kRowPad), colpad = round_up(N, kColPad);
   ctx->storedJ_kind = 0;
-  hipError_t e = ctx->storedJ.ensure(sizeof(float) * (size_t)ld * colpad);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_fill_zero_f32, dim3(8 * ctx->cu_count), dim3(kBlock), 0, ctx->stream,
-                     ctx->storedJ.as<float>(), ld * colpad);
-  const int site_blocks = (S + kBlock - 1) / kBlock;
-  const long waves_per_chunk = (long)site_blocks * (kBlock / 64);
-  long want = std::max<long>(1, std::min<long>(N, (16384 + waves_per_chunk - 1) / waves_per_chunk));
-  const long per = (N + want - 1) / want;
-  const int nchunks = (int)((N + per - 1) / per);
-  const int ny = (int)ctx->ny, nz = (int)ctx->nz;
-  FieldDir fd = field_dir(ctx);
-  {
-    ProfScope ps(ctx, "storedj_build_fp64");
-#define JGM_BUILD(M)                                                                            \
-  hipLaunchKernelGGL((k_storedj_build<M>), dim3(site_blocks, nchunks), dim3(kBlock), 0,        \
-                     ctx->stream, ctx->d_sx.as<double>(), ctx->d_sy.as<double>(),               \
-                     ctx->d_sz.as<double>(), S, ctx->d_xlo.as<double>(), ctx->d_ylo.as<double>(), \
-                     ctx->d_zlo.as<double>(), ctx->d_xsz.as<double>(), ctx->d_ysz.as<double>(), \
-                     ctx->d_zsz.as<double>(), ny, nz, N, per, kind, fd, ld,                    \
-                     ctx->storedJ.as<float>())
-    switch (mask_of(kind)) {
-      case kMaskScalar: JGM_BUILD(kMaskScalar); break;
-      case kMaskAll: JGM_BUILD(kMaskAll); break;
-      default: JGM_BUILD(kMaskTensor); break;
+  release_host_storedj(ctx);
+  hipError_t e;
+  if (block_cells == 0) {
+    if ((e = ctx->storedJ.ensure(sizeof(float) * (size_t)ld * colpad)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fill_zero_f32, dim3(8 * ctx->cu_count), dim3(kBlock), 0, ctx->stream,
+                       ctx->storedJ.as<float>(), ld * colpad);
+    if ((e = build_columns(ctx, kind, 0, N, ld, ctx->storedJ.as<float>())) != hipSuccess)
+      return e;
+  } else {
+    const long blk = std::min<long>(round_up((long)block_cells, kColPad), colpad);
+    // the staging buffers replace the HBM copy of J
+    ctx->storedJ.release();
+    if ((e = ctx->storedJ.ensure(sizeof(float) * (size_t)(2 * blk * ld))) != hipSuccess) return e;
+    void* host = nullptr;
+    if ((e = hipHostMalloc(&host, sizeof(float) * (size_t)ld * colpad, hipHostMallocDefault)) !=
+        hipSuccess)
+      return e;
+    ctx->storedJ_host = static_cast<float*>(host);
+    ctx->storedJ_block = (size_t)blk;
+    float* stage = ctx->storedJ.as<float>();
+    for (long c0 = 0; c0 < colpad; c0 += blk) {
+      const long nc = std::min(blk, colpad - c0);
+      hipLaunchKernelGGL(k_fill_zero_f32, dim3(8 * ctx->cu_count), dim3(kBlock), 0, ctx->stream,
+                         stage, nc * ld);
+      if (c0 < N && (e = build_columns(ctx, kind, c0, std::min(c0 + nc, N), ld, stage)) !=
+                        hipSuccess)
+        break;
+      if ((e = hipMemcpyAsync(ctx->storedJ_host + c0 * ld, stage, sizeof(float) * nc * ld,
+                              hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess)
+        break;
     }
-#undef JGM_BUILD
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+      release_host_storedj(ctx);
+      return e;
+    }
   }
   e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -544,37 +673,56 @@ namespace {
 hipError_t run_jm(jgm_context* ctx, int gemm_precision, const float* Xt, int P, int nrhs,
                   float* out) {
   const long R = (long)ctx->storedJ_rows;
-  const long ld = (long)ctx->storedJ_ld, colpad = (long)ctx->storedJ_colpad;
-  const float* J = ctx->storedJ.as<const float>();
+  const long ld = (long)ctx->storedJ_ld;
   hipStream_t st = ctx->stream;
   const int rhs_tiles = P / 64;
   const int row_tiles = (int)((ld + 255) / 256);
-  // one workgroup per CU: a second co-resident workgroup slows both down (the CU's
-  // memory pipeline is the limit); splits in multiples of 8 when there are enough of
-  // them to group by XCD
-  long nsplit = std::max<long>(1, (long)ctx->cu_count / ((long)row_tiles * rhs_tiles));
-  nsplit = std::min<long>(nsplit, colpad / 32);
-  if (nsplit >= 16) nsplit = nsplit / 8 * 8;
-  const long per = round_up((colpad + nsplit - 1) / nsplit, 2 * kChunk);
-  hipError_t e = ctx->partial.ensure(sizeof(float) * (size_t)nsplit * R * P);
-  if (e != hipSuccess) return e;
-  dim3 grid((unsigned)(nsplit * row_tiles), (unsigned)rhs_tiles);
-  {
-    ProfScope ps(ctx, gemm_precision == JGM_GEMM_FP32   ? "storedj_jm_fp32"
-                      : gemm_precision == JGM_GEMM_BF16 ? "storedj_jm_bf16"
-                                                        : "storedj_jm_bf16_refined");
-#define JGM_JM(PR)                                                                             \
-  hipLaunchKernelGGL((k_gemm_jm<PR>), grid, dim3(kBlock), 0, st, J, ld, R, colpad, Xt, P, per, \
-                     row_tiles, (int)nsplit, ctx->partial.as<float>())
-    switch (gemm_precision) {
-      case JGM_GEMM_FP32: JGM_JM(JGM_GEMM_FP32); break;
-      case JGM_GEMM_BF16: JGM_JM(JGM_GEMM_BF16); break;
-      default: JGM_JM(JGM_GEMM_BF16_REFINED); break;
-    }
-#undef JGM_JM
+  const bool streamed = ctx->storedJ_host != nullptr;
+  hipError_t e;
+  if (streamed) {
+    if ((e = ctx->acc64.ensure(sizeof(double) * (size_t)R * nrhs)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fill_zero_f64, dim3(4 * ctx->cu_count), dim3(kBlock), 0, st,
+                       ctx->acc64.as<double>(), R * (long)nrhs);
   }
-  hipLaunchKernelGGL(k_gemm_jm_reduce, dim3(blocks_for(R * (long)nrhs)), dim3(kBlock), 0, st,
-                     ctx->partial.as<const float>(), (int)nsplit, R, P, nrhs, out);
+  e = for_each_block(ctx, [&](const float* J, long c0, long ncols) -> hipError_t {
+    // one workgroup per CU: a second co-resident workgroup slows both down (the CU's
+    // memory pipeline is the limit); splits in multiples of 8 when there are enough of
+    // them to group by XCD
+    long nsplit = std::max<long>(1, (long)ctx->cu_count / ((long)row_tiles * rhs_tiles));
+    nsplit = std::min<long>(nsplit, ncols / 32);
+    if (nsplit >= 16) nsplit = nsplit / 8 * 8;
+    const long per = round_up((ncols + nsplit - 1) / nsplit, 2 * kChunk);
+    hipError_t e2 = ctx->partial.ensure(sizeof(float) * (size_t)nsplit * R * P);
+    if (e2 != hipSuccess) return e2;
+    dim3 grid((unsigned)(nsplit * row_tiles), (unsigned)rhs_tiles);
+    const float* Mt = Xt + c0 * P;
+    {
+      ProfScope ps(ctx, gemm_precision == JGM_GEMM_FP32   ? "storedj_jm_fp32"
+                        : gemm_precision == JGM_GEMM_BF16 ? "storedj_jm_bf16"
+                                                          : "storedj_jm_bf16_refined");
+#define JGM_JM(PR)                                                                              \
+  hipLaunchKernelGGL((k_gemm_jm<PR>), grid, dim3(kBlock), 0, st, J, ld, R, ncols, Mt, P, per,   \
+                     row_tiles, (int)nsplit, ctx->partial.as<float>())
+      switch (gemm_precision) {
+        case JGM_GEMM_FP32: JGM_JM(JGM_GEMM_FP32); break;
+        case JGM_GEMM_BF16: JGM_JM(JGM_GEMM_BF16); break;
+        default: JGM_JM(JGM_GEMM_BF16_REFINED); break;
+      }
+#undef JGM_JM
+    }
+    if (streamed)
+      hipLaunchKernelGGL(k_gemm_jm_accumulate, dim3(blocks_for(R * (long)nrhs)), dim3(kBlock), 0,
+                         st, ctx->partial.as<const float>(), (int)nsplit, R, P, nrhs,
+                         ctx->acc64.as<double>());
+    else
+      hipLaunchKernelGGL(k_gemm_jm_reduce, dim3(blocks_for(R * (long)nrhs)), dim3(kBlock), 0, st,
+                         ctx->partial.as<const float>(), (int)nsplit, R, P, nrhs, out);
+    return hipGetLastError();
+  });
+  if (e != hipSuccess) return e;
+  if (streamed)
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(blocks_for(R * (long)nrhs)), dim3(kBlock), 0, st,
+                       ctx->acc64.as<const double>(), R * (long)nrhs, out);
   return hipGetLastError();
 }
 
@@ -582,24 +730,27 @@ hipError_t run_jm(jgm_context* ctx, int gemm_precision, const float* Xt, int P, 
 hipError_t run_jtr(jgm_context* ctx, int gemm_precision, const float* Xt, int P, int nrhs,
                    float* out) {
   const long N = (long)ctx->storedJ_cols;
-  const long ld = (long)ctx->storedJ_ld, colpad = (long)ctx->storedJ_colpad;
-  const float* J = ctx->storedJ.as<const float>();
+  const long ld = (long)ctx->storedJ_ld;
   hipStream_t st = ctx->stream;
-  const long nblk = colpad / kColPad;
-  dim3 grid((unsigned)std::min<long>(nblk, 65535), (unsigned)(P / 64));
-  ProfScope ps(ctx, gemm_precision == JGM_GEMM_FP32   ? "storedj_jtr_fp32"
-                    : gemm_precision == JGM_GEMM_BF16 ? "storedj_jtr_bf16"
-                                                      : "storedj_jtr_bf16_refined");
+  return for_each_block(ctx, [&](const float* J, long c0, long ncols) -> hipError_t {
+    const long nblk = ncols / kColPad;
+    const long nvalid = std::max<long>(0, std::min(ncols, N - c0));
+    if (nvalid == 0) return hipSuccess;
+    dim3 grid((unsigned)std::min<long>(nblk, 65535), (unsigned)(P / 64));
+    ProfScope ps(ctx, gemm_precision == JGM_GEMM_FP32   ? "storedj_jtr_fp32"
+                      : gemm_precision == JGM_GEMM_BF16 ? "storedj_jtr_bf16"
+                                                        : "storedj_jtr_bf16_refined");
 #define JGM_JTR(PR)                                                                          \
-  hipLaunchKernelGGL((k_gemm_jtr<PR>), grid, dim3(kBlock), 0, st, J, ld, N, nblk, Xt, P, out, \
-                     nrhs)
-  switch (gemm_precision) {
-    case JGM_GEMM_FP32: JGM_JTR(JGM_GEMM_FP32); break;
-    case JGM_GEMM_BF16: JGM_JTR(JGM_GEMM_BF16); break;
-    default: JGM_JTR(JGM_GEMM_BF16_REFINED); break;
-  }
+  hipLaunchKernelGGL((k_gemm_jtr<PR>), grid, dim3(kBlock), 0, st, J, ld, nvalid, nblk, Xt, P, \
+                     out + c0, N, nrhs)
+    switch (gemm_precision) {
+      case JGM_GEMM_FP32: JGM_JTR(JGM_GEMM_FP32); break;
+      case JGM_GEMM_BF16: JGM_JTR(JGM_GEMM_BF16); break;
+      default: JGM_JTR(JGM_GEMM_BF16_REFINED); break;
+    }
 #undef JGM_JTR
-  return hipGetLastError();
+    return hipGetLastError();
+  });
 }
 
 }  // namespace

@@ -290,13 +290,15 @@ class FullSensitivityGravMagCalculator(CachedGravMagCalculator):
     builds J and returns the imp's direct result; later calls are J m (plus the
     background layers) and 2 [J B]^T D^T misfit on the matrix cores in `gemm_precision`."""
 
+    host_block_cells = None
+
     def __init__(self, imp, gemm_precision=nat.GEMM_FP32):
         super().__init__(imp)
         self.gemm_precision = gemm_precision
 
     def CalculateNewModel(self, model, data):
         ctx = self.Imp.context(model, data)
-        ctx.storedj_build(self.Imp.kind, self.Imp.precision)
+        ctx.storedj_build(self.Imp.kind, self.Imp.precision, self.host_block_cells)
         return self.Imp.Calculate(model, data)
 
     def CalculateCachedResult(self, model, data):
@@ -322,3 +324,15 @@ class FullSensitivityGravMagCalculator(CachedGravMagCalculator):
         if misfits.ndim != 2 or misfits.shape[1] != need:
             raise ValueError(f"misfits must have shape (B, {need})")
         return ctx.storedj_gradient_batch(self.gemm_precision, models, misfits)
+
+
+class DiskGravMagCalculator(FullSensitivityGravMagCalculator):
+    """The role of jif3D/GravMag/DiskGravMagCalculator.h:100-200 (a J larger than the
+    memory that computes with it, kept elsewhere and read back for every cached product):
+    J is kept in page-locked host memory and streamed to HBM in blocks of
+    `block_cells` cells, each block's copy overlapping the products on the previous one.
+    The cached gradient keeps the factor 2 that the reference's Disk calculator omits."""
+
+    def __init__(self, imp, gemm_precision=nat.GEMM_FP32, block_cells=16384):
+        super().__init__(imp, gemm_precision)
+        self.host_block_cells = int(block_cells)
