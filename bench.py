@@ -17,7 +17,11 @@ right-hand sides on a 7168 x 131072 FP32 J) in FP32, BF16 and BF16-refined MFMA 
 with HBM and matrix-core fractions.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp64|both]
-                    [--workload all|headline|storedj]
+                    [--workload all|headline|storedj|configs]
+
+`--workload configs` prints instead one table row per benchmark configuration (gz
+forward; FTG forward; gz+FTG adjoint with a finite-difference check; magnetic total-field
+forward + adjoint), each with the oracle's CPU rate beside it.
 """
 import argparse
 import json
@@ -213,6 +217,180 @@ def run_storedj(args):
     return res
 
 
+FE_FWD = {"gz": (55.0, 10.0), "ftg": (106.0, 60.0), "gz+ftg": (133.0, 70.0),
+          "magnetic": (106.0, 72.0)}  # flop-equivalents per pair: a * rho_c + b
+
+
+def run_configs(args):
+    """The survey's five benchmark configurations, one table row each: GPU pairs/s and
+    flop-equivalent rate (FP64 and mixed where both exist), error against the FP64 path
+    and against the oracle's long double, and the oracle's CPU rate on a sample of sites
+    in the same process.  Forward-only rows count forward pairs; adjoint rows adjoint
+    pairs; the fwd+adj rows count a pair once per forward + adjoint."""
+    from jif3d_amd import _native as nat, synthetic
+    from oracle import oracle as orc
+    reps = max(3, args.steps)
+    rows = []
+
+    def timed(ctx, fn):
+        fn()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ctx.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    def cpu_rate(fn, nsites_total, budget=2.0):
+        t0 = time.perf_counter()
+        fn(1)
+        t1 = time.perf_counter() - t0
+        n = int(max(1, min(nsites_total, budget / max(t1, 1e-4))))
+        t0 = time.perf_counter()
+        fn(n)
+        return n, (time.perf_counter() - t0)
+
+    def row(cfg, what, model, data, times, fe_key, errors, cpu, passes=1):
+        S, N = data.nsites, model.ncells
+        rho_c = (model.nx + 1) * (model.ny + 1) * (model.nz + 1) / N
+        a, b = FE_FWD[fe_key]
+        fe = passes * (a * rho_c + b)  # forward and adjoint count the same
+        r = {"config": cfg, "what": what, "cells": N, "sites": S, "rho_c": rho_c,
+             "fe_per_pair": fe}
+        for prec, t in times.items():
+            pairs = S * N / t
+            peak = PEAK["fp64"] if prec == "fp64" else PEAK["fp32"]
+            r[prec] = {"ms": t * 1e3, "pairs_per_s": pairs, "fe_per_s": pairs * fe,
+                       "frac_of_vector_peak": pairs * fe / peak}
+        if "fp64" in times and "mixed" in times:
+            r["speedup_mixed_vs_fp64"] = times["fp64"] / times["mixed"]
+        r["errors"] = errors
+        n, t = cpu
+        r["cpu"] = {"pairs_per_s": n * N / t, "cores": orc.num_threads(),
+                    "sample_sites": n}
+        rows.append(r)
+        log(f"[bench] config {cfg} {what}: " + ", ".join(
+            f"{p} {v['ms']:.3f} ms" for p, v in r.items() if isinstance(v, dict) and "ms" in v))
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    def comp_rel(a, b, k):
+        a, b = a.reshape(-1, k), b.reshape(-1, k)
+        floor = 1e-6 * np.max(np.abs(b))
+        return float(max(np.max(np.abs(a[:, c] - b[:, c])) / max(np.max(np.abs(b[:, c])), floor)
+                         for c in range(k)))
+
+    # config 0: gz forward, FP64, against the oracle on every site
+    m, d = synthetic.config0()
+    ctx = nat.Context(0)
+    ctx.configure(m, d)
+    mv, x = m.GetModelParameters(), d.positions
+    dm, out = ctx.to_device(mv), ctx.alloc(d.nsites)
+    t64 = timed(ctx, lambda: ctx.forward(nat.KIND_SCALAR, nat.FP64, dm, out))
+    g = out.to_host()
+    errs = {"fp64_vs_oracle_fp64": rel(g, orc.calculate(orc.SCALAR, m, mv, x)),
+            "fp64_vs_long_double": rel(g, orc.calculate_ld(orc.SCALAR, m, mv, x))}
+    cpu = cpu_rate(lambda n: orc.calculate(orc.SCALAR, m, mv, tuple(c[:n] for c in x)),
+                   d.nsites)
+    row(0, "gz forward", m, d, {"fp64": t64}, "gz", errs, cpu)
+
+    # config 1: FTG forward, FP64 and mixed
+    m, d = synthetic.config1()
+    ctx = nat.Context(0)
+    ctx.configure(m, d)
+    mv, x = m.GetModelParameters(), d.positions
+    dm, out = ctx.to_device(mv), ctx.alloc(d.nsites * 9)
+    res, times = {}, {}
+    for name, prec in (("fp64", nat.FP64), ("mixed", nat.MIXED)):
+        times[name] = timed(ctx, lambda: ctx.forward(nat.KIND_TENSOR, prec, dm, out))
+        res[name] = out.to_host()
+    sub = np.random.default_rng(11).choice(d.nsites, 24, replace=False)
+    xs = tuple(c[sub] for c in x)
+    ld = orc.calculate_ld(orc.TENSOR, m, mv, xs)
+    errs = {"mixed_vs_fp64_per_component": comp_rel(res["mixed"], res["fp64"], 9),
+            "fp64_vs_long_double_24_sites": comp_rel(res["fp64"].reshape(-1, 9)[sub], ld, 9),
+            "mixed_vs_long_double_24_sites": comp_rel(res["mixed"].reshape(-1, 9)[sub], ld, 9)}
+    cpu = cpu_rate(lambda n: orc.calculate(orc.TENSOR, m, mv, tuple(c[:n] for c in x)),
+                   d.nsites)
+    row(1, "FTG forward", m, d, times, "ftg", errs, cpu)
+
+    # config 2: gz + FTG adjoint (gradient of sum |d - d_obs|^2)
+    m, d = synthetic.config2()
+    ctx = nat.Context(0)
+    ctx.configure(m, d)
+    kind = nat.KIND_SCALAR_TENSOR
+    true = m.GetModelParameters()
+    start = true.copy()
+    start[:m.ncells] = 2670.0
+    rng = np.random.default_rng(21)
+    d_true = ctx.forward(kind, nat.FP64, true)
+    scale = np.abs(d_true.reshape(-1, 10)).max(axis=0)
+    d_obs = d_true + (0.01 * rng.normal(size=d_true.size).reshape(-1, 10) * scale).reshape(-1)
+    r = ctx.forward(kind, nat.FP64, start) - d_obs
+    dm, dr, grad = ctx.to_device(start), ctx.to_device(r), ctx.alloc(ctx.n_params())
+    res, times = {}, {}
+    for name, prec in (("fp64", nat.FP64), ("mixed", nat.MIXED)):
+        times[name] = timed(ctx, lambda: ctx.adjoint(kind, prec, dm, dr, grad))
+        res[name] = grad.to_host()
+    cells = np.random.default_rng(22).choice(m.ncells, 20, replace=False)
+    fd = np.empty(cells.size)
+    for i, c in enumerate(cells):
+        p_, q_ = start.copy(), start.copy()
+        p_[c] += 0.1
+        q_[c] -= 0.1
+        dd = ctx.forward(kind, nat.FP64, p_) - ctx.forward(kind, nat.FP64, q_)
+        fd[i] = 2.0 * np.dot(r, dd) / 0.2
+    gmax = np.max(np.abs(res["fp64"]))
+    errs = {"mixed_vs_fp64": rel(res["mixed"], res["fp64"]),
+            "fp64_vs_central_fd_20_cells": float(np.max(np.abs(res["fp64"][cells] - fd)) / gmax),
+            "mixed_vs_central_fd_20_cells": float(np.max(np.abs(res["mixed"][cells] - fd)) / gmax)}
+    x = d.positions
+    rr = r.reshape(-1, 10)
+
+    def cpu_adj(n):
+        xs = tuple(c[:n] for c in x)
+        orc.lqderivative(orc.SCALAR, m, start, xs, rr[:n, 0].copy())
+        orc.lqderivative(orc.TENSOR, m, start, xs, rr[:n, 1:].reshape(-1).copy())
+    cpu = cpu_rate(cpu_adj, d.nsites)
+    row(2, "gz+FTG adjoint", m, d, times, "gz+ftg", errs, cpu)
+
+    # config 4: magnetic total-field anomaly forward + adjoint
+    m, d, field = synthetic.config4_magnetic()
+    ctx = nat.Context(0)
+    ctx.configure(m, d, field)
+    mv, x = m.GetModelParameters(), d.positions
+    kind = nat.KIND_MAGNETIC
+    sub = np.random.default_rng(31).choice(d.nsites, 6, replace=False)
+    xs = tuple(c[sub] for c in x)
+    b64 = ctx.forward(kind, nat.FP64, mv).reshape(-1, 3)
+    ld = orc.calculate_ld(orc.MAGNETIC, m, mv, xs, field=field)
+    ctx.set_transform(nat.TF_TOTAL_FIELD_ANOMALY, field[0], field[1])
+    dm, out = ctx.to_device(mv), ctx.alloc(d.nsites)
+    dr, grad = ctx.to_device(np.random.default_rng(32).normal(size=d.nsites)), ctx.alloc(ctx.n_params())
+    res, times = {}, {}
+    for name, prec in (("fp64", nat.FP64), ("mixed", nat.MIXED)):
+        def step(prec=prec):
+            ctx.forward(kind, prec, dm, out)
+            ctx.adjoint(kind, prec, dm, dr, grad)
+        times[name] = timed(ctx, step)
+        res[name] = (out.to_host(), grad.to_host())
+    errs = {"mixed_vs_fp64_tfa": rel(res["mixed"][0], res["fp64"][0]),
+            "mixed_vs_fp64_gradient": rel(res["mixed"][1], res["fp64"][1]),
+            "fp64_vs_long_double_B_6_sites": comp_rel(b64[sub], ld, 3)}
+    rtf = np.random.default_rng(33).normal(size=d.nsites)
+
+    def cpu_mag(n):
+        xn = tuple(c[:n] for c in x)
+        orc.calculate(orc.MAGNETIC, m, mv, xn, field=field,
+                      transform=orc.TF_TOTAL_FIELD_ANOMALY, tf_angles=field[:2])
+        orc.lqderivative(orc.MAGNETIC, m, mv, xn, rtf[:n], field=field,
+                         transform=orc.TF_TOTAL_FIELD_ANOMALY, tf_angles=field[:2])
+    cpu = cpu_rate(cpu_mag, d.nsites, budget=4.0)
+    row(4, "magnetic TFA forward + adjoint", m, d, times, "magnetic", errs, cpu, passes=2)
+    return rows
+
+
 def cpu_baseline(model, data, budget_s):
     from oracle import oracle as orc
     mv = model.GetModelParameters()
@@ -255,7 +433,7 @@ def main():
     ap.add_argument("--small", action="store_true", help="quarter-size mesh for quick checks")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 512 MiB device-to-device copy (counter calibration)")
-    ap.add_argument("--workload", default="all", choices=["all", "headline", "storedj"],
+    ap.add_argument("--workload", default="all", choices=["all", "headline", "storedj", "configs"],
                     help="headline: gz+FTG forward+adjoint only; storedj: batched stored-J only")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -265,6 +443,9 @@ def main():
 
     if args.workload == "storedj":
         print(json.dumps({"storedj": run_storedj(args)}), flush=True)
+        return
+    if args.workload == "configs":
+        print(json.dumps({"configs": run_configs(args)}), flush=True)
         return
 
     from jif3d_amd import synthetic
