@@ -189,7 +189,8 @@ jgm_status jgm_destroy(jgm_context* ctx) {
                                &ctx->near.zcen, &ctx->near.adj_table, &ctx->near.corner_off,
                                &ctx->near.cell_off, &ctx->near_h, &ctx->near_v, &ctx->vsites, &ctx->coef_mono,
                                &ctx->near.site_order, &ctx->near.mid_box, &ctx->near.xoff,
-                               &ctx->near.yoff, &ctx->near.zoff, &ctx->acc64};
+                               &ctx->near.yoff, &ctx->near.zoff, &ctx->acc64,
+                               &ctx->raw_cache_model, &ctx->raw_cache_raw, &ctx->raw_cache_flag};
   for (auto* b : bufs) b->release();
   jgm::release_host_storedj(ctx);
   if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
@@ -271,6 +272,7 @@ jgm_status jgm_set_mesh(jgm_context* ctx, size_t nx, size_t ny, size_t nz, const
   if ((st = upload(ctx->d_zsz, ctx->zs.data(), nz, ctx->stream, "mesh")) != JGM_OK) return st;
   hipError_t e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return hip_fail(e, "mesh upload");
+  ++ctx->config_gen;
   ctx->mesh_set = true;
   ctx->bg_valid = false;
   ctx->near.valid = false;
@@ -282,6 +284,7 @@ jgm_status jgm_set_background(jgm_context* ctx, size_t nlayers, const double* th
   if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
   if (nlayers > 0 && !thicknesses) return fail(JGM_ERR_INVALID_ARGUMENT, "null thicknesses");
   if (nlayers > 4096) return fail(JGM_ERR_UNSUPPORTED, "too many background layers");
+  ++ctx->config_gen;
   ctx->bgthick.assign(thicknesses, thicknesses + nlayers);
   ctx->bg_valid = false;
   ctx->storedJ_kind = 0;
@@ -308,6 +311,7 @@ jgm_status jgm_set_sites(jgm_context* ctx, size_t nsites, const double* x, const
   if ((st = upload(ctx->d_sz, z, nsites, ctx->stream, "sites")) != JGM_OK) return st;
   hipError_t e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return hip_fail(e, "site upload");
+  ++ctx->config_gen;
   ctx->sites_set = true;
   ctx->bg_valid = false;
   ctx->near.valid = false;
@@ -322,6 +326,7 @@ jgm_status jgm_set_field(jgm_context* ctx, double inclination, double declinatio
     return fail(JGM_ERR_INVALID_ARGUMENT, "non-finite field parameter");
   const bool changed = inclination != ctx->inclination || declination != ctx->declination ||
                        fieldstrength != ctx->fieldstrength;
+  ++ctx->config_gen;
   ctx->inclination = inclination;
   ctx->declination = declination;
   ctx->fieldstrength = fieldstrength;
@@ -335,6 +340,7 @@ jgm_status jgm_set_transform(jgm_context* ctx, jgm_transform tf, double inclinat
   if (tf != JGM_TF_NONE && tf != JGM_TF_FTG_INVARIANT && tf != JGM_TF_TOTAL_FIELD &&
       tf != JGM_TF_TOTAL_FIELD_ANOMALY)
     return fail(JGM_ERR_INVALID_ARGUMENT, "unknown transform %d", (int)tf);
+  ++ctx->config_gen;
   ctx->transform = tf;
   ctx->tf_inc = inclination;
   ctx->tf_dec = declination;
@@ -344,6 +350,7 @@ jgm_status jgm_set_transform(jgm_context* ctx, jgm_transform tf, double inclinat
 jgm_status jgm_set_mixed_threshold(jgm_context* ctx, double q0) {
   if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
   if (!(q0 >= 2.0 && q0 <= 1e6)) return fail(JGM_ERR_INVALID_ARGUMENT, "q0 must be in [2, 1e6]");
+  ++ctx->config_gen;
   ctx->mixed_q0 = q0;
   ctx->near.valid = false;
   return JGM_OK;
@@ -353,6 +360,7 @@ jgm_status jgm_set_mixed_far_radius(jgm_context* ctx, double q1) {
   if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
   if (!(q1 == 0.0 || (q1 >= 2.0 && q1 <= 1e6)))
     return fail(JGM_ERR_INVALID_ARGUMENT, "q1 must be 0 or in [2, 1e6]");
+  ++ctx->config_gen;
   ctx->mixed_q1 = q1;
   return JGM_OK;
 }
@@ -377,11 +385,27 @@ jgm_status jgm_forward(jgm_context* ctx, jgm_kind kind, jgm_precision prec, cons
   if (ctx->S == 0) return JGM_OK;
   if ((st = ensure_background(ctx)) != JGM_OK) return st;
   hipError_t e;
+  // with a transform, keep the raw data and the model for an adjoint at the same model
+  double* raw_out = nullptr;
+  const size_t nmodel = jgm::nparams(ctx, kind), nraw = ctx->S * jgm::raw_per_site(kind);
+  ctx->raw_cache.valid = false;
+  if (ctx->transform != JGM_TF_NONE) {
+    if ((e = ctx->raw_cache_raw.ensure(sizeof(double) * nraw)) != hipSuccess ||
+        (e = ctx->raw_cache_model.ensure(sizeof(double) * (nmodel ? nmodel : 1))) != hipSuccess)
+      return hip_fail(e, "forward");
+    raw_out = ctx->raw_cache_raw.as<double>();
+  }
   if (prec == JGM_MIXED)
-    e = jgm::launch_forward_mixed(ctx, kind, model, out, nullptr);
+    e = jgm::launch_forward_mixed(ctx, kind, model, out, raw_out);
   else
-    e = jgm::launch_forward_fp64(ctx, kind, model, out, nullptr);
+    e = jgm::launch_forward_fp64(ctx, kind, model, out, raw_out);
   if (e != hipSuccess) return hip_fail(e, "forward");
+  if (raw_out) {
+    if (nmodel && (e = hipMemcpyAsync(ctx->raw_cache_model.ptr, model, sizeof(double) * nmodel,
+                                      hipMemcpyDeviceToDevice, ctx->stream)) != hipSuccess)
+      return hip_fail(e, "forward");
+    ctx->raw_cache = {true, (int)kind, (int)prec, ctx->config_gen, nmodel};
+  }
   return JGM_OK;
 }
 
@@ -406,20 +430,39 @@ jgm_status jgm_adjoint(jgm_context* ctx, jgm_kind kind, jgm_precision prec, cons
   const double* rm = misfit;
   if (ctx->transform != JGM_TF_NONE) {
     // raw data at the model for the transform Jacobian (LQDerivative recomputes it
-    // per site, ThreeDGravMagImplementation.h:209-214)
-    if ((e = ctx->raw.ensure(sizeof(double) * S * kraw)) != hipSuccess) return hip_fail(e, "adjoint");
-    if ((e = ctx->out_stage.ensure(sizeof(double) * S)) != hipSuccess) return hip_fail(e, "adjoint");
+    // per site, ThreeDGravMagImplementation.h:209-214): taken from the last forward when
+    // that ran with the same kind, precision and settings on a bitwise equal model
     if ((e = ctx->misfit_stage.ensure(sizeof(double) * S * kraw)) != hipSuccess)
       return hip_fail(e, "adjoint");
-    if (prec == JGM_MIXED)
-      e = jgm::launch_forward_mixed(ctx, kind, model, ctx->out_stage.as<double>(),
-                                    ctx->raw.as<double>());
-    else
-      e = jgm::launch_forward_fp64(ctx, kind, model, ctx->out_stage.as<double>(),
-                                   ctx->raw.as<double>());
-    if (e != hipSuccess) return hip_fail(e, "adjoint (raw forward)");
-    e = jgm::launch_misfit_to_raw(ctx, kind, misfit, ctx->raw.as<const double>(),
-                                  ctx->misfit_stage.as<double>());
+    const double* raw = nullptr;
+    const jgm_context::RawCache& rc = ctx->raw_cache;
+    if (rc.valid && rc.kind == (int)kind && rc.prec == (int)prec && rc.gen == ctx->config_gen &&
+        rc.nmodel == jgm::nparams(ctx, kind)) {
+      int differ = 1;
+      if ((e = ctx->raw_cache_flag.ensure(sizeof(int))) != hipSuccess ||
+          (e = jgm::launch_models_differ(ctx, model, ctx->raw_cache_model.as<const double>(),
+                                         rc.nmodel, ctx->raw_cache_flag.as<int>())) != hipSuccess ||
+          (e = hipMemcpyAsync(&differ, ctx->raw_cache_flag.ptr, sizeof(int),
+                              hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
+          (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+        return hip_fail(e, "adjoint (model comparison)");
+      if (!differ) raw = ctx->raw_cache_raw.as<const double>();
+    }
+    if (!raw) {
+      if ((e = ctx->raw.ensure(sizeof(double) * S * kraw)) != hipSuccess)
+        return hip_fail(e, "adjoint");
+      if ((e = ctx->out_stage.ensure(sizeof(double) * S)) != hipSuccess)
+        return hip_fail(e, "adjoint");
+      if (prec == JGM_MIXED)
+        e = jgm::launch_forward_mixed(ctx, kind, model, ctx->out_stage.as<double>(),
+                                      ctx->raw.as<double>());
+      else
+        e = jgm::launch_forward_fp64(ctx, kind, model, ctx->out_stage.as<double>(),
+                                     ctx->raw.as<double>());
+      if (e != hipSuccess) return hip_fail(e, "adjoint (raw forward)");
+      raw = ctx->raw.as<const double>();
+    }
+    e = jgm::launch_misfit_to_raw(ctx, kind, misfit, raw, ctx->misfit_stage.as<double>());
     if (e != hipSuccess) return hip_fail(e, "adjoint (transform)");
     rm = ctx->misfit_stage.as<const double>();
   }

@@ -124,6 +124,19 @@ struct jgm_context {
   double mixed_q0 = 8.0;
   double mixed_q1 = 24.0;
 
+  // incremented by every setter: caches record the generation they were made for
+  unsigned long config_gen = 1;
+  // raw (untransformed) data of the last jgm_forward made with a data transform, and a
+  // copy of its model: an adjoint at the same model takes the transform's Jacobian from
+  // it instead of a second forward
+  struct RawCache {
+    bool valid = false;
+    int kind = 0, prec = 0;
+    unsigned long gen = 0;
+    size_t nmodel = 0;
+  } raw_cache;
+  jgm::DeviceBuffer raw_cache_model, raw_cache_raw, raw_cache_flag;
+
   // caches
   jgm::DeviceBuffer bgT;  // [S][L][7] background box terms, valid when bg_valid
   bool bg_valid = false;
@@ -178,6 +191,9 @@ hipError_t launch_adjoint_mixed(jgm_context* ctx, int kind, const double* model,
                                 const double* raw_misfit, double* grad);
 hipError_t build_near_boxes(jgm_context* ctx);
 hipError_t launch_fill_zero(jgm_context* ctx, double* p, size_t n);
+// flag[0] = 0 if a[0..n) and b[0..n) are bitwise equal, 1 otherwise
+hipError_t launch_models_differ(jgm_context* ctx, const double* a, const double* b, size_t n,
+                                int* flag);
 hipError_t launch_background_gradient(jgm_context* ctx, int kind, const double* raw_misfit,
                                       double* grad_bg);
 hipError_t launch_storedj_build(jgm_context* ctx, int kind, int precision);

@@ -855,6 +855,29 @@ hipError_t magnetization_adjoint(jgm_context* ctx, const double* rm, double* gra
 
 }  // namespace
 
+namespace {
+__global__ __launch_bounds__(kBlock) void k_zero_flag(int* __restrict__ flag) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) flag[0] = 0;
+}
+// every lane that finds a differing element stores the same value 1
+__global__ __launch_bounds__(kBlock) void k_models_differ(const double* __restrict__ a,
+                                                          const double* __restrict__ b, long n,
+                                                          int* __restrict__ flag) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    if (__double_as_longlong(a[i]) != __double_as_longlong(b[i])) flag[0] = 1;
+}
+}  // namespace
+
+hipError_t launch_models_differ(jgm_context* ctx, const double* a, const double* b, size_t n,
+                                int* flag) {
+  hipLaunchKernelGGL(k_zero_flag, dim3(1), dim3(kBlock), 0, ctx->stream, flag);
+  if (n > 0)
+    hipLaunchKernelGGL(k_models_differ, dim3((unsigned)std::min<size_t>(blocks_for((long)n), 1024)),
+                       dim3(kBlock), 0, ctx->stream, a, b, (long)n, flag);
+  return hipGetLastError();
+}
+
 hipError_t launch_background_gradient(jgm_context* ctx, int kind, const double* rm,
                                       double* grad_bg) {
   const int L = (int)ctx->bgthick.size();
