@@ -143,7 +143,12 @@ jgm_status jgm_forward_host(jgm_context *ctx, jgm_kind kind, jgm_precision prec,
 
 /* Least-squares derivative 2 J^T D^T misfit, matrix free (J recomputed, never stored):
  * ThreeDGravMagImplementation::LQDerivative (:177-231).  model is needed for the
- * transform Jacobian D.  misfit: nsites * data_per_site; grad: n_params (device). */
+ * transform Jacobian D.  misfit: nsites * data_per_site; grad: n_params (device).
+ * With a transform set, jgm_forward keeps its raw data and a copy of its model, and
+ * jgm_adjoint uses them when its model is bitwise equal to that copy and no setting
+ * has changed since (it then waits for the context's stream once, to read the
+ * comparison); otherwise it computes the raw data again.  Results are the same
+ * either way. */
 jgm_status jgm_adjoint(jgm_context *ctx, jgm_kind kind, jgm_precision prec, const double *model,
                        const double *misfit, double *grad);
 jgm_status jgm_adjoint_host(jgm_context *ctx, jgm_kind kind, jgm_precision prec,
