@@ -190,7 +190,8 @@ jgm_status jgm_destroy(jgm_context* ctx) {
                                &ctx->near.cell_off, &ctx->near_h, &ctx->near_v, &ctx->vsites, &ctx->coef_mono,
                                &ctx->near.site_order, &ctx->near.mid_box, &ctx->near.xoff,
                                &ctx->near.yoff, &ctx->near.zoff, &ctx->acc64,
-                               &ctx->raw_cache_model, &ctx->raw_cache_raw, &ctx->raw_cache_flag};
+                               &ctx->raw_cache_model, &ctx->raw_cache_raw, &ctx->raw_cache_flag,
+                               &ctx->partsum, &ctx->chunk_scratch};
   for (auto* b : bufs) b->release();
   jgm::release_host_storedj(ctx);
   if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);

@@ -144,7 +144,8 @@ struct jgm_context {
 
   // scratch
   jgm::DeviceBuffer weights, partial, raw, coef, hpart, hsum, model_stage, out_stage, misfit_stage,
-      grad_stage, farpart, nearpart, coef32, coef_mono, near_h, near_v, vsites;
+      grad_stage, farpart, nearpart, coef32, coef_mono, near_h, near_v, vsites, partsum,
+      chunk_scratch;
 
   jgm::Profiler prof;
 
@@ -191,6 +192,9 @@ hipError_t launch_adjoint_mixed(jgm_context* ctx, int kind, const double* model,
                                 const double* raw_misfit, double* grad);
 hipError_t build_near_boxes(jgm_context* ctx);
 hipError_t launch_fill_zero(jgm_context* ctx, double* p, size_t n);
+// out[i] = sum over c < nchunks of part[c * n + i] (fixed order, two passes for many chunks)
+hipError_t launch_reduce_chunks(jgm_context* ctx, const double* part, int nchunks, long n,
+                                double* out);
 // flag[0] = 0 if a[0..n) and b[0..n) are bitwise equal, 1 otherwise
 hipError_t launch_models_differ(jgm_context* ctx, const double* a, const double* b, size_t n,
                                 int* flag);

@@ -272,6 +272,20 @@ __global__ __launch_bounds__(kBlock) void k_sum_chunks(const double* __restrict_
   out[q] = a;
 }
 
+// First pass of the chunk reduction: group g sums chunks [g per, (g + 1) per) of every
+// row; rows are coalesced across lanes.
+__global__ __launch_bounds__(kBlock) void k_sum_chunk_groups(const double* __restrict__ part,
+                                                             int nchunks, int per, long n,
+                                                             double* __restrict__ grp) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int c0 = (int)blockIdx.y * per;
+  const int c1 = c0 + per < nchunks ? c0 + per : nchunks;
+  double a = 0.0;
+  for (int c = c0; c < c1; ++c) a += part[(long)c * n + q];
+  grp[(long)blockIdx.y * n + q] = a;
+}
+
 // g_c = 2 sum_{a,b,c} sigma(a,b,c) H(i+a, j+b, k+c)
 __global__ __launch_bounds__(kBlock) void k_corner_difference(const double* __restrict__ H,
                                                               int nx, int ny, int nz,
@@ -461,12 +475,19 @@ hipError_t forward_impl(jgm_context* ctx, int kind, int nb, const double* models
   const double tf_bx = std::cos(ctx->tf_inc) * std::cos(ctx->tf_dec);
   const double tf_by = std::cos(ctx->tf_inc) * std::sin(ctx->tf_dec);
   const double tf_bz = std::sin(ctx->tf_inc);
-  for (int b = 0; b < nb; ++b)
+  if (nchunks > 0 && (e = ctx->partsum.ensure(sizeof(double) * (size_t)NC * S)) != hipSuccess)
+    return e;
+  for (int b = 0; b < nb; ++b) {
+    if (nchunks > 0 &&
+        (e = launch_reduce_chunks(ctx, ctx->partial.as<const double>() + (size_t)b * nchunks * NC * S,
+                                  nchunks, (long)NC * S, ctx->partsum.as<double>())) != hipSuccess)
+      return e;
     hipLaunchKernelGGL((k_forward_finish<MASK>), dim3(blocks_for(S)), dim3(kBlock), 0, st,
-                       ctx->partial.as<const double>() + (size_t)b * nchunks * NC * S, nchunks, S,
+                       ctx->partsum.as<const double>(), nchunks > 0 ? 1 : 0, S,
                        ctx->bgT.as<const double>(), models + b * model_ld + N, L, kind,
                        field_dir(ctx), ctx->transform, tf_bx, tf_by, tf_bz, out + b * out_ld,
                        raw_out ? raw_out + b * raw_ld : nullptr);
+  }
   return hipGetLastError();
 }
 
@@ -733,8 +754,14 @@ hipError_t maggrad_forward(jgm_context* ctx, const double* model, double* out,
                        ctx->d_Yc.as<double>(), ctx->d_Zc.as<double>(), ncy, ncz,
                        ctx->weights.as<const double>(), ncols, per, ctx->partial.as<double>());
   }
+  if (nchunks > 0) {
+    if ((e = ctx->partsum.ensure(sizeof(double) * 6 * (size_t)S2)) != hipSuccess) return e;
+    if ((e = launch_reduce_chunks(ctx, ctx->partial.as<const double>(), (int)nchunks, 6L * S2,
+                                  ctx->partsum.as<double>())) != hipSuccess)
+      return e;
+  }
   hipLaunchKernelGGL(k_maggrad_finish, dim3(blocks_for(S)), dim3(kBlock), 0, st,
-                     ctx->partial.as<const double>(), (int)nchunks, S, field_dir(ctx),
+                     ctx->partsum.as<const double>(), nchunks > 0 ? 1 : 0, S, field_dir(ctx),
                      ctx->fieldstrength, out, raw_out);
   return hipGetLastError();
 }
@@ -812,8 +839,14 @@ hipError_t magnetization_forward(jgm_context* ctx, const double* model, double* 
   const double tf_bx = std::cos(ctx->tf_inc) * std::cos(ctx->tf_dec);
   const double tf_by = std::cos(ctx->tf_inc) * std::sin(ctx->tf_dec);
   const double tf_bz = std::sin(ctx->tf_inc);
+  if (nchunks > 0) {
+    if ((e = ctx->partsum.ensure(sizeof(double) * 3 * (size_t)S)) != hipSuccess) return e;
+    if ((e = launch_reduce_chunks(ctx, ctx->partial.as<const double>(), (int)nchunks, 3L * S,
+                                  ctx->partsum.as<double>())) != hipSuccess)
+      return e;
+  }
   hipLaunchKernelGGL(k_mag3_finish, dim3(blocks_for(S)), dim3(kBlock), 0, st,
-                     ctx->partial.as<const double>(), (int)nchunks, S, magnetization_scale(),
+                     ctx->partsum.as<const double>(), nchunks > 0 ? 1 : 0, S, magnetization_scale(),
                      ctx->transform, tf_bx, tf_by, tf_bz, out, raw_out);
   return hipGetLastError();
 }
@@ -854,6 +887,31 @@ hipError_t magnetization_adjoint(jgm_context* ctx, const double* rm, double* gra
 }
 
 }  // namespace
+
+// out[i] = sum_c part[c * n + i].  A per-row loop over all chunks leaves the GPU idle
+// when the rows are few (a forward over few sites has few rows and many chunks), so
+// many chunks go through a grouped first pass (about 256k threads) and a second pass
+// over the group totals.  The order of additions is fixed.
+hipError_t launch_reduce_chunks(jgm_context* ctx, const double* part, int nchunks, long n,
+                                double* out) {
+  if (n <= 0) return hipSuccess;
+  hipStream_t st = ctx->stream;
+  if (nchunks <= 8) {
+    hipLaunchKernelGGL(k_sum_chunks, dim3(blocks_for(n)), dim3(kBlock), 0, st, part, nchunks, n,
+                       out);
+    return hipGetLastError();
+  }
+  long groups = std::max<long>(1, std::min<long>((262144 + n - 1) / n, nchunks / 4));
+  const int per = (int)((nchunks + groups - 1) / groups);
+  groups = (nchunks + per - 1) / per;
+  hipError_t e = ctx->chunk_scratch.ensure(sizeof(double) * (size_t)groups * n);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_sum_chunk_groups, dim3(blocks_for(n), (unsigned)groups), dim3(kBlock), 0,
+                     st, part, nchunks, per, n, ctx->chunk_scratch.as<double>());
+  hipLaunchKernelGGL(k_sum_chunks, dim3(blocks_for(n)), dim3(kBlock), 0, st,
+                     ctx->chunk_scratch.as<const double>(), (int)groups, n, out);
+  return hipGetLastError();
+}
 
 namespace {
 __global__ __launch_bounds__(kBlock) void k_zero_flag(int* __restrict__ flag) {

@@ -867,8 +867,15 @@ hipError_t forward_mixed_impl(jgm_context* ctx, int kind, const double* model, d
   const double tf_bx = std::cos(ctx->tf_inc) * std::cos(ctx->tf_dec);
   const double tf_by = std::cos(ctx->tf_inc) * std::sin(ctx->tf_dec);
   const double tf_bz = std::sin(ctx->tf_inc);
+  if (nchunks > 0) {
+    if ((e = ctx->partsum.ensure(sizeof(double) * (size_t)NC * S)) != hipSuccess) return e;
+    if ((e = launch_reduce_chunks(ctx, ctx->farpart.as<const double>(), nchunks, (long)NC * S,
+                                  ctx->partsum.as<double>())) != hipSuccess)
+      return e;
+  }
   hipLaunchKernelGGL((k_mixed_forward_finish<MASK>), dim3(blocks_for(S)), dim3(kBlock), 0, st,
-                     ctx->farpart.as<const double>(), nchunks, ctx->nearpart.as<const double>(), S,
+                     ctx->partsum.as<const double>(), nchunks > 0 ? 1 : 0,
+                     ctx->nearpart.as<const double>(), S,
                      ctx->bgT.as<const double>(), model + N, L, kind, field_dir(ctx),
                      ctx->transform, tf_bx, tf_by, tf_bz, out, raw_out);
   return hipGetLastError();
