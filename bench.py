@@ -50,8 +50,8 @@ def log(msg):
 
 # library timer names -> kernel symbols in the rocprofv3 summaries (cubic mesh)
 KERNEL_SYMBOL = {
-    "forward_corners_fp64_gz_ftg": "k_forward_corners<7>",
-    "adjoint_corners_fp64_gz_ftg": "k_adjoint_corners<7>",
+    "forward_corners_fp64_gz_ftg": "k_forward_corners<7, 1>",
+    "adjoint_corners_fp64_gz_ftg": "k_adjoint_corners<7, 1>",
     "far_forward_fp32_gz_ftg": "k_far_forward<7, 0>",
     "far_adjoint_fp32_gz_ftg": "k_far_adjoint<7, 0>",
 }
