@@ -62,9 +62,7 @@ struct NearBoxes {
   DeviceBuffer adj_table;  // FP32 per-cell table with unit density (G V, B...)
   bool adj_table_valid = false;
   DeviceBuffer site_order;  // int32 [S]: sites in Morton order of (x, y), for compact waves
-  DeviceBuffer mid_box;     // int32 [S][6]: cells within the second radius (hexadecapole kept)
   double half_diag = 0.0;   // largest cell half-diagonal (m)
-  double mid_q1 = -1.0;     // second radius the mid boxes were built for
 };
 
 // Optional per-kernel timing: HIP events recorded on the context's stream around
@@ -122,7 +120,7 @@ struct jgm_context {
   // in half-diagonals) beyond which the far field keeps only monopole (and quadrupole)
   // terms; 0 keeps the hexadecapole everywhere
   double mixed_q0 = 8.0;
-  double mixed_q1 = 24.0;
+  double mixed_q1 = 26.0;
 
   // incremented by every setter: caches record the generation they were made for
   unsigned long config_gen = 1;

@@ -14,9 +14,10 @@
 // U_ij = G rho V d2Phi/dx_i dx_j.  Odd orders vanish by symmetry, so the neglected
 // term is O((h/d)^6) relative.  Cubes have B = 0.  The formulas are evaluated in
 // direction cosines u = x/r so that no power of 1/r below r^-7 is formed (FP32
-// range).  Beyond a second radius (q1 half-diagonals, default 24; the per-site "mid
-// box" holds every cell closer than that) the hexadecapole, O((h/d)^4) of a cell's
-// field there, is dropped; for cubes the remaining monopole is evaluated in a
+// range).  Beyond a second radius (q1 half-diagonals, default 26, tested per tile
+// against the distance from the site to the tile's nearest cell centre) the
+// hexadecapole, O((h/d)^4) of a cell's field there, is dropped; for cubes the remaining
+// monopole is evaluated in a
 // regrouped form (running sums of w s^3 and w s^5 along a column of cells in the
 // forward, the site's coefficients folded into one quadratic form in the adjoint).
 // Site offsets are formed in FP64 against the first cell of a tile and rounded once
@@ -25,7 +26,7 @@
 // 4 cells or 4 columns (forward) or 8 sites (adjoint): the rounding error of an FP32
 // run grows with the run length times its partial sum.  Forward waves take their 64
 // sites in Morton order of (x, y) so that a wave's sites cluster and tiles outside all
-// their near and mid boxes take the cheap path for the whole wave.
+// their near boxes and second radii take the cheap path for the whole wave.
 //
 // Layouts in HBM
 //   cell table   FP32: uniform mesh [c] = {w}; otherwise [c][12] =
@@ -347,27 +348,24 @@ __device__ __forceinline__ void far_tile_mono(const FarGeom& g, const float* __r
 
 // Forward far field: one lane per site; each wave walks a chunk of 4x4x4 tiles.
 // Lanes take sites in `order` (spatially compact waves).  Cells inside the lane's
-// near box contribute nothing here (FP64 near pass).  A tile outside the mid box of
-// every site of the wave (the box holding all cells within the second radius) keeps
-// only the monopole (and quadrupole) terms.  Site offsets are formed in FP64 against
+// near box contribute nothing here (FP64 near pass).  A tile whose cells are all beyond
+// the second radius of every site of the wave keeps only the monopole (and quadrupole)
+// terms.  Site offsets are formed in FP64 against
 // the tile's first cell and rounded once per tile; cell offsets within a tile are exact
 // for uniform meshes.
 template <int MASK, int MESH>
 __global__ __launch_bounds__(kBlock) void k_far_forward(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
-    int S, const int* __restrict__ order, const int* __restrict__ box,
-    const int* __restrict__ mid, FarGeom g, const float* __restrict__ table, MpConst uc,
-    long ntiles, long tiles_per_chunk, double* __restrict__ farpart) {
+    int S, const int* __restrict__ order, const int* __restrict__ box, float r1sq, FarGeom g,
+    const float* __restrict__ table, MpConst uc, long ntiles, long tiles_per_chunk,
+    double* __restrict__ farpart) {
   constexpr int NC = Ncomp<MASK>::value;
   const int li = blockIdx.x * blockDim.x + threadIdx.x;
   const int s = order[li < S ? li : S - 1];
   const double px = sx[s], py = sy[s], pz = sz[s];
-  int bx[6], mb[6];
+  int bx[6];
 #pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    bx[q] = box[s * 6 + q];
-    mb[q] = mid[s * 6 + q];
-  }
+  for (int q = 0; q < 6; ++q) bx[q] = box[s * 6 + q];
   const long t0 = (long)blockIdx.y * tiles_per_chunk;
   const long t1 = t0 + tiles_per_chunk < ntiles ? t0 + tiles_per_chunk : ntiles;
   double acc64[NC];
@@ -384,8 +382,12 @@ __global__ __launch_bounds__(kBlock) void k_far_forward(
                 Zt = (float)(pz - g.zcen[k0]);
     const bool hit = !(bx[1] < i0 || bx[0] > i0 + ni - 1 || bx[3] < j0 || bx[2] > j0 + nj - 1 ||
                        bx[5] < k0 || bx[4] > k0 + nk - 1);
-    const bool need = !(mb[1] < i0 || mb[0] > i0 + ni - 1 || mb[3] < j0 ||
-                        mb[2] > j0 + nj - 1 || mb[5] < k0 || mb[4] > k0 + nk - 1);
+    // some cell of the tile closer than the second radius: keep the hexadecapole (the
+    // offsets to the tile's cell centres span [Xt - extent, Xt] along each axis)
+    const float dx = fmaxf(0.0f, fmaxf(-Xt, Xt - g.xoff[i0 + ni - 1]));
+    const float dy = fmaxf(0.0f, fmaxf(-Yt, Yt - g.yoff[j0 + nj - 1]));
+    const float dz = fmaxf(0.0f, fmaxf(-Zt, Zt - g.zoff[k0 + nk - 1]));
+    const bool need = fmaf(dx, dx, fmaf(dy, dy, dz * dz)) < r1sq;
     if (__any(hit))
       far_tile<MASK, MESH, true, true>(g, table, uc, i0, j0, k0, ni, nj, nk, Xt, Yt, Zt, bx,
                                        acc64);
@@ -513,15 +515,15 @@ __global__ __launch_bounds__(kBlock) void k_mixed_coef(const double* __restrict_
 // sum_m coef_m(s) T_m(s, c) in FP32, flushed to FP64 every 8 sites; cells in the site's
 // near box are skipped (FP64 near pass).  Site offsets are formed in FP64 against the
 // tile's first cell once per site and lane and shared by the lane's cells; a site whose
-// mid box misses the tile keeps only the monopole (and quadrupole) terms, a
-// wave-uniform branch.
+// second radius misses every cell of the tile keeps only the monopole (and quadrupole)
+// terms, a wave-uniform branch.
 constexpr int kQ = 4;  // cells per lane of the far adjoint
 template <int MASK, int MESH>
 __global__ __launch_bounds__(kBlock) void k_far_adjoint(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     const float* __restrict__ coef, const float* __restrict__ mono, const int* __restrict__ box,
-    const int* __restrict__ mid, int S, int sites_per_chunk, FarGeom g,
-    const float* __restrict__ table, MpConst uc, long ntiles, double* __restrict__ gpart) {
+    float r1sq, int S, int sites_per_chunk, FarGeom g, const float* __restrict__ table,
+    MpConst uc, long ntiles, double* __restrict__ gpart) {
   constexpr int NC = Ncomp<MASK>::value;
   constexpr int TW = TableWidth<MESH>::value;
   constexpr bool QUAD = MESH != kMeshCubes;
@@ -541,6 +543,9 @@ __global__ __launch_bounds__(kBlock) void k_far_adjoint(
   const int i = i0 + (vij ? ii : 0), j = j0 + (vij ? jj : 0);
   const double x0 = g.xcen[i0], y0 = g.ycen[j0], z0 = g.zcen[k0];
   const float ox = (float)(g.xcen[i] - x0), oy = (float)(g.ycen[j] - y0);
+  // extent of the tile's cell centres from its first cell, per axis
+  const float ex = (float)(g.xcen[i0 + ni - 1] - x0), ey = (float)(g.ycen[j0 + nj - 1] - y0),
+              ez = (float)(g.zcen[k0 + nk - 1] - z0);
   bool vq[kQ];
   int kq[kQ];
   long cq[kQ];
@@ -566,11 +571,8 @@ __global__ __launch_bounds__(kBlock) void k_far_adjoint(
   int nacc = 0;
   for (int s = s0; s < s1; ++s) {
     const int* bx = box + s * 6;
-    const int* mb = mid + s * 6;
     const bool tile_hit = !(bx[1] < i0 || bx[0] > i0 + ni - 1 || bx[3] < j0 ||
                             bx[2] > j0 + nj - 1 || bx[5] < k0 || bx[4] > k0 + nk - 1);
-    const bool need = !(mb[1] < i0 || mb[0] > i0 + ni - 1 || mb[3] < j0 ||
-                        mb[2] > j0 + nj - 1 || mb[5] < k0 || mb[4] > k0 + nk - 1);
     bool nq[kQ];
 #pragma unroll
     for (int q = 0; q < kQ; ++q) nq[q] = false;
@@ -581,6 +583,9 @@ __global__ __launch_bounds__(kBlock) void k_far_adjoint(
     }
     const float Xt = (float)(sx[s] - x0), Yt = (float)(sy[s] - y0), Zt = (float)(sz[s] - z0);
     const float X = Xt - ox, Y = Yt - oy;
+    const float dx = fmaxf(0.0f, fmaxf(-Xt, Xt - ex)), dy = fmaxf(0.0f, fmaxf(-Yt, Yt - ey)),
+                dz = fmaxf(0.0f, fmaxf(-Zt, Zt - ez));
+    const bool need = fmaf(dx, dx, fmaf(dy, dy, dz * dz)) < r1sq;
     float Zq[kQ], pq[kQ];
 #pragma unroll
     for (int q = 0; q < kQ; ++q) Zq[q] = nq[q] ? 1.0e6f : Zt - ozq[q];
@@ -754,6 +759,14 @@ FarGeom far_geom(const jgm_context* ctx) {
   return g;
 }
 
+// Square of the second radius in m^2 (cells closer than it keep the hexadecapole);
+// every cell when the radius is 0.
+float second_radius_sq(const jgm_context* ctx) {
+  if (!(ctx->mixed_q1 > 0.0)) return FLT_MAX;
+  const double r1 = ctx->mixed_q1 * ctx->near.half_diag;
+  return (float)(r1 * r1);
+}
+
 MpConst uniform_const(const jgm_context* ctx) {
   if (ctx->nx == 0 || ctx->ny == 0 || ctx->nz == 0) return MpConst{};
   return mp_const(ctx->xs[0], ctx->ys[0], ctx->zs[0]);
@@ -848,7 +861,7 @@ hipError_t forward_mixed_impl(jgm_context* ctx, int kind, const double* model, d
       hipLaunchKernelGGL((k_far_forward<MASK, MESH>), dim3(site_blocks, nchunks), dim3(kBlock), 0,
                          st, ctx->d_sx.as<double>(), ctx->d_sy.as<double>(),
                          ctx->d_sz.as<double>(), S, ctx->near.site_order.as<const int>(),
-                         ctx->near.box.as<const int>(), ctx->near.mid_box.as<const int>(), g,
+                         ctx->near.box.as<const int>(), second_radius_sq(ctx), g,
                          ctx->weights.as<const float>(), uniform_const(ctx), ntiles, per,
                          ctx->farpart.as<double>());
     }
@@ -922,7 +935,7 @@ hipError_t adjoint_mixed_impl(jgm_context* ctx, int kind, const double* rm, doub
                          dim3(kBlock), 0, st, ctx->d_sx.as<double>(), ctx->d_sy.as<double>(),
                          ctx->d_sz.as<double>(), ctx->coef32.as<const float>(),
                          ctx->coef_mono.as<const float>(), nb.box.as<const int>(),
-                         nb.mid_box.as<const int>(), S, per, g,
+                         second_radius_sq(ctx), S, per, g,
                          nb.adj_table.as<const float>(), uniform_const(ctx), ntiles8,
                          ctx->hpart.as<double>());
     }
@@ -1095,21 +1108,6 @@ hipError_t build_near_boxes(jgm_context* ctx) {
   if ((e = upload_vec(nb.zoff, zoff, st)) != hipSuccess) return e;
   const std::vector<int> order = morton_order(ctx);
   if ((e = upload_vec(nb.site_order, order, st)) != hipSuccess) return e;
-  // mid boxes: every cell within the second radius q1 of the site (all cells for q1 = 0)
-  std::vector<int> mid(std::max<size_t>(1, S) * 6);
-  const double R1 = ctx->mixed_q1 * nb.half_diag;
-  for (size_t s = 0; s < S; ++s) {
-    int* o = &mid[s * 6];
-    if (!(ctx->mixed_q1 > 0.0)) {
-      o[0] = 0; o[1] = nx - 1; o[2] = 0; o[3] = ny - 1; o[4] = 0; o[5] = nz - 1;
-      continue;
-    }
-    axis_range(xcen, ctx->hsx[s], R1, &o[0], &o[1]);
-    axis_range(ycen, ctx->hsy[s], R1, &o[2], &o[3]);
-    axis_range(zcen, ctx->hsz[s], R1, &o[4], &o[5]);
-  }
-  if ((e = upload_vec(nb.mid_box, mid, st)) != hipSuccess) return e;
-  nb.mid_q1 = ctx->mixed_q1;
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
   nb.adj_table_valid = false;
   nb.valid = true;
@@ -1118,7 +1116,7 @@ hipError_t build_near_boxes(jgm_context* ctx) {
 
 hipError_t launch_forward_mixed(jgm_context* ctx, int kind, const double* model, double* out,
                                 double* raw_out) {
-  if (!ctx->near.valid || ctx->near.mid_q1 != ctx->mixed_q1) {
+  if (!ctx->near.valid) {
     hipError_t e = build_near_boxes(ctx);
     if (e != hipSuccess) return e;
   }
@@ -1131,7 +1129,7 @@ hipError_t launch_forward_mixed(jgm_context* ctx, int kind, const double* model,
 
 hipError_t launch_adjoint_mixed(jgm_context* ctx, int kind, const double* /*model*/,
                                 const double* raw_misfit, double* grad) {
-  if (!ctx->near.valid || ctx->near.mid_q1 != ctx->mixed_q1) {
+  if (!ctx->near.valid) {
     hipError_t e = build_near_boxes(ctx);
     if (e != hipSuccess) return e;
   }
