@@ -163,9 +163,10 @@ jgm_status jgm_adjoint_host(jgm_context *ctx, jgm_kind kind, jgm_precision prec,
  * misfits[b * misfit_ld ...] and its gradient grads[b * grad_ld ...] (grad_ld >=
  * jgm_model_size); all device memory.  With JGM_FP64 and the gravity and
  * susceptibility kinds up to 8 models share each corner-function evaluation, so a
- * batch of 8 costs little more than one model; each result is bit-identical to the
- * single-model call.  JGM_MIXED and the vertical-gradient and magnetization kinds run
- * the models one after another.  nmodels == 0 is a no-op. */
+ * batch of 8 costs little more than one model; with JGM_MIXED up to 4 models share each
+ * far-field multipole evaluation.  Each result is bit-identical to the single-model
+ * call.  The vertical-gradient and magnetization kinds run the models one after
+ * another.  nmodels == 0 is a no-op. */
 jgm_status jgm_forward_batch(jgm_context *ctx, jgm_kind kind, jgm_precision prec, size_t nmodels,
                              const double *models, size_t model_ld, double *out, size_t out_ld);
 jgm_status jgm_adjoint_batch(jgm_context *ctx, jgm_kind kind, jgm_precision prec, size_t nmodels,

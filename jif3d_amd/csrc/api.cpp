@@ -557,8 +557,8 @@ jgm_status jgm_forward_batch(jgm_context* ctx, jgm_kind kind, jgm_precision prec
   if ((st = ensure_background(ctx)) != JGM_OK) return st;
   hipError_t e = hipSuccess;
   if (prec == JGM_MIXED) {
-    for (size_t b = 0; b < nmodels && e == hipSuccess; ++b)
-      e = jgm::launch_forward_mixed(ctx, kind, models + b * model_ld, out + b * out_ld, nullptr);
+    e = jgm::launch_forward_mixed_batch(ctx, kind, nmodels, models, model_ld, out, out_ld,
+                                        nullptr, 0);
   } else {
     e = jgm::launch_forward_fp64_batch(ctx, kind, nmodels, models, model_ld, out, out_ld, nullptr,
                                        0);
@@ -603,9 +603,8 @@ jgm_status jgm_adjoint_batch(jgm_context* ctx, jgm_kind kind, jgm_precision prec
     double* raw = ctx->raw.as<double>();
     double* outs = ctx->out_stage.as<double>();
     if (prec == JGM_MIXED) {
-      for (size_t b = 0; b < nmodels && e == hipSuccess; ++b)
-        e = jgm::launch_forward_mixed(ctx, kind, models + b * model_ld, outs + b * S,
-                                      raw + b * nraw);
+      e = jgm::launch_forward_mixed_batch(ctx, kind, nmodels, models, model_ld, outs, S, raw,
+                                          nraw);
     } else {
       e = jgm::launch_forward_fp64_batch(ctx, kind, nmodels, models, model_ld, outs, S, raw,
                                          nraw);
@@ -619,9 +618,7 @@ jgm_status jgm_adjoint_batch(jgm_context* ctx, jgm_kind kind, jgm_precision prec
     rm_ld = nraw;
   }
   if (prec == JGM_MIXED) {
-    for (size_t b = 0; b < nmodels && e == hipSuccess; ++b)
-      e = jgm::launch_adjoint_mixed(ctx, kind, models ? models + b * model_ld : nullptr,
-                                    rm + b * rm_ld, grads + b * grad_ld);
+    e = jgm::launch_adjoint_mixed_batch(ctx, kind, nmodels, rm, rm_ld, grads, grad_ld);
   } else {
     e = jgm::launch_adjoint_fp64_batch(ctx, kind, nmodels, rm, rm_ld, grads, grad_ld);
   }

@@ -186,8 +186,15 @@ hipError_t launch_sensitivity_rows(jgm_context* ctx, int kind, size_t site0, siz
                                    double* rows);
 hipError_t launch_forward_mixed(jgm_context* ctx, int kind, const double* model, double* out,
                                 double* raw_out);
+hipError_t launch_forward_mixed_batch(jgm_context* ctx, int kind, size_t nmodels,
+                                      const double* models, size_t model_ld, double* out,
+                                      size_t out_ld, double* raw_out, size_t raw_ld);
 hipError_t launch_adjoint_mixed(jgm_context* ctx, int kind, const double* model,
                                 const double* raw_misfit, double* grad);
+// nmodels gradients, in groups of 4, 2 and 1 that share the far-field kernel terms
+hipError_t launch_adjoint_mixed_batch(jgm_context* ctx, int kind, size_t nmodels,
+                                      const double* raw_misfits, size_t rm_ld, double* grads,
+                                      size_t grad_ld);
 hipError_t build_near_boxes(jgm_context* ctx);
 hipError_t launch_fill_zero(jgm_context* ctx, double* p, size_t n);
 // out[i] = sum over c < nchunks of part[c * n + i] (fixed order, two passes for many chunks)

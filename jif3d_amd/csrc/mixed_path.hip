@@ -220,6 +220,24 @@ __global__ __launch_bounds__(kBlock) void k_cell_table(const double* __restrict_
   }
 }
 
+// Cell weights of nb models: wt[c * nb + b] = G rho_b V (FP32), rho_b at rho + b * model_ld.
+__global__ __launch_bounds__(kBlock) void k_cell_weights(const double* __restrict__ rho,
+                                                         long model_ld, int nb,
+                                                         const double* __restrict__ xsz,
+                                                         const double* __restrict__ ysz,
+                                                         const double* __restrict__ zsz, int nx,
+                                                         int ny, int nz, float* __restrict__ wt) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = (long)nx * ny * nz;
+  if (c >= n) return;
+  const int k = (int)(c % nz);
+  const long t = c / nz;
+  const int j = (int)(t % ny);
+  const int i = (int)(t / ny);
+  const double vol = xsz[i] * ysz[j] * zsz[k];
+  for (int b = 0; b < nb; ++b) wt[c * nb + b] = (float)(kGravConst * rho[b * model_ld + c] * vol);
+}
+
 struct FarGeom {
   const double* xcen;
   const double* ycen;
@@ -237,15 +255,17 @@ struct TableWidth {
 };
 
 // Sum over the cells of one 4x4x4 tile for one lane (site at Xt, Yt, Zt from the
-// centre of the tile's first cell), flushed to FP64 per column of cells.  MASKED
-// skips the lane's near-box cells (evaluated at a harmless far point, weight 0).
-template <int MASK, int MESH, bool HEXA, bool MASKED>
-__device__ __forceinline__ void far_tile(const FarGeom& g, const float* __restrict__ table,
-                                         const MpConst& uc, int i0, int j0, int k0, int ni,
-                                         int nj, int nk, float Xt, float Yt, float Zt,
-                                         const int (&bx)[6], double* acc64) {
+// centre of the tile's first cell), flushed to FP64 per column of cells, for NB models
+// at once (weights wt[c * NB + b]; the multipole terms of a pair are evaluated once).
+// MASKED skips the lane's near-box cells (evaluated at a harmless far point, weight 0).
+// shape: the unit-density cell table holding the shape constants of general meshes.
+template <int MASK, int MESH, bool HEXA, bool MASKED, int NB>
+__device__ __forceinline__ void far_tile(const FarGeom& g, const float* __restrict__ shape,
+                                         const float* __restrict__ wt, const MpConst& uc,
+                                         int i0, int j0, int k0, int ni, int nj, int nk,
+                                         float Xt, float Yt, float Zt, const int (&bx)[6],
+                                         double (&acc64)[NB][Ncomp<MASK>::value]) {
   constexpr int NC = Ncomp<MASK>::value;
-  constexpr int TW = TableWidth<MESH>::value;
   constexpr bool QUAD = MESH != kMeshCubes;
   for (int ii = 0; ii < ni; ++ii) {
     const float X = Xt - g.xoff[i0 + ii];
@@ -254,21 +274,28 @@ __device__ __forceinline__ void far_tile(const FarGeom& g, const float* __restri
       const float Y = Yt - g.yoff[j0 + jj];
       const bool inxy = inx && (j0 + jj >= bx[2]) && (j0 + jj <= bx[3]);
       const long cb = ((long)(i0 + ii) * g.ny + (j0 + jj)) * g.nz + k0;
-      float acc[NC];
+      float acc[NB][NC];
 #pragma unroll
-      for (int m = 0; m < NC; ++m) acc[m] = 0.0f;
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int m = 0; m < NC; ++m) acc[b][m] = 0.0f;
       for (int kk = 0; kk < nk; ++kk) {
         const bool near = inxy && (k0 + kk >= bx[4]) && (k0 + kk <= bx[5]);
         const float Z = near ? 1.0e6f : Zt - g.zoff[k0 + kk];
-        const MpConst cc = cell_const<MESH>(table, cb + kk, uc);
+        const MpConst cc = cell_const<MESH>(shape, cb + kk, uc);
         float T[NC];
         multipole_f32<MASK, QUAD, HEXA>(X, Y, Z, cc, T);
-        const float w = near ? 0.0f : table[(cb + kk) * TW];
 #pragma unroll
-        for (int m = 0; m < NC; ++m) acc[m] = fmaf(w, T[m], acc[m]);
+        for (int b = 0; b < NB; ++b) {
+          const float w = near ? 0.0f : wt[(cb + kk) * NB + b];
+#pragma unroll
+          for (int m = 0; m < NC; ++m) acc[b][m] = fmaf(w, T[m], acc[b][m]);
+        }
       }
 #pragma unroll
-      for (int m = 0; m < NC; ++m) acc64[m] += (double)acc[m];
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int m = 0; m < NC; ++m) acc64[b][m] += (double)acc[b][m];
     }
   }
 }
@@ -279,11 +306,12 @@ __device__ __forceinline__ void far_tile(const FarGeom& g, const float* __restri
 //   SZZ5 = sum w Z^2 s^5            (s = 1/r),
 // and gz = -SZ3, Txx = 3X^2 S5 - S3, Tyy = 3Y^2 S5 - S3, Tzz = 3 SZZ5 - S3,
 // Txy = 3XY S5, Txz = 3X SZ5, Tyz = 3Y SZ5: the monopole terms of multipole_f32
-// regrouped, about half the operations per cell.
-template <int MASK>
-__device__ __forceinline__ void far_tile_mono(const FarGeom& g, const float* __restrict__ table,
+// regrouped, about half the operations per cell.  The NB models share s.
+template <int MASK, int NB>
+__device__ __forceinline__ void far_tile_mono(const FarGeom& g, const float* __restrict__ wt,
                                               int i0, int j0, int k0, int ni, int nj, int nk,
-                                              float Xt, float Yt, float Zt, double* acc64) {
+                                              float Xt, float Yt, float Zt,
+                                              double (&acc64)[NB][Ncomp<MASK>::value]) {
   constexpr int NC = Ncomp<MASK>::value;
   constexpr bool GZ = (MASK & kGz) != 0, DIAG = (MASK & kDiag) != 0, OFF = (MASK & kOff) != 0;
   // z offsets of the tile's cells; a cell past the mesh edge sits far away with weight 0
@@ -297,14 +325,18 @@ __device__ __forceinline__ void far_tile_mono(const FarGeom& g, const float* __r
   for (int ii = 0; ii < ni; ++ii) {
     const float X = Xt - g.xoff[i0 + ii];
     const float X3 = 3.0f * X;
-    float row[NC];  // FP32 sums of this ii's columns, flushed to FP64 per ii
+    float row[NB][NC];  // FP32 sums of this ii's columns, flushed to FP64 per ii
 #pragma unroll
-    for (int m = 0; m < NC; ++m) row[m] = 0.0f;
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int m = 0; m < NC; ++m) row[b][m] = 0.0f;
     for (int jj = 0; jj < nj; ++jj) {
       const float Y = Yt - g.yoff[j0 + jj];
       const float XXYY = fmaf(X, X, Y * Y);
-      const float* wc = table + ((long)(i0 + ii) * g.ny + (j0 + jj)) * g.nz + k0;
-      float S3 = 0.0f, S5 = 0.0f, SZ3 = 0.0f, SZ5 = 0.0f, SZZ5 = 0.0f;
+      const float* wc = wt + (((long)(i0 + ii) * g.ny + (j0 + jj)) * g.nz + k0) * NB;
+      float S3[NB], S5[NB], SZ3[NB], SZ5[NB], SZZ5[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) S3[b] = S5[b] = SZ3[b] = SZ5[b] = SZZ5[b] = 0.0f;
 #pragma unroll
       for (int kk = 0; kk < kTile; ++kk) {
         const float Z = Zc[kk];
@@ -314,35 +346,45 @@ __device__ __forceinline__ void far_tile_mono(const FarGeom& g, const float* __r
         const float e = fmaf(-(r2 * sv), sv, 1.0f);  // one Newton step
         sv = fmaf(0.5f * sv, e, sv);
         const float s2 = sv * sv;
-        const float w = kv[kk] ? wc[kk] : 0.0f;
-        const float q3 = w * (s2 * sv);
-        if constexpr (GZ) SZ3 = fmaf(Z, q3, SZ3);
-        if constexpr (DIAG || OFF) {
-          const float q5 = q3 * s2;
-          S5 += q5;
-          if constexpr (DIAG) {
-            S3 += q3;
-            SZZ5 = fmaf(ZZ, q5, SZZ5);
+        const float s3 = s2 * sv;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const float w = kv[kk] ? wc[kk * NB + b] : 0.0f;
+          const float q3 = w * s3;
+          if constexpr (GZ) SZ3[b] = fmaf(Z, q3, SZ3[b]);
+          if constexpr (DIAG || OFF) {
+            const float q5 = q3 * s2;
+            S5[b] += q5;
+            if constexpr (DIAG) {
+              S3[b] += q3;
+              SZZ5[b] = fmaf(ZZ, q5, SZZ5[b]);
+            }
+            if constexpr (OFF) SZ5[b] = fmaf(Z, q5, SZ5[b]);
           }
-          if constexpr (OFF) SZ5 = fmaf(Z, q5, SZ5);
         }
       }
-      int k = 0;
       const float Y3 = 3.0f * Y;
-      if constexpr (GZ) row[k++] -= SZ3;
-      if constexpr (DIAG) {
-        row[k] = fmaf(X3 * X, S5, row[k] - S3), ++k;
-        row[k] = fmaf(Y3 * Y, S5, row[k] - S3), ++k;
-        row[k] = fmaf(3.0f, SZZ5, row[k] - S3), ++k;
-      }
-      if constexpr (OFF) {
-        row[k] = fmaf(X3 * Y, S5, row[k]), ++k;
-        row[k] = fmaf(X3, SZ5, row[k]), ++k;
-        row[k] = fmaf(Y3, SZ5, row[k]), ++k;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        float* rw = row[b];
+        int k = 0;
+        if constexpr (GZ) rw[k++] -= SZ3[b];
+        if constexpr (DIAG) {
+          rw[k] = fmaf(X3 * X, S5[b], rw[k] - S3[b]), ++k;
+          rw[k] = fmaf(Y3 * Y, S5[b], rw[k] - S3[b]), ++k;
+          rw[k] = fmaf(3.0f, SZZ5[b], rw[k] - S3[b]), ++k;
+        }
+        if constexpr (OFF) {
+          rw[k] = fmaf(X3 * Y, S5[b], rw[k]), ++k;
+          rw[k] = fmaf(X3, SZ5[b], rw[k]), ++k;
+          rw[k] = fmaf(Y3, SZ5[b], rw[k]), ++k;
+        }
       }
     }
 #pragma unroll
-    for (int m = 0; m < NC; ++m) acc64[m] += (double)row[m];
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int m = 0; m < NC; ++m) acc64[b][m] += (double)row[b][m];
   }
 }
 
@@ -350,15 +392,16 @@ __device__ __forceinline__ void far_tile_mono(const FarGeom& g, const float* __r
 // Lanes take sites in `order` (spatially compact waves).  Cells inside the lane's
 // near box contribute nothing here (FP64 near pass).  A tile whose cells are all beyond
 // the second radius of every site of the wave keeps only the monopole (and quadrupole)
-// terms.  Site offsets are formed in FP64 against
-// the tile's first cell and rounded once per tile; cell offsets within a tile are exact
-// for uniform meshes.
-template <int MASK, int MESH>
+// terms.  Site offsets are formed in FP64 against the tile's first cell and rounded once
+// per tile; cell offsets within a tile are exact for uniform meshes.  NB models share
+// every pair's multipole terms; model b's sums go to
+// farpart[((b * nchunks + chunk) * NC + m) * S + s].
+template <int MASK, int MESH, int NB>
 __global__ __launch_bounds__(kBlock) void k_far_forward(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     int S, const int* __restrict__ order, const int* __restrict__ box, float r1sq, FarGeom g,
-    const float* __restrict__ table, MpConst uc, long ntiles, long tiles_per_chunk,
-    double* __restrict__ farpart) {
+    const float* __restrict__ shape, const float* __restrict__ wt, MpConst uc, long ntiles,
+    long tiles_per_chunk, double* __restrict__ farpart) {
   constexpr int NC = Ncomp<MASK>::value;
   const int li = blockIdx.x * blockDim.x + threadIdx.x;
   const int s = order[li < S ? li : S - 1];
@@ -368,9 +411,11 @@ __global__ __launch_bounds__(kBlock) void k_far_forward(
   for (int q = 0; q < 6; ++q) bx[q] = box[s * 6 + q];
   const long t0 = (long)blockIdx.y * tiles_per_chunk;
   const long t1 = t0 + tiles_per_chunk < ntiles ? t0 + tiles_per_chunk : ntiles;
-  double acc64[NC];
+  double acc64[NB][NC];
 #pragma unroll
-  for (int m = 0; m < NC; ++m) acc64[m] = 0.0;
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int m = 0; m < NC; ++m) acc64[b][m] = 0.0;
   for (long t = t0; t < t1; ++t) {
     const int tk = (int)(t % g.ntz);
     const long tt = t / g.ntz;
@@ -389,20 +434,23 @@ __global__ __launch_bounds__(kBlock) void k_far_forward(
     const float dz = fmaxf(0.0f, fmaxf(-Zt, Zt - g.zoff[k0 + nk - 1]));
     const bool need = fmaf(dx, dx, fmaf(dy, dy, dz * dz)) < r1sq;
     if (__any(hit))
-      far_tile<MASK, MESH, true, true>(g, table, uc, i0, j0, k0, ni, nj, nk, Xt, Yt, Zt, bx,
-                                       acc64);
+      far_tile<MASK, MESH, true, true, NB>(g, shape, wt, uc, i0, j0, k0, ni, nj, nk, Xt, Yt, Zt,
+                                           bx, acc64);
     else if (__any(need))
-      far_tile<MASK, MESH, true, false>(g, table, uc, i0, j0, k0, ni, nj, nk, Xt, Yt, Zt, bx,
-                                        acc64);
+      far_tile<MASK, MESH, true, false, NB>(g, shape, wt, uc, i0, j0, k0, ni, nj, nk, Xt, Yt,
+                                            Zt, bx, acc64);
     else if constexpr (MESH == kMeshCubes)
-      far_tile_mono<MASK>(g, table, i0, j0, k0, ni, nj, nk, Xt, Yt, Zt, acc64);
+      far_tile_mono<MASK, NB>(g, wt, i0, j0, k0, ni, nj, nk, Xt, Yt, Zt, acc64);
     else
-      far_tile<MASK, MESH, false, false>(g, table, uc, i0, j0, k0, ni, nj, nk, Xt, Yt, Zt, bx,
-                                         acc64);
+      far_tile<MASK, MESH, false, false, NB>(g, shape, wt, uc, i0, j0, k0, ni, nj, nk, Xt, Yt,
+                                             Zt, bx, acc64);
   }
   if (li < S) {
 #pragma unroll
-    for (int m = 0; m < NC; ++m) farpart[((long)blockIdx.y * NC + m) * S + s] = acc64[m];
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int m = 0; m < NC; ++m)
+        farpart[(((long)b * gridDim.y + blockIdx.y) * NC + m) * S + s] = acc64[b][m];
   }
 }
 
@@ -516,9 +564,11 @@ __global__ __launch_bounds__(kBlock) void k_mixed_coef(const double* __restrict_
 // near box are skipped (FP64 near pass).  Site offsets are formed in FP64 against the
 // tile's first cell once per site and lane and shared by the lane's cells; a site whose
 // second radius misses every cell of the tile keeps only the monopole (and quadrupole)
-// terms, a wave-uniform branch.
+// terms, a wave-uniform branch.  NB misfit vectors share the kernel terms T_m; model b's
+// coefficients sit at coef + b S NC and mono + b S 8, its partial sums at
+// gpart[(b nchunks + chunk) ncell + c].
 constexpr int kQ = 4;  // cells per lane of the far adjoint
-template <int MASK, int MESH>
+template <int MASK, int MESH, int NB>
 __global__ __launch_bounds__(kBlock) void k_far_adjoint(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     const float* __restrict__ coef, const float* __restrict__ mono, const int* __restrict__ box,
@@ -561,13 +611,15 @@ __global__ __launch_bounds__(kBlock) void k_far_adjoint(
   }
   const int s0 = blockIdx.y * sites_per_chunk;
   const int s1 = s0 + sites_per_chunk < S ? s0 + sites_per_chunk : S;
-  double a64[kQ];
-  float acc[kQ];
+  double a64[NB][kQ];
+  float acc[NB][kQ];
 #pragma unroll
-  for (int q = 0; q < kQ; ++q) {
-    a64[q] = 0.0;
-    acc[q] = 0.0f;
-  }
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+      a64[b][q] = 0.0;
+      acc[b][q] = 0.0f;
+    }
   int nacc = 0;
   for (int s = s0; s < s1; ++s) {
     const int* bx = box + s * 6;
@@ -586,16 +638,20 @@ __global__ __launch_bounds__(kBlock) void k_far_adjoint(
     const float dx = fmaxf(0.0f, fmaxf(-Xt, Xt - ex)), dy = fmaxf(0.0f, fmaxf(-Yt, Yt - ey)),
                 dz = fmaxf(0.0f, fmaxf(-Zt, Zt - ez));
     const bool need = fmaf(dx, dx, fmaf(dy, dy, dz * dz)) < r1sq;
-    float Zq[kQ], pq[kQ];
+    float Zq[kQ], pq[NB][kQ];
 #pragma unroll
     for (int q = 0; q < kQ; ++q) Zq[q] = nq[q] ? 1.0e6f : Zt - ozq[q];
     if (MESH == kMeshCubes && !need) {
       // monopole: s^5 Q - s^3 (c_gz Z + C) with the site's regrouped coefficients
-      const float* mc = mono + (long)s * 8;
-      const float XX = X * X, YY = Y * Y;
+      const float XX = X * X, YY = Y * Y, XY = X * Y;
       const float XXYY = XX + YY;
-      const float Qxy = fmaf(mc[1], XX, fmaf(mc[2], YY, mc[4] * (X * Y)));
-      const float Lz = fmaf(mc[5], X, mc[6] * Y);
+      float Qxy[NB], Lz[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const float* mc = mono + ((long)b * S + s) * 8;
+        Qxy[b] = fmaf(mc[1], XX, fmaf(mc[2], YY, mc[4] * XY));
+        Lz[b] = fmaf(mc[5], X, mc[6] * Y);
+      }
 #pragma unroll
       for (int q = 0; q < kQ; ++q) {
         const float Z = Zq[q];
@@ -604,11 +660,14 @@ __global__ __launch_bounds__(kBlock) void k_far_adjoint(
         const float e = fmaf(-(r2 * sv), sv, 1.0f);  // one Newton step
         sv = fmaf(0.5f * sv, e, sv);
         const float s2 = sv * sv, s3 = s2 * sv;
-        const float Q = fmaf(Z, fmaf(mc[3], Z, Lz), Qxy);
-        pq[q] = fmaf(Q, s3 * s2, -s3 * fmaf(mc[0], Z, mc[7]));
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const float* mc = mono + ((long)b * S + s) * 8;
+          const float Q = fmaf(Z, fmaf(mc[3], Z, Lz[b]), Qxy[b]);
+          pq[b][q] = fmaf(Q, s3 * s2, -s3 * fmaf(mc[0], Z, mc[7]));
+        }
       }
     } else {
-      const float* cs = coef + (long)s * NC;
 #pragma unroll
       for (int q = 0; q < kQ; ++q) {
         float T[NC];
@@ -616,29 +675,40 @@ __global__ __launch_bounds__(kBlock) void k_far_adjoint(
           multipole_f32<MASK, QUAD, true>(X, Y, Zq[q], ccq[q], T);
         else
           multipole_f32<MASK, QUAD, false>(X, Y, Zq[q], ccq[q], T);
-        float p = 0.0f;
 #pragma unroll
-        for (int m = 0; m < NC; ++m) p = fmaf(cs[m], T[m], p);
-        pq[q] = p;
+        for (int b = 0; b < NB; ++b) {
+          const float* cs = coef + ((long)b * S + s) * NC;
+          float p = 0.0f;
+#pragma unroll
+          for (int m = 0; m < NC; ++m) p = fmaf(cs[m], T[m], p);
+          pq[b][q] = p;
+        }
       }
     }
 #pragma unroll
-    for (int q = 0; q < kQ; ++q) acc[q] += nq[q] ? 0.0f : pq[q];
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int q = 0; q < kQ; ++q) acc[b][q] += nq[q] ? 0.0f : pq[b][q];
     if (++nacc == 8) {
 #pragma unroll
-      for (int q = 0; q < kQ; ++q) {
-        a64[q] += (double)acc[q];
-        acc[q] = 0.0f;
-      }
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {
+          a64[b][q] += (double)acc[b][q];
+          acc[b][q] = 0.0f;
+        }
       nacc = 0;
     }
   }
   const long ncell = (long)g.nx * g.ny * g.nz;
 #pragma unroll
   for (int q = 0; q < kQ; ++q) {
-    a64[q] += (double)acc[q];
-    if (vq[q])  // G V
-      gpart[(long)blockIdx.y * ncell + cq[q]] = 2.0 * (double)table[cq[q] * TW] * a64[q];
+    const double gv = 2.0 * (double)table[cq[q] * TW];  // G V
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      a64[b][q] += (double)acc[b][q];
+      if (vq[q]) gpart[((long)b * gridDim.y + blockIdx.y) * ncell + cq[q]] = gv * a64[b][q];
+    }
   }
 }
 
@@ -828,9 +898,40 @@ const char* far_name(int mask, bool adjoint) {
                              : "far_forward_fp32_ftg";
 }
 
+// Unit-density cell table of the mesh (G V and, for general meshes, the shape
+// constants), built once per geometry; the adjoint's weights and the forward's shapes.
+template <int MESH>
+hipError_t ensure_shape_table(jgm_context* ctx) {
+  NearBoxes& nb = ctx->near;
+  if (nb.adj_table_valid) return hipSuccess;
+  const FarGeom g = far_geom(ctx);
+  const long N = (long)ncells(ctx);
+  hipError_t e = nb.adj_table.ensure(sizeof(float) * (size_t)N * TableWidth<MESH>::value);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_cell_table<MESH>), dim3(blocks_for(N)), dim3(kBlock), 0, ctx->stream,
+                     (const double*)nullptr, ctx->d_xsz.as<const double>(),
+                     ctx->d_ysz.as<const double>(), ctx->d_zsz.as<const double>(), g.nx, g.ny,
+                     g.nz, nb.adj_table.as<float>());
+  nb.adj_table_valid = true;
+  return hipGetLastError();
+}
+
+template <int MASK, int MESH, class... Args>
+void launch_far_forward(int nb, dim3 grid, hipStream_t st, Args... args) {
+  switch (nb) {
+    case 4: hipLaunchKernelGGL((k_far_forward<MASK, MESH, 4>), grid, dim3(kBlock), 0, st, args...); break;
+    case 2: hipLaunchKernelGGL((k_far_forward<MASK, MESH, 2>), grid, dim3(kBlock), 0, st, args...); break;
+    default: hipLaunchKernelGGL((k_far_forward<MASK, MESH, 1>), grid, dim3(kBlock), 0, st, args...); break;
+  }
+}
+
+// Forward of nb in {1, 2, 4} models (model b at models + b * model_ld; data at
+// out + b * out_ld, raw data at raw_out + b * raw_ld when given): one far-field pass for
+// all of them, the FP64 near field and the epilogue per model.
 template <int MASK, int MESH>
-hipError_t forward_mixed_impl(jgm_context* ctx, int kind, const double* model, double* out,
-                              double* raw_out) {
+hipError_t forward_mixed_impl(jgm_context* ctx, int kind, int nbm, const double* models,
+                              size_t model_ld, double* out, size_t out_ld, double* raw_out,
+                              size_t raw_ld) {
   constexpr int NC = Ncomp<MASK>::value;
   hipStream_t st = ctx->stream;
   const int S = (int)ctx->S;
@@ -840,7 +941,7 @@ hipError_t forward_mixed_impl(jgm_context* ctx, int kind, const double* model, d
   const long ntiles = (long)g.ntx * g.nty * g.ntz;
   hipError_t e;
   int nchunks = 0;
-  if ((e = ctx->nearpart.ensure(sizeof(double) * (size_t)S * NC)) != hipSuccess) return e;
+  if ((e = ctx->nearpart.ensure(sizeof(double) * (size_t)nbm * S * NC)) != hipSuccess) return e;
   if (N > 0) {
     const int site_blocks = (S + kBlock - 1) / kBlock;
     const long waves_per_chunk = (long)site_blocks * (kBlock / 64);
@@ -848,99 +949,115 @@ hipError_t forward_mixed_impl(jgm_context* ctx, int kind, const double* model, d
     want = std::max<long>(1, std::min<long>(want, ntiles));
     const long per = (ntiles + want - 1) / want;
     nchunks = (int)((ntiles + per - 1) / per);
-    if ((e = ctx->weights.ensure(sizeof(float) * (size_t)N * TableWidth<MESH>::value)) !=
-        hipSuccess)
+    if ((e = ctx->weights.ensure(sizeof(float) * (size_t)N * nbm)) != hipSuccess) return e;
+    if ((e = ctx->farpart.ensure(sizeof(double) * (size_t)nbm * nchunks * NC * S)) != hipSuccess)
       return e;
-    if ((e = ctx->farpart.ensure(sizeof(double) * (size_t)nchunks * NC * S)) != hipSuccess)
-      return e;
-    hipLaunchKernelGGL((k_cell_table<MESH>), dim3(blocks_for(N)), dim3(kBlock), 0, st, model,
-                       ctx->d_xsz.as<const double>(), ctx->d_ysz.as<const double>(),
-                       ctx->d_zsz.as<const double>(), g.nx, g.ny, g.nz, ctx->weights.as<float>());
+    if constexpr (MESH == kMeshGeneral) {
+      if ((e = ensure_shape_table<MESH>(ctx)) != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_cell_weights, dim3(blocks_for(N)), dim3(kBlock), 0, st, models,
+                       (long)model_ld, nbm, ctx->d_xsz.as<const double>(),
+                       ctx->d_ysz.as<const double>(), ctx->d_zsz.as<const double>(), g.nx, g.ny,
+                       g.nz, ctx->weights.as<float>());
     {
       ProfScope ps(ctx, far_name(MASK, false));
-      hipLaunchKernelGGL((k_far_forward<MASK, MESH>), dim3(site_blocks, nchunks), dim3(kBlock), 0,
-                         st, ctx->d_sx.as<double>(), ctx->d_sy.as<double>(),
-                         ctx->d_sz.as<double>(), S, ctx->near.site_order.as<const int>(),
-                         ctx->near.box.as<const int>(), second_radius_sq(ctx), g,
-                         ctx->weights.as<const float>(), uniform_const(ctx), ntiles, per,
-                         ctx->farpart.as<double>());
+      launch_far_forward<MASK, MESH>(nbm, dim3(site_blocks, nchunks), st,
+                                     ctx->d_sx.as<const double>(), ctx->d_sy.as<const double>(),
+                                     ctx->d_sz.as<const double>(), S,
+                                     ctx->near.site_order.as<const int>(),
+                                     ctx->near.box.as<const int>(), second_radius_sq(ctx), g,
+                                     ctx->near.adj_table.as<const float>(),
+                                     ctx->weights.as<const float>(), uniform_const(ctx), ntiles,
+                                     per, ctx->farpart.as<double>());
     }
     {
       ProfScope ps(ctx, "near_forward_fp64");
-      hipLaunchKernelGGL((k_near_forward<MASK>), dim3(S), dim3(kBlock), 0, st,
-                         ctx->d_sx.as<double>(), ctx->d_sy.as<double>(), ctx->d_sz.as<double>(),
-                         ctx->near.box.as<const int>(), ctx->d_Xc.as<double>(),
-                         ctx->d_Yc.as<double>(), ctx->d_Zc.as<double>(), model, g.nx, g.ny, g.nz,
-                         ctx->nearpart.as<double>());
+      for (int b = 0; b < nbm; ++b)
+        hipLaunchKernelGGL((k_near_forward<MASK>), dim3(S), dim3(kBlock), 0, st,
+                           ctx->d_sx.as<double>(), ctx->d_sy.as<double>(),
+                           ctx->d_sz.as<double>(), ctx->near.box.as<const int>(),
+                           ctx->d_Xc.as<double>(), ctx->d_Yc.as<double>(),
+                           ctx->d_Zc.as<double>(), models + b * model_ld, g.nx, g.ny, g.nz,
+                           ctx->nearpart.as<double>() + (size_t)b * S * NC);
     }
   } else {
-    hipLaunchKernelGGL(k_fill_zero, dim3(blocks_for((long)S * NC)), dim3(kBlock), 0, st,
-                       ctx->nearpart.as<double>(), (size_t)S * NC);
+    hipLaunchKernelGGL(k_fill_zero, dim3(blocks_for((long)nbm * S * NC)), dim3(kBlock), 0, st,
+                       ctx->nearpart.as<double>(), (size_t)nbm * S * NC);
   }
   const double tf_bx = std::cos(ctx->tf_inc) * std::cos(ctx->tf_dec);
   const double tf_by = std::cos(ctx->tf_inc) * std::sin(ctx->tf_dec);
   const double tf_bz = std::sin(ctx->tf_inc);
-  if (nchunks > 0) {
-    if ((e = ctx->partsum.ensure(sizeof(double) * (size_t)NC * S)) != hipSuccess) return e;
-    if ((e = launch_reduce_chunks(ctx, ctx->farpart.as<const double>(), nchunks, (long)NC * S,
-                                  ctx->partsum.as<double>())) != hipSuccess)
+  if (nchunks > 0 && (e = ctx->partsum.ensure(sizeof(double) * (size_t)NC * S)) != hipSuccess)
+    return e;
+  for (int b = 0; b < nbm; ++b) {
+    if (nchunks > 0 &&
+        (e = launch_reduce_chunks(ctx, ctx->farpart.as<const double>() + (size_t)b * nchunks * NC * S,
+                                  nchunks, (long)NC * S, ctx->partsum.as<double>())) != hipSuccess)
       return e;
+    hipLaunchKernelGGL((k_mixed_forward_finish<MASK>), dim3(blocks_for(S)), dim3(kBlock), 0, st,
+                       ctx->partsum.as<const double>(), nchunks > 0 ? 1 : 0,
+                       ctx->nearpart.as<const double>() + (size_t)b * S * NC, S,
+                       ctx->bgT.as<const double>(), models + b * model_ld + N, L, kind,
+                       field_dir(ctx), ctx->transform, tf_bx, tf_by, tf_bz, out + b * out_ld,
+                       raw_out ? raw_out + b * raw_ld : nullptr);
   }
-  hipLaunchKernelGGL((k_mixed_forward_finish<MASK>), dim3(blocks_for(S)), dim3(kBlock), 0, st,
-                     ctx->partsum.as<const double>(), nchunks > 0 ? 1 : 0,
-                     ctx->nearpart.as<const double>(), S,
-                     ctx->bgT.as<const double>(), model + N, L, kind, field_dir(ctx),
-                     ctx->transform, tf_bx, tf_by, tf_bz, out, raw_out);
   return hipGetLastError();
 }
 
+template <int MASK, int MESH, class... Args>
+void launch_far_adjoint(int nbm, dim3 grid, hipStream_t st, Args... args) {
+  switch (nbm) {
+    case 4: hipLaunchKernelGGL((k_far_adjoint<MASK, MESH, 4>), grid, dim3(kBlock), 0, st, args...);
+      break;
+    case 2: hipLaunchKernelGGL((k_far_adjoint<MASK, MESH, 2>), grid, dim3(kBlock), 0, st, args...);
+      break;
+    default:
+      hipLaunchKernelGGL((k_far_adjoint<MASK, MESH, 1>), grid, dim3(kBlock), 0, st, args...);
+  }
+}
+
+// Gradients of nbm (1, 2 or 4) misfit vectors rm + b rm_ld into grad + b grad_ld.
 template <int MASK, int MESH>
-hipError_t adjoint_mixed_impl(jgm_context* ctx, int kind, const double* rm, double* grad) {
+hipError_t adjoint_mixed_impl(jgm_context* ctx, int kind, int nbm, const double* rm,
+                              size_t rm_ld, double* grad, size_t grad_ld) {
   constexpr int NC = Ncomp<MASK>::value;
   hipStream_t st = ctx->stream;
   const int S = (int)ctx->S;
   const long N = (long)ncells(ctx);
   const FarGeom g = far_geom(ctx);
-  const long ntiles = (long)g.ntx * g.nty * g.ntz;
   NearBoxes& nb = ctx->near;
   hipError_t e;
   if (N > 0) {
-    if ((e = ctx->coef.ensure(sizeof(double) * (size_t)S * NC)) != hipSuccess) return e;
-    if ((e = ctx->coef32.ensure(sizeof(float) * (size_t)S * NC)) != hipSuccess) return e;
-    if ((e = ctx->coef_mono.ensure(sizeof(float) * (size_t)S * 8)) != hipSuccess) return e;
-    hipLaunchKernelGGL((k_mixed_coef<MASK>), dim3(blocks_for(S)), dim3(kBlock), 0, st, rm, S,
-                       kind, field_dir(ctx), ctx->coef.as<double>(), ctx->coef32.as<float>(),
-                       ctx->coef_mono.as<float>());
-    if (!nb.adj_table_valid) {
-      if ((e = nb.adj_table.ensure(sizeof(float) * (size_t)N * TableWidth<MESH>::value)) !=
-          hipSuccess)
-        return e;
-      hipLaunchKernelGGL((k_cell_table<MESH>), dim3(blocks_for(N)), dim3(kBlock), 0, st,
-                         (const double*)nullptr, ctx->d_xsz.as<const double>(),
-                         ctx->d_ysz.as<const double>(), ctx->d_zsz.as<const double>(), g.nx, g.ny,
-                         g.nz, nb.adj_table.as<float>());
-      nb.adj_table_valid = true;
-    }
+    const size_t SB = (size_t)S * nbm;
+    if ((e = ctx->coef.ensure(sizeof(double) * SB * NC)) != hipSuccess) return e;
+    if ((e = ctx->coef32.ensure(sizeof(float) * SB * NC)) != hipSuccess) return e;
+    if ((e = ctx->coef_mono.ensure(sizeof(float) * SB * 8)) != hipSuccess) return e;
+    for (int b = 0; b < nbm; ++b)
+      hipLaunchKernelGGL((k_mixed_coef<MASK>), dim3(blocks_for(S)), dim3(kBlock), 0, st,
+                         rm + b * rm_ld, S, kind, field_dir(ctx),
+                         ctx->coef.as<double>() + (size_t)b * S * NC,
+                         ctx->coef32.as<float>() + (size_t)b * S * NC,
+                         ctx->coef_mono.as<float>() + (size_t)b * S * 8);
+    if ((e = ensure_shape_table<MESH>(ctx)) != hipSuccess) return e;
     // the adjoint walks 4x4x(4 kQ) tiles, kQ cells per lane
     const long ntiles8 = (long)g.ntx * g.nty * ((g.nz + kQ * kTile - 1) / (kQ * kTile));
     int nchunks = (int)std::min<long>(
         S, std::max<long>(1, (kTargetWaves + ntiles8 - 1) / ntiles8));
     const int per = (S + nchunks - 1) / nchunks;
     nchunks = (S + per - 1) / per;
-    if ((e = ctx->hpart.ensure(sizeof(double) * (size_t)nchunks * N)) != hipSuccess) return e;
+    if ((e = ctx->hpart.ensure(sizeof(double) * (size_t)nbm * nchunks * N)) != hipSuccess)
+      return e;
     if ((e = ctx->hsum.ensure(sizeof(double) * (size_t)N)) != hipSuccess) return e;
     {
       ProfScope ps(ctx, far_name(MASK, true));
-      hipLaunchKernelGGL((k_far_adjoint<MASK, MESH>), dim3(blocks_for(ntiles8 * 64), nchunks),
-                         dim3(kBlock), 0, st, ctx->d_sx.as<double>(), ctx->d_sy.as<double>(),
-                         ctx->d_sz.as<double>(), ctx->coef32.as<const float>(),
-                         ctx->coef_mono.as<const float>(), nb.box.as<const int>(),
-                         second_radius_sq(ctx), S, per, g,
-                         nb.adj_table.as<const float>(), uniform_const(ctx), ntiles8,
-                         ctx->hpart.as<double>());
+      launch_far_adjoint<MASK, MESH>(nbm, dim3(blocks_for(ntiles8 * 64), nchunks), st,
+                                     ctx->d_sx.as<const double>(), ctx->d_sy.as<const double>(),
+                                     ctx->d_sz.as<const double>(), ctx->coef32.as<const float>(),
+                                     ctx->coef_mono.as<const float>(), nb.box.as<const int>(),
+                                     second_radius_sq(ctx), S, per, g,
+                                     nb.adj_table.as<const float>(), uniform_const(ctx), ntiles8,
+                                     ctx->hpart.as<double>());
     }
-    hipLaunchKernelGGL(k_fill_zero, dim3(blocks_for(N)), dim3(kBlock), 0, st,
-                       ctx->hsum.as<double>(), (size_t)N);
     if (nb.ncols > 0) {
       if ((e = ctx->near_h.ensure(sizeof(double) * std::max<size_t>(1, nb.total_near_corners))) !=
           hipSuccess)
@@ -948,47 +1065,67 @@ hipError_t adjoint_mixed_impl(jgm_context* ctx, int kind, const double* rm, doub
       if ((e = ctx->near_v.ensure(sizeof(double) * std::max<size_t>(1, nb.total_near_cells))) !=
           hipSuccess)
         return e;
-      ProfScope ps(ctx, "near_adjoint_fp64");
-      hipLaunchKernelGGL((k_near_adjoint_site<MASK>), dim3(S), dim3(kBlock), 0, st,
-                         ctx->d_sx.as<double>(), ctx->d_sy.as<double>(), ctx->d_sz.as<double>(),
-                         nb.box.as<const int>(), ctx->coef.as<const double>(),
-                         ctx->d_Xc.as<double>(), ctx->d_Yc.as<double>(), ctx->d_Zc.as<double>(),
-                         nb.corner_off.as<const long>(), nb.cell_off.as<const long>(),
-                         ctx->near_h.as<double>(), ctx->near_v.as<double>());
-      hipLaunchKernelGGL(k_near_adjoint_gather, dim3((unsigned)nb.ncols), dim3(64), 0, st,
-                         nb.col_ptr.as<const int>(), nb.col_sites.as<const int>(),
-                         nb.col_ids.as<const int>(), nb.col_kmax.as<const int>(),
-                         nb.box.as<const int>(), nb.cell_off.as<const long>(),
-                         ctx->near_v.as<const double>(), g.ny, g.nz, ctx->hsum.as<double>());
     }
-    hipLaunchKernelGGL(k_adjoint_mixed_finish, dim3(blocks_for(N)), dim3(kBlock), 0, st,
-                       ctx->hpart.as<const double>(), nchunks, ctx->hsum.as<const double>(), N,
-                       grad);
+    for (int b = 0; b < nbm; ++b) {
+      hipLaunchKernelGGL(k_fill_zero, dim3(blocks_for(N)), dim3(kBlock), 0, st,
+                         ctx->hsum.as<double>(), (size_t)N);
+      if (nb.ncols > 0) {
+        ProfScope ps(ctx, "near_adjoint_fp64");
+        hipLaunchKernelGGL((k_near_adjoint_site<MASK>), dim3(S), dim3(kBlock), 0, st,
+                           ctx->d_sx.as<double>(), ctx->d_sy.as<double>(),
+                           ctx->d_sz.as<double>(), nb.box.as<const int>(),
+                           ctx->coef.as<const double>() + (size_t)b * S * NC,
+                           ctx->d_Xc.as<double>(), ctx->d_Yc.as<double>(),
+                           ctx->d_Zc.as<double>(), nb.corner_off.as<const long>(),
+                           nb.cell_off.as<const long>(), ctx->near_h.as<double>(),
+                           ctx->near_v.as<double>());
+        hipLaunchKernelGGL(k_near_adjoint_gather, dim3((unsigned)nb.ncols), dim3(64), 0, st,
+                           nb.col_ptr.as<const int>(), nb.col_sites.as<const int>(),
+                           nb.col_ids.as<const int>(), nb.col_kmax.as<const int>(),
+                           nb.box.as<const int>(), nb.cell_off.as<const long>(),
+                           ctx->near_v.as<const double>(), g.ny, g.nz, ctx->hsum.as<double>());
+      }
+      hipLaunchKernelGGL(k_adjoint_mixed_finish, dim3(blocks_for(N)), dim3(kBlock), 0, st,
+                         ctx->hpart.as<const double>() + (size_t)b * nchunks * N, nchunks,
+                         ctx->hsum.as<const double>(), N, grad + b * grad_ld);
+    }
   }
   if (!ctx->bgthick.empty()) {
-    e = launch_background_gradient(ctx, kind, rm, grad + N);
-    if (e != hipSuccess) return e;
+    for (int b = 0; b < nbm; ++b) {
+      e = launch_background_gradient(ctx, kind, rm + b * rm_ld, grad + b * grad_ld + N);
+      if (e != hipSuccess) return e;
+    }
   }
   return hipGetLastError();
 }
 
 template <int MASK>
-hipError_t forward_dispatch(jgm_context* ctx, int kind, const double* model, double* out,
-                            double* raw_out) {
+hipError_t forward_dispatch(jgm_context* ctx, int kind, int nbm, const double* models,
+                            size_t model_ld, double* out, size_t out_ld, double* raw_out,
+                            size_t raw_ld) {
   switch (ctx->near.mesh_class) {
-    case kMeshCubes: return forward_mixed_impl<MASK, kMeshCubes>(ctx, kind, model, out, raw_out);
+    case kMeshCubes:
+      return forward_mixed_impl<MASK, kMeshCubes>(ctx, kind, nbm, models, model_ld, out, out_ld,
+                                                  raw_out, raw_ld);
     case kMeshUniform:
-      return forward_mixed_impl<MASK, kMeshUniform>(ctx, kind, model, out, raw_out);
-    default: return forward_mixed_impl<MASK, kMeshGeneral>(ctx, kind, model, out, raw_out);
+      return forward_mixed_impl<MASK, kMeshUniform>(ctx, kind, nbm, models, model_ld, out, out_ld,
+                                                    raw_out, raw_ld);
+    default:
+      return forward_mixed_impl<MASK, kMeshGeneral>(ctx, kind, nbm, models, model_ld, out, out_ld,
+                                                    raw_out, raw_ld);
   }
 }
 
 template <int MASK>
-hipError_t adjoint_dispatch(jgm_context* ctx, int kind, const double* rm, double* grad) {
+hipError_t adjoint_dispatch(jgm_context* ctx, int kind, int nbm, const double* rm, size_t rm_ld,
+                            double* grad, size_t grad_ld) {
   switch (ctx->near.mesh_class) {
-    case kMeshCubes: return adjoint_mixed_impl<MASK, kMeshCubes>(ctx, kind, rm, grad);
-    case kMeshUniform: return adjoint_mixed_impl<MASK, kMeshUniform>(ctx, kind, rm, grad);
-    default: return adjoint_mixed_impl<MASK, kMeshGeneral>(ctx, kind, rm, grad);
+    case kMeshCubes:
+      return adjoint_mixed_impl<MASK, kMeshCubes>(ctx, kind, nbm, rm, rm_ld, grad, grad_ld);
+    case kMeshUniform:
+      return adjoint_mixed_impl<MASK, kMeshUniform>(ctx, kind, nbm, rm, rm_ld, grad, grad_ld);
+    default:
+      return adjoint_mixed_impl<MASK, kMeshGeneral>(ctx, kind, nbm, rm, rm_ld, grad, grad_ld);
   }
 }
 
@@ -1116,28 +1253,69 @@ hipError_t build_near_boxes(jgm_context* ctx) {
 
 hipError_t launch_forward_mixed(jgm_context* ctx, int kind, const double* model, double* out,
                                 double* raw_out) {
+  return launch_forward_mixed_batch(ctx, kind, 1, model, 0, out, 0, raw_out, 0);
+}
+
+hipError_t launch_forward_mixed_batch(jgm_context* ctx, int kind, size_t nmodels,
+                                      const double* models, size_t model_ld, double* out,
+                                      size_t out_ld, double* raw_out, size_t raw_ld) {
   if (!ctx->near.valid) {
     hipError_t e = build_near_boxes(ctx);
     if (e != hipSuccess) return e;
   }
-  switch (mask_of(kind)) {
-    case kMaskScalar: return forward_dispatch<kMaskScalar>(ctx, kind, model, out, raw_out);
-    case kMaskAll: return forward_dispatch<kMaskAll>(ctx, kind, model, out, raw_out);
-    default: return forward_dispatch<kMaskTensor>(ctx, kind, model, out, raw_out);
+  for (size_t b0 = 0; b0 < nmodels;) {
+    const size_t left = nmodels - b0;
+    const int nbm = left >= 4 ? 4 : left >= 2 ? 2 : 1;
+    const double* m = models + b0 * model_ld;
+    double* o = out + b0 * out_ld;
+    double* r = raw_out ? raw_out + b0 * raw_ld : nullptr;
+    hipError_t e;
+    switch (mask_of(kind)) {
+      case kMaskScalar:
+        e = forward_dispatch<kMaskScalar>(ctx, kind, nbm, m, model_ld, o, out_ld, r, raw_ld);
+        break;
+      case kMaskAll:
+        e = forward_dispatch<kMaskAll>(ctx, kind, nbm, m, model_ld, o, out_ld, r, raw_ld);
+        break;
+      default:
+        e = forward_dispatch<kMaskTensor>(ctx, kind, nbm, m, model_ld, o, out_ld, r, raw_ld);
+        break;
+    }
+    if (e != hipSuccess) return e;
+    b0 += nbm;
   }
+  return hipSuccess;
 }
 
 hipError_t launch_adjoint_mixed(jgm_context* ctx, int kind, const double* /*model*/,
                                 const double* raw_misfit, double* grad) {
+  return launch_adjoint_mixed_batch(ctx, kind, 1, raw_misfit, 0, grad, 0);
+}
+
+hipError_t launch_adjoint_mixed_batch(jgm_context* ctx, int kind, size_t nmodels,
+                                      const double* raw_misfits, size_t rm_ld, double* grads,
+                                      size_t grad_ld) {
   if (!ctx->near.valid) {
     hipError_t e = build_near_boxes(ctx);
     if (e != hipSuccess) return e;
   }
-  switch (mask_of(kind)) {
-    case kMaskScalar: return adjoint_dispatch<kMaskScalar>(ctx, kind, raw_misfit, grad);
-    case kMaskAll: return adjoint_dispatch<kMaskAll>(ctx, kind, raw_misfit, grad);
-    default: return adjoint_dispatch<kMaskTensor>(ctx, kind, raw_misfit, grad);
+  for (size_t b0 = 0; b0 < nmodels;) {
+    const size_t left = nmodels - b0;
+    const int nbm = left >= 4 ? 4 : left >= 2 ? 2 : 1;
+    const double* r = raw_misfits + b0 * rm_ld;
+    double* gr = grads + b0 * grad_ld;
+    hipError_t e;
+    switch (mask_of(kind)) {
+      case kMaskScalar:
+        e = adjoint_dispatch<kMaskScalar>(ctx, kind, nbm, r, rm_ld, gr, grad_ld);
+        break;
+      case kMaskAll: e = adjoint_dispatch<kMaskAll>(ctx, kind, nbm, r, rm_ld, gr, grad_ld); break;
+      default: e = adjoint_dispatch<kMaskTensor>(ctx, kind, nbm, r, rm_ld, gr, grad_ld); break;
+    }
+    if (e != hipSuccess) return e;
+    b0 += nbm;
   }
+  return hipSuccess;
 }
 
 }  // namespace jgm
