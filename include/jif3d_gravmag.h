@@ -190,11 +190,14 @@ jgm_status jgm_sensitivity_rows(jgm_context *ctx, jgm_kind kind, size_t site0, s
 
 /* Stored J in HBM, FP32, compact layout: rows = nsites * k with, per site,
  * gz (scalar, k = 1); Uxx, Uxy, Uxz, Uyy, Uyz, Uzz (tensor, k = 6); Bx, By, Bz per unit
- * susceptibility (magnetic, k = 3); gz then the six tensor values (scalar+tensor, k = 7).
- * Columns = ncells (background layers are not stored), column-major (rows fastest).
+ * susceptibility (magnetic, k = 3); gz then the six tensor values (scalar+tensor, k = 7);
+ * Bz at the site minus Bz 1 m below (vertical gradient, k = 1); Bx, By, Bz
+ * (magnetization, k = 3).  Columns = ncells (background layers are not stored; the
+ * vertical-gradient kind's background gradient is zero, as in OMPMagneticGradImp), or
+ * 3 ncells [Mx; My; Mz] for the magnetization kind; column-major (rows fastest).
  * Replaces the ublas matrix of FullSensitivityGravMagCalculator (:40-41, :116-130).
  * Every entry is evaluated with the FP64 closed form and rounded once to FP32, so
- * prec (JGM_FP64 or JGM_MIXED) does not change J. */
+ * prec (JGM_FP64, or JGM_MIXED for the kinds that have it) does not change J. */
 jgm_status jgm_storedj_build(jgm_context *ctx, jgm_kind kind, jgm_precision prec);
 size_t jgm_storedj_rows(const jgm_context *ctx);
 /* Y = J M for nrhs model vectors (M: ncells x nrhs column-major, Y: rows x nrhs

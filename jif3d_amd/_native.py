@@ -216,6 +216,7 @@ class Context:
         self.handle = h
         self.device = device
         self.nsites = 0
+        self.storedj_kind = 0
 
     def close(self):
         if getattr(self, "handle", None):
@@ -359,7 +360,8 @@ class Context:
         return self.lib.jgm_storedj_rows(self.handle)
 
     def storedj_cols(self):
-        return self.ncells
+        """Parameter columns of the stored J: three per cell for the magnetization kind."""
+        return 3 * self.ncells if self.storedj_kind == KIND_MAGNETIZATION else self.ncells
 
     def alloc(self, shape, dtype=np.float64):
         return DeviceArray(self, shape, dtype)
@@ -429,14 +431,15 @@ class Context:
             return out
         m = _f64(model) if model is not None else None
         r = _f64(misfit)
-        g = np.zeros(self.n_params())
+        g = np.zeros(self.model_size(self.storedj_kind))
         check(self.lib.jgm_storedj_gradient_host(self.handle, prec, _pd(m), _pd(r), _pd(g)))
         return g
 
     def storedj_forward_batch(self, prec, models):
         """Cached forward of several models (host (B, n_params) -> (B, nout)); up to 64
         models share one pass over J."""
-        m = np.ascontiguousarray(np.asarray(models, dtype=np.float64).reshape(-1, self.n_params()))
+        n = self.model_size(self.storedj_kind)
+        m = np.ascontiguousarray(np.asarray(models, dtype=np.float64).reshape(-1, n))
         res = np.zeros((m.shape[0], self.nsites * self.data_per_site(self.storedj_kind)))
         check(self.lib.jgm_storedj_forward_batch_host(self.handle, prec, m.shape[0], _pd(m),
                                                       _pd(res)))
@@ -450,8 +453,8 @@ class Context:
         m = None
         if models is not None:
             m = np.ascontiguousarray(np.asarray(models, dtype=np.float64).reshape(
-                -1, self.n_params()))
-        g = np.zeros((r.shape[0], self.n_params()))
+                -1, self.model_size(self.storedj_kind)))
+        g = np.zeros((r.shape[0], self.model_size(self.storedj_kind)))
         check(self.lib.jgm_storedj_gradient_batch_host(self.handle, prec, r.shape[0],
                                                        _pd(m) if m is not None else None,
                                                        _pd(r), _pd(g)))

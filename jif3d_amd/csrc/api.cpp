@@ -680,7 +680,6 @@ jgm_status jgm_sensitivity_rows(jgm_context* ctx, jgm_kind kind, size_t site0, s
 jgm_status jgm_storedj_build(jgm_context* ctx, jgm_kind kind, jgm_precision prec) {
   jgm_status st = check_ready(ctx, kind, prec);
   if (st != JGM_OK) return st;
-  if (fp64_only(kind)) return fail(JGM_ERR_UNSUPPORTED, "no stored J for kind %d", (int)kind);
   DeviceGuard g(ctx->device);
   hipError_t e = jgm::launch_storedj_build(ctx, kind, prec);
   if (e == hipErrorNotSupported) return fail(JGM_ERR_UNSUPPORTED, "stored J not available");
@@ -692,7 +691,6 @@ jgm_status jgm_storedj_build_host(jgm_context* ctx, jgm_kind kind, jgm_precision
                                   size_t block_cells) {
   jgm_status st = check_ready(ctx, kind, prec);
   if (st != JGM_OK) return st;
-  if (fp64_only(kind)) return fail(JGM_ERR_UNSUPPORTED, "no stored J for kind %d", (int)kind);
   if (block_cells == 0) return fail(JGM_ERR_INVALID_ARGUMENT, "block_cells must be positive");
   DeviceGuard g(ctx->device);
   hipError_t e = jgm::launch_storedj_build_resident(ctx, kind, prec, block_cells);
@@ -799,7 +797,8 @@ jgm_status jgm_storedj_forward_batch(jgm_context* ctx, jgm_gemm_precision prec, 
   if (st != JGM_OK) return st;
   if (nmodels == 0) return JGM_OK;
   if (!models || !out) return fail(JGM_ERR_INVALID_ARGUMENT, "null models or output pointer");
-  const size_t np = jgm::nparams(ctx), nout = ctx->S * data_per_site(ctx, ctx->storedJ_kind);
+  const size_t np = jgm::nparams(ctx, ctx->storedJ_kind),
+               nout = ctx->S * data_per_site(ctx, ctx->storedJ_kind);
   if (nmodels > 1 && (model_ld < np || out_ld < nout))
     return fail(JGM_ERR_INVALID_ARGUMENT, "leading dimensions too small");
   DeviceGuard g(ctx->device);
@@ -823,7 +822,7 @@ jgm_status jgm_storedj_gradient_batch(jgm_context* ctx, jgm_gemm_precision prec,
     return fail(JGM_ERR_INVALID_ARGUMENT, "a transform needs the models for its Jacobian");
   const int kind = ctx->storedJ_kind;
   const size_t S = ctx->S, kraw = jgm::raw_per_site(kind);
-  const size_t np = jgm::nparams(ctx), nmis = S * data_per_site(ctx, kind);
+  const size_t np = jgm::nparams(ctx, kind), nmis = S * data_per_site(ctx, kind);
   if (nmodels > 1 && (misfit_ld < nmis || grad_ld < np ||
                       (ctx->transform != JGM_TF_NONE && model_ld < np)))
     return fail(JGM_ERR_INVALID_ARGUMENT, "leading dimensions too small");
@@ -863,7 +862,8 @@ jgm_status jgm_storedj_forward_batch_host(jgm_context* ctx, jgm_gemm_precision p
   if (nmodels == 0) return JGM_OK;
   if (!models || !out) return fail(JGM_ERR_INVALID_ARGUMENT, "null models or output pointer");
   DeviceGuard g(ctx->device);
-  const size_t np = jgm::nparams(ctx), nout = ctx->S * data_per_site(ctx, ctx->storedJ_kind);
+  const size_t np = jgm::nparams(ctx, ctx->storedJ_kind),
+               nout = ctx->S * data_per_site(ctx, ctx->storedJ_kind);
   const HostArg args[2] = {{models, nullptr, nmodels * np}, {nullptr, out, nmodels * nout}};
   return run_staged(ctx, "storedj_forward_batch_host", args,
                     [&](const double* const* in, double* const* o) {
@@ -879,7 +879,8 @@ jgm_status jgm_storedj_gradient_batch_host(jgm_context* ctx, jgm_gemm_precision 
   if (nmodels == 0) return JGM_OK;
   if (!misfits || !grads) return fail(JGM_ERR_INVALID_ARGUMENT, "null misfit or gradient pointer");
   DeviceGuard g(ctx->device);
-  const size_t np = jgm::nparams(ctx), nmis = ctx->S * data_per_site(ctx, ctx->storedJ_kind);
+  const size_t np = jgm::nparams(ctx, ctx->storedJ_kind),
+               nmis = ctx->S * data_per_site(ctx, ctx->storedJ_kind);
   const HostArg args[3] = {{models, nullptr, models ? nmodels * np : 0},
                            {misfits, nullptr, nmodels * nmis},
                            {nullptr, grads, nmodels * np}};
@@ -897,7 +898,7 @@ jgm_status jgm_storedj_forward_host(jgm_context* ctx, jgm_gemm_precision prec,
   if (st != JGM_OK) return st;
   if (!model || !out) return fail(JGM_ERR_INVALID_ARGUMENT, "null model or output pointer");
   DeviceGuard g(ctx->device);
-  const size_t np = jgm::nparams(ctx);
+  const size_t np = jgm::nparams(ctx, ctx->storedJ_kind);
   const size_t nout = ctx->S * data_per_site(ctx, ctx->storedJ_kind);
   hipError_t e;
   if ((e = ctx->model_stage.ensure(sizeof(double) * (np ? np : 1))) != hipSuccess)
@@ -923,7 +924,7 @@ jgm_status jgm_storedj_gradient_host(jgm_context* ctx, jgm_gemm_precision prec,
   if (st != JGM_OK) return st;
   if (!misfit || !grad) return fail(JGM_ERR_INVALID_ARGUMENT, "null misfit or gradient pointer");
   DeviceGuard g(ctx->device);
-  const size_t np = jgm::nparams(ctx);
+  const size_t np = jgm::nparams(ctx, ctx->storedJ_kind);
   const size_t nmis = ctx->S * data_per_site(ctx, ctx->storedJ_kind);
   jgm::DeviceBuffer dmis;  // separate from the internal misfit_stage used by transforms
   hipError_t e = hipSuccess;

@@ -57,8 +57,21 @@ __host__ __device__ inline int compact_rows(int kind) {
     case JGM_KIND_SCALAR: return 1;
     case JGM_KIND_TENSOR: return 6;
     case JGM_KIND_MAGNETIC: return 3;
+    case JGM_KIND_MAGNETIC_GRADIENT: return 1;
+    case JGM_KIND_MAGNETIZATION: return 3;
     default: return 7;
   }
+}
+
+// parameter columns of the stored J: three per cell for the magnetization kind
+inline long storedj_columns(const jgm_context* ctx, int kind) {
+  const long n = (long)ncells(ctx);
+  return kind == JGM_KIND_MAGNETIZATION ? 3 * n : n;
+}
+
+// the vertical-gradient and magnetization kinds have no background columns
+__host__ __device__ inline bool has_background(int kind) {
+  return kind != JGM_KIND_MAGNETIC_GRADIENT && kind != JGM_KIND_MAGNETIZATION;
 }
 
 int mask_of(int kind) {
@@ -74,24 +87,31 @@ __global__ __launch_bounds__(kBlock) void k_fill_zero_f32(float* __restrict__ p,
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0.0f;
 }
 
-// Build: one lane per site, the wave walks a chunk of cells (wave-uniform); FP64
-// closed form per cell (8 corners), stored as FP32 in the compact layout.
+// Build: one lane per site, the wave walks a chunk of columns (wave-uniform); FP64
+// closed form per cell (8 corners), stored as FP32 in the compact layout.  Column col
+// is cell col, except for the magnetization kind, where it is component p = col / N of
+// cell col % N (J[3 s + i][p N + c] = mscale T_ip, OMPMagnetizationImp.cpp:25-117); the
+// vertical-gradient kind stores Bz at the site minus Bz 1 m below
+// (OMPMagneticGradImp.cpp:13-106), mscale = F 1e-7 / G.
 template <int MASK>
 __global__ __launch_bounds__(kBlock) void k_storedj_build(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     int S, const double* __restrict__ xlo, const double* __restrict__ ylo,
     const double* __restrict__ zlo, const double* __restrict__ xsz,
-    const double* __restrict__ ysz, const double* __restrict__ zsz, int ny, int nz, long cbase,
-    long cend, long cells_per_chunk, int kind, FieldDir fd, long ld, float* __restrict__ J) {
+    const double* __restrict__ ysz, const double* __restrict__ zsz, int ny, int nz, long ncell,
+    long cbase, long cend, long cells_per_chunk, int kind, FieldDir fd, double mscale, long ld,
+    float* __restrict__ J) {
   constexpr int NC = Ncomp<MASK>::value;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int sl = s < S ? s : S - 1;
   const double px = sx[sl], py = sy[sl], pz = sz[sl];
   const int kJ = compact_rows(kind);
-  // cells [cbase, cend) of the mesh; column c - cbase of J
+  // columns [cbase, cend); column col is stored at J + (col - cbase) ld
   const long c0 = cbase + (long)blockIdx.y * cells_per_chunk;
   const long c1 = c0 + cells_per_chunk < cend ? c0 + cells_per_chunk : cend;
-  for (long c = c0; c < c1; ++c) {
+  for (long col = c0; col < c1; ++col) {
+    const int p = kind == JGM_KIND_MAGNETIZATION ? (int)(col / ncell) : 0;
+    const long c = col - p * ncell;
     const int k = (int)(c % nz);
     const long t = c / nz;
     const int j = (int)(t % ny);
@@ -99,9 +119,26 @@ __global__ __launch_bounds__(kBlock) void k_storedj_build(
     double v[NC], u[7];
     box_terms_f64<MASK>(px, py, pz, xlo[i], ylo[j], zlo[k], xsz[i], ysz[j], zsz[k], v);
     spread_components<MASK>(v, u);  // gz, Uxx, Uyy, Uzz, Uxy, Uxz, Uyz
+    double w[7];
+    if (MASK == kMaskTensor && kind == JGM_KIND_MAGNETIC_GRADIENT) {
+      box_terms_f64<MASK>(px, py, pz + 1.0, xlo[i], ylo[j], zlo[k], xsz[i], ysz[j], zsz[k], v);
+      spread_components<MASK>(v, w);
+    }
     if (s >= S) continue;
-    float* o = J + (c - cbase) * ld + (long)s * kJ;
-    if (kind == JGM_KIND_SCALAR) {
+    float* o = J + (col - cbase) * ld + (long)s * kJ;
+    if (kind == JGM_KIND_MAGNETIC_GRADIENT) {
+      const double b0 = (u[5] * fd.bx + u[6] * fd.by + u[3] * fd.bz) * mscale;
+      const double b1 = (w[5] * fd.bx + w[6] * fd.by + w[3] * fd.bz) * mscale;
+      o[0] = (float)(b0 - b1);
+    } else if (kind == JGM_KIND_MAGNETIZATION) {
+      // column p of T = [[xx, xy, xz], [xy, yy, yz], [xz, yz, zz]]
+      const double t0 = p == 0 ? u[1] : p == 1 ? u[4] : u[5];
+      const double t1 = p == 0 ? u[4] : p == 1 ? u[2] : u[6];
+      const double t2 = p == 0 ? u[5] : p == 1 ? u[6] : u[3];
+      o[0] = (float)(t0 * mscale);
+      o[1] = (float)(t1 * mscale);
+      o[2] = (float)(t2 * mscale);
+    } else if (kind == JGM_KIND_SCALAR) {
       o[0] = (float)u[0];
     } else if (kind == JGM_KIND_MAGNETIC) {
       o[0] = (float)((u[1] * fd.bx + u[4] * fd.by + u[5] * fd.bz) * fd.scale);
@@ -423,9 +460,9 @@ __global__ __launch_bounds__(kBlock) void k_rm_to_compact(const double* __restri
     const double* rm = rm0 + b * rm_ld;
     const long s = row / kJ;
     int q = (int)(row % kJ);
-    if (kind == JGM_KIND_SCALAR) {
+    if (kind == JGM_KIND_SCALAR || kind == JGM_KIND_MAGNETIC_GRADIENT) {
       v = (float)rm[s];
-    } else if (kind == JGM_KIND_MAGNETIC) {
+    } else if (kind == JGM_KIND_MAGNETIC || kind == JGM_KIND_MAGNETIZATION) {
       v = (float)rm[s * 3 + q];
     } else {
       const int kraw = kind == JGM_KIND_SCALAR_TENSOR ? 10 : 9;
@@ -463,6 +500,12 @@ __global__ __launch_bounds__(kBlock) void k_storedj_finish(
   if (s >= S) return;
   const int kJ = compact_rows(kind);
   const float* y = Y + (long)s * kJ;
+  if (!has_background(kind)) {  // J rows are the data themselves
+    double raw[3];
+    for (int m = 0; m < kJ; ++m) raw[m] = y[m];
+    store_site(raw, kJ, s, tf, tfbx, tfby, tfbz, out, raw_out);
+    return;
+  }
   double u[7] = {0, 0, 0, 0, 0, 0, 0};  // gz, Uxx, Uyy, Uzz, Uxy, Uxz, Uyz
   if (kind == JGM_KIND_SCALAR) {
     u[0] = y[0];
@@ -531,13 +574,18 @@ hipError_t build_columns(jgm_context* ctx, int kind, long c0, long c1, long ld, 
   const int nchunks = (int)((n + per - 1) / per);
   const int ny = (int)ctx->ny, nz = (int)ctx->nz;
   FieldDir fd = field_dir(ctx);
+  // unit factors of the magnetic kinds that the J rows carry (the tensor terms include G)
+  const double mu0 = 4e-7 * kPi;  // OMPMagnetizationImp.cpp:35
+  const double mscale = kind == JGM_KIND_MAGNETIZATION ? 1.0e9 * mu0 / (4 * kPi * kGravConst)
+                                                       : ctx->fieldstrength * (1e-7 / kGravConst);
   ProfScope ps(ctx, "storedj_build_fp64");
 #define JGM_BUILD(M)                                                                            \
   hipLaunchKernelGGL((k_storedj_build<M>), dim3(site_blocks, nchunks), dim3(kBlock), 0,        \
                      ctx->stream, ctx->d_sx.as<double>(), ctx->d_sy.as<double>(),               \
                      ctx->d_sz.as<double>(), S, ctx->d_xlo.as<double>(), ctx->d_ylo.as<double>(), \
                      ctx->d_zlo.as<double>(), ctx->d_xsz.as<double>(), ctx->d_ysz.as<double>(), \
-                     ctx->d_zsz.as<double>(), ny, nz, c0, c1, per, kind, fd, ld, J)
+                     ctx->d_zsz.as<double>(), ny, nz, (long)ncells(ctx), c0, c1, per, kind, fd, \
+                     mscale, ld, J)
   switch (mask_of(kind)) {
     case kMaskScalar: JGM_BUILD(kMaskScalar); break;
     case kMaskAll: JGM_BUILD(kMaskAll); break;
@@ -614,7 +662,7 @@ hipError_t launch_storedj_build_resident(jgm_context* ctx, int kind, int precisi
                                          size_t block_cells) {
   (void)precision;  // the build evaluates the FP64 closed form; J is stored in FP32
   const int S = (int)ctx->S;
-  const long N = (long)ncells(ctx);
+  const long N = storedj_columns(ctx, kind);
   const int kJ = compact_rows(kind);
   const long R = (long)S * kJ;
   if (S == 0 || N == 0) return hipErrorInvalidValue;
@@ -784,7 +832,7 @@ hipError_t launch_storedj_forward_batch(jgm_context* ctx, int gemm_precision, si
   const long colpad = (long)ctx->storedJ_colpad;
   const int kind = ctx->storedJ_kind;
   const int S = (int)ctx->S;
-  const int L = (int)ctx->bgthick.size();
+  const int L = has_background(kind) ? (int)ctx->bgthick.size() : 0;
   const int P = 64;
   hipError_t e = ctx->coef32.ensure(sizeof(float) * (size_t)colpad * P);
   if (e != hipSuccess) return e;
@@ -806,8 +854,8 @@ hipError_t launch_storedj_forward_batch(jgm_context* ctx, int gemm_precision, si
     for (int b = 0; b < nb; ++b)
       hipLaunchKernelGGL(k_storedj_finish, dim3(blocks_for(S)), dim3(kBlock), 0, ctx->stream,
                          Y + (size_t)b * R, S, kind, ctx->bgT.as<const double>(),
-                         m + b * model_ld + N, L, field_dir(ctx), ctx->transform, tf_bx, tf_by,
-                         tf_bz, out + (b0 + b) * out_ld,
+                         m + b * model_ld + (L > 0 ? N : 0), L, field_dir(ctx), ctx->transform,
+                         tf_bx, tf_by, tf_bz, out + (b0 + b) * out_ld,
                          raw_out ? raw_out + (b0 + b) * raw_ld : nullptr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
@@ -845,8 +893,11 @@ hipError_t launch_storedj_gradient_batch(jgm_context* ctx, int gemm_precision, s
       hipLaunchKernelGGL(k_scale_to_f64, dim3(blocks_for(N)), dim3(kBlock), 0, ctx->stream,
                          G + (size_t)b * N, N, 2.0, g);
       if ((e = hipGetLastError()) != hipSuccess) return e;
-      if ((e = launch_background_gradient(ctx, kind, rm + b * misfit_ld, g + N)) != hipSuccess)
-        return e;
+      if (has_background(kind))
+        e = launch_background_gradient(ctx, kind, rm + b * misfit_ld, g + N);
+      else if (kind == JGM_KIND_MAGNETIC_GRADIENT)  // zero background columns
+        e = launch_fill_zero(ctx, g + N, ctx->bgthick.size());
+      if (e != hipSuccess) return e;
     }
   }
   return hipSuccess;

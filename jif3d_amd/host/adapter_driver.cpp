@@ -14,8 +14,9 @@
 // input  (little-endian): int32 kind, int32 precision, int32 nx, ny, nz, nbg, nsites;
 //        then doubles: xcoords[nx+1], ycoords[ny+1], zcoords[nz+1], xsizes[nx],
 //        ysizes[ny], zsizes[nz], bg[nbg], sx[ns], sy[ns], sz[ns], field[3],
-//        model[nx*ny*nz + nbg], misfit[ns*raw_per_site]
-// output: doubles forward[ns*raw_per_site], gradient[nx*ny*nz + nbg] (per model for
+//        model[nx*ny*nz + nbg] (3*nx*ny*nz for the magnetization kind),
+//        misfit[ns*raw_per_site]
+// output: doubles forward[ns*raw_per_site], gradient[model length] (per model for
 //         --batch, all forwards first)
 #include <cstdint>
 #include <cstdio>
@@ -109,9 +110,12 @@ int main(int argc, char** argv) {
     sites.y = read_doubles(in, ns);
     sites.z = read_doubles(in, ns);
     const std::vector<double> field = read_doubles(in, 3);
-    const std::vector<double> model = read_doubles(in, nx * ny * nz + nbg);
+    const size_t nmodel =
+        kind == JGM_KIND_MAGNETIZATION ? 3 * nx * ny * nz : nx * ny * nz + nbg;
+    const std::vector<double> model = read_doubles(in, nmodel);
     jif3d_amd::HipGravMagImplementation imp(kind, prec);
-    if (kind == JGM_KIND_MAGNETIC) imp.SetField(field[0], field[1], field[2]);
+    if (kind == JGM_KIND_MAGNETIC || kind == JGM_KIND_MAGNETIC_GRADIENT)
+      imp.SetField(field[0], field[1], field[2]);
     const std::vector<double> misfit = read_doubles(in, ns * imp.RawDataPerMeasurement());
     std::vector<double> fwd, grad;
     if (middle) {

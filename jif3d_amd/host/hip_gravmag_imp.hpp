@@ -208,7 +208,7 @@ class HipFullSensitivityCalculator {
     if (!Cached(mesh, sites)) return imp_.LQDerivative(mesh, sites, model, misfit);
     if (misfit.size() != sites.size() * imp_.GetDataPerMeasurement())
       throw FatalException("misfit length does not match the number of data");
-    std::vector<double> grad(jgm_n_params(imp_.context()));
+    std::vector<double> grad(imp_.ModelSize());
     check(jgm_storedj_gradient_host(imp_.context(), prec_, model.data(), misfit.data(),
                                     grad.data()),
           "jgm_storedj_gradient_host");

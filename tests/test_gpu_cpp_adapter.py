@@ -1,5 +1,6 @@
 """The C++ host adapter (jif3d_amd/host/hip_gravmag_imp.hpp) through its driver program:
-Calculate and LQDerivative of scalar, tensor and magnetic imps against the oracle."""
+Calculate and LQDerivative of the scalar, tensor, magnetic, vertical-gradient and
+magnetization imps against the oracle."""
 import os
 import subprocess
 import tempfile
@@ -8,7 +9,8 @@ import numpy as np
 import pytest
 
 from golden_paths import ROOT
-from jif3d_amd.model import ThreeDGravityModel
+from jif3d_amd.model import (ThreeDGravityModel, ThreeDSusceptibilityModel,
+                             ThreeDMagnetizationModel)
 
 pytestmark = pytest.mark.gpu
 DRIVER = os.path.join(ROOT, "jif3d_amd", "host", "adapter_driver")
@@ -57,6 +59,44 @@ def test_cpp_adapter_matches_oracle(orc, kind, okind, k, full):
     ref = orc.calculate(okind, m, m.GetModelParameters(), sites, field=fld)
     gref = orc.lqderivative(okind, m, m.GetModelParameters(), sites, misfit, field=fld)
     tf, tg = (2e-6, 2e-6) if full else (1e-11, 1e-10)
+    assert np.max(np.abs(fwd - ref)) <= tf * np.max(np.abs(ref))
+    assert np.max(np.abs(grad - gref)) <= tg * np.max(np.abs(gref))
+
+
+@pytest.mark.parametrize("full", [False, True])
+@pytest.mark.parametrize("kind,okind,k", [(5, 4, 1), (6, 5, 3)])
+def test_cpp_adapter_magnetic_extensions(orc, kind, okind, k, full):
+    """Vertical-gradient and magnetization imps, direct and through the stored J."""
+    rng = np.random.default_rng(60 + kind)
+    m = ThreeDMagnetizationModel() if kind == 6 else ThreeDSusceptibilityModel()
+    m.SetMeshSize(7, 6, 5)
+    m.SetXCellSizes(rng.integers(10, 35, 7).astype(float))
+    m.SetYCellSizes(rng.integers(10, 35, 6).astype(float))
+    m.SetZCellSizes(rng.integers(10, 35, 5).astype(float))
+    n = m.ncells
+    if kind == 6:
+        m.SetMagnetization(rng.normal(size=n), rng.normal(size=n), rng.normal(size=n))
+    else:
+        m.SetSusceptibilities(rng.random(n) * 0.05)
+        m.SetBackgroundThicknesses([30.0, 500.0])
+        m.SetBackgroundSusceptibilities([0.01, 0.02])
+    ns = 9
+    sites = (rng.random(ns) * 150, rng.random(ns) * 150, -rng.random(ns) * 30 - 1)
+    field = (0.6, 0.2, 48000.0)
+    misfit = rng.normal(size=ns * k)
+    with tempfile.TemporaryDirectory() as td:
+        inp, out = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        write_problem(inp, kind, 0, m, sites, field, misfit)
+        res = subprocess.run([DRIVER, inp, out] + (["--full"] if full else []),
+                             capture_output=True, text=True, timeout=120)
+        assert res.returncode == 0, res.stderr
+        data = np.fromfile(out, dtype="<f8")
+    mv = m.GetModelParameters()
+    fwd, grad = data[: ns * k], data[ns * k:]
+    assert grad.size == mv.size
+    ref = orc.calculate(okind, m, mv, sites, field=field)
+    gref = orc.lqderivative(okind, m, mv, sites, misfit, field=field)
+    tf, tg = (1e-5, 1e-5) if full else (1e-11, 1e-10)
     assert np.max(np.abs(fwd - ref)) <= tf * np.max(np.abs(ref))
     assert np.max(np.abs(grad - gref)) <= tg * np.max(np.abs(gref))
 
