@@ -558,28 +558,99 @@ __global__ __launch_bounds__(kBlock) void k_mixed_coef(const double* __restrict_
   o[7] = (float)(c7[1] + c7[2] + c7[3]);
 }
 
+// Adjoint far field, one site's terms for a lane's kQ cells, added to acc.  HIT: the
+// site's near box touches the tile, and the lane's cells inside it (nq) are moved far
+// away and dropped (FP64 near pass); otherwise no cell needs a mask.
+constexpr int kQ = 4;  // cells per lane of the far adjoint
+template <int MASK, int MESH, int NB, bool HIT>
+__device__ __forceinline__ void far_adjoint_site(
+    const float* __restrict__ coef, const float4* mono4, int S, int s, float Xt, float Yt,
+    float Zt, bool need, float ox, float oy, const float (&ozq)[kQ], const MpConst (&ccq)[kQ],
+    const bool (&nq)[kQ], float (&acc)[NB][kQ]) {
+  constexpr int NC = Ncomp<MASK>::value;
+  constexpr bool QUAD = MESH != kMeshCubes;
+  const float X = Xt - ox, Y = Yt - oy;
+  float Zq[kQ], pq[NB][kQ];
+#pragma unroll
+  for (int q = 0; q < kQ; ++q) Zq[q] = (HIT && nq[q]) ? 1.0e6f : Zt - ozq[q];
+  if (MESH == kMeshCubes && !need) {
+    // monopole: s^5 Q - s^3 (c_gz Z + C) with the site's regrouped coefficients
+    const float XX = X * X, YY = Y * Y, XY = X * Y;
+    const float XXYY = XX + YY;
+    float Qxy[NB], Lz[NB], m0[NB], m3[NB], m7[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const float4 lo = mono4[2 * b], hi = mono4[2 * b + 1];  // mc[0..3], mc[4..7]
+      Qxy[b] = fmaf(lo.y, XX, fmaf(lo.z, YY, hi.x * XY));
+      Lz[b] = fmaf(hi.y, X, hi.z * Y);
+      m0[b] = lo.x;
+      m3[b] = lo.w;
+      m7[b] = hi.w;
+    }
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+      const float Z = Zq[q];
+      const float r2 = fmaf(Z, Z, XXYY);
+      float sv = __builtin_amdgcn_rsqf(r2);
+      const float e = fmaf(-(r2 * sv), sv, 1.0f);  // one Newton step
+      sv = fmaf(0.5f * sv, e, sv);
+      const float s2 = sv * sv, s3 = s2 * sv;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const float Q = fmaf(Z, fmaf(m3[b], Z, Lz[b]), Qxy[b]);
+        pq[b][q] = fmaf(Q, s3 * s2, -s3 * fmaf(m0[b], Z, m7[b]));
+      }
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+      float T[NC];
+      if (need)
+        multipole_f32<MASK, QUAD, true>(X, Y, Zq[q], ccq[q], T);
+      else
+        multipole_f32<MASK, QUAD, false>(X, Y, Zq[q], ccq[q], T);
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const float* cs = coef + ((long)b * S + s) * NC;
+        float p = 0.0f;
+#pragma unroll
+        for (int m = 0; m < NC; ++m) p = fmaf(cs[m], T[m], p);
+        pq[b][q] = p;
+      }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) acc[b][q] += (HIT && nq[q]) ? 0.0f : pq[b][q];
+}
+
 // Adjoint far field: one lane per kQ cells (k, k + 4, k + 8, ...) of a 4x4x(4 kQ) tile;
 // the wave walks a chunk of sites (wave-uniform) and accumulates
 // sum_m coef_m(s) T_m(s, c) in FP32, flushed to FP64 every 8 sites; cells in the site's
 // near box are skipped (FP64 near pass).  Site offsets are formed in FP64 against the
-// tile's first cell once per site and lane and shared by the lane's cells; a site whose
-// second radius misses every cell of the tile keeps only the monopole (and quadrupole)
-// terms, a wave-uniform branch.  NB misfit vectors share the kernel terms T_m; model b's
-// coefficients sit at coef + b S NC and mono + b S 8, its partial sums at
-// gpart[(b nchunks + chunk) ncell + c].
-constexpr int kQ = 4;  // cells per lane of the far adjoint
+// tile's first cell and rounded once; a site whose second radius misses every cell of
+// the tile keeps only the monopole (and quadrupole) terms, a wave-uniform branch.  These
+// per-site values, whether the site's near box touches the tile and the site's monopole
+// coefficients are the same for every lane, so each lane forms or loads them for one site
+// of a run of 64 and the wave reads them back from LDS (no scalar-cache misses per site).  NB misfit vectors share the kernel
+// terms T_m; model b's coefficients sit at coef + b S NC and mono + b S 8, its partial
+// sums at gpart[(b nchunks + chunk) ncell + c].
 template <int MASK, int MESH, int NB>
 __global__ __launch_bounds__(kBlock) void k_far_adjoint(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     const float* __restrict__ coef, const float* __restrict__ mono, const int* __restrict__ box,
     float r1sq, int S, int sites_per_chunk, FarGeom g, const float* __restrict__ table,
     MpConst uc, long ntiles, double* __restrict__ gpart) {
-  constexpr int NC = Ncomp<MASK>::value;
   constexpr int TW = TableWidth<MESH>::value;
-  constexpr bool QUAD = MESH != kMeshCubes;
   constexpr int KT = kQ * kTile;  // tile depth in k
+  // per-site values of the wave's run of 64 sites: offsets and flags, and the NB sets of
+  // monopole coefficients (two float4 each)
+  __shared__ float4 sitev[kBlock];
+  __shared__ float4 monov[kBlock * 2 * NB];
   const long t = ((long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
   const int lane = threadIdx.x & 63;
+  const int run = threadIdx.x - lane;
   if (t >= ntiles) return;
   const int ntz = (g.nz + KT - 1) / KT;
   const int tk = (int)(t % ntz);
@@ -621,84 +692,68 @@ __global__ __launch_bounds__(kBlock) void k_far_adjoint(
       acc[b][q] = 0.0f;
     }
   int nacc = 0;
-  for (int s = s0; s < s1; ++s) {
-    const int* bx = box + s * 6;
-    const bool tile_hit = !(bx[1] < i0 || bx[0] > i0 + ni - 1 || bx[3] < j0 ||
-                            bx[2] > j0 + nj - 1 || bx[5] < k0 || bx[4] > k0 + nk - 1);
-    bool nq[kQ];
-#pragma unroll
-    for (int q = 0; q < kQ; ++q) nq[q] = false;
-    if (tile_hit) {
-      const bool inxy = i >= bx[0] && i <= bx[1] && j >= bx[2] && j <= bx[3];
-#pragma unroll
-      for (int q = 0; q < kQ; ++q) nq[q] = inxy && kq[q] >= bx[4] && kq[q] <= bx[5];
-    }
-    const float Xt = (float)(sx[s] - x0), Yt = (float)(sy[s] - y0), Zt = (float)(sz[s] - z0);
-    const float X = Xt - ox, Y = Yt - oy;
-    const float dx = fmaxf(0.0f, fmaxf(-Xt, Xt - ex)), dy = fmaxf(0.0f, fmaxf(-Yt, Yt - ey)),
-                dz = fmaxf(0.0f, fmaxf(-Zt, Zt - ez));
-    const bool need = fmaf(dx, dx, fmaf(dy, dy, dz * dz)) < r1sq;
-    float Zq[kQ], pq[NB][kQ];
-#pragma unroll
-    for (int q = 0; q < kQ; ++q) Zq[q] = nq[q] ? 1.0e6f : Zt - ozq[q];
-    if (MESH == kMeshCubes && !need) {
-      // monopole: s^5 Q - s^3 (c_gz Z + C) with the site's regrouped coefficients
-      const float XX = X * X, YY = Y * Y, XY = X * Y;
-      const float XXYY = XX + YY;
-      float Qxy[NB], Lz[NB];
-#pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        const float* mc = mono + ((long)b * S + s) * 8;
-        Qxy[b] = fmaf(mc[1], XX, fmaf(mc[2], YY, mc[4] * XY));
-        Lz[b] = fmaf(mc[5], X, mc[6] * Y);
-      }
-#pragma unroll
-      for (int q = 0; q < kQ; ++q) {
-        const float Z = Zq[q];
-        const float r2 = fmaf(Z, Z, XXYY);
-        float sv = __builtin_amdgcn_rsqf(r2);
-        const float e = fmaf(-(r2 * sv), sv, 1.0f);  // one Newton step
-        sv = fmaf(0.5f * sv, e, sv);
-        const float s2 = sv * sv, s3 = s2 * sv;
+  for (int sb = s0; sb < s1; sb += 64) {
+    const int nsb = min(64, s1 - sb);
+    {
+      // this lane's site of the run: offsets from the tile, the second-radius test (bit
+      // 0), whether its near box touches the tile (bit 1), its monopole coefficients
+      const int sl = sb + (lane < nsb ? lane : 0);
+      const float Xl = (float)(sx[sl] - x0), Yl = (float)(sy[sl] - y0),
+                  Zl = (float)(sz[sl] - z0);
+      const float dx = fmaxf(0.0f, fmaxf(-Xl, Xl - ex)), dy = fmaxf(0.0f, fmaxf(-Yl, Yl - ey)),
+                  dz = fmaxf(0.0f, fmaxf(-Zl, Zl - ez));
+      const bool nl = fmaf(dx, dx, fmaf(dy, dy, dz * dz)) < r1sq;
+      const int* bl = box + sl * 6;
+      const bool hl = !(bl[1] < i0 || bl[0] > i0 + ni - 1 || bl[3] < j0 ||
+                        bl[2] > j0 + nj - 1 || bl[5] < k0 || bl[4] > k0 + nk - 1);
+      sitev[threadIdx.x] = make_float4(Xl, Yl, Zl, (float)((nl ? 1 : 0) | (hl ? 2 : 0)));
+      if (MESH == kMeshCubes) {
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-          const float* mc = mono + ((long)b * S + s) * 8;
-          const float Q = fmaf(Z, fmaf(mc[3], Z, Lz[b]), Qxy[b]);
-          pq[b][q] = fmaf(Q, s3 * s2, -s3 * fmaf(mc[0], Z, mc[7]));
-        }
-      }
-    } else {
-#pragma unroll
-      for (int q = 0; q < kQ; ++q) {
-        float T[NC];
-        if (need)
-          multipole_f32<MASK, QUAD, true>(X, Y, Zq[q], ccq[q], T);
-        else
-          multipole_f32<MASK, QUAD, false>(X, Y, Zq[q], ccq[q], T);
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-          const float* cs = coef + ((long)b * S + s) * NC;
-          float p = 0.0f;
-#pragma unroll
-          for (int m = 0; m < NC; ++m) p = fmaf(cs[m], T[m], p);
-          pq[b][q] = p;
+          const float4* m4 = reinterpret_cast<const float4*>(mono + ((long)b * S + sl) * 8);
+          monov[threadIdx.x * 2 * NB + 2 * b] = m4[0];
+          monov[threadIdx.x * 2 * NB + 2 * b + 1] = m4[1];
         }
       }
     }
+    // the wave's own LDS writes are complete before its reads below
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int js = 0; js < nsb; ++js) {
+      const int s = sb + js;
+      const float4 v = sitev[run + js];
+      const int flags = __builtin_amdgcn_readfirstlane((int)v.w);
+      const bool need = (flags & 1) != 0;
+      const float4* m4 = monov + (run + js) * 2 * NB;
+      bool nq[kQ];
+      if (flags & 2) {
+        const int* bx = box + s * 6;
+        const bool inxy = i >= bx[0] && i <= bx[1] && j >= bx[2] && j <= bx[3];
 #pragma unroll
-    for (int b = 0; b < NB; ++b)
+        for (int q = 0; q < kQ; ++q) nq[q] = inxy && kq[q] >= bx[4] && kq[q] <= bx[5];
+        far_adjoint_site<MASK, MESH, NB, true>(coef, m4, S, s, v.x, v.y, v.z, need, ox, oy, ozq,
+                                               ccq, nq, acc);
+      } else {
 #pragma unroll
-      for (int q = 0; q < kQ; ++q) acc[b][q] += nq[q] ? 0.0f : pq[b][q];
-    if (++nacc == 8) {
+        for (int q = 0; q < kQ; ++q) nq[q] = false;
+        far_adjoint_site<MASK, MESH, NB, false>(coef, m4, S, s, v.x, v.y, v.z, need, ox, oy,
+                                                ozq, ccq, nq, acc);
+      }
+      if (++nacc == 8) {
 #pragma unroll
-      for (int b = 0; b < NB; ++b)
+        for (int b = 0; b < NB; ++b)
 #pragma unroll
-        for (int q = 0; q < kQ; ++q) {
-          a64[b][q] += (double)acc[b][q];
-          acc[b][q] = 0.0f;
-        }
-      nacc = 0;
+          for (int q = 0; q < kQ; ++q) {
+            a64[b][q] += (double)acc[b][q];
+            acc[b][q] = 0.0f;
+          }
+        nacc = 0;
+      }
     }
+    // every read of this run is done before the next run overwrites the slots
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
   }
   const long ncell = (long)g.nx * g.ny * g.nz;
 #pragma unroll
