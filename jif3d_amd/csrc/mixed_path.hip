@@ -300,19 +300,22 @@ __device__ __forceinline__ void far_tile(const FarGeom& g, const float* __restri
   }
 }
 
-// Monopole-only sum over a tile of cubes.  Along a column of cells X and Y are
-// fixed, so sum_k w T_m needs five running sums per column,
-//   S3 = sum w s^3, S5 = sum w s^5, SZ3 = sum w Z s^3, SZ5 = sum w Z s^5,
-//   SZZ5 = sum w Z^2 s^5            (s = 1/r),
-// and gz = -SZ3, Txx = 3X^2 S5 - S3, Tyy = 3Y^2 S5 - S3, Tzz = 3 SZZ5 - S3,
-// Txy = 3XY S5, Txz = 3X SZ5, Tyz = 3Y SZ5: the monopole terms of multipole_f32
-// regrouped, about half the operations per cell.  The NB models share s.
+// Monopole-only sum over a tile of cubes.  With s = 1/r and w the cell weights, the
+// monopole terms of multipole_f32 regrouped are
+//   gz = -sum w Z s^3,  Txx = 3X^2 sum w s^5 - sum w s^3,  Tyy = 3 sum w Y^2 s^5 - sum w s^3,
+//   Tzz = 3 sum w Z^2 s^5 - sum w s^3,  Txy = 3X sum w Y s^5,  Txz = 3X sum w Z s^5,
+//   Tyz = 3 sum w Y Z s^5,
+// so a row of the tile (fixed X) needs seven running sums: the Y-free ones accumulate per
+// cell, the Y-weighted ones per column from its sums of w s^5 and w Z s^5; sum w s^3 is
+// not summed but formed as X^2 sum w s^5 + sum w Y^2 s^5 + sum w Z^2 s^5 (r^2 s^5 = s^3),
+// which makes the diagonal exactly trace-free, Txx = 2 X^2 A5 - AYY5 - AZZ5 and so on.
+// The components are formed once per row, about half the operations per cell of the full
+// multipole.  The NB models share s.
 template <int MASK, int NB>
 __device__ __forceinline__ void far_tile_mono(const FarGeom& g, const float* __restrict__ wt,
                                               int i0, int j0, int k0, int ni, int nj, int nk,
                                               float Xt, float Yt, float Zt,
                                               double (&acc64)[NB][Ncomp<MASK>::value]) {
-  constexpr int NC = Ncomp<MASK>::value;
   constexpr bool GZ = (MASK & kGz) != 0, DIAG = (MASK & kDiag) != 0, OFF = (MASK & kOff) != 0;
   // z offsets of the tile's cells; a cell past the mesh edge sits far away with weight 0
   float Zc[kTile];
@@ -324,19 +327,18 @@ __device__ __forceinline__ void far_tile_mono(const FarGeom& g, const float* __r
   }
   for (int ii = 0; ii < ni; ++ii) {
     const float X = Xt - g.xoff[i0 + ii];
-    const float X3 = 3.0f * X;
-    float row[NB][NC];  // FP32 sums of this ii's columns, flushed to FP64 per ii
+    // FP32 sums over this row's cells, flushed to FP64 per row
+    float A5[NB], AY5[NB], AYY5[NB], AZ3[NB], AZZ5[NB], AZ5[NB], AYZ5[NB];
 #pragma unroll
-    for (int b = 0; b < NB; ++b)
-#pragma unroll
-      for (int m = 0; m < NC; ++m) row[b][m] = 0.0f;
+    for (int b = 0; b < NB; ++b) A5[b] = AY5[b] = AYY5[b] = AZ3[b] = AZZ5[b] = AZ5[b] = AYZ5[b] = 0.0f;
     for (int jj = 0; jj < nj; ++jj) {
       const float Y = Yt - g.yoff[j0 + jj];
-      const float XXYY = fmaf(X, X, Y * Y);
+      const float YY = Y * Y;
+      const float XXYY = fmaf(X, X, YY);
       const float* wc = wt + (((long)(i0 + ii) * g.ny + (j0 + jj)) * g.nz + k0) * NB;
-      float S3[NB], S5[NB], SZ3[NB], SZ5[NB], SZZ5[NB];
+      float S5[NB], SZ5[NB];  // this column's sums of w s^5 and w Z s^5
 #pragma unroll
-      for (int b = 0; b < NB; ++b) S3[b] = S5[b] = SZ3[b] = SZ5[b] = SZZ5[b] = 0.0f;
+      for (int b = 0; b < NB; ++b) S5[b] = SZ5[b] = 0.0f;
 #pragma unroll
       for (int kk = 0; kk < kTile; ++kk) {
         const float Z = Zc[kk];
@@ -351,40 +353,43 @@ __device__ __forceinline__ void far_tile_mono(const FarGeom& g, const float* __r
         for (int b = 0; b < NB; ++b) {
           const float w = kv[kk] ? wc[kk * NB + b] : 0.0f;
           const float q3 = w * s3;
-          if constexpr (GZ) SZ3[b] = fmaf(Z, q3, SZ3[b]);
+          if constexpr (GZ) AZ3[b] = fmaf(Z, q3, AZ3[b]);
           if constexpr (DIAG || OFF) {
             const float q5 = q3 * s2;
             S5[b] += q5;
-            if constexpr (DIAG) {
-              S3[b] += q3;
-              SZZ5[b] = fmaf(ZZ, q5, SZZ5[b]);
-            }
+            if constexpr (DIAG) AZZ5[b] = fmaf(ZZ, q5, AZZ5[b]);
             if constexpr (OFF) SZ5[b] = fmaf(Z, q5, SZ5[b]);
           }
         }
       }
-      const float Y3 = 3.0f * Y;
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
-        float* rw = row[b];
-        int k = 0;
-        if constexpr (GZ) rw[k++] -= SZ3[b];
-        if constexpr (DIAG) {
-          rw[k] = fmaf(X3 * X, S5[b], rw[k] - S3[b]), ++k;
-          rw[k] = fmaf(Y3 * Y, S5[b], rw[k] - S3[b]), ++k;
-          rw[k] = fmaf(3.0f, SZZ5[b], rw[k] - S3[b]), ++k;
-        }
+        if constexpr (DIAG || OFF) A5[b] += S5[b];
+        if constexpr (DIAG) AYY5[b] = fmaf(YY, S5[b], AYY5[b]);
         if constexpr (OFF) {
-          rw[k] = fmaf(X3 * Y, S5[b], rw[k]), ++k;
-          rw[k] = fmaf(X3, SZ5[b], rw[k]), ++k;
-          rw[k] = fmaf(Y3, SZ5[b], rw[k]), ++k;
+          AY5[b] = fmaf(Y, S5[b], AY5[b]);
+          AZ5[b] += SZ5[b];
+          AYZ5[b] = fmaf(Y, SZ5[b], AYZ5[b]);
         }
       }
     }
+    const float X3 = 3.0f * X;
 #pragma unroll
-    for (int b = 0; b < NB; ++b)
-#pragma unroll
-      for (int m = 0; m < NC; ++m) acc64[b][m] += (double)row[b][m];
+    for (int b = 0; b < NB; ++b) {
+      int k = 0;
+      if constexpr (GZ) acc64[b][k++] += (double)(-AZ3[b]);
+      if constexpr (DIAG) {
+        const float P = (X * X) * A5[b];
+        acc64[b][k++] += (double)fmaf(2.0f, P, -(AYY5[b] + AZZ5[b]));
+        acc64[b][k++] += (double)fmaf(2.0f, AYY5[b], -(P + AZZ5[b]));
+        acc64[b][k++] += (double)fmaf(2.0f, AZZ5[b], -(P + AYY5[b]));
+      }
+      if constexpr (OFF) {
+        acc64[b][k++] += (double)(X3 * AY5[b]);
+        acc64[b][k++] += (double)(X3 * AZ5[b]);
+        acc64[b][k++] += (double)(3.0f * AYZ5[b]);
+      }
+    }
   }
 }
 
@@ -550,12 +555,18 @@ __global__ __launch_bounds__(kBlock) void k_mixed_coef(const double* __restrict_
     c64[(long)s * NC + m] = c[m] * sc[m];
     c32[(long)s * NC + m] = (float)c[m];
   }
-  // monopole form of sum_m c_m T_m = s^5 Q(X, Y, Z) - s^3 (c_gz Z + c_xx + c_yy + c_zz),
-  // Q = 3 (c_xx X^2 + c_yy Y^2 + c_zz Z^2 + c_xy XY + c_xz XZ + c_yz YZ)
+  // monopole form of sum_m c_m T_m: the monopole tensor is trace-free, so the diagonal
+  // coefficients may drop their mean t/3 (t = c_xx + c_yy + c_zz), and with s^3 = r^2 s^5
+  //   sum_m c_m T_m = s^5 (Q(X, Y, Z) - c_gz Z r^2),
+  //   Q = 3 ((c_xx - t/3) X^2 + (c_yy - t/3) Y^2 + (c_zz - t/3) Z^2 + c_xy XY + c_xz XZ
+  //       + c_yz YZ);
+  // stored: -c_gz, the six factors of Q, 0
   float* o = mono + (long)s * 8;
-  o[0] = (float)c7[0];
-  for (int m = 1; m < 7; ++m) o[m] = (float)(3.0 * c7[m]);
-  o[7] = (float)(c7[1] + c7[2] + c7[3]);
+  const double t3 = (c7[1] + c7[2] + c7[3]) / 3.0;
+  o[0] = (float)(-c7[0]);
+  for (int m = 1; m < 4; ++m) o[m] = (float)(3.0 * (c7[m] - t3));
+  for (int m = 4; m < 7; ++m) o[m] = (float)(3.0 * c7[m]);
+  o[7] = 0.0f;
 }
 
 // Adjoint far field, one site's terms for a lane's kQ cells, added to acc.  HIT: the
@@ -574,18 +585,18 @@ __device__ __forceinline__ void far_adjoint_site(
 #pragma unroll
   for (int q = 0; q < kQ; ++q) Zq[q] = (HIT && nq[q]) ? 1.0e6f : Zt - ozq[q];
   if (MESH == kMeshCubes && !need) {
-    // monopole: s^5 Q - s^3 (c_gz Z + C) with the site's regrouped coefficients
+    // monopole with the site's regrouped, trace-free coefficients:
+    // s^5 (Q - c_gz Z r^2) = s^5 (Qxy + Z (Lz - c_gz (X^2 + Y^2) + Z (3 c'_zz - c_gz Z)))
     const float XX = X * X, YY = Y * Y, XY = X * Y;
     const float XXYY = XX + YY;
-    float Qxy[NB], Lz[NB], m0[NB], m3[NB], m7[NB];
+    float Qxy[NB], L2[NB], ngz[NB], m3[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const float4 lo = mono4[2 * b], hi = mono4[2 * b + 1];  // mc[0..3], mc[4..7]
       Qxy[b] = fmaf(lo.y, XX, fmaf(lo.z, YY, hi.x * XY));
-      Lz[b] = fmaf(hi.y, X, hi.z * Y);
-      m0[b] = lo.x;
+      L2[b] = fmaf(lo.x, XXYY, fmaf(hi.y, X, hi.z * Y));
+      ngz[b] = lo.x;
       m3[b] = lo.w;
-      m7[b] = hi.w;
     }
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {
@@ -594,12 +605,11 @@ __device__ __forceinline__ void far_adjoint_site(
       float sv = __builtin_amdgcn_rsqf(r2);
       const float e = fmaf(-(r2 * sv), sv, 1.0f);  // one Newton step
       sv = fmaf(0.5f * sv, e, sv);
-      const float s2 = sv * sv, s3 = s2 * sv;
+      const float s2 = sv * sv;
+      const float s5 = (s2 * sv) * s2;
 #pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        const float Q = fmaf(Z, fmaf(m3[b], Z, Lz[b]), Qxy[b]);
-        pq[b][q] = fmaf(Q, s3 * s2, -s3 * fmaf(m0[b], Z, m7[b]));
-      }
+      for (int b = 0; b < NB; ++b)
+        pq[b][q] = fmaf(Z, fmaf(Z, fmaf(ngz[b], Z, m3[b]), L2[b]), Qxy[b]) * s5;
     }
   } else {
 #pragma unroll
