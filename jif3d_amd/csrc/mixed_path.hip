@@ -330,15 +330,14 @@ __device__ __forceinline__ void far_tile_mono(const FarGeom& g, const float* __r
     // FP32 sums over this row's cells, flushed to FP64 per row
     float A5[NB], AY5[NB], AYY5[NB], AZ3[NB], AZZ5[NB], AZ5[NB], AYZ5[NB];
 #pragma unroll
-    for (int b = 0; b < NB; ++b) A5[b] = AY5[b] = AYY5[b] = AZ3[b] = AZZ5[b] = AZ5[b] = AYZ5[b] = 0.0f;
+    for (int b = 0; b < NB; ++b)
+      A5[b] = AY5[b] = AYY5[b] = AZ3[b] = AZZ5[b] = AZ5[b] = AYZ5[b] = 0.0f;
     for (int jj = 0; jj < nj; ++jj) {
       const float Y = Yt - g.yoff[j0 + jj];
       const float YY = Y * Y;
       const float XXYY = fmaf(X, X, YY);
       const float* wc = wt + (((long)(i0 + ii) * g.ny + (j0 + jj)) * g.nz + k0) * NB;
       float S5[NB], SZ5[NB];  // this column's sums of w s^5 and w Z s^5
-#pragma unroll
-      for (int b = 0; b < NB; ++b) S5[b] = SZ5[b] = 0.0f;
 #pragma unroll
       for (int kk = 0; kk < kTile; ++kk) {
         const float Z = Zc[kk];
@@ -356,9 +355,10 @@ __device__ __forceinline__ void far_tile_mono(const FarGeom& g, const float* __r
           if constexpr (GZ) AZ3[b] = fmaf(Z, q3, AZ3[b]);
           if constexpr (DIAG || OFF) {
             const float q5 = q3 * s2;
-            S5[b] += q5;
+            // the column sums start at their first cell's terms (no additions of zero)
+            S5[b] = kk == 0 ? q5 : S5[b] + q5;
             if constexpr (DIAG) AZZ5[b] = fmaf(ZZ, q5, AZZ5[b]);
-            if constexpr (OFF) SZ5[b] = fmaf(Z, q5, SZ5[b]);
+            if constexpr (OFF) SZ5[b] = kk == 0 ? Z * q5 : fmaf(Z, q5, SZ5[b]);
           }
         }
       }
