@@ -52,8 +52,8 @@ def log(msg):
 KERNEL_SYMBOL = {
     "forward_corners_fp64_gz_ftg": "k_forward_corners<7, 1>",
     "adjoint_corners_fp64_gz_ftg": "k_adjoint_corners<7, 1>",
-    "far_forward_fp32_gz_ftg": "k_far_forward<7, 0>",
-    "far_adjoint_fp32_gz_ftg": "k_far_adjoint<7, 0>",
+    "far_forward_fp32_gz_ftg": "k_far_forward<7, 0, 1>",
+    "far_adjoint_fp32_gz_ftg": "k_far_adjoint<7, 0, 1>",
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gz_ftg_128x128x64_hbm_traffic.md")
 VALU_FILE = os.path.join(ROOT, "profiles", "gz_ftg_128x128x64_valu_counters.md")
@@ -81,7 +81,7 @@ def profiled_valu(kname):
         return None
     for line in open(VALU_FILE):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 6 and cells[0] == f"`{sym}`":
+        if len(cells) >= 6 and cells[0] == f"`{sym}`":
             return float(cells[1]) * 64, (float(cells[2]) + float(cells[3])) * 64
     return None
 
