@@ -401,8 +401,14 @@ __device__ __forceinline__ void far_tile_mono(const FarGeom& g, const float* __r
 // per tile; cell offsets within a tile are exact for uniform meshes.  NB models share
 // every pair's multipole terms; model b's sums go to
 // farpart[((b * nchunks + chunk) * NC + m) * S + s].
+// One model: at least six waves per SIMD (at most 80 VGPRs; the few spills stay outside the
+// cell loops), measured faster than the five that 93 VGPRs allow and than seven or eight,
+// whose spills reach the loops.
+constexpr int far_waves_per_simd(int nb) { return nb == 1 ? 6 : 1; }
+
 template <int MASK, int MESH, int NB>
-__global__ __launch_bounds__(kBlock) void k_far_forward(
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(far_waves_per_simd(NB)))) void k_far_forward(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     int S, const int* __restrict__ order, const int* __restrict__ box, float r1sq, FarGeom g,
     const float* __restrict__ shape, const float* __restrict__ wt, MpConst uc, long ntiles,
@@ -647,7 +653,8 @@ __device__ __forceinline__ void far_adjoint_site(
 // terms T_m; model b's coefficients sit at coef + b S NC and mono + b S 8, its partial
 // sums at gpart[(b nchunks + chunk) ncell + c].
 template <int MASK, int MESH, int NB>
-__global__ __launch_bounds__(kBlock) void k_far_adjoint(
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(far_waves_per_simd(NB)))) void k_far_adjoint(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     const float* __restrict__ coef, const float* __restrict__ mono, const int* __restrict__ box,
     float r1sq, int S, int sites_per_chunk, FarGeom g, const float* __restrict__ table,
