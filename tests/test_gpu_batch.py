@@ -204,3 +204,48 @@ def test_evaluate_along_direction():
     assert chis.shape == (4,) and grads.shape == (4, m0.size)
     assert abs(chis[0] - obj.CalcMisfit(m0)) <= 1e-12 * chis[0]
     assert chis[1] < chis[0]  # a small step along the negative gradient decreases chi^2
+
+
+def regular_context(kind, cell, n=(48, 44, 16), nsx=12, seed=11):
+    """A mesh of equal cells (cubes or one box shape) under a grid of sites (waves of
+    64 sites cover a patch of the survey each); the far field's second radius is set to 9
+    half-diagonals so that most tiles take the monopole-only loops."""
+    rng = np.random.default_rng(seed)
+    m = ThreeDGravityModel()
+    m.SetMeshSize(*n)
+    m.SetXCellSizes(np.full(n[0], cell[0]))
+    m.SetYCellSizes(np.full(n[1], cell[1]))
+    m.SetZCellSizes(np.full(n[2], cell[2]))
+    m.SetBackgroundThicknesses([15.0, 200.0, 1e4])
+    d = TensorGravityData()
+    xl, yl = n[0] * cell[0], n[1] * cell[1]
+    for i in range(nsx):
+        for j in range(nsx):
+            d.AddMeasurementPoint((i + 0.3) * xl / nsx, (j + 0.6) * yl / nsx,
+                                  -rng.uniform(1, 30))
+    ctx = nat.Context(0)
+    ctx.configure(m, d, FIELD)
+    ctx.set_mixed_far_radius(9.0)
+    return ctx, rng
+
+
+@pytest.mark.parametrize("cell", [(10.0, 10.0, 10.0), (12.0, 8.0, 5.0)])
+@pytest.mark.parametrize("kind", [nat.KIND_SCALAR, nat.KIND_TENSOR, nat.KIND_MAGNETIC,
+                                  nat.KIND_SCALAR_TENSOR])
+@pytest.mark.parametrize("nb", [3, 8])
+def test_mixed_batch_equals_single_regular_mesh(kind, cell, nb):
+    """Cubes take the regrouped monopole loops of both far-field kernels, equal boxes the
+    uniform-mesh multipole; groups of 4, 2 and 1 models must give the single results."""
+    ctx, rng = regular_context(kind, cell)
+    M = models_for(ctx, kind, rng, nb)
+    F = ctx.forward_batch(kind, nat.MIXED, M)
+    R = rng.normal(size=F.shape)
+    G = ctx.adjoint_batch(kind, nat.MIXED, M, R)
+    for b in range(nb):
+        assert np.array_equal(F[b], ctx.forward(kind, nat.MIXED, M[b])), b
+        assert np.array_equal(G[b], ctx.adjoint(kind, nat.MIXED, M[b], R[b])), b
+    # the loops beyond the second radius did run: with the hexadecapole kept everywhere
+    # (radius 0) both the forward and the gradient differ
+    ctx.set_mixed_far_radius(0.0)
+    assert not np.array_equal(F[0], ctx.forward(kind, nat.MIXED, M[0]))
+    assert not np.array_equal(G[0], ctx.adjoint(kind, nat.MIXED, M[0], R[0]))
