@@ -200,14 +200,15 @@ jgm_status jgm_sensitivity_rows(jgm_context *ctx, jgm_kind kind, size_t site0, s
  * prec (JGM_FP64, or JGM_MIXED for the kinds that have it) does not change J. */
 jgm_status jgm_storedj_build(jgm_context *ctx, jgm_kind kind, jgm_precision prec);
 size_t jgm_storedj_rows(const jgm_context *ctx);
-/* Y = J M for nrhs model vectors (M: ncells x nrhs column-major, Y: rows x nrhs
+/* Y = J M for nrhs model vectors (M: cols x nrhs column-major with cols the stored J's
+ * column count, ncells or 3 ncells for the magnetization kind; Y: rows x nrhs
  * column-major; device pointers), batching CalculateRawData (:137-152) over L-BFGS
  * trial models.  JGM_GEMM_FP32 multiplies FP32 operands; JGM_GEMM_BF16 rounds J and M
  * to BF16 (FP32 accumulation); JGM_GEMM_BF16_REFINED adds the products of the BF16
  * remainders (hi*hi + hi*lo + lo*hi), close to FP32 accuracy at BF16 matrix-core rate. */
 jgm_status jgm_storedj_apply(jgm_context *ctx, jgm_gemm_precision prec, const float *M,
                              size_t nrhs, float *Y);
-/* G = J^T R for nrhs residual vectors (R: rows x nrhs, G: ncells x nrhs), the batched
+/* G = J^T R for nrhs residual vectors (R: rows x nrhs, G: cols x nrhs), the batched
  * form of CalculateRawLQDerivative (:214-221) without its factor 2. */
 jgm_status jgm_storedj_apply_t(jgm_context *ctx, jgm_gemm_precision prec, const float *R,
                                size_t nrhs, float *G);

@@ -394,7 +394,7 @@ class Context:
 
     def storedj_apply(self, prec, M, Y):
         """Y = J M for device arrays of FP32 right-hand sides, one per row: M of shape
-        (nrhs, ncells), Y of shape (nrhs, storedj_rows())."""
+        (nrhs, storedj_cols()), Y of shape (nrhs, storedj_rows())."""
         _check_rhs(M, Y, self.storedj_cols(), self.storedj_rows())
         check(self.lib.jgm_storedj_apply(self.handle, prec, _devptr(M), M.shape[0],
                                          _devptr(Y)))
@@ -402,7 +402,7 @@ class Context:
 
     def storedj_apply_t(self, prec, R, G):
         """G = J^T R for device arrays: R of shape (nrhs, storedj_rows()), G of shape
-        (nrhs, ncells)."""
+        (nrhs, storedj_cols())."""
         _check_rhs(R, G, self.storedj_rows(), self.storedj_cols())
         check(self.lib.jgm_storedj_apply_t(self.handle, prec, _devptr(R), R.shape[0],
                                            _devptr(G)))
