@@ -184,7 +184,9 @@ jgm_status jgm_adjoint_batch_host(jgm_context *ctx, jgm_kind kind, jgm_precision
 /* Raw (untransformed) FP64 sensitivity rows of sites [site0, site0+nsites) in the
  * layout FullSensitivityGravMagCalculator::HandleSensitivities builds
  * (jif3D/GravMag/FullSensitivityGravMagCalculator.h:99-114): nsites*raw_per_site rows
- * of n_params doubles, row-major (device). */
+ * of jgm_model_size(kind) doubles, row-major (device); the vertical-gradient kind's
+ * background columns are zero (OMPMagneticGradImp.h:27-34), the magnetization kind's
+ * columns are [Mx; My; Mz]. */
 jgm_status jgm_sensitivity_rows(jgm_context *ctx, jgm_kind kind, size_t site0, size_t nsites,
                                 double *rows);
 

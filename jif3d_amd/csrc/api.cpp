@@ -662,8 +662,6 @@ jgm_status jgm_sensitivity_rows(jgm_context* ctx, jgm_kind kind, size_t site0, s
                                 double* rows) {
   jgm_status st = check_ready(ctx, kind, JGM_FP64);
   if (st != JGM_OK) return st;
-  if (fp64_only(kind))
-    return fail(JGM_ERR_UNSUPPORTED, "no sensitivity rows for kind %d", (int)kind);
   if (!rows && nsites) return fail(JGM_ERR_INVALID_ARGUMENT, "null rows pointer");
   if (site0 > ctx->S || nsites > ctx->S - site0)
     return fail(JGM_ERR_INVALID_ARGUMENT, "site range [%zu, %zu) outside [0, %zu)", site0,

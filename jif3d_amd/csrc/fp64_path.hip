@@ -355,14 +355,45 @@ __global__ __launch_bounds__(kBlock) void k_sensitivity_rows(
     int site0, const double* __restrict__ xlo, const double* __restrict__ ylo,
     const double* __restrict__ zlo, const double* __restrict__ xsz,
     const double* __restrict__ ysz, const double* __restrict__ zsz, int nx, int ny, int nz,
-    const double* __restrict__ bgT, int L, int kind, FieldDir fd, double* __restrict__ rows) {
+    const double* __restrict__ bgT, int L, int kind, FieldDir fd, double mscale,
+    double* __restrict__ rows) {
   constexpr int NC = Ncomp<MASK>::value;
   const long N = (long)nx * ny * nz;
-  const long np = N + L;
+  const long np = kind == JGM_KIND_MAGNETIZATION ? 3 * N : N + L;
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= np) return;
   const int sl = blockIdx.y;  // local site index
   const int s = site0 + sl;
+  if (kind == JGM_KIND_MAGNETIC_GRADIENT || kind == JGM_KIND_MAGNETIZATION) {
+    // OMPMagneticGradImp.cpp:13-106 (no background columns: zeros) and
+    // OMPMagnetizationImp.cpp:25-117 (column p N + cell of component p)
+    double* base = rows + (long)sl * raw_per_site(kind) * np + c;
+    const int p = kind == JGM_KIND_MAGNETIZATION ? (int)(c / N) : 0;
+    const long cell = c - p * N;
+    if (cell >= N) {
+      base[0] = 0.0;
+      return;
+    }
+    const int k = (int)(cell % nz);
+    const long t = cell / nz;
+    const int j = (int)(t % ny);
+    const int i = (int)(t / ny);
+    double v[NC], u[7], w[7];
+    box_terms_f64<MASK>(sx[s], sy[s], sz[s], xlo[i], ylo[j], zlo[k], xsz[i], ysz[j], zsz[k], v);
+    spread_components<MASK>(v, u);  // gz, Uxx, Uyy, Uzz, Uxy, Uxz, Uyz
+    if (kind == JGM_KIND_MAGNETIC_GRADIENT) {
+      box_terms_f64<MASK>(sx[s], sy[s], sz[s] + 1.0, xlo[i], ylo[j], zlo[k], xsz[i], ysz[j],
+                          zsz[k], v);
+      spread_components<MASK>(v, w);
+      base[0] = (u[5] * fd.bx + u[6] * fd.by + u[3] * fd.bz) * mscale -
+                (w[5] * fd.bx + w[6] * fd.by + w[3] * fd.bz) * mscale;
+    } else {  // column p of T = [[xx, xy, xz], [xy, yy, yz], [xz, yz, zz]]
+      base[0] = (p == 0 ? u[1] : p == 1 ? u[4] : u[5]) * mscale;
+      base[np] = (p == 0 ? u[4] : p == 1 ? u[2] : u[6]) * mscale;
+      base[2 * np] = (p == 0 ? u[5] : p == 1 ? u[6] : u[3]) * mscale;
+    }
+    return;
+  }
   double u[7] = {0, 0, 0, 0, 0, 0, 0};
   if (c < N) {
     const int k = (int)(c % nz);
@@ -1064,8 +1095,12 @@ hipError_t launch_adjoint_fp64_batch(jgm_context* ctx, int kind, size_t nsets,
 
 hipError_t launch_sensitivity_rows(jgm_context* ctx, int kind, size_t site0, size_t nsites,
                                    double* rows) {
-  const long np = (long)nparams(ctx);
+  const long np = (long)nparams(ctx, kind);
   const int L = (int)ctx->bgthick.size();
+  // unit factors of the vertical-gradient and magnetization rows (the terms include G)
+  const double mscale = kind == JGM_KIND_MAGNETIZATION
+                            ? magnetization_scale() / kGravConst
+                            : ctx->fieldstrength * (1e-7 / kGravConst);
   dim3 grid(blocks_for(np), (unsigned)nsites);
   const double* a[6] = {ctx->d_xlo.as<double>(), ctx->d_ylo.as<double>(), ctx->d_zlo.as<double>(),
                         ctx->d_xsz.as<double>(), ctx->d_ysz.as<double>(), ctx->d_zsz.as<double>()};
@@ -1073,7 +1108,8 @@ hipError_t launch_sensitivity_rows(jgm_context* ctx, int kind, size_t site0, siz
   hipLaunchKernelGGL((k_sensitivity_rows<M>), grid, dim3(kBlock), 0, ctx->stream,                \
                      ctx->d_sx.as<double>(), ctx->d_sy.as<double>(), ctx->d_sz.as<double>(),     \
                      (int)site0, a[0], a[1], a[2], a[3], a[4], a[5], (int)ctx->nx, (int)ctx->ny, \
-                     (int)ctx->nz, ctx->bgT.as<const double>(), L, kind, field_dir(ctx), rows)
+                     (int)ctx->nz, ctx->bgT.as<const double>(), L, kind, field_dir(ctx), mscale, \
+                     rows)
   switch (mask_of(kind)) {
     case kMaskScalar: JGM_ROWS(kMaskScalar); break;
     case kMaskAll: JGM_ROWS(kMaskAll); break;

@@ -131,7 +131,7 @@ class HipGravMagImplementation:
         ctx = self.context(model, data)
         nsites = data.nsites - site0 if nsites is None else nsites
         k = self.RawDataPerMeasurement()
-        out = ctx.alloc((nsites * k, ctx.n_params()))
+        out = ctx.alloc((nsites * k, ctx.model_size(self.kind)))
         ctx.sensitivity_rows(self.kind, site0, nsites, out)
         ctx.synchronize()
         host = out.to_host()

@@ -139,9 +139,5 @@ def test_fp64_only_kinds_reject_other_paths():
         with pytest.raises(nat.JgmError) as e:  # the stored J is built from the FP64 path
             ctx.storedj_build(kind, nat.MIXED)
         assert e.value.status == nat.JGM_ERR_UNSUPPORTED
-        rows = ctx.alloc((1, ctx.model_size(kind)))
-        with pytest.raises(nat.JgmError) as e:  # no reference-layout row writer for these
-            ctx.sensitivity_rows(kind, 0, 1, rows)
-        assert e.value.status == nat.JGM_ERR_UNSUPPORTED
     assert ctx.model_size(nat.KIND_MAGNETIZATION) == 3 * m.ncells
     assert ctx.model_size(nat.KIND_MAGNETIC_GRADIENT) == m.ncells + 2

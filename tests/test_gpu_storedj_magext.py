@@ -161,3 +161,30 @@ def test_calculators(kind, calc_cls):
     for b in range(len(trials)):
         assert np.array_equal(F[b], cached[b][0])
         assert np.array_equal(G[b], cached[b][2])
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_sensitivity_rows_match_oracle(orc, kind):
+    """FP64 rows in the reference layout: 1 row per site of ncells + nbg columns (zero
+    background columns) or 3 rows of 3 ncells columns, against the oracle's matrix."""
+    m, d = make_model(kind, 12)
+    ctx = nat.Context(0)
+    ctx.configure(m, d, FIELD)
+    k = 3 if kind == nat.KIND_MAGNETIZATION else 1
+    n = ctx.model_size(kind)
+    s0, ns = 4, 9
+    rows = ctx.alloc((ns * k, n))
+    ctx.sensitivity_rows(kind, s0, ns, rows)
+    ctx.synchronize()
+    got = rows.to_host()
+    _, ref = orc.calculate(oracle_kind(orc, kind), m, m.GetModelParameters(), d.positions,
+                           field=FIELD, want_sens=True)
+    ref = ref[s0 * k:(s0 + ns) * k]
+    assert got.shape == ref.shape
+    if kind == nat.KIND_MAGNETIC_GRADIENT:
+        assert np.all(got[:, m.ncells:] == 0.0)
+    assert np.max(np.abs(got - ref)) <= 1e-11 * np.max(np.abs(ref))
+    # the rows reproduce the forward: J m
+    mv = m.GetModelParameters()
+    fwd = ctx.forward(kind, nat.FP64, mv)[s0 * k:(s0 + ns) * k]
+    assert np.max(np.abs(got @ mv - fwd)) <= 1e-11 * np.max(np.abs(fwd))
