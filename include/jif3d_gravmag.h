@@ -51,7 +51,8 @@ typedef enum jgm_status {
  * no background).  JGM_KIND_MAGNETIZATION: Bx, By, Bz of a magnetization model with
  * three values per cell, [Mx; My; Mz], B = 1e9 mu0/(4 pi G) T M (OMPMagnetizationImp,
  * jif3D/Magnetics/OMPMagnetizationImp.cpp:25-117; no background, model length
- * 3 ncells).  These two kinds run in JGM_FP64 only. */
+ * 3 ncells; JGM_MIXED evaluates the tensor far field once per site-cell pair for the
+ * three components).  The vertical-gradient kind runs in JGM_FP64 only. */
 typedef enum jgm_kind {
   JGM_KIND_SCALAR = 1,
   JGM_KIND_TENSOR = 2,
@@ -166,7 +167,8 @@ jgm_status jgm_adjoint_host(jgm_context *ctx, jgm_kind kind, jgm_precision prec,
  * batch of 8 costs little more than one model; with JGM_MIXED up to 4 models share each
  * far-field multipole evaluation.  Each result is bit-identical to the single-model
  * call.  The vertical-gradient and magnetization kinds run the models one after
- * another.  nmodels == 0 is a no-op. */
+ * another (the magnetization kind's three components share the far-field terms within
+ * each model).  nmodels == 0 is a no-op. */
 jgm_status jgm_forward_batch(jgm_context *ctx, jgm_kind kind, jgm_precision prec, size_t nmodels,
                              const double *models, size_t model_ld, double *out, size_t out_ld);
 jgm_status jgm_adjoint_batch(jgm_context *ctx, jgm_kind kind, jgm_precision prec, size_t nmodels,

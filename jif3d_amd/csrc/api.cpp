@@ -52,10 +52,8 @@ bool kind_ok(int kind) {
          kind == JGM_KIND_MAGNETIZATION;
 }
 
-// the vertical-gradient and magnetization kinds have FP64 kernels only
-bool fp64_only(int kind) {
-  return kind == JGM_KIND_MAGNETIC_GRADIENT || kind == JGM_KIND_MAGNETIZATION;
-}
+// the vertical-gradient kind has FP64 kernels only
+bool fp64_only(int kind) { return kind == JGM_KIND_MAGNETIC_GRADIENT; }
 
 jgm_status check_ready(const jgm_context* ctx, int kind, int prec) {
   if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
@@ -191,7 +189,7 @@ jgm_status jgm_destroy(jgm_context* ctx) {
                                &ctx->near.site_order, &ctx->near.xoff,
                                &ctx->near.yoff, &ctx->near.zoff, &ctx->acc64,
                                &ctx->raw_cache_model, &ctx->raw_cache_raw, &ctx->raw_cache_flag,
-                               &ctx->partsum, &ctx->chunk_scratch};
+                               &ctx->partsum, &ctx->chunk_scratch, &ctx->mag_part};
   for (auto* b : bufs) b->release();
   jgm::release_host_storedj(ctx);
   if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);

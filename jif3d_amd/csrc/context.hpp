@@ -143,7 +143,7 @@ struct jgm_context {
   // scratch
   jgm::DeviceBuffer weights, partial, raw, coef, hpart, hsum, model_stage, out_stage, misfit_stage,
       grad_stage, farpart, nearpart, coef32, coef_mono, near_h, near_v, vsites, partsum,
-      chunk_scratch;
+      chunk_scratch, mag_part;
 
   jgm::Profiler prof;
 

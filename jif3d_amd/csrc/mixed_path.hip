@@ -542,6 +542,60 @@ __global__ __launch_bounds__(kBlock) void k_mixed_forward_finish(
   finish_site(u, s, bgT, bgvals, L, kind, fd, tf, tfbx, tfby, tfbz, out, raw_out);
 }
 
+// Near plus far sums of one weight set, kept per site (component order of MASK).
+template <int MASK>
+__global__ __launch_bounds__(kBlock) void k_mixed_forward_sums(const double* __restrict__ farpart,
+                                                               int nchunks,
+                                                               const double* __restrict__ nearout,
+                                                               int S, double* __restrict__ sums) {
+  constexpr int NC = Ncomp<MASK>::value;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+#pragma unroll
+  for (int m = 0; m < NC; ++m) {
+    double a = nearout[(long)s * NC + m];
+    for (int c = 0; c < nchunks; ++c) a += farpart[((long)c * NC + m) * S + s];
+    sums[(long)s * NC + m] = a;
+  }
+}
+
+// Magnetization (OMPMagnetizationImp.cpp:25-117): B_i = scale sum_j T_ij M_j from the
+// tensor sums U^j of the three weight sets Mx, My, Mz (sums[j][s][6], tensor order
+// xx, yy, zz, xy, xz, yz), then the data transform.
+__global__ __launch_bounds__(kBlock) void k_mag3_mixed_finish(const double* __restrict__ sums,
+                                                              int S, double scale, int tf,
+                                                              double tfbx, double tfby,
+                                                              double tfbz,
+                                                              double* __restrict__ out,
+                                                              double* __restrict__ raw_out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  double u[3][7];
+  for (int j = 0; j < 3; ++j) spread_components<kMaskTensor>(sums + ((long)j * S + s) * 6, u[j]);
+  // u[.]: gz, Uxx, Uyy, Uzz, Uxy, Uxz, Uyz
+  double raw[3];
+  raw[0] = scale * (u[0][1] + u[1][4] + u[2][5]);
+  raw[1] = scale * (u[0][4] + u[1][2] + u[2][6]);
+  raw[2] = scale * (u[0][5] + u[1][6] + u[2][3]);
+  store_site(raw, 3, s, tf, tfbx, tfby, tfbz, out, raw_out);
+}
+
+// Magnetization adjoint as three tensor adjoints: dB_i / dM_j = scale T_ij, so the
+// gradient of M_j is the tensor gradient of the raw tensor misfit R^j with
+// R^j_(i, j) = scale r_i and zeros elsewhere (raw tensor order xx, xy, xz, yx, ...).
+__global__ __launch_bounds__(kBlock) void k_mag3_tensor_misfit(const double* __restrict__ rm,
+                                                               int S, double scale,
+                                                               double* __restrict__ rmt) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= 27L * S) return;
+  const int j = (int)(idx / (9L * S));
+  const long rest = idx - (long)j * 9 * S;
+  const int s = (int)(rest / 9);
+  const int q = (int)(rest % 9);
+  const int i = q / 3;
+  rmt[idx] = (q % 3 == j) ? scale * rm[(long)s * 3 + i] : 0.0;
+}
+
 // Adjoint coefficients: coef (FP64, with the sign and G of each corner sum, for
 // the near pass) and coef32 (FP32 physical values, for the far pass).
 template <int MASK>
@@ -1001,9 +1055,11 @@ void launch_far_forward(int nb, dim3 grid, hipStream_t st, Args... args) {
 // out + b * out_ld, raw data at raw_out + b * raw_ld when given): one far-field pass for
 // all of them, the FP64 near field and the epilogue per model.
 template <int MASK, int MESH>
+// With sums non-null the near plus far sums of weight set b go to sums + b S NC and no
+// data are formed (the magnetization kind combines three sets).
 hipError_t forward_mixed_impl(jgm_context* ctx, int kind, int nbm, const double* models,
                               size_t model_ld, double* out, size_t out_ld, double* raw_out,
-                              size_t raw_ld) {
+                              size_t raw_ld, double* sums) {
   constexpr int NC = Ncomp<MASK>::value;
   hipStream_t st = ctx->stream;
   const int S = (int)ctx->S;
@@ -1066,12 +1122,18 @@ hipError_t forward_mixed_impl(jgm_context* ctx, int kind, int nbm, const double*
         (e = launch_reduce_chunks(ctx, ctx->farpart.as<const double>() + (size_t)b * nchunks * NC * S,
                                   nchunks, (long)NC * S, ctx->partsum.as<double>())) != hipSuccess)
       return e;
-    hipLaunchKernelGGL((k_mixed_forward_finish<MASK>), dim3(blocks_for(S)), dim3(kBlock), 0, st,
-                       ctx->partsum.as<const double>(), nchunks > 0 ? 1 : 0,
-                       ctx->nearpart.as<const double>() + (size_t)b * S * NC, S,
-                       ctx->bgT.as<const double>(), models + b * model_ld + N, L, kind,
-                       field_dir(ctx), ctx->transform, tf_bx, tf_by, tf_bz, out + b * out_ld,
-                       raw_out ? raw_out + b * raw_ld : nullptr);
+    if (sums)
+      hipLaunchKernelGGL((k_mixed_forward_sums<MASK>), dim3(blocks_for(S)), dim3(kBlock), 0, st,
+                         ctx->partsum.as<const double>(), nchunks > 0 ? 1 : 0,
+                         ctx->nearpart.as<const double>() + (size_t)b * S * NC, S,
+                         sums + (size_t)b * S * NC);
+    else
+      hipLaunchKernelGGL((k_mixed_forward_finish<MASK>), dim3(blocks_for(S)), dim3(kBlock), 0,
+                         st, ctx->partsum.as<const double>(), nchunks > 0 ? 1 : 0,
+                         ctx->nearpart.as<const double>() + (size_t)b * S * NC, S,
+                         ctx->bgT.as<const double>(), models + b * model_ld + N, L, kind,
+                         field_dir(ctx), ctx->transform, tf_bx, tf_by, tf_bz, out + b * out_ld,
+                         raw_out ? raw_out + b * raw_ld : nullptr);
   }
   return hipGetLastError();
 }
@@ -1090,8 +1152,9 @@ void launch_far_adjoint(int nbm, dim3 grid, hipStream_t st, Args... args) {
 
 // Gradients of nbm (1, 2 or 4) misfit vectors rm + b rm_ld into grad + b grad_ld.
 template <int MASK, int MESH>
+// with_bg false: no background-layer entries after the cells (magnetization kind)
 hipError_t adjoint_mixed_impl(jgm_context* ctx, int kind, int nbm, const double* rm,
-                              size_t rm_ld, double* grad, size_t grad_ld) {
+                              size_t rm_ld, double* grad, size_t grad_ld, bool with_bg) {
   constexpr int NC = Ncomp<MASK>::value;
   hipStream_t st = ctx->stream;
   const int S = (int)ctx->S;
@@ -1162,7 +1225,7 @@ hipError_t adjoint_mixed_impl(jgm_context* ctx, int kind, int nbm, const double*
                          ctx->hsum.as<const double>(), N, grad + b * grad_ld);
     }
   }
-  if (!ctx->bgthick.empty()) {
+  if (with_bg && !ctx->bgthick.empty()) {
     for (int b = 0; b < nbm; ++b) {
       e = launch_background_gradient(ctx, kind, rm + b * rm_ld, grad + b * grad_ld + N);
       if (e != hipSuccess) return e;
@@ -1174,31 +1237,78 @@ hipError_t adjoint_mixed_impl(jgm_context* ctx, int kind, int nbm, const double*
 template <int MASK>
 hipError_t forward_dispatch(jgm_context* ctx, int kind, int nbm, const double* models,
                             size_t model_ld, double* out, size_t out_ld, double* raw_out,
-                            size_t raw_ld) {
+                            size_t raw_ld, double* sums = nullptr) {
   switch (ctx->near.mesh_class) {
     case kMeshCubes:
       return forward_mixed_impl<MASK, kMeshCubes>(ctx, kind, nbm, models, model_ld, out, out_ld,
-                                                  raw_out, raw_ld);
+                                                  raw_out, raw_ld, sums);
     case kMeshUniform:
       return forward_mixed_impl<MASK, kMeshUniform>(ctx, kind, nbm, models, model_ld, out, out_ld,
-                                                    raw_out, raw_ld);
+                                                    raw_out, raw_ld, sums);
     default:
       return forward_mixed_impl<MASK, kMeshGeneral>(ctx, kind, nbm, models, model_ld, out, out_ld,
-                                                    raw_out, raw_ld);
+                                                    raw_out, raw_ld, sums);
   }
 }
 
 template <int MASK>
 hipError_t adjoint_dispatch(jgm_context* ctx, int kind, int nbm, const double* rm, size_t rm_ld,
-                            double* grad, size_t grad_ld) {
+                            double* grad, size_t grad_ld, bool with_bg = true) {
   switch (ctx->near.mesh_class) {
     case kMeshCubes:
-      return adjoint_mixed_impl<MASK, kMeshCubes>(ctx, kind, nbm, rm, rm_ld, grad, grad_ld);
+      return adjoint_mixed_impl<MASK, kMeshCubes>(ctx, kind, nbm, rm, rm_ld, grad, grad_ld,
+                                                  with_bg);
     case kMeshUniform:
-      return adjoint_mixed_impl<MASK, kMeshUniform>(ctx, kind, nbm, rm, rm_ld, grad, grad_ld);
+      return adjoint_mixed_impl<MASK, kMeshUniform>(ctx, kind, nbm, rm, rm_ld, grad, grad_ld,
+                                                    with_bg);
     default:
-      return adjoint_mixed_impl<MASK, kMeshGeneral>(ctx, kind, nbm, rm, rm_ld, grad, grad_ld);
+      return adjoint_mixed_impl<MASK, kMeshGeneral>(ctx, kind, nbm, rm, rm_ld, grad, grad_ld,
+                                                    with_bg);
   }
+}
+
+// 1e9 mu0 / (4 pi G): magnetization in A/m to nT through the tensor terms (which carry G)
+double magnetization_unit() {
+  const double mu0 = 4e-7 * kPi;  // OMPMagnetizationImp.cpp:35
+  return 1.0e9 * mu0 / (4 * kPi * kGravConst);
+}
+
+// Magnetization kind in mixed precision: the tensor far and near field of the weight
+// sets Mx, My (a group of two) and Mz share each pair's multipole terms.
+hipError_t magnetization_forward_mixed(jgm_context* ctx, const double* model, double* out,
+                                       double* raw_out) {
+  const int S = (int)ctx->S;
+  const size_t N = ncells(ctx);
+  hipError_t e = ctx->mag_part.ensure(sizeof(double) * 3 * 6 * std::max(1, S));
+  if (e != hipSuccess) return e;
+  double* sums = ctx->mag_part.as<double>();
+  e = forward_dispatch<kMaskTensor>(ctx, JGM_KIND_TENSOR, 2, model, N, nullptr, 0, nullptr, 0,
+                                    sums);
+  if (e != hipSuccess) return e;
+  e = forward_dispatch<kMaskTensor>(ctx, JGM_KIND_TENSOR, 1, model + 2 * N, N, nullptr, 0,
+                                    nullptr, 0, sums + 2 * 6 * (size_t)S);
+  if (e != hipSuccess) return e;
+  const double tf_bx = std::cos(ctx->tf_inc) * std::cos(ctx->tf_dec);
+  const double tf_by = std::cos(ctx->tf_inc) * std::sin(ctx->tf_dec);
+  const double tf_bz = std::sin(ctx->tf_inc);
+  hipLaunchKernelGGL(k_mag3_mixed_finish, dim3(blocks_for(S)), dim3(kBlock), 0, ctx->stream,
+                     ctx->mag_part.as<const double>(), S, magnetization_unit(), ctx->transform,
+                     tf_bx, tf_by, tf_bz, out, raw_out);
+  return hipGetLastError();
+}
+
+hipError_t magnetization_adjoint_mixed(jgm_context* ctx, const double* rm, double* grad) {
+  const int S = (int)ctx->S;
+  const size_t N = ncells(ctx);
+  hipError_t e = ctx->mag_part.ensure(sizeof(double) * 27 * std::max(1, S));
+  if (e != hipSuccess) return e;
+  double* rmt = ctx->mag_part.as<double>();
+  hipLaunchKernelGGL(k_mag3_tensor_misfit, dim3(blocks_for(27L * S)), dim3(kBlock), 0,
+                     ctx->stream, rm, S, magnetization_unit(), rmt);
+  e = adjoint_dispatch<kMaskTensor>(ctx, JGM_KIND_TENSOR, 2, rmt, 9 * (size_t)S, grad, N, false);
+  if (e != hipSuccess) return e;
+  return adjoint_dispatch<kMaskTensor>(ctx, JGM_KIND_TENSOR, 1, rmt + 18 * (size_t)S,
+                                       9 * (size_t)S, grad + 2 * N, N, false);
 }
 
 }  // namespace
@@ -1335,6 +1445,14 @@ hipError_t launch_forward_mixed_batch(jgm_context* ctx, int kind, size_t nmodels
     hipError_t e = build_near_boxes(ctx);
     if (e != hipSuccess) return e;
   }
+  if (kind == JGM_KIND_MAGNETIZATION) {  // three weight sets per model
+    for (size_t b = 0; b < nmodels; ++b) {
+      hipError_t e = magnetization_forward_mixed(ctx, models + b * model_ld, out + b * out_ld,
+                                                 raw_out ? raw_out + b * raw_ld : nullptr);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
   for (size_t b0 = 0; b0 < nmodels;) {
     const size_t left = nmodels - b0;
     const int nbm = left >= 4 ? 4 : left >= 2 ? 2 : 1;
@@ -1370,6 +1488,14 @@ hipError_t launch_adjoint_mixed_batch(jgm_context* ctx, int kind, size_t nmodels
   if (!ctx->near.valid) {
     hipError_t e = build_near_boxes(ctx);
     if (e != hipSuccess) return e;
+  }
+  if (kind == JGM_KIND_MAGNETIZATION) {
+    for (size_t b = 0; b < nmodels; ++b) {
+      hipError_t e =
+          magnetization_adjoint_mixed(ctx, raw_misfits + b * rm_ld, grads + b * grad_ld);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
   }
   for (size_t b0 = 0; b0 < nmodels;) {
     const size_t left = nmodels - b0;

@@ -179,11 +179,12 @@ class HipMagneticGradImp(HipGravMagImplementation):
 
 class HipMagnetizationImp(HipGravMagImplementation):
     """OMPMagnetizationImp (jif3D/Magnetics/OMPMagnetizationImp.cpp:25-125): Bx, By, Bz
-    of a ThreeDMagnetizationModel (model vector [Mx; My; Mz], no background); FP64."""
+    of a ThreeDMagnetizationModel (model vector [Mx; My; Mz], no background).  FP64 by
+    default; nat.MIXED runs the tensor far field once per pair for the three components."""
     kind = nat.KIND_MAGNETIZATION
 
-    def __init__(self, device=0, stream=None):
-        super().__init__(nat.FP64, device, stream)
+    def __init__(self, precision=nat.FP64, device=0, stream=None):
+        super().__init__(precision, device, stream)
 
 
 class ThreeDGravMagCalculator:
