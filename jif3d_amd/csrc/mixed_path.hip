@@ -401,14 +401,18 @@ __device__ __forceinline__ void far_tile_mono(const FarGeom& g, const float* __r
 // per tile; cell offsets within a tile are exact for uniform meshes.  NB models share
 // every pair's multipole terms; model b's sums go to
 // farpart[((b * nchunks + chunk) * NC + m) * S + s].
-// One model: at least six waves per SIMD (at most 80 VGPRs; the few spills stay outside the
-// cell loops), measured faster than the five that 93 VGPRs allow and than seven or eight,
-// whose spills reach the loops.
-constexpr int far_waves_per_simd(int nb) { return nb == 1 ? 6 : 1; }
+// Minimum waves per SIMD the far-field kernels are built for, by models per launch (1:
+// the register count decides).  One model: six (at most 80 VGPRs; the few spills stay
+// outside the cell loops), measured faster than the five that 93 VGPRs allow and than
+// seven or eight, whose spills reach the loops.  Two models in the forward and four in
+// the adjoint: four waves (2.12 -> 1.96 and 1.35 -> 1.28 ms per model at 128x128x64
+// cells, 4096 sites); more waves for the others were no faster or spilled in the loops.
+constexpr int far_fwd_waves(int nb) { return nb == 1 ? 6 : nb == 2 ? 4 : 1; }
+constexpr int far_adj_waves(int nb) { return nb == 1 ? 6 : nb == 4 ? 4 : 1; }
 
 template <int MASK, int MESH, int NB>
 __global__ __launch_bounds__(kBlock)
-__attribute__((amdgpu_waves_per_eu(far_waves_per_simd(NB)))) void k_far_forward(
+__attribute__((amdgpu_waves_per_eu(far_fwd_waves(NB)))) void k_far_forward(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     int S, const int* __restrict__ order, const int* __restrict__ box, float r1sq, FarGeom g,
     const float* __restrict__ shape, const float* __restrict__ wt, MpConst uc, long ntiles,
@@ -708,7 +712,7 @@ __device__ __forceinline__ void far_adjoint_site(
 // sums at gpart[(b nchunks + chunk) ncell + c].
 template <int MASK, int MESH, int NB>
 __global__ __launch_bounds__(kBlock)
-__attribute__((amdgpu_waves_per_eu(far_waves_per_simd(NB)))) void k_far_adjoint(
+__attribute__((amdgpu_waves_per_eu(far_adj_waves(NB)))) void k_far_adjoint(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     const float* __restrict__ coef, const float* __restrict__ mono, const int* __restrict__ box,
     float r1sq, int S, int sites_per_chunk, FarGeom g, const float* __restrict__ table,
