@@ -192,3 +192,72 @@ def test_far_radius_settings():
     ctx = nat.Context(0)
     with pytest.raises(nat.JgmError):
         ctx.set_mixed_far_radius(1.0)
+
+
+def cube_mesh(n, nz, h=20.0):
+    m = ThreeDGravityModel()
+    m.SetMeshSize(n, n, nz)
+    for f, k in zip((m.SetXCellSizes, m.SetYCellSizes, m.SetZCellSizes), (n, n, nz)):
+        f(np.full(k, h))
+    return m
+
+
+def test_mixed_sites_inside_the_mesh():
+    """Borehole sites: inside cells, on cell faces, edges and corners.  The cells around
+    such a site fall in its FP64 near box; the far field sees only distant cells.  At a
+    site on a mesh line the reference's guarded log gives every cell along that line an
+    off-diagonal artifact (tests/test_oracle_guard.py) that the FP64 path reproduces and
+    the far-field expansion does not, so there only gz and the diagonal are compared."""
+    m = cube_mesh(48, 24)
+    rng = np.random.default_rng(21)
+    m.SetDensities(2000.0 + 800.0 * rng.random(m.ncells))
+    d = TensorGravityData()
+    for _ in range(250):
+        d.AddMeasurementPoint(rng.uniform(0, 960), rng.uniform(0, 960), rng.uniform(1, 470))
+    for p in [(200.0, 300.0, 100.0), (200.0, 310.0, 100.0), (210.0, 310.0, 120.0),
+              (480.0, 480.0, 240.0), (0.0, 0.0, 0.0), (960.0, 960.0, 480.0)]:
+        d.AddMeasurementPoint(*p)  # corner, edge, face, centre line, mesh corners
+    ctx = nat.Context(0)
+    ctx.configure(m, d)
+    K = nat.KIND_SCALAR_TENSOR
+    mv = m.GetModelParameters()
+    f64 = ctx.forward(K, nat.FP64, mv).reshape(-1, 10)
+    mix = ctx.forward(K, nat.MIXED, mv).reshape(-1, 10)
+    assert np.all(np.isfinite(f64)) and np.all(np.isfinite(mix))
+    scale = np.abs(f64).max(axis=0)
+    err = np.abs(mix - f64) / scale
+    assert err[:250].max() <= TOL  # sites off the mesh lines: every component
+    gz_diag = [0, 1, 5, 9]  # gz, Uxx, Uyy, Uzz
+    assert err[250:, gz_diag].max() <= TOL
+    r = rng.normal(size=f64.size)
+    r[2500:] = 0.0  # the gradient of the sites off the mesh lines
+    g64 = ctx.adjoint(K, nat.FP64, mv, r)
+    gmx = ctx.adjoint(K, nat.MIXED, mv, r)
+    assert rel_inf(gmx, g64) <= TOL
+
+
+def test_mixed_large_ragged_survey():
+    """20011 sites (not a multiple of the 64-lane wave) over 64x64x16 cubes: forward and
+    gradient against FP64, and a batch of two equal to single calls."""
+    m = cube_mesh(64, 16)
+    rng = np.random.default_rng(22)
+    m.SetDensities(2500.0 + 300.0 * rng.standard_normal(m.ncells))
+    d = TensorGravityData()
+    for _ in range(20011):
+        d.AddMeasurementPoint(rng.uniform(-100, 1380), rng.uniform(-100, 1380),
+                              -rng.uniform(1, 50))
+    ctx = nat.Context(0)
+    ctx.configure(m, d)
+    K = nat.KIND_SCALAR_TENSOR
+    mv = m.GetModelParameters()
+    f64 = ctx.forward(K, nat.FP64, mv)
+    mix = ctx.forward(K, nat.MIXED, mv)
+    assert comp_err(mix, f64, 10) <= TOL
+    r = rng.normal(size=f64.size)
+    g64 = ctx.adjoint(K, nat.FP64, mv, r)
+    gmx = ctx.adjoint(K, nat.MIXED, mv, r)
+    assert rel_inf(gmx, g64) <= TOL
+    M = np.stack([mv, mv * (1.0 + 0.01 * rng.standard_normal(mv.size))])
+    F = ctx.forward_batch(K, nat.MIXED, M)
+    assert np.array_equal(F[0], mix)
+    assert np.array_equal(F[1], ctx.forward(K, nat.MIXED, M[1]))
