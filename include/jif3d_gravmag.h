@@ -136,7 +136,8 @@ size_t jgm_model_size(const jgm_context *ctx, jgm_kind kind);
  * (jif3D/GravMag/ThreeDGravMagImplementation.h:116-168) with the OMP imps'
  * CalcGridded/CalcBackground (jif3D/Gravity/ScalarOMPGravityImp.cpp:124-180,
  * TensorOMPGravityImp.cpp:112-182, OMPMagneticSusceptibilityImp.cpp:31-194).
- * model: n_params doubles (device); out: nsites * data_per_site doubles (device). */
+ * model: jgm_model_size(kind) doubles (device); out: nsites * data_per_site doubles
+ * (device). */
 jgm_status jgm_forward(jgm_context *ctx, jgm_kind kind, jgm_precision prec, const double *model,
                        double *out);
 jgm_status jgm_forward_host(jgm_context *ctx, jgm_kind kind, jgm_precision prec,
@@ -144,7 +145,8 @@ jgm_status jgm_forward_host(jgm_context *ctx, jgm_kind kind, jgm_precision prec,
 
 /* Least-squares derivative 2 J^T D^T misfit, matrix free (J recomputed, never stored):
  * ThreeDGravMagImplementation::LQDerivative (:177-231).  model is needed for the
- * transform Jacobian D.  misfit: nsites * data_per_site; grad: n_params (device).
+ * transform Jacobian D.  misfit: nsites * data_per_site; grad: jgm_model_size(kind)
+ * doubles (device).
  * With a transform set, jgm_forward keeps its raw data and a copy of its model, and
  * jgm_adjoint uses them when its model is bitwise equal to that copy and no setting
  * has changed since (it then waits for the context's stream once, to read the
