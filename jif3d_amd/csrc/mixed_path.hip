@@ -1046,21 +1046,32 @@ hipError_t ensure_shape_table(jgm_context* ctx) {
   return hipGetLastError();
 }
 
+// nb in {1, 2, 4}, or 3 with the tensor mask (the magnetization kind's Mx, My, Mz)
 template <int MASK, int MESH, class... Args>
 void launch_far_forward(int nb, dim3 grid, hipStream_t st, Args... args) {
   switch (nb) {
-    case 4: hipLaunchKernelGGL((k_far_forward<MASK, MESH, 4>), grid, dim3(kBlock), 0, st, args...); break;
-    case 2: hipLaunchKernelGGL((k_far_forward<MASK, MESH, 2>), grid, dim3(kBlock), 0, st, args...); break;
-    default: hipLaunchKernelGGL((k_far_forward<MASK, MESH, 1>), grid, dim3(kBlock), 0, st, args...); break;
+    case 4:
+      hipLaunchKernelGGL((k_far_forward<MASK, MESH, 4>), grid, dim3(kBlock), 0, st, args...);
+      break;
+    case 3:
+      if constexpr (MASK == kMaskTensor)
+        hipLaunchKernelGGL((k_far_forward<MASK, MESH, 3>), grid, dim3(kBlock), 0, st, args...);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_far_forward<MASK, MESH, 2>), grid, dim3(kBlock), 0, st, args...);
+      break;
+    default:
+      hipLaunchKernelGGL((k_far_forward<MASK, MESH, 1>), grid, dim3(kBlock), 0, st, args...);
+      break;
   }
 }
 
-// Forward of nb in {1, 2, 4} models (model b at models + b * model_ld; data at
-// out + b * out_ld, raw data at raw_out + b * raw_ld when given): one far-field pass for
-// all of them, the FP64 near field and the epilogue per model.
+// Forward of nb models (model b at models + b * model_ld; data at out + b * out_ld, raw
+// data at raw_out + b * raw_ld when given): one far-field pass for all of them, the FP64
+// near field and the epilogue per model.  With sums non-null the near plus far sums of
+// weight set b go to sums + b S NC and no data are formed (the magnetization kind
+// combines its three sets).
 template <int MASK, int MESH>
-// With sums non-null the near plus far sums of weight set b go to sums + b S NC and no
-// data are formed (the magnetization kind combines three sets).
 hipError_t forward_mixed_impl(jgm_context* ctx, int kind, int nbm, const double* models,
                               size_t model_ld, double* out, size_t out_ld, double* raw_out,
                               size_t raw_ld, double* sums) {
@@ -1142,21 +1153,28 @@ hipError_t forward_mixed_impl(jgm_context* ctx, int kind, int nbm, const double*
   return hipGetLastError();
 }
 
+// nbm in {1, 2, 4}, or 3 with the tensor mask (the magnetization kind)
 template <int MASK, int MESH, class... Args>
 void launch_far_adjoint(int nbm, dim3 grid, hipStream_t st, Args... args) {
   switch (nbm) {
-    case 4: hipLaunchKernelGGL((k_far_adjoint<MASK, MESH, 4>), grid, dim3(kBlock), 0, st, args...);
+    case 4:
+      hipLaunchKernelGGL((k_far_adjoint<MASK, MESH, 4>), grid, dim3(kBlock), 0, st, args...);
       break;
-    case 2: hipLaunchKernelGGL((k_far_adjoint<MASK, MESH, 2>), grid, dim3(kBlock), 0, st, args...);
+    case 3:
+      if constexpr (MASK == kMaskTensor)
+        hipLaunchKernelGGL((k_far_adjoint<MASK, MESH, 3>), grid, dim3(kBlock), 0, st, args...);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_far_adjoint<MASK, MESH, 2>), grid, dim3(kBlock), 0, st, args...);
       break;
     default:
       hipLaunchKernelGGL((k_far_adjoint<MASK, MESH, 1>), grid, dim3(kBlock), 0, st, args...);
   }
 }
 
-// Gradients of nbm (1, 2 or 4) misfit vectors rm + b rm_ld into grad + b grad_ld.
+// Gradients of nbm misfit vectors rm + b rm_ld into grad + b grad_ld; with_bg false: no
+// background-layer entries after the cells (magnetization kind).
 template <int MASK, int MESH>
-// with_bg false: no background-layer entries after the cells (magnetization kind)
 hipError_t adjoint_mixed_impl(jgm_context* ctx, int kind, int nbm, const double* rm,
                               size_t rm_ld, double* grad, size_t grad_ld, bool with_bg) {
   constexpr int NC = Ncomp<MASK>::value;
@@ -1277,8 +1295,8 @@ double magnetization_unit() {
   return 1.0e9 * mu0 / (4 * kPi * kGravConst);
 }
 
-// Magnetization kind in mixed precision: the tensor far and near field of the weight
-// sets Mx, My (a group of two) and Mz share each pair's multipole terms.
+// Magnetization kind in mixed precision: the tensor far field of the weight sets Mx, My,
+// Mz (one group of three) shares each pair's multipole terms; near field per set.
 hipError_t magnetization_forward_mixed(jgm_context* ctx, const double* model, double* out,
                                        double* raw_out) {
   const int S = (int)ctx->S;
@@ -1286,11 +1304,8 @@ hipError_t magnetization_forward_mixed(jgm_context* ctx, const double* model, do
   hipError_t e = ctx->mag_part.ensure(sizeof(double) * 3 * 6 * std::max(1, S));
   if (e != hipSuccess) return e;
   double* sums = ctx->mag_part.as<double>();
-  e = forward_dispatch<kMaskTensor>(ctx, JGM_KIND_TENSOR, 2, model, N, nullptr, 0, nullptr, 0,
+  e = forward_dispatch<kMaskTensor>(ctx, JGM_KIND_TENSOR, 3, model, N, nullptr, 0, nullptr, 0,
                                     sums);
-  if (e != hipSuccess) return e;
-  e = forward_dispatch<kMaskTensor>(ctx, JGM_KIND_TENSOR, 1, model + 2 * N, N, nullptr, 0,
-                                    nullptr, 0, sums + 2 * 6 * (size_t)S);
   if (e != hipSuccess) return e;
   const double tf_bx = std::cos(ctx->tf_inc) * std::cos(ctx->tf_dec);
   const double tf_by = std::cos(ctx->tf_inc) * std::sin(ctx->tf_dec);
@@ -1309,10 +1324,8 @@ hipError_t magnetization_adjoint_mixed(jgm_context* ctx, const double* rm, doubl
   double* rmt = ctx->mag_part.as<double>();
   hipLaunchKernelGGL(k_mag3_tensor_misfit, dim3(blocks_for(27L * S)), dim3(kBlock), 0,
                      ctx->stream, rm, S, magnetization_unit(), rmt);
-  e = adjoint_dispatch<kMaskTensor>(ctx, JGM_KIND_TENSOR, 2, rmt, 9 * (size_t)S, grad, N, false);
-  if (e != hipSuccess) return e;
-  return adjoint_dispatch<kMaskTensor>(ctx, JGM_KIND_TENSOR, 1, rmt + 18 * (size_t)S,
-                                       9 * (size_t)S, grad + 2 * N, N, false);
+  return adjoint_dispatch<kMaskTensor>(ctx, JGM_KIND_TENSOR, 3, rmt, 9 * (size_t)S, grad, N,
+                                       false);
 }
 
 }  // namespace
