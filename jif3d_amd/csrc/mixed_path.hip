@@ -898,27 +898,41 @@ __global__ __launch_bounds__(kBlock) void k_near_adjoint_site(
   }
 }
 
-// Adjoint near field, pass 2: one block per listed column (i, j); each cell sums
-// the values of the sites whose box holds it, in ascending site order.
+// Adjoint near field, pass 2: one 64-lane block per listed column (i, j); each cell sums
+// the values of the sites whose box holds it, in ascending site order.  The column's sites
+// (their box's k range and the offset of the column in their value buffer) are staged in
+// LDS 64 at a time, so the value loads of a cell do not wait on one another.
 __global__ __launch_bounds__(64) void k_near_adjoint_gather(
     const int* __restrict__ col_ptr, const int* __restrict__ col_sites,
     const int* __restrict__ col_ids, const int* __restrict__ col_kmax,
     const int* __restrict__ box, const long* __restrict__ cell_off,
     const double* __restrict__ vbuf, int ny, int nz, double* __restrict__ gnear) {
+  __shared__ long soff[64];  // vbuf index of (i, j, k = 0) of the site's box run
+  __shared__ int sk0[64], sk1[64];
   const int col = col_ids[blockIdx.x];
   const int i = col / ny, j = col % ny;
   const int p0 = col_ptr[blockIdx.x], p1 = col_ptr[blockIdx.x + 1];
   const int kmax = col_kmax[blockIdx.x];
-  for (int k = threadIdx.x; k <= kmax; k += blockDim.x) {
+  for (int kb = 0; kb <= kmax; kb += 64) {
+    const int k = kb + (int)threadIdx.x;
     double acc = 0.0;
-    for (int p = p0; p < p1; ++p) {
-      const int s = col_sites[p];
-      const int* b = box + s * 6;
-      if (k < b[4] || k > b[5]) continue;
-      const int nj = b[3] - b[2] + 1, nk = b[5] - b[4] + 1;
-      acc += vbuf[cell_off[s] + ((long)(i - b[0]) * nj + (j - b[2])) * nk + (k - b[4])];
+    for (int pb = p0; pb < p1; pb += 64) {
+      const int np = min(64, p1 - pb);
+      __syncthreads();  // the previous run of sites has been read
+      if ((int)threadIdx.x < np) {
+        const int s = col_sites[pb + threadIdx.x];
+        const int* b = box + s * 6;
+        const int nj = b[3] - b[2] + 1, nk = b[5] - b[4] + 1;
+        soff[threadIdx.x] = cell_off[s] + ((long)(i - b[0]) * nj + (j - b[2])) * nk - b[4];
+        sk0[threadIdx.x] = b[4];
+        sk1[threadIdx.x] = b[5];
+      }
+      __syncthreads();
+      if (k <= kmax)
+        for (int q = 0; q < np; ++q)
+          if (k >= sk0[q] && k <= sk1[q]) acc += vbuf[soff[q] + k];
     }
-    gnear[((long)i * ny + j) * nz + k] = acc;
+    if (k <= kmax) gnear[((long)i * ny + j) * nz + k] = acc;
   }
 }
 
