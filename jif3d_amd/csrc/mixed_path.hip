@@ -53,7 +53,13 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kTile = 4;  // far-field tile edge (cells)
-constexpr int kTargetWaves = 16384;
+// Waves per far-field launch (the sites or tiles are cut into chunks to reach about this
+// many).  The tiles' costs differ (monopole, hexadecapole, masked near-box tiles), and many
+// small chunks balance the CUs: at 128x128x64 cells and 4096 sites, measured against 16384,
+// the forward is fastest at 65536 (2.40 -> 2.04 ms) and the adjoint at 98304 (2.41 -> 2.18
+// ms).  The partial sums this leaves (~0.3 GB per model) are reduced in fixed order.
+constexpr int kFarForwardWaves = 65536;
+constexpr int kFarAdjointWaves = 98304;
 constexpr int kTableWidth = 12;  // floats per cell of a non-uniform mesh
 
 // Mesh classes of the far-field kernels.
@@ -1102,7 +1108,7 @@ hipError_t forward_mixed_impl(jgm_context* ctx, int kind, int nbm, const double*
   if (N > 0) {
     const int site_blocks = (S + kBlock - 1) / kBlock;
     const long waves_per_chunk = (long)site_blocks * (kBlock / 64);
-    long want = (kTargetWaves + waves_per_chunk - 1) / waves_per_chunk;
+    long want = (kFarForwardWaves + waves_per_chunk - 1) / waves_per_chunk;
     want = std::max<long>(1, std::min<long>(want, ntiles));
     const long per = (ntiles + want - 1) / want;
     nchunks = (int)((ntiles + per - 1) / per);
@@ -1213,7 +1219,7 @@ hipError_t adjoint_mixed_impl(jgm_context* ctx, int kind, int nbm, const double*
     // the adjoint walks 4x4x(4 kQ) tiles, kQ cells per lane
     const long ntiles8 = (long)g.ntx * g.nty * ((g.nz + kQ * kTile - 1) / (kQ * kTile));
     int nchunks = (int)std::min<long>(
-        S, std::max<long>(1, (kTargetWaves + ntiles8 - 1) / ntiles8));
+        S, std::max<long>(1, (kFarAdjointWaves + ntiles8 - 1) / ntiles8));
     const int per = (S + nchunks - 1) / nchunks;
     nchunks = (S + per - 1) / per;
     if ((e = ctx->hpart.ensure(sizeof(double) * (size_t)nbm * nchunks * N)) != hipSuccess)
