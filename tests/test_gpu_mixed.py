@@ -261,3 +261,59 @@ def test_mixed_large_ragged_survey():
     F = ctx.forward_batch(K, nat.MIXED, M)
     assert np.array_equal(F[0], mix)
     assert np.array_equal(F[1], ctx.forward(K, nat.MIXED, M[1]))
+
+
+@pytest.mark.parametrize("imp_cls,k", [(ScalarHipGravityImp, 1), (ScalarTensorHipGravityImp, 10)])
+def test_mixed_edge_cases(orc, imp_cls, k):
+    """Mixed precision on the edge cases of the FP64 path: background only (no cells), no
+    sites, one cell and one site, a ragged 1 x 3 x 2 mesh with sites off the mesh."""
+    # background only: the far and near passes have no cells; the layered background is FP64
+    m = ThreeDGravityModel()
+    m.SetMeshSize(0, 0, 0)
+    m.SetBackgroundThicknesses([200.0, 300.0, 3500.0, 1000.0])
+    m.SetBackgroundDensities([1.0, 1.0, 5.0, 5.0])
+    d = TensorGravityData()
+    d.SetMeasurementPoints([3e4, 5e4], [2e4, 6e4], [0.0, -10.0])
+    ref = imp_cls(nat.FP64).Calculate(m, d)
+    mix = imp_cls(nat.MIXED).Calculate(m, d)
+    np.testing.assert_allclose(mix, ref, rtol=1e-13, atol=0)  # the same FP64 background terms
+    r = np.random.default_rng(3).normal(size=ref.size)
+    g64 = imp_cls(nat.FP64).LQDerivative(m, d, r)
+    gmx = imp_cls(nat.MIXED).LQDerivative(m, d, r)
+    assert gmx.shape == (4,)
+    np.testing.assert_allclose(gmx, g64, rtol=1e-13, atol=0)
+    # one cell, one site
+    m = ThreeDGravityModel()
+    m.SetMeshSize(1, 1, 1)
+    for f in (m.SetXCellSizes, m.SetYCellSizes, m.SetZCellSizes):
+        f([20.0])
+    m.SetDensities([300.0])
+    d = TensorGravityData()
+    d.AddMeasurementPoint(5.0, 7.0, -3.0)
+    mv = m.GetModelParameters()
+    ref = orc.calculate(orc.SCALAR, m, mv, d.positions)
+    if k == 10:  # gz, then the nine tensor components
+        ref = np.concatenate([ref, orc.calculate(orc.TENSOR, m, mv, d.positions)])
+    mix = imp_cls(nat.MIXED).Calculate(m, d)
+    assert comp_err(mix, ref, k) <= 1e-12  # one cell is always near: FP64 throughout
+    # ragged mesh, sites inside, beside and far from it
+    m = ThreeDGravityModel()
+    m.SetMeshSize(1, 3, 2)
+    m.SetXCellSizes([7.0])
+    m.SetYCellSizes([1.0, 13.0, 2.5])
+    m.SetZCellSizes([3.0, 40.0])
+    m.SetDensities(np.arange(6) + 1.0)
+    d = TensorGravityData()
+    d.SetMeasurementPoints([2.0, 30.0, 4000.0], [3.0, -20.0, 9000.0], [-0.5, 10.0, -100.0])
+    ref = imp_cls(nat.FP64).Calculate(m, d)
+    mix = imp_cls(nat.MIXED).Calculate(m, d)
+    assert comp_err(mix, ref, k) <= TOL
+    r = np.random.default_rng(4).normal(size=ref.size)
+    g64 = imp_cls(nat.FP64).LQDerivative(m, d, r)
+    gmx = imp_cls(nat.MIXED).LQDerivative(m, d, r)
+    assert rel_inf(gmx, g64) <= TOL
+    # no sites
+    d0 = TensorGravityData()
+    assert imp_cls(nat.MIXED).Calculate(m, d0).size == 0
+    g0 = imp_cls(nat.MIXED).LQDerivative(m, d0, np.zeros(0))
+    assert g0.size == 6 and np.all(g0 == 0)
