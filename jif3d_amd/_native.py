@@ -11,6 +11,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libjif3d_amd.so")
 
+API_VERSION = 1  # JGM_API_VERSION of include/jif3d_gravmag.h this binding follows
 JGM_OK = 0
 (JGM_ERR_INVALID_ARGUMENT, JGM_ERR_NOT_READY, JGM_ERR_DEVICE, JGM_ERR_OUT_OF_MEMORY,
  JGM_ERR_UNSUPPORTED) = 1, 2, 3, 4, 5
@@ -125,6 +126,9 @@ def load():
     lib.jgm_storedj_rows.argtypes = [vp]
     lib.jgm_profile_count.restype = sz
     lib.jgm_profile_count.argtypes = [vp]
+    if lib.jgm_api_version() != API_VERSION:
+        raise RuntimeError(f"{LIBPATH} has C ABI version {lib.jgm_api_version()}, this binding "
+                           f"expects {API_VERSION}: rebuild with `python -m jif3d_amd.build`")
     _lib = lib
     return lib
 

@@ -57,6 +57,10 @@ class HipGravMagImplementation {
   HipGravMagImplementation(jgm_kind kind, jgm_precision prec = JGM_FP64, int device = 0,
                            void* stream = nullptr)
       : kind_(kind), prec_(prec) {
+    if (jgm_api_version() != JGM_API_VERSION)
+      throw FatalException("libjif3d_amd.so has C ABI version " +
+                           std::to_string(jgm_api_version()) + ", this header " +
+                           std::to_string(JGM_API_VERSION));
     check(jgm_create(device, stream, &ctx_), "jgm_create");
   }
   ~HipGravMagImplementation() {

@@ -36,6 +36,16 @@ def test_api_version_and_kinds():
     assert [lib.jgm_raw_per_site(k) for k in (1, 2, 3, 4, 99)] == [1, 9, 3, 10, 0]
 
 
+def test_binding_rejects_another_abi_version(monkeypatch):
+    """The ctypes binding refuses a library built from another header version."""
+    from jif3d_amd import _native
+    _native.load()
+    monkeypatch.setattr(_native, "_lib", None)
+    monkeypatch.setattr(_native, "API_VERSION", _native.API_VERSION + 1)
+    with pytest.raises(RuntimeError, match="C ABI version"):
+        _native.load()
+
+
 def test_null_context_is_rejected():
     from jif3d_amd import _native
     lib = _native.load()
