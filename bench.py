@@ -175,8 +175,11 @@ def run_storedj(args):
         for key, kname in (("J_M", f"storedj_jm_{name}"), ("JT_R", f"storedj_jtr_{name}")):
             kms, n = prof[kname]
             avg = kms / n * 1e-3
-            entry[key] = {"kernel_ms": avg * 1e3, "J_stream_GB_s": jbytes / avg / 1e9,
-                          "hbm_frac": jbytes / avg / HBM_PEAK,
+            # the BF16 J M streams the packed BF16 copy of J (2 bytes per element); the
+            # other products stream the FP32 J
+            jb = jbytes / 2 if (key == "J_M" and prec == nat.GEMM_BF16) else jbytes
+            entry[key] = {"kernel_ms": avg * 1e3, "J_bytes_per_element": jb / (R * N),
+                          "J_stream_GB_s": jb / avg / 1e9, "hbm_frac": jb / avg / HBM_PEAK,
                           "TFLOP_s": flops / avg / 1e12, "mfma_frac": flops / avg / peak}
         y, g = Y.to_host().astype(np.float64), G.to_host().astype(np.float64)
         if ref is None:

@@ -211,7 +211,11 @@ size_t jgm_storedj_rows(const jgm_context *ctx);
  * column-major; device pointers), batching CalculateRawData (:137-152) over L-BFGS
  * trial models.  JGM_GEMM_FP32 multiplies FP32 operands; JGM_GEMM_BF16 rounds J and M
  * to BF16 (FP32 accumulation); JGM_GEMM_BF16_REFINED adds the products of the BF16
- * remainders (hi*hi + hi*lo + lo*hi), close to FP32 accuracy at BF16 matrix-core rate. */
+ * remainders (hi*hi + hi*lo + lo*hi), close to FP32 accuracy at BF16 matrix-core rate.
+ * The first JGM_GEMM_BF16 product of an HBM-resident J makes a BF16 copy of J (2 bytes
+ * per element of device memory, kept until J is rebuilt or released) that the J M
+ * products read instead of the FP32 J, with the same results; when that memory cannot
+ * be allocated they read the FP32 J. */
 jgm_status jgm_storedj_apply(jgm_context *ctx, jgm_gemm_precision prec, const float *M,
                              size_t nrhs, float *Y);
 /* G = J^T R for nrhs residual vectors (R: rows x nrhs, G: cols x nrhs), the batched

@@ -958,6 +958,8 @@ jgm_status jgm_storedj_release(jgm_context* ctx) {
   DeviceGuard g(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   ctx->storedJ.release();
+  ctx->storedJ_bf16.release();
+  ++ctx->storedJ_gen;
   jgm::release_host_storedj(ctx);
   ctx->acc64.release();
   ctx->storedJ_kind = 0;

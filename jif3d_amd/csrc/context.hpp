@@ -162,6 +162,10 @@ struct jgm_context {
   hipEvent_t ev_copied[2] = {nullptr, nullptr}, ev_used[2] = {nullptr, nullptr};
   hipEvent_t ev_start = nullptr;
   jgm::DeviceBuffer acc64;  // FP64 totals of J M over the streamed blocks
+  // BF16 copy of an HBM-resident J for the BF16 J M, 8 cells packed per row; made from
+  // the J of generation storedJ_bf16_gen (storedJ_gen counts builds)
+  jgm::DeviceBuffer storedJ_bf16;
+  size_t storedJ_gen = 1, storedJ_bf16_gen = 0;
 };
 
 namespace jgm {

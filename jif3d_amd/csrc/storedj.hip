@@ -25,6 +25,7 @@
 // v_mfma_f32_32x32x16_bf16 (FP32 accumulate), with the "refined" variant splitting
 // both operands into hi + lo BF16 parts in registers and summing hi*hi + hi*lo +
 // lo*hi (per-product error <= 3 * 2^-16 relative instead of 2^-7).
+#include <type_traits>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -223,6 +224,19 @@ __device__ __forceinline__ void mma_block(const float (&a)[2][8], const float (&
   }
 }
 
+// BF16 J M on the packed BF16 copy of J: A arrives as the lane's eight BF16 k values
+// (one 16-byte load), B is rounded in registers as in mma_block.
+__device__ __forceinline__ void mma_block_packed(const bf16x8 (&a)[2], const float (&b)[2][8],
+                                                 f32x16 (&acc)[2][2]) {
+  bf16x8 bh[2], bl[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) to_bf16<false>(b[t], bh[t], bl[t]);
+#pragma unroll
+  for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb) acc[ta][tb] = mfma_bf16(a[ta], bh[tb], acc[ta][tb]);
+}
+
 // Two-level accumulation: the MFMA accumulators are added into FP32 totals and reset
 // every 256 k, so no FP32 running sum grows longer than 256 k (the error of one long
 // FP32 sum grows with its length).
@@ -247,11 +261,25 @@ __device__ __forceinline__ int chunk_k(int s, int lk) {
 // Y partials: Yp[split][row][P].  Grid: x = nsplit * row_tiles workgroups (XCD-grouped
 // when nsplit is a multiple of 8 so that the workgroups of one split share an L2), y =
 // right-hand-side tiles of 64.  Each split covers `per` cells (a multiple of 32).
-template <int PREC>
-__global__ __launch_bounds__(kBlock) void k_gemm_jm(const float* __restrict__ J, long ld, long R,
+// PACKED (BF16 only): J is the packed BF16 copy (k_pack_bf16_cells), read with one
+// 16-byte load per lane and 32-row tile instead of eight 4-byte loads: half the bytes of
+// the FP32 J, the same rounded values in the same k order, so the same results.
+struct JmFrag {
+  float a[2][8], b[2][8];
+};
+struct JmFragPacked {
+  bf16x8 a[2];
+  float b[2][8];
+};
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+template <int PREC, bool PACKED>
+__global__ __launch_bounds__(kBlock) void k_gemm_jm(const void* __restrict__ Jv, long ld, long R,
                                                     long colpad, const float* __restrict__ Mt,
                                                     int P, long per, int row_tiles, int nsplit,
                                                     float* __restrict__ Yp) {
+  static_assert(!PACKED || PREC == JGM_GEMM_BF16, "the packed copy holds BF16 J only");
+  using Frag = typename std::conditional<PACKED, JmFragPacked, JmFrag>::type;
   const int w = blockIdx.x;
   int split, rt;
   if ((nsplit & 7) == 0) {
@@ -271,7 +299,8 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm(const float* __restrict__ J,
   const long c0 = (long)split * per;
   const long cend = c0 + per < colpad ? c0 + per : colpad;
   const int nchunk = cend > c0 ? (int)((cend - c0) / kChunk) : 0;  // even
-  const float* Jw = J + row0 + li;
+  const float* Jw = static_cast<const float*>(Jv) + row0 + li;
+  const u32x4* Jp = static_cast<const u32x4*>(Jv) + row0 + li;
   const float* Mw = Mt + b0 + li;
   f32x16 acc[2][2];
 #pragma unroll
@@ -287,30 +316,44 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm(const float* __restrict__ J,
     for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
       for (int r = 0; r < 16; ++r) tot[ta][tb][r] = 0.0f;
-  float xa[2][8], xb[2][8], ya[2][8], yb[2][8];
-  auto load = [&](float (&a)[2][8], float (&b)[2][8], long cb) {
+  Frag x, y;
+  auto load = [&](Frag& f, long cb) {
+    if constexpr (PACKED) {
+      // this lane's eight cells cb + 8 lk .. cb + 8 lk + 7 are one packed group
+      const long grp = (cb >> 3) + lk;
+      f.a[0] = __builtin_bit_cast(bf16x8, __builtin_nontemporal_load(Jp + grp * ld));
+      f.a[1] = __builtin_bit_cast(bf16x8, __builtin_nontemporal_load(Jp + grp * ld + 32));
+    }
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
       const long cell = cb + chunk_k<PREC>(s, lk);
-      a[0][s] = __builtin_nontemporal_load(Jw + cell * ld);
-      a[1][s] = __builtin_nontemporal_load(Jw + cell * ld + 32);
-      b[0][s] = Mw[cell * P];
-      b[1][s] = Mw[cell * P + 32];
+      if constexpr (!PACKED) {
+        f.a[0][s] = __builtin_nontemporal_load(Jw + cell * ld);
+        f.a[1][s] = __builtin_nontemporal_load(Jw + cell * ld + 32);
+      }
+      f.b[0][s] = Mw[cell * P];
+      f.b[1][s] = Mw[cell * P + 32];
     }
+  };
+  auto mma = [&](const Frag& f) {
+    if constexpr (PACKED)
+      mma_block_packed(f.a, f.b, acc);
+    else
+      mma_block<PREC>(f.a, f.b, acc);
   };
   // sched_barrier keeps the scheduler from sinking the next chunk's loads down to
   // their uses, which would expose the full memory latency once per chunk
   if (nchunk > 0) {
-    load(xa, xb, c0);
+    load(x, c0);
     for (int q = 0; q < nchunk; q += 2) {
-      load(ya, yb, c0 + (long)(q + 1) * kChunk);
+      load(y, c0 + (long)(q + 1) * kChunk);
       __builtin_amdgcn_sched_barrier(0);
-      mma_block<PREC>(xa, xb, acc);
+      mma(x);
       __builtin_amdgcn_sched_barrier(0);
       const int qn = q + 2 < nchunk ? q + 2 : nchunk - 1;  // the last reload is unused
-      load(xa, xb, c0 + (long)qn * kChunk);
+      load(x, c0 + (long)qn * kChunk);
       __builtin_amdgcn_sched_barrier(0);
-      mma_block<PREC>(ya, yb, acc);
+      mma(y);
       __builtin_amdgcn_sched_barrier(0);
       if (((q + 2) & 15) == 0) flush(acc, tot);  // every 256 cells
     }
@@ -328,6 +371,21 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm(const float* __restrict__ J,
         out[row * P + b0 + 32 + li] = tot[ta][1][r];
       }
     }
+}
+
+// BF16 copy of J for the BF16 J M: element (row, cell) at
+// Jp[((cell / 8) ld + row) 8 + cell % 8], rounded to nearest even as mma_block rounds it.
+__global__ __launch_bounds__(kBlock) void k_pack_bf16_cells(const float* __restrict__ J, long ld,
+                                                            long ngroups, bf16x8* __restrict__ Jp) {
+  const long n = ngroups * ld;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (long)gridDim.x * blockDim.x) {
+    const long grp = idx / ld, row = idx - grp * ld;
+    bf16x8 v;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = (__bf16)J[(8 * grp + e) * ld + row];
+    Jp[idx] = v;
+  }
 }
 
 // Y[row + b*R] = sum over splits of Yp[split][row][b]
@@ -679,6 +737,8 @@ hipError_t launch_storedj_build_resident(jgm_context* ctx, int kind, int precisi
   if (S == 0 || N == 0) return hipErrorInvalidValue;
   const long ld = round_up(R, kRowPad), colpad = round_up(N, kColPad);
   ctx->storedJ_kind = 0;
+  ++ctx->storedJ_gen;  // the packed BF16 copy no longer matches
+  ctx->storedJ_bf16.release();
   release_host_storedj(ctx);
   hipError_t e;
   if (block_cells == 0) {
@@ -728,6 +788,24 @@ hipError_t launch_storedj_build_resident(jgm_context* ctx, int kind, int precisi
 
 namespace {
 
+// Makes the packed BF16 copy of the HBM-resident J if it is missing or older than J.
+// false when device memory for it (2 bytes per J element) cannot be had: the BF16
+// product then rounds the FP32 J in registers, with the same results.
+bool ensure_bf16_copy(jgm_context* ctx) {
+  if (ctx->storedJ_bf16_gen == ctx->storedJ_gen) return true;
+  const long ld = (long)ctx->storedJ_ld, ngroups = (long)ctx->storedJ_colpad / 8;
+  if (ctx->storedJ_bf16.ensure(sizeof(__bf16) * (size_t)ld * ngroups * 8) != hipSuccess) {
+    (void)hipGetLastError();
+    ctx->storedJ_bf16.release();
+    return false;
+  }
+  hipLaunchKernelGGL(k_pack_bf16_cells, dim3(8 * ctx->cu_count), dim3(kBlock), 0, ctx->stream,
+                     ctx->storedJ.as<const float>(), ld, ngroups,
+                     ctx->storedJ_bf16.as<bf16x8>());
+  ctx->storedJ_bf16_gen = ctx->storedJ_gen;
+  return true;
+}
+
 // J M on a transposed, padded operand Xt[colpad][P] -> out (R x nrhs, column-major)
 hipError_t run_jm(jgm_context* ctx, int gemm_precision, const float* Xt, int P, int nrhs,
                   float* out) {
@@ -738,6 +816,8 @@ hipError_t run_jm(jgm_context* ctx, int gemm_precision, const float* Xt, int P, 
   const int row_tiles = (int)((ld + 255) / 256);
   const bool streamed = ctx->storedJ_host != nullptr;
   hipError_t e;
+  // the BF16 product of an HBM-resident J reads the packed BF16 copy, made once per J
+  const bool packed = gemm_precision == JGM_GEMM_BF16 && !streamed && ensure_bf16_copy(ctx);
   if (streamed) {
     if ((e = ctx->acc64.ensure(sizeof(double) * (size_t)R * nrhs)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_fill_zero_f64, dim3(4 * ctx->cu_count), dim3(kBlock), 0, st,
@@ -759,13 +839,18 @@ hipError_t run_jm(jgm_context* ctx, int gemm_precision, const float* Xt, int P, 
       ProfScope ps(ctx, gemm_precision == JGM_GEMM_FP32   ? "storedj_jm_fp32"
                         : gemm_precision == JGM_GEMM_BF16 ? "storedj_jm_bf16"
                                                           : "storedj_jm_bf16_refined");
-#define JGM_JM(PR)                                                                              \
-  hipLaunchKernelGGL((k_gemm_jm<PR>), grid, dim3(kBlock), 0, st, J, ld, R, ncols, Mt, P, per,   \
+#define JGM_JM(PR, PK, JV)                                                                      \
+  hipLaunchKernelGGL((k_gemm_jm<PR, PK>), grid, dim3(kBlock), 0, st, JV, ld, R, ncols, Mt, P, per, \
                      row_tiles, (int)nsplit, ctx->partial.as<float>())
       switch (gemm_precision) {
-        case JGM_GEMM_FP32: JGM_JM(JGM_GEMM_FP32); break;
-        case JGM_GEMM_BF16: JGM_JM(JGM_GEMM_BF16); break;
-        default: JGM_JM(JGM_GEMM_BF16_REFINED); break;
+        case JGM_GEMM_FP32: JGM_JM(JGM_GEMM_FP32, false, J); break;
+        case JGM_GEMM_BF16:
+          if (packed)
+            JGM_JM(JGM_GEMM_BF16, true, ctx->storedJ_bf16.as<const void>());
+          else
+            JGM_JM(JGM_GEMM_BF16, false, J);
+          break;
+        default: JGM_JM(JGM_GEMM_BF16_REFINED, false, J); break;
       }
 #undef JGM_JM
     }

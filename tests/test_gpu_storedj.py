@@ -263,3 +263,29 @@ def test_batched_full_size(orc):
     rhs = np.sum(M32 * G, axis=1)
     scale = np.sum(np.abs(Y) * np.abs(R32), axis=1)
     assert np.all(np.abs(lhs - rhs) <= 1e-5 * scale)
+
+
+def test_bf16_copy_follows_a_rebuilt_J():
+    """The BF16 J M of an HBM-resident J reads a packed BF16 copy of J made on first use: a
+    rebuild (here of another kind, with other rows) or a release must not leave the old copy
+    in use, and the same J gives the same products again."""
+    m, d = small_mesh()
+    ctx = make_ctx(m, d)
+    rng = np.random.default_rng(7)
+    N = m.ncells
+    M = rng.normal(size=(5, N)).astype(np.float32).astype(np.float64)
+    Rr = rng.normal(size=(5, 7 * d.nsites)).astype(np.float32).astype(np.float64)
+    ys = []
+    for step, kind in enumerate((nat.KIND_SCALAR_TENSOR, nat.KIND_SCALAR,
+                                 nat.KIND_SCALAR_TENSOR)):
+        ctx.storedj_build(kind, nat.FP64)
+        J = read_J(ctx)
+        R = J.shape[0]
+        Y, _ = products(ctx, nat.GEMM_BF16, M, Rr[:, :R])
+        assert Y.shape == (5, R)
+        bound = eps_eff(nat.GEMM_BF16, N) * (np.abs(M) @ np.abs(J).T) + 1e-45
+        assert np.all(np.abs(Y - M @ J.T) <= bound)
+        ys.append(Y)
+        if step == 1:
+            ctx.storedj_release()
+    assert np.array_equal(ys[0], ys[2])
