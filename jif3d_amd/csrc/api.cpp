@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -163,6 +164,7 @@ jgm_status jgm_create(int device, void* stream, jgm_context** out) {
   if (!ctx) return fail(JGM_ERR_OUT_OF_MEMORY, "host allocation failed");
   ctx->device = device;
   ctx->stream = static_cast<hipStream_t>(stream);
+  if (const char* v = std::getenv("JIF3D_AMD_BF16_COPY")) ctx->bf16_copy_enabled = v[0] != '0';
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
       cus > 0)
@@ -182,6 +184,7 @@ jgm_status jgm_destroy(jgm_context* ctx) {
                                &ctx->coef, &ctx->hpart, &ctx->hsum, &ctx->model_stage,
                                &ctx->out_stage, &ctx->misfit_stage, &ctx->grad_stage,
                                &ctx->farpart, &ctx->nearpart, &ctx->coef32, &ctx->storedJ, &ctx->storedJ_vec,
+                               &ctx->storedJ_bf16,
                                &ctx->near.col_ptr, &ctx->near.col_sites, &ctx->near.col_ids,
                                &ctx->near.col_kmax, &ctx->near.xcen, &ctx->near.ycen,
                                &ctx->near.zcen, &ctx->near.adj_table, &ctx->near.corner_off,
@@ -329,7 +332,11 @@ jgm_status jgm_set_field(jgm_context* ctx, double inclination, double declinatio
   ctx->inclination = inclination;
   ctx->declination = declination;
   ctx->fieldstrength = fieldstrength;
-  if (changed && ctx->storedJ_kind == JGM_KIND_MAGNETIC) ctx->storedJ_kind = 0;
+  // the stored J of the susceptibility and vertical-gradient kinds holds the field
+  // direction (and the gradient kind its strength): it no longer applies
+  if (changed &&
+      (ctx->storedJ_kind == JGM_KIND_MAGNETIC || ctx->storedJ_kind == JGM_KIND_MAGNETIC_GRADIENT))
+    ctx->storedJ_kind = 0;
   return JGM_OK;
 }
 

@@ -12,10 +12,15 @@
 
 namespace jgm {
 
-// Grow-only device allocation owned by a context.
+// Grow-only device allocation owned by a context; freed by release() or on
+// destruction, never copied.
 struct DeviceBuffer {
   void* ptr = nullptr;
   size_t bytes = 0;
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  ~DeviceBuffer() { release(); }
   hipError_t ensure(size_t need) {
     if (need <= bytes) return hipSuccess;
     if (ptr) {
@@ -163,9 +168,13 @@ struct jgm_context {
   hipEvent_t ev_start = nullptr;
   jgm::DeviceBuffer acc64;  // FP64 totals of J M over the streamed blocks
   // BF16 copy of an HBM-resident J for the BF16 J M, 8 cells packed per row; made from
-  // the J of generation storedJ_bf16_gen (storedJ_gen counts builds)
+  // the J of generation storedJ_bf16_gen (storedJ_gen counts builds).  A failed
+  // allocation of the copy is remembered for the generation it failed on, so it is not
+  // tried again until J is rebuilt; bf16_copy_enabled = false (environment variable
+  // JIF3D_AMD_BF16_COPY=0 when the context is created) never makes the copy.
   jgm::DeviceBuffer storedJ_bf16;
-  size_t storedJ_gen = 1, storedJ_bf16_gen = 0;
+  size_t storedJ_gen = 1, storedJ_bf16_gen = 0, storedJ_bf16_failed_gen = 0;
+  bool bf16_copy_enabled = true;
 };
 
 namespace jgm {

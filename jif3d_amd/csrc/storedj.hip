@@ -793,10 +793,12 @@ namespace {
 // product then rounds the FP32 J in registers, with the same results.
 bool ensure_bf16_copy(jgm_context* ctx) {
   if (ctx->storedJ_bf16_gen == ctx->storedJ_gen) return true;
+  if (!ctx->bf16_copy_enabled || ctx->storedJ_bf16_failed_gen == ctx->storedJ_gen) return false;
   const long ld = (long)ctx->storedJ_ld, ngroups = (long)ctx->storedJ_colpad / 8;
   if (ctx->storedJ_bf16.ensure(sizeof(__bf16) * (size_t)ld * ngroups * 8) != hipSuccess) {
     (void)hipGetLastError();
     ctx->storedJ_bf16.release();
+    ctx->storedJ_bf16_failed_gen = ctx->storedJ_gen;  // not retried for this J
     return false;
   }
   hipLaunchKernelGGL(k_pack_bf16_cells, dim3(8 * ctx->cu_count), dim3(kBlock), 0, ctx->stream,
@@ -836,8 +838,11 @@ hipError_t run_jm(jgm_context* ctx, int gemm_precision, const float* Xt, int P, 
     dim3 grid((unsigned)(nsplit * row_tiles), (unsigned)rhs_tiles);
     const float* Mt = Xt + c0 * P;
     {
+      // the packed BF16 product is profiled under its own name, so that timings know
+      // whether 2 or 4 bytes of J were read per element
       ProfScope ps(ctx, gemm_precision == JGM_GEMM_FP32   ? "storedj_jm_fp32"
-                        : gemm_precision == JGM_GEMM_BF16 ? "storedj_jm_bf16"
+                        : gemm_precision == JGM_GEMM_BF16 ? (packed ? "storedj_jm_bf16_packed"
+                                                                    : "storedj_jm_bf16")
                                                           : "storedj_jm_bf16_refined");
 #define JGM_JM(PR, PK, JV)                                                                      \
   hipLaunchKernelGGL((k_gemm_jm<PR, PK>), grid, dim3(kBlock), 0, st, JV, ld, R, ncols, Mt, P, per, \

@@ -59,6 +59,19 @@
 enum { ORC_SCALAR = 1, ORC_TENSOR = 2, ORC_MAGNETIC = 3, ORC_MAGGRAD = 4, ORC_MAGNETIZATION = 5 };
 enum { ORC_TF_NONE = 0, ORC_TF_FTG_INVARIANT = 1, ORC_TF_TOTAL_FIELD = 2, ORC_TF_TOTAL_FIELD_ANOMALY = 3 };
 
+/* Planted mistakes for the tests that show the transform-chain pins can fail: built
+ * only with -DORC_MUTANT=n (oracle/Makefile target liboracle_mutant<n>.so); 0, the
+ * default, is the faithful restatement.
+ *   1  TotalField Jacobian B/|B|^2 instead of B/|B|
+ *   2  FTGInvariant Jacobian drops U22 from dI/dU00
+ *   3  TotalFieldAnomaly Jacobian with cos(dec) and sin(dec) exchanged
+ *   4  transformed LQDerivative contracts the Jacobian with the wrong J row
+ *   5  TotalField value drops the Bz term
+ *   6  transformed LQDerivative without the factor 2 */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
 /* jif3D::sign, jif3D/Global/NumUtil.h:27-33 */
 static inline int orc_sign(double n) { return n >= 0 ? 1 : -1; }
 
@@ -438,7 +451,8 @@ static void tf_apply(int tf, double inc, double dec, const double *in, double *o
              in[2] * in[6];
   } else if (tf == ORC_TF_TOTAL_FIELD) {
     /* jif3D/Magnetics/MagneticTransforms.h:27-30 */
-    out[0] = sqrt(in[0] * in[0] + in[1] * in[1] + in[2] * in[2]);
+    out[0] = ORC_MUTANT == 5 ? sqrt(in[0] * in[0] + in[1] * in[1])
+                             : sqrt(in[0] * in[0] + in[1] * in[1] + in[2] * in[2]);
   } else if (tf == ORC_TF_TOTAL_FIELD_ANOMALY) {
     /* jif3D/Magnetics/MagneticTransforms.h:121-128, constructor :143-149 */
     const double bx = cos(inc) * cos(dec), by = cos(inc) * sin(dec), bz = sin(inc);
@@ -449,15 +463,21 @@ static void tf_apply(int tf, double inc, double dec, const double *in, double *o
 static void tf_derivative(int tf, double inc, double dec, const double *in, double *d) {
   if (tf == ORC_TF_FTG_INVARIANT) {
     /* jif3D/Gravity/GravityTransforms.h:80-91 */
-    d[0] = in[4] + in[8]; d[1] = -in[3]; d[2] = -in[6];
+    d[0] = ORC_MUTANT == 2 ? in[4] : in[4] + in[8]; d[1] = -in[3]; d[2] = -in[6];
     d[3] = -in[1]; d[4] = in[0] + in[8]; d[5] = -in[7];
     d[6] = -in[2]; d[7] = -in[5]; d[8] = in[0] + in[4];
   } else if (tf == ORC_TF_TOTAL_FIELD) {
     /* jif3D/Magnetics/MagneticTransforms.h:66-72 */
-    const double t = sqrt(in[0] * in[0] + in[1] * in[1] + in[2] * in[2]);
+    const double t2 = in[0] * in[0] + in[1] * in[1] + in[2] * in[2];
+    const double t = ORC_MUTANT == 1 ? t2 : sqrt(t2);
     d[0] = in[0] / t; d[1] = in[1] / t; d[2] = in[2] / t;
   } else if (tf == ORC_TF_TOTAL_FIELD_ANOMALY) {
-    d[0] = cos(inc) * cos(dec); d[1] = cos(inc) * sin(dec); d[2] = sin(inc);
+    /* jif3D/Magnetics/MagneticTransforms.h:131-141 */
+    if (ORC_MUTANT == 3) {
+      d[0] = cos(inc) * sin(dec); d[1] = cos(inc) * cos(dec); d[2] = sin(inc);
+    } else {
+      d[0] = cos(inc) * cos(dec); d[1] = cos(inc) * sin(dec); d[2] = sin(inc);
+    }
   }
 }
 
@@ -533,10 +553,11 @@ int orc_lqderivative(int kind, size_t nx, size_t ny, size_t nz, const double *xc
       tf_derivative(tf, tf_inc, tf_dec, raw, d);
       for (size_t c = 0; c < np; ++c) {
         double s = 0.0;
-        for (size_t r = 0; r < kraw; ++r) s += d[r] * rows[r * np + c];
+        for (size_t r = 0; r < kraw; ++r)
+          s += d[r] * rows[(ORC_MUTANT == 4 ? (r + 1) % kraw : r) * np + c];
         newrow[c] = s;
       }
-      const double mf = misfit[i * kout];
+      const double mf = ORC_MUTANT == 6 ? 0.5 * misfit[i * kout] : misfit[i * kout];
       for (size_t c = 0; c < np; ++c) grad[c] += mf * newrow[c];
     } else {
       /* DerivMod += Misfit(i*k+j) * row j, :224-228 */

@@ -14,6 +14,7 @@ mmoorkamp/jif3D tree):
 * lqderivative      -- ThreeDGravMagImplementation::LQDerivative, :177-231
 * calculate_ld      -- the forward problem in 80-bit long double (truth oracle)
 """
+import contextlib
 import ctypes
 import os
 
@@ -40,39 +41,60 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIBPATH):
             build()
-        _lib = ctypes.CDLL(_LIBPATH)
-        d = ctypes.c_double
-        p = ctypes.POINTER(ctypes.c_double)
-        sz = ctypes.c_size_t
-        _lib.orc_grav_box_term.restype = d
-        _lib.orc_grav_box_term.argtypes = [d] * 9
-        _lib.orc_grav_box_term_ld.restype = d
-        _lib.orc_grav_box_term_ld.argtypes = [d] * 9
-        _lib.orc_tensor_box_term.restype = None
-        _lib.orc_tensor_box_term.argtypes = [d] * 9 + [p]
-        _lib.orc_tensor_box_term_ld.restype = None
-        _lib.orc_tensor_box_term_ld.argtypes = [d] * 9 + [p]
-        _lib.orc_inf_sheet_term.restype = d
-        _lib.orc_inf_sheet_term.argtypes = [d] * 3
-        _lib.orc_uzz_inf_sheet_term.restype = d
-        _lib.orc_uzz_inf_sheet_term.argtypes = [d] * 3
-        _lib.orc_num_threads.restype = ctypes.c_int
-        common = [ctypes.c_int, sz, sz, sz, p, p, p, p, p, p, sz, p, p, sz, p, p, p, d, d, d]
-        _lib.orc_calculate.restype = ctypes.c_int
-        _lib.orc_calculate.argtypes = common + [ctypes.c_int, d, d, p, p]
-        _lib.orc_lqderivative.restype = ctypes.c_int
-        _lib.orc_lqderivative.argtypes = common + [ctypes.c_int, d, d, p, p]
-        _lib.orc_calculate_ld.restype = ctypes.c_int
-        _lib.orc_calculate_ld.argtypes = common + [p]
-        _lib.orc_j_times.restype = None
-        _lib.orc_j_times.argtypes = [sz, sz, p, p, p]
-        _lib.orc_jt_times2.restype = None
-        _lib.orc_jt_times2.argtypes = [sz, sz, p, p, p]
-        _lib.orc_transform.restype = None
-        _lib.orc_transform.argtypes = [ctypes.c_int, d, d, p, p]
-        _lib.orc_transform_derivative.restype = None
-        _lib.orc_transform_derivative.argtypes = [ctypes.c_int, d, d, p, p]
+        _lib = _load(_LIBPATH)
     return _lib
+
+
+@contextlib.contextmanager
+def planted(n):
+    """Use, inside the block, the oracle built with planted mistake n (ORC_MUTANT in
+    gravmag_oracle.c), so a test can show that its pins fail on a plausible mistake."""
+    global _lib
+    name = f"liboracle_mutant{int(n)}.so"
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", _HERE, name])
+    saved = lib()
+    _lib = _load(os.path.join(_HERE, name))
+    try:
+        yield
+    finally:
+        _lib = saved
+
+
+def _load(path):
+    lib_ = ctypes.CDLL(path)
+    d = ctypes.c_double
+    p = ctypes.POINTER(ctypes.c_double)
+    sz = ctypes.c_size_t
+    lib_.orc_grav_box_term.restype = d
+    lib_.orc_grav_box_term.argtypes = [d] * 9
+    lib_.orc_grav_box_term_ld.restype = d
+    lib_.orc_grav_box_term_ld.argtypes = [d] * 9
+    lib_.orc_tensor_box_term.restype = None
+    lib_.orc_tensor_box_term.argtypes = [d] * 9 + [p]
+    lib_.orc_tensor_box_term_ld.restype = None
+    lib_.orc_tensor_box_term_ld.argtypes = [d] * 9 + [p]
+    lib_.orc_inf_sheet_term.restype = d
+    lib_.orc_inf_sheet_term.argtypes = [d] * 3
+    lib_.orc_uzz_inf_sheet_term.restype = d
+    lib_.orc_uzz_inf_sheet_term.argtypes = [d] * 3
+    lib_.orc_num_threads.restype = ctypes.c_int
+    common = [ctypes.c_int, sz, sz, sz, p, p, p, p, p, p, sz, p, p, sz, p, p, p, d, d, d]
+    lib_.orc_calculate.restype = ctypes.c_int
+    lib_.orc_calculate.argtypes = common + [ctypes.c_int, d, d, p, p]
+    lib_.orc_lqderivative.restype = ctypes.c_int
+    lib_.orc_lqderivative.argtypes = common + [ctypes.c_int, d, d, p, p]
+    lib_.orc_calculate_ld.restype = ctypes.c_int
+    lib_.orc_calculate_ld.argtypes = common + [p]
+    lib_.orc_j_times.restype = None
+    lib_.orc_j_times.argtypes = [sz, sz, p, p, p]
+    lib_.orc_jt_times2.restype = None
+    lib_.orc_jt_times2.argtypes = [sz, sz, p, p, p]
+    lib_.orc_transform.restype = None
+    lib_.orc_transform.argtypes = [ctypes.c_int, d, d, p, p]
+    lib_.orc_transform_derivative.restype = None
+    lib_.orc_transform_derivative.argtypes = [ctypes.c_int, d, d, p, p]
+    return lib_
 
 
 def _ptr(a):

@@ -317,3 +317,34 @@ def test_mixed_edge_cases(orc, imp_cls, k):
     assert imp_cls(nat.MIXED).Calculate(m, d0).size == 0
     g0 = imp_cls(nat.MIXED).LQDerivative(m, d0, np.zeros(0))
     assert g0.size == 6 and np.all(g0 == 0)
+
+
+def test_mixed_adjoint_deep_mesh_borehole():
+    """16x16x96 cubes of 10 m with borehole sites down to 900 m: the near boxes of the deep
+    sites reach layers k >= 64, which the near-field adjoint gathers in a second pass of
+    64 layers.  Mixed gradient and forward against FP64."""
+    rng = np.random.default_rng(41)
+    m = ThreeDGravityModel()
+    m.SetMeshSize(16, 16, 96)
+    m.SetXCellSizes(np.full(16, 10.0))
+    m.SetYCellSizes(np.full(16, 10.0))
+    m.SetZCellSizes(np.full(96, 10.0))
+    m.SetDensities(2670 + rng.normal(0, 100, m.ncells))
+    d = TensorGravityData()
+    xs, ys, zs = [], [], []
+    for bx, by in ((35.0, 42.0), (81.0, 120.0), (150.0, 7.5)):
+        for z in np.linspace(-5.0, 900.0, 19):
+            xs.append(bx)
+            ys.append(by)
+            zs.append(z)
+    d.SetMeasurementPoints(np.array(xs), np.array(ys), np.array(zs))
+    imp = ScalarTensorHipGravityImp(nat.MIXED)
+    imp64 = ScalarTensorHipGravityImp(nat.FP64)
+    assert comp_err(imp.Calculate(m, d), imp64.Calculate(m, d), 10) <= TOL
+    r = rng.normal(size=d.nsites * 10) * 1e-9
+    g = imp.LQDerivative(m, d, r)
+    g64 = imp64.LQDerivative(m, d, r)
+    assert rel_inf(g, g64) <= TOL
+    # the deepest layers carry gradient from the deep sites' near boxes
+    gz = g64[:m.ncells].reshape(16, 16, 96)
+    assert np.max(np.abs(gz[:, :, 64:])) > 0.05 * np.max(np.abs(gz))

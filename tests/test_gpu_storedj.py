@@ -289,3 +289,27 @@ def test_bf16_copy_follows_a_rebuilt_J():
         if step == 1:
             ctx.storedj_release()
     assert np.array_equal(ys[0], ys[2])
+
+
+def test_bf16_packed_copy_equals_rounding_in_registers(monkeypatch):
+    """The packed BF16 copy of J holds the BF16 roundings of the FP32 J in the k order of
+    the MFMA steps, so the BF16 J M that reads it equals, bit for bit, the one that rounds
+    the FP32 J in registers (the copy switched off with JIF3D_AMD_BF16_COPY=0 when the
+    context is created)."""
+    m, d = small_mesh()
+    rng = np.random.default_rng(17)
+    M = rng.normal(size=(6, m.ncells)).astype(np.float32)
+    outs = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("JIF3D_AMD_BF16_COPY", flag)
+        ctx = make_ctx(m, d)
+        ctx.storedj_build(nat.KIND_SCALAR_TENSOR, nat.FP64)
+        R = ctx.storedj_rows()
+        Y = ctx.alloc((6, R), np.float32)
+        ctx.profile_enable(True)
+        ctx.storedj_apply(nat.GEMM_BF16, ctx.to_device(M), Y)
+        ctx.synchronize()
+        names = set(ctx.profile())
+        assert ("storedj_jm_bf16_packed" in names) == (flag == "1"), names
+        outs.append(Y.to_host())
+    assert np.array_equal(outs[0], outs[1])

@@ -188,3 +188,31 @@ def test_sensitivity_rows_match_oracle(orc, kind):
     mv = m.GetModelParameters()
     fwd = ctx.forward(kind, nat.FP64, mv)[s0 * k:(s0 + ns) * k]
     assert np.max(np.abs(got @ mv - fwd)) <= 1e-11 * np.max(np.abs(fwd))
+
+
+def test_field_change_drops_the_vertical_gradient_J():
+    """The vertical-gradient J holds the inducing field's direction and strength
+    (B = F/(4 pi G) T b, scaled by 1e-7/G): after another field is set the stored J is
+    gone (NOT_READY), and a rebuilt J gives the direct forward of the new field."""
+    kind = nat.KIND_MAGNETIC_GRADIENT
+    m, d = make_model(kind, 21)
+    ctx = nat.Context(0)
+    ctx.configure(m, d, FIELD)
+    ctx.storedj_build(kind, nat.FP64)
+    assert ctx.storedj_rows() == d.nsites
+    mv = m.GetModelParameters()
+    before = ctx.storedj_forward(nat.GEMM_FP32, mv)
+    ctx.set_field(FIELD[0], FIELD[1], FIELD[2])  # unchanged field keeps J
+    assert ctx.storedj_rows() == d.nsites
+    new_field = (0.3, 0.9, 47000.0)
+    ctx.set_field(*new_field)
+    assert ctx.storedj_rows() == 0
+    out = ctx.alloc(d.nsites)
+    status = ctx.lib.jgm_storedj_forward(ctx.handle, nat.GEMM_FP32,
+                                         nat._devptr(ctx.to_device(mv)), nat._devptr(out))
+    assert status == nat.JGM_ERR_NOT_READY
+    ctx.storedj_build(kind, nat.FP64)
+    got = ctx.storedj_forward(nat.GEMM_FP32, mv)
+    want = ctx.forward(kind, nat.FP64, mv)
+    assert np.max(np.abs(got - want)) <= 1e-5 * np.max(np.abs(want))
+    assert np.max(np.abs(got - before)) > 1e-3 * np.max(np.abs(want))
