@@ -73,19 +73,22 @@ def build(verbose=False, asm=False):
     return LIB
 
 
-DRIVER = os.path.join(HERE, "host", "adapter_driver")
+DRIVERS = ["adapter_driver", "bridge_driver"]
 
 
 def build_driver():
-    """The C++ adapter's command-line driver, linked against libjif3d_amd.so."""
+    """The C++ host drivers (the adapter, and the jif3D imp behind the bridge classes),
+    linked against libjif3d_amd.so."""
     cxx = shutil.which("g++") or "g++"
-    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, "-I", os.path.join(HERE, "host"),
-           os.path.join(HERE, "host", "adapter_driver.cpp"), "-o", DRIVER + ".tmp",
-           "-L", HERE, "-ljif3d_amd", "-Wl,-rpath,$ORIGIN/.."]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"building the C++ adapter driver failed:\n{res.stderr}")
-    os.replace(DRIVER + ".tmp", DRIVER)
+    for name in DRIVERS:
+        exe = os.path.join(HERE, "host", name)
+        cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", INCLUDE, "-I",
+               os.path.join(HERE, "host"), os.path.join(HERE, "host", name + ".cpp"), "-o",
+               exe + ".tmp", "-L", HERE, "-ljif3d_amd", "-Wl,-rpath,$ORIGIN/.."]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"building the C++ driver {name} failed:\n{res.stderr}")
+        os.replace(exe + ".tmp", exe)
 
 
 if __name__ == "__main__":
