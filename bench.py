@@ -125,7 +125,7 @@ def run_gpu(args, model, data, prec_name):
     return result
 
 
-MFMA_PEAK = {"fp32": 157.3e12, "bf16": 2.5e15}  # MI355X dense matrix peaks (flop/s)
+MFMA_PEAK = {"fp32": 157.3e12, "bf16": 2.5e15, "fp64": 78.6e12}  # MI355X dense matrix peaks
 HBM_PEAK = 8.0e12
 PCIE_PEAK = 64.0e9  # PCIe 5.0 x16, one direction
 
@@ -195,6 +195,41 @@ def run_storedj(args):
                 "JT_R": float(np.max(np.abs(g - ref[1])) / np.max(np.abs(ref[1])))}
         res[name] = entry
         log(f"[bench] stored J {name}: {wall:.3f} ms per J M + J^T R")
+    ctx.storedj_release()
+    # the same J in FP64 (the reference's precision) on the FP64 matrix cores
+    t0 = time.perf_counter()
+    ctx.storedj_build(nat.KIND_SCALAR_TENSOR, nat.FP64, fp64=True)
+    ctx.synchronize()
+    f64 = {"build_ms": (time.perf_counter() - t0) * 1e3}
+    Md64, Rd64 = ctx.to_device(M.astype(np.float64)), ctx.to_device(Rr.astype(np.float64))
+    Y64, G64 = ctx.alloc((nrhs, R)), ctx.alloc((nrhs, N))
+    for _ in range(args.warmup):
+        ctx.storedj_apply_f64(Md64, Y64)
+        ctx.storedj_apply_t_f64(Rd64, G64)
+    ctx.synchronize()
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.storedj_apply_f64(Md64, Y64)
+        ctx.storedj_apply_t_f64(Rd64, G64)
+    ctx.synchronize()
+    f64["ms_per_pair_of_products"] = (time.perf_counter() - t0) * 1e3 / args.steps
+    prof = ctx.profile()
+    jb64 = 8.0 * R * N
+    for key, kname in (("J_M", "storedj_jm_fp64"), ("JT_R", "storedj_jtr_fp64")):
+        kms, n = prof[kname]
+        avg = kms / n * 1e-3
+        f64[key] = {"kernel": kname, "kernel_ms": avg * 1e3, "J_bytes_per_element": 8.0,
+                    "J_stream_GB_s": jb64 / avg / 1e9, "hbm_frac": jb64 / avg / HBM_PEAK,
+                    "TFLOP_s": flops / avg / 1e12, "mfma_frac": flops / avg / MFMA_PEAK["fp64"]}
+    y64, g64 = Y64.to_host(), G64.to_host()
+    f64["rel_diff_vs_fp32"] = {
+        "J_M": float(np.max(np.abs(y64 - ref[0])) / np.max(np.abs(y64))),
+        "JT_R": float(np.max(np.abs(g64 - ref[1])) / np.max(np.abs(g64)))}
+    res["fp64"] = f64
+    log(f"[bench] stored J fp64: {f64['ms_per_pair_of_products']:.3f} ms per J M + J^T R")
+    for a in (Md64, Rd64, Y64, G64):
+        a.free()
     ctx.storedj_release()
     # the same J kept in page-locked host memory and streamed over PCIe in blocks of
     # 16384 cells (J larger than HBM); FP32 products, one warm-up pair, one timed pair

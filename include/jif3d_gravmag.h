@@ -81,7 +81,8 @@ typedef enum jgm_transform {
 typedef enum jgm_gemm_precision {
   JGM_GEMM_FP32 = 0,        /* FP32 MFMA, FP32 accumulate */
   JGM_GEMM_BF16 = 1,        /* BF16 operands, FP32 accumulate */
-  JGM_GEMM_BF16_REFINED = 2 /* BF16 hi/lo split of both operands, three products */
+  JGM_GEMM_BF16_REFINED = 2, /* BF16 hi/lo split of both operands, three products */
+  JGM_GEMM_FP64 = 3          /* FP64 J (jgm_storedj_build_fp64) on FP64 MFMA, FP64 accumulate */
 } jgm_gemm_precision;
 
 const char *jgm_last_error(void);
@@ -260,6 +261,23 @@ jgm_status jgm_storedj_gradient_batch_host(jgm_context *ctx, jgm_gemm_precision 
                                            size_t nmodels, const double *models,
                                            const double *misfits, double *grads);
 jgm_status jgm_storedj_release(jgm_context *ctx);
+
+/* Stored J in FP64, the precision of the reference's FullSensitivity matrix (a
+ * ublas::matrix<double>, FullSensitivityGravMagCalculator.h:40-41, :116-130): same compact
+ * layout as jgm_storedj_build, 8 bytes per element (7.5 GB for 7168 x 131072).  The cached
+ * forward and gradient (jgm_storedj_forward / _gradient and their batch and host forms)
+ * then take JGM_GEMM_FP64: model and residuals stay FP64 and J m, J^T r run on the FP64
+ * matrix cores with FP64 accumulation, so cached results equal the direct FP64 path to
+ * FP64 rounding (the reference's own caching tests ask for float epsilon).  The other
+ * GEMM precisions need the FP32 J and return JGM_ERR_UNSUPPORTED on an FP64 J, and
+ * JGM_GEMM_FP64 on an FP32 J.  Held in HBM only (no host-resident form). */
+jgm_status jgm_storedj_build_fp64(jgm_context *ctx, jgm_kind kind);
+/* Y = J M and G = J^T R on the FP64 J: M cols x nrhs, Y rows x nrhs, R rows x nrhs,
+ * G cols x nrhs, column-major FP64 device arrays; the FP64 forms of jgm_storedj_apply /
+ * _apply_t (CalculateRawData :137-152 and CalculateRawLQDerivative :214-221 without its
+ * factor 2, batched over right-hand sides). */
+jgm_status jgm_storedj_apply_f64(jgm_context *ctx, const double *M, size_t nrhs, double *Y);
+jgm_status jgm_storedj_apply_t_f64(jgm_context *ctx, const double *R, size_t nrhs, double *G);
 
 /* Stored J for problems whose J does not fit in device memory: the role of the
  * reference's DiskGravMagCalculator (jif3D/GravMag/DiskGravMagCalculator.h:100-200),

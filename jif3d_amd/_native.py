@@ -19,7 +19,7 @@ KIND_SCALAR, KIND_TENSOR, KIND_MAGNETIC, KIND_SCALAR_TENSOR = 1, 2, 3, 4
 KIND_MAGNETIC_GRADIENT, KIND_MAGNETIZATION = 5, 6
 FP64, MIXED = 0, 1
 TF_NONE, TF_FTG_INVARIANT, TF_TOTAL_FIELD, TF_TOTAL_FIELD_ANOMALY = 0, 1, 2, 3
-GEMM_FP32, GEMM_BF16, GEMM_BF16_REFINED = 0, 1, 2
+GEMM_FP32, GEMM_BF16, GEMM_BF16_REFINED, GEMM_FP64 = 0, 1, 2, 3
 RAW_PER_SITE = {KIND_SCALAR: 1, KIND_TENSOR: 9, KIND_MAGNETIC: 3, KIND_SCALAR_TENSOR: 10,
                 KIND_MAGNETIC_GRADIENT: 1, KIND_MAGNETIZATION: 3}
 
@@ -37,7 +37,8 @@ EXPORTS = [
     "jgm_storedj_apply_t", "jgm_storedj_forward", "jgm_storedj_gradient",
     "jgm_storedj_forward_host", "jgm_storedj_gradient_host", "jgm_storedj_release", "jgm_device_alloc", "jgm_device_free",
     "jgm_copy_to_device", "jgm_copy_to_host", "jgm_copy_device", "jgm_profile_enable", "jgm_profile_reset",
-    "jgm_profile_count", "jgm_profile_get",
+    "jgm_profile_count", "jgm_profile_get", "jgm_storedj_build_fp64", "jgm_storedj_apply_f64",
+    "jgm_storedj_apply_t_f64",
 ]
 
 
@@ -100,6 +101,9 @@ def load():
         "jgm_storedj_forward_batch_host": [vp, i, sz, pd, pd],
         "jgm_storedj_gradient_batch_host": [vp, i, sz, pd, pd, pd],
         "jgm_storedj_release": [vp],
+        "jgm_storedj_build_fp64": [vp, i],
+        "jgm_storedj_apply_f64": [vp, vp, sz, vp],
+        "jgm_storedj_apply_t_f64": [vp, vp, sz, vp],
         "jgm_device_alloc": [vp, sz, ctypes.POINTER(vp)],
         "jgm_device_free": [vp, vp],
         "jgm_copy_to_device": [vp, vp, vp, sz],
@@ -146,13 +150,13 @@ def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
-def _check_rhs(x, y, nin, nout):
-    """Shapes of a batched stored-J product: x (nrhs, nin) and y (nrhs, nout), FP32."""
+def _check_rhs(x, y, nin, nout, dtype=np.float32):
+    """Shapes of a batched stored-J product: x (nrhs, nin) and y (nrhs, nout), of dtype."""
     if len(x.shape) != 2 or tuple(y.shape) != (x.shape[0], nout) or x.shape[1] != nin:
         raise ValueError(f"stored-J operands must be (nrhs, {nin}) and (nrhs, {nout}), "
                          f"got {tuple(x.shape)} and {tuple(y.shape)}")
-    if np.dtype(x.dtype) != np.float32 or np.dtype(y.dtype) != np.float32:
-        raise ValueError("stored-J operands are float32")
+    if np.dtype(x.dtype) != np.dtype(dtype) or np.dtype(y.dtype) != np.dtype(dtype):
+        raise ValueError(f"stored-J operands are {np.dtype(dtype).name}")
 
 
 def _devptr(t):
@@ -348,10 +352,15 @@ class Context:
         check(self.lib.jgm_sensitivity_rows(self.handle, kind, site0, nsites, _devptr(out)))
         return out
 
-    def storedj_build(self, kind, prec, host_block_cells=None):
+    def storedj_build(self, kind, prec, host_block_cells=None, fp64=False):
         """Build the stored J in HBM, or with host_block_cells in page-locked host
-        memory, streamed to the device in blocks of that many cells."""
-        if host_block_cells:
+        memory, streamed to the device in blocks of that many cells; fp64 keeps J in FP64
+        (HBM only) for the GEMM_FP64 products."""
+        if fp64:
+            if host_block_cells:
+                raise ValueError("the FP64 stored J is held in HBM only")
+            check(self.lib.jgm_storedj_build_fp64(self.handle, kind))
+        elif host_block_cells:
             check(self.lib.jgm_storedj_build_host(self.handle, kind, prec, int(host_block_cells)))
         else:
             check(self.lib.jgm_storedj_build(self.handle, kind, prec))
@@ -410,6 +419,20 @@ class Context:
         _check_rhs(R, G, self.storedj_rows(), self.storedj_cols())
         check(self.lib.jgm_storedj_apply_t(self.handle, prec, _devptr(R), R.shape[0],
                                            _devptr(G)))
+        return G
+
+    def storedj_apply_f64(self, M, Y):
+        """Y = J M on the FP64 J: device FP64 arrays, M (nrhs, storedj_cols()), Y (nrhs,
+        storedj_rows())."""
+        _check_rhs(M, Y, self.storedj_cols(), self.storedj_rows(), np.float64)
+        check(self.lib.jgm_storedj_apply_f64(self.handle, _devptr(M), M.shape[0], _devptr(Y)))
+        return Y
+
+    def storedj_apply_t_f64(self, R, G):
+        """G = J^T R on the FP64 J: R (nrhs, storedj_rows()), G (nrhs, storedj_cols())."""
+        _check_rhs(R, G, self.storedj_rows(), self.storedj_cols(), np.float64)
+        check(self.lib.jgm_storedj_apply_t_f64(self.handle, _devptr(R), R.shape[0],
+                                               _devptr(G)))
         return G
 
     def storedj_forward(self, prec, model, out=None):

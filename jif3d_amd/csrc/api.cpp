@@ -184,7 +184,7 @@ jgm_status jgm_destroy(jgm_context* ctx) {
                                &ctx->coef, &ctx->hpart, &ctx->hsum, &ctx->model_stage,
                                &ctx->out_stage, &ctx->misfit_stage, &ctx->grad_stage,
                                &ctx->farpart, &ctx->nearpart, &ctx->coef32, &ctx->storedJ, &ctx->storedJ_vec,
-                               &ctx->storedJ_bf16,
+                               &ctx->storedJ_bf16, &ctx->storedJ_bf16c,
                                &ctx->near.col_ptr, &ctx->near.col_sites, &ctx->near.col_ids,
                                &ctx->near.col_kmax, &ctx->near.xcen, &ctx->near.ycen,
                                &ctx->near.zcen, &ctx->near.adj_table, &ctx->near.corner_off,
@@ -690,6 +690,16 @@ jgm_status jgm_storedj_build(jgm_context* ctx, jgm_kind kind, jgm_precision prec
   return JGM_OK;
 }
 
+jgm_status jgm_storedj_build_fp64(jgm_context* ctx, jgm_kind kind) {
+  jgm_status st = check_ready(ctx, kind, JGM_FP64);
+  if (st != JGM_OK) return st;
+  DeviceGuard g(ctx->device);
+  hipError_t e = jgm::launch_storedj_build_f64(ctx, kind);
+  if (e == hipErrorInvalidValue) return fail(JGM_ERR_INVALID_ARGUMENT, "no sites or no cells");
+  if (e != hipSuccess) return hip_fail(e, "FP64 stored J build");
+  return JGM_OK;
+}
+
 jgm_status jgm_storedj_build_host(jgm_context* ctx, jgm_kind kind, jgm_precision prec,
                                   size_t block_cells) {
   jgm_status st = check_ready(ctx, kind, prec);
@@ -715,6 +725,9 @@ static jgm_status storedj_apply(jgm_context* ctx, jgm_gemm_precision prec, int t
   if (ctx->storedJ_kind == 0) return fail(JGM_ERR_NOT_READY, "no stored J (jgm_storedj_build)");
   if (prec != JGM_GEMM_FP32 && prec != JGM_GEMM_BF16 && prec != JGM_GEMM_BF16_REFINED)
     return fail(JGM_ERR_INVALID_ARGUMENT, "unknown GEMM precision %d", (int)prec);
+  if (ctx->storedJ_fp64)
+    return fail(JGM_ERR_UNSUPPORTED,
+                "the stored J is FP64: its products are jgm_storedj_apply_f64 / _apply_t_f64");
   if (!in || !out) return fail(JGM_ERR_INVALID_ARGUMENT, "null operand");
   if (nrhs == 0) return JGM_OK;
   if (nrhs > 4096) return fail(JGM_ERR_UNSUPPORTED, "at most 4096 right-hand sides");
@@ -735,11 +748,40 @@ jgm_status jgm_storedj_apply_t(jgm_context* ctx, jgm_gemm_precision prec, const 
   return storedj_apply(ctx, prec, 1, R, nrhs, G);
 }
 
+static jgm_status storedj_apply_f64(jgm_context* ctx, int transpose, const double* in,
+                                    size_t nrhs, double* out) {
+  if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
+  if (ctx->storedJ_kind == 0)
+    return fail(JGM_ERR_NOT_READY, "no stored J (jgm_storedj_build_fp64)");
+  if (!ctx->storedJ_fp64)
+    return fail(JGM_ERR_UNSUPPORTED, "the stored J is FP32: build it with jgm_storedj_build_fp64");
+  if (!in || !out) return fail(JGM_ERR_INVALID_ARGUMENT, "null operand");
+  if (nrhs == 0) return JGM_OK;
+  if (nrhs > 4096) return fail(JGM_ERR_UNSUPPORTED, "at most 4096 right-hand sides");
+  DeviceGuard g(ctx->device);
+  hipError_t e = jgm::launch_storedj_apply_f64(ctx, transpose, in, nrhs, out);
+  if (e != hipSuccess) return hip_fail(e, "FP64 stored J product");
+  return JGM_OK;
+}
+
+jgm_status jgm_storedj_apply_f64(jgm_context* ctx, const double* M, size_t nrhs, double* Y) {
+  return storedj_apply_f64(ctx, 0, M, nrhs, Y);
+}
+
+jgm_status jgm_storedj_apply_t_f64(jgm_context* ctx, const double* R, size_t nrhs, double* G) {
+  return storedj_apply_f64(ctx, 1, R, nrhs, G);
+}
+
 static jgm_status storedj_ready(jgm_context* ctx, jgm_gemm_precision prec) {
   if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
   if (ctx->storedJ_kind == 0) return fail(JGM_ERR_NOT_READY, "no stored J (jgm_storedj_build)");
-  if (prec != JGM_GEMM_FP32 && prec != JGM_GEMM_BF16 && prec != JGM_GEMM_BF16_REFINED)
+  if (prec != JGM_GEMM_FP32 && prec != JGM_GEMM_BF16 && prec != JGM_GEMM_BF16_REFINED &&
+      prec != JGM_GEMM_FP64)
     return fail(JGM_ERR_INVALID_ARGUMENT, "unknown GEMM precision %d", (int)prec);
+  if ((prec == JGM_GEMM_FP64) != ctx->storedJ_fp64)
+    return fail(JGM_ERR_UNSUPPORTED,
+                ctx->storedJ_fp64 ? "the stored J is FP64: use JGM_GEMM_FP64"
+                                  : "JGM_GEMM_FP64 needs the FP64 J of jgm_storedj_build_fp64");
   return check_ready(ctx, ctx->storedJ_kind, JGM_FP64);
 }
 
@@ -966,10 +1008,12 @@ jgm_status jgm_storedj_release(jgm_context* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   ctx->storedJ.release();
   ctx->storedJ_bf16.release();
+  ctx->storedJ_bf16c.release();
   ++ctx->storedJ_gen;
   jgm::release_host_storedj(ctx);
   ctx->acc64.release();
   ctx->storedJ_kind = 0;
+  ctx->storedJ_fp64 = false;
   ctx->storedJ_rows = ctx->storedJ_cols = 0;
   return JGM_OK;
 }

@@ -68,6 +68,12 @@ struct NearBoxes {
   bool adj_table_valid = false;
   DeviceBuffer site_order;  // int32 [S]: sites in Morton order of (x, y), for compact waves
   double half_diag = 0.0;   // largest cell half-diagonal (m)
+  // sites on mesh lines (two coordinates equal to corner coordinates of the mesh): the
+  // mixed path evaluates them with the FP64 closed form; nline of them, their survey
+  // indices (int32) and coordinates (FP64 x[nline], y[nline], z[nline])
+  int nline = 0;
+  DeviceBuffer line_idx, line_xyz;
+  std::vector<int> line_sites;  // host copy of the indices
 };
 
 // Optional per-kernel timing: HIP events recorded on the context's stream around
@@ -149,6 +155,10 @@ struct jgm_context {
   jgm::DeviceBuffer weights, partial, raw, coef, hpart, hsum, model_stage, out_stage, misfit_stage,
       grad_stage, farpart, nearpart, coef32, coef_mono, near_h, near_v, vsites, partsum,
       chunk_scratch, mag_part;
+  // FP64 evaluation of the sites the mixed path hands back (corner weights, partials,
+  // sums, gathered misfits, coefficients, corner sums, a gradient, the residuals of the
+  // other sites)
+  jgm::DeviceBuffer rt_w, rt_part, rt_sum, rt_rm, rt_coef, rt_h, rt_tmp, rt_rm0;
 
   jgm::Profiler prof;
 
@@ -157,6 +167,7 @@ struct jgm_context {
   // multiple of 256); the padding holds zeros
   jgm::DeviceBuffer storedJ, storedJ_vec;
   int storedJ_kind = 0;
+  bool storedJ_fp64 = false;  // J held in FP64 (jgm_storedj_build_fp64) instead of FP32
   size_t storedJ_rows = 0, storedJ_cols = 0, storedJ_ld = 0, storedJ_colpad = 0;
   // host-resident stored J: the whole J in page-locked host memory, streamed in column
   // blocks of storedJ_block cells through two HBM staging buffers (storedJ then holds
@@ -175,6 +186,10 @@ struct jgm_context {
   jgm::DeviceBuffer storedJ_bf16;
   size_t storedJ_gen = 1, storedJ_bf16_gen = 0, storedJ_bf16_failed_gen = 0;
   bool bf16_copy_enabled = true;
+  // BF16 copy of J in J's own column-major layout (a cell's rows contiguous) for the BF16
+  // J^T R, made and dropped like the copy above (same switch)
+  jgm::DeviceBuffer storedJ_bf16c, storedJ_rhs16;  // the copy; its BF16 right-hand sides
+  size_t storedJ_bf16c_gen = 0, storedJ_bf16c_failed_gen = 0;
 };
 
 namespace jgm {
@@ -209,6 +224,16 @@ hipError_t launch_adjoint_mixed_batch(jgm_context* ctx, int kind, size_t nmodels
                                       const double* raw_misfits, size_t rm_ld, double* grads,
                                       size_t grad_ld);
 hipError_t build_near_boxes(jgm_context* ctx);
+// FP64 closed form for F of the sites (coordinates fx, fy, fz, survey indices idx):
+// the forward writes their data into out / raw_out at their own indices; the adjoint adds
+// their cell gradients (rm: raw misfits of the whole survey) into grad
+hipError_t launch_forward_fp64_sites(jgm_context* ctx, int kind, const double* model, int F,
+                                     const double* fx, const double* fy, const double* fz,
+                                     const int* idx, double* out, double* raw_out);
+hipError_t launch_adjoint_fp64_sites_add(jgm_context* ctx, int kind, int F, const double* fx,
+                                         const double* fy, const double* fz, const int* idx,
+                                         const double* rm, double* grad);
+hipError_t launch_zero_site_rows(jgm_context* ctx, double* a, const int* idx, int F, int k);
 hipError_t launch_fill_zero(jgm_context* ctx, double* p, size_t n);
 // out[i] = sum over c < nchunks of part[c * n + i] (fixed order, two passes for many chunks)
 hipError_t launch_reduce_chunks(jgm_context* ctx, const double* part, int nchunks, long n,
@@ -224,6 +249,9 @@ hipError_t launch_storedj_build_resident(jgm_context* ctx, int kind, int precisi
 void release_host_storedj(jgm_context* ctx);
 hipError_t launch_storedj_apply(jgm_context* ctx, int gemm_precision, int transpose,
                                 const float* in, size_t nrhs, float* out);
+hipError_t launch_storedj_build_f64(jgm_context* ctx, int kind);
+hipError_t launch_storedj_apply_f64(jgm_context* ctx, int transpose, const double* in,
+                                    size_t nrhs, double* out);
 hipError_t launch_storedj_forward(jgm_context* ctx, int gemm_precision, const double* model,
                                   double* out, double* raw_out);
 hipError_t launch_storedj_gradient(jgm_context* ctx, int gemm_precision,

@@ -927,26 +927,226 @@ hipError_t magnetization_adjoint(jgm_context* ctx, const double* rm, double* gra
 // when the rows are few (a forward over few sites has few rows and many chunks), so
 // many chunks go through a grouped first pass (about 256k threads) and a second pass
 // over the group totals.  The order of additions is fixed.
+namespace {
+// groups of the first pass for nchunks chunks of n rows (1: a single pass)
+long reduce_groups(int nchunks, long n, int* per) {
+  if (nchunks <= 8) {
+    *per = nchunks;
+    return 1;
+  }
+  long groups = std::max<long>(1, std::min<long>((262144 + n - 1) / n, nchunks / 4));
+  *per = (int)((nchunks + groups - 1) / groups);
+  return (nchunks + *per - 1) / *per;
+}
+// the reduction with the group totals (groups * n doubles) in `gsum`
+void reduce_chunks_with(hipStream_t st, const double* part, int nchunks, long n, double* out,
+                        double* gsum) {
+  int per = 0;
+  const long groups = reduce_groups(nchunks, n, &per);
+  if (groups == 1) {
+    hipLaunchKernelGGL(k_sum_chunks, dim3(blocks_for(n)), dim3(kBlock), 0, st, part, nchunks, n,
+                       out);
+    return;
+  }
+  hipLaunchKernelGGL(k_sum_chunk_groups, dim3(blocks_for(n), (unsigned)groups), dim3(kBlock), 0,
+                     st, part, nchunks, per, n, gsum);
+  hipLaunchKernelGGL(k_sum_chunks, dim3(blocks_for(n)), dim3(kBlock), 0, st,
+                     static_cast<const double*>(gsum), (int)groups, n, out);
+}
+}  // namespace
+
 hipError_t launch_reduce_chunks(jgm_context* ctx, const double* part, int nchunks, long n,
                                 double* out) {
   if (n <= 0) return hipSuccess;
-  hipStream_t st = ctx->stream;
-  if (nchunks <= 8) {
-    hipLaunchKernelGGL(k_sum_chunks, dim3(blocks_for(n)), dim3(kBlock), 0, st, part, nchunks, n,
-                       out);
-    return hipGetLastError();
+  int per = 0;
+  const long groups = reduce_groups(nchunks, n, &per);
+  if (groups > 1) {
+    hipError_t e = ctx->chunk_scratch.ensure(sizeof(double) * (size_t)groups * n);
+    if (e != hipSuccess) return e;
   }
-  long groups = std::max<long>(1, std::min<long>((262144 + n - 1) / n, nchunks / 4));
-  const int per = (int)((nchunks + groups - 1) / groups);
-  groups = (nchunks + per - 1) / per;
-  hipError_t e = ctx->chunk_scratch.ensure(sizeof(double) * (size_t)groups * n);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_sum_chunk_groups, dim3(blocks_for(n), (unsigned)groups), dim3(kBlock), 0,
-                     st, part, nchunks, per, n, ctx->chunk_scratch.as<double>());
-  hipLaunchKernelGGL(k_sum_chunks, dim3(blocks_for(n)), dim3(kBlock), 0, st,
-                     ctx->chunk_scratch.as<const double>(), (int)groups, n, out);
+  reduce_chunks_with(ctx->stream, part, nchunks, n, out, ctx->chunk_scratch.as<double>());
   return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// FP64 closed form for a subset of the sites: F sites by their coordinates (fx, fy, fz)
+// and their indices idx in the survey.  The mixed-precision path hands these sites over
+// (sites on mesh lines, where the reference's guarded logarithms differ from the field the
+// far-field expansion computes); results go to the sites' own places in out / raw_out and
+// their cell gradients are added to grad.
+namespace {
+template <int MASK>
+__global__ __launch_bounds__(kBlock) void k_forward_finish_sites(
+    const double* __restrict__ psum, int F, const int* __restrict__ idx,
+    const double* __restrict__ bgT, const double* __restrict__ bgvals, int L, int kind,
+    FieldDir fd, int tf, double tfbx, double tfby, double tfbz, double* __restrict__ out,
+    double* __restrict__ raw_out) {
+  constexpr int NC = Ncomp<MASK>::value;
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  double acc[NC], sc[NC];
+  comp_scales<MASK>(sc);
+#pragma unroll
+  for (int m = 0; m < NC; ++m) acc[m] = psum[(long)m * F + f] * sc[m];
+  double u[7];
+  spread_components<MASK>(acc, u);
+  const int s = idx[f];
+  double raw[10];
+  const int kraw = assemble_raw(u, s, bgT, bgvals, L, kind, fd, raw);
+  store_site(raw, kraw, s, tf, tfbx, tfby, tfbz, out, raw_out);
+}
+
+// dst[f k + m] = src[idx[f] k + m]
+__global__ __launch_bounds__(kBlock) void k_gather_site_rows(const double* __restrict__ src,
+                                                             const int* __restrict__ idx, int F,
+                                                             int k, double* __restrict__ dst) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)F * k) return;
+  const int f = (int)(t / k), m = (int)(t % k);
+  dst[t] = src[(long)idx[f] * k + m];
+}
+
+// rows idx[f] of a k-wide site array set to zero
+__global__ __launch_bounds__(kBlock) void k_zero_site_rows(double* __restrict__ a,
+                                                           const int* __restrict__ idx, int F,
+                                                           int k) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)F * k) return;
+  a[(long)idx[t / k] * k + t % k] = 0.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_add_f64(double* __restrict__ dst,
+                                                    const double* __restrict__ src, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] += src[i];
+}
+
+template <int MASK>
+hipError_t forward_sites_impl(jgm_context* ctx, int kind, const double* model, int F,
+                              const double* fx, const double* fy, const double* fz,
+                              const int* idx, double* out, double* raw_out) {
+  constexpr int NC = Ncomp<MASK>::value;
+  hipStream_t st = ctx->stream;
+  const long N = (long)ncells(ctx);
+  const int L = (int)ctx->bgthick.size();
+  const int ncx = (int)ctx->nx + 1, ncy = (int)ctx->ny + 1, ncz = (int)ctx->nz + 1;
+  const long ncols = (long)ncx * ncy;
+  const int site_blocks = (F + kBlock - 1) / kBlock;
+  const long waves_per_chunk = (long)site_blocks * (kBlock / 64);
+  long want = std::min<long>((kTargetWaves + waves_per_chunk - 1) / waves_per_chunk, ncols);
+  want = std::max<long>(1, want);
+  const long per = (ncols + want - 1) / want;
+  const int nchunks = (int)((ncols + per - 1) / per);
+  hipError_t e;
+  if ((e = ctx->rt_w.ensure(sizeof(double) * (size_t)ncols * ncz)) != hipSuccess ||
+      (e = ctx->rt_part.ensure(sizeof(double) * (size_t)nchunks * NC * F)) != hipSuccess ||
+      (e = ctx->rt_sum.ensure(sizeof(double) * (size_t)NC * F)) != hipSuccess)
+    return e;
+  hipLaunchKernelGGL(k_corner_weights, dim3(blocks_for(ncols * ncz)), dim3(kBlock), 0, st, model,
+                     0L, 1, (int)ctx->nx, (int)ctx->ny, (int)ctx->nz, ctx->rt_w.as<double>());
+  {
+    ProfScope ps(ctx, "forward_corners_fp64_line_sites");
+    launch_forward_corners<MASK>(1, dim3(site_blocks, nchunks), st, fx, fy, fz, F,
+                                 ctx->d_Xc.as<const double>(), ctx->d_Yc.as<const double>(),
+                                 ctx->d_Zc.as<const double>(), ncy, ncz,
+                                 ctx->rt_w.as<const double>(), ncols, per,
+                                 ctx->rt_part.as<double>());
+  }
+  if ((e = launch_reduce_chunks(ctx, ctx->rt_part.as<const double>(), nchunks, (long)NC * F,
+                                ctx->rt_sum.as<double>())) != hipSuccess)
+    return e;
+  const double tf_bx = std::cos(ctx->tf_inc) * std::cos(ctx->tf_dec);
+  const double tf_by = std::cos(ctx->tf_inc) * std::sin(ctx->tf_dec);
+  const double tf_bz = std::sin(ctx->tf_inc);
+  hipLaunchKernelGGL((k_forward_finish_sites<MASK>), dim3(blocks_for(F)), dim3(kBlock), 0, st,
+                     ctx->rt_sum.as<const double>(), F, idx, ctx->bgT.as<const double>(),
+                     model + N, L, kind, field_dir(ctx), ctx->transform, tf_bx, tf_by, tf_bz,
+                     out, raw_out);
+  return hipGetLastError();
+}
+
+template <int MASK>
+hipError_t adjoint_sites_impl(jgm_context* ctx, int kind, int F, const double* fx,
+                              const double* fy, const double* fz, const int* idx,
+                              const double* rm, double* grad) {
+  constexpr int NC = Ncomp<MASK>::value;
+  hipStream_t st = ctx->stream;
+  const long N = (long)ncells(ctx);
+  const int kraw = (int)raw_per_site(kind);
+  const int ncy = (int)ctx->ny + 1, ncz = (int)ctx->nz + 1;
+  const long nq = (long)ncorners(ctx);
+  const long corner_waves = (nq + 63) / 64;
+  int nchunks = (int)std::min<long>(F, std::max<long>(1, (kTargetWaves + corner_waves - 1) /
+                                                            corner_waves));
+  const int per = (F + nchunks - 1) / nchunks;
+  nchunks = (F + per - 1) / per;
+  hipError_t e;
+  if ((e = ctx->rt_rm.ensure(sizeof(double) * (size_t)F * kraw)) != hipSuccess ||
+      (e = ctx->rt_coef.ensure(sizeof(double) * (size_t)F * NC)) != hipSuccess ||
+      (e = ctx->rt_h.ensure(sizeof(double) * (size_t)nchunks * nq)) != hipSuccess ||
+      (e = ctx->rt_sum.ensure(sizeof(double) * (size_t)std::max<long>(nq, N))) != hipSuccess ||
+      (e = ctx->rt_tmp.ensure(sizeof(double) * (size_t)N)) != hipSuccess)
+    return e;
+  hipLaunchKernelGGL(k_gather_site_rows, dim3(blocks_for((long)F * kraw)), dim3(kBlock), 0, st, rm,
+                     idx, F, kraw, ctx->rt_rm.as<double>());
+  hipLaunchKernelGGL((k_adjoint_coef<MASK>), dim3(blocks_for(F)), dim3(kBlock), 0, st,
+                     ctx->rt_rm.as<const double>(), F, kind, field_dir(ctx),
+                     ctx->rt_coef.as<double>(), NC);
+  {
+    ProfScope ps(ctx, "adjoint_corners_fp64_line_sites");
+    launch_adjoint_corners<MASK>(1, dim3(blocks_for(nq), nchunks), st, fx, fy, fz,
+                                 ctx->rt_coef.as<const double>(), F, per,
+                                 ctx->d_Xc.as<const double>(), ctx->d_Yc.as<const double>(),
+                                 ctx->d_Zc.as<const double>(), ncy, ncz, nq,
+                                 ctx->rt_h.as<double>());
+  }
+  const double* H = ctx->rt_h.as<const double>();
+  if (nchunks > 1) {
+    hipLaunchKernelGGL(k_sum_chunks, dim3(blocks_for(nq)), dim3(kBlock), 0, st, H, nchunks, nq,
+                       ctx->rt_sum.as<double>());
+    H = ctx->rt_sum.as<const double>();
+  }
+  hipLaunchKernelGGL(k_corner_difference, dim3(blocks_for(N)), dim3(kBlock), 0, st, H,
+                     (int)ctx->nx, (int)ctx->ny, (int)ctx->nz, ctx->rt_tmp.as<double>());
+  hipLaunchKernelGGL(k_add_f64, dim3(blocks_for(N)), dim3(kBlock), 0, st, grad,
+                     ctx->rt_tmp.as<const double>(), N);
+  return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_forward_fp64_sites(jgm_context* ctx, int kind, const double* model, int F,
+                                     const double* fx, const double* fy, const double* fz,
+                                     const int* idx, double* out, double* raw_out) {
+  if (F <= 0 || ncells(ctx) == 0) return hipSuccess;
+  switch (mask_of(kind)) {
+    case kMaskScalar:
+      return forward_sites_impl<kMaskScalar>(ctx, kind, model, F, fx, fy, fz, idx, out, raw_out);
+    case kMaskAll:
+      return forward_sites_impl<kMaskAll>(ctx, kind, model, F, fx, fy, fz, idx, out, raw_out);
+    default:
+      return forward_sites_impl<kMaskTensor>(ctx, kind, model, F, fx, fy, fz, idx, out, raw_out);
+  }
+}
+
+hipError_t launch_adjoint_fp64_sites_add(jgm_context* ctx, int kind, int F, const double* fx,
+                                         const double* fy, const double* fz, const int* idx,
+                                         const double* rm, double* grad) {
+  if (F <= 0 || ncells(ctx) == 0) return hipSuccess;
+  switch (mask_of(kind)) {
+    case kMaskScalar:
+      return adjoint_sites_impl<kMaskScalar>(ctx, kind, F, fx, fy, fz, idx, rm, grad);
+    case kMaskAll: return adjoint_sites_impl<kMaskAll>(ctx, kind, F, fx, fy, fz, idx, rm, grad);
+    default: return adjoint_sites_impl<kMaskTensor>(ctx, kind, F, fx, fy, fz, idx, rm, grad);
+  }
+}
+
+hipError_t launch_zero_site_rows(jgm_context* ctx, double* a, const int* idx, int F, int k) {
+  if (F <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_site_rows, dim3(blocks_for((long)F * k)), dim3(kBlock), 0,
+                     ctx->stream, a, idx, F, k);
+  return hipGetLastError();
+}
+
 
 namespace {
 __global__ __launch_bounds__(kBlock) void k_zero_flag(int* __restrict__ flag) {

@@ -1464,11 +1464,52 @@ hipError_t build_near_boxes(jgm_context* ctx) {
   if ((e = upload_vec(nb.zoff, zoff, st)) != hipSuccess) return e;
   const std::vector<int> order = morton_order(ctx);
   if ((e = upload_vec(nb.site_order, order, st)) != hipSuccess) return e;
+  // sites with two coordinates on corner planes of the mesh lie on a mesh line: there the
+  // reference's guarded logarithms (log(max(u + r, eps)), BasicGravElements.cpp:53-59)
+  // give every cell with an edge on that line a term that the far-field expansion does not
+  // have, so these sites take the FP64 closed form over the whole mesh
+  nb.line_sites.clear();
+  for (size_t s = 0; s < S; ++s) {
+    const int on = (int)std::binary_search(ctx->xc.begin(), ctx->xc.end(), ctx->hsx[s]) +
+                   (int)std::binary_search(ctx->yc.begin(), ctx->yc.end(), ctx->hsy[s]) +
+                   (int)std::binary_search(ctx->zc.begin(), ctx->zc.end(), ctx->hsz[s]);
+    if (on >= 2) nb.line_sites.push_back((int)s);
+  }
+  nb.nline = (int)nb.line_sites.size();
+  if (nb.nline > 0) {
+    std::vector<double> xyz(3 * (size_t)nb.nline);
+    for (int f = 0; f < nb.nline; ++f) {
+      xyz[f] = ctx->hsx[nb.line_sites[f]];
+      xyz[nb.nline + f] = ctx->hsy[nb.line_sites[f]];
+      xyz[2 * (size_t)nb.nline + f] = ctx->hsz[nb.line_sites[f]];
+    }
+    if ((e = upload_vec(nb.line_idx, nb.line_sites, st)) != hipSuccess) return e;
+    if ((e = upload_vec(nb.line_xyz, xyz, st)) != hipSuccess) return e;
+  }
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
   nb.adj_table_valid = false;
   nb.valid = true;
   return hipSuccess;
 }
+
+namespace {
+hipError_t route_line_sites_forward(jgm_context* ctx, int kind, size_t nmodels,
+                                    const double* models, size_t model_ld, double* out,
+                                    size_t out_ld, double* raw_out, size_t raw_ld) {
+  const NearBoxes& nb = ctx->near;
+  if (nb.nline == 0 || ncells(ctx) == 0) return hipSuccess;
+  const double* xyz = nb.line_xyz.as<const double>();
+  const int F = nb.nline;
+  for (size_t b = 0; b < nmodels; ++b) {
+    hipError_t e = launch_forward_fp64_sites(ctx, kind, models + b * model_ld, F, xyz, xyz + F,
+                                             xyz + 2 * F, nb.line_idx.as<const int>(),
+                                             out + b * out_ld,
+                                             raw_out ? raw_out + b * raw_ld : nullptr);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+}  // namespace
 
 hipError_t launch_forward_mixed(jgm_context* ctx, int kind, const double* model, double* out,
                                 double* raw_out) {
@@ -1511,7 +1552,9 @@ hipError_t launch_forward_mixed_batch(jgm_context* ctx, int kind, size_t nmodels
     if (e != hipSuccess) return e;
     b0 += nbm;
   }
-  return hipSuccess;
+  // sites on mesh lines: the FP64 closed form in place of the mixed result
+  return route_line_sites_forward(ctx, kind, nmodels, models, model_ld, out, out_ld, raw_out,
+                                  raw_ld);
 }
 
 hipError_t launch_adjoint_mixed(jgm_context* ctx, int kind, const double* /*model*/,
@@ -1534,6 +1577,27 @@ hipError_t launch_adjoint_mixed_batch(jgm_context* ctx, int kind, size_t nmodels
     }
     return hipSuccess;
   }
+  // sites on mesh lines: their misfits are taken out of the mixed adjoint and their
+  // cell gradients added from the FP64 closed form below
+  const NearBoxes& nbx = ctx->near;
+  const bool route = nbx.nline > 0 && ncells(ctx) > 0;
+  const double* all_misfits = raw_misfits;
+  const size_t all_ld = rm_ld;
+  if (route) {
+    const size_t kraw = raw_per_site(kind), n = ctx->S * kraw;
+    hipError_t e = ctx->rt_rm0.ensure(sizeof(double) * n * nmodels);
+    if (e != hipSuccess) return e;
+    double* rm0 = ctx->rt_rm0.as<double>();
+    for (size_t b = 0; b < nmodels; ++b) {
+      if ((e = hipMemcpyAsync(rm0 + b * n, raw_misfits + b * rm_ld, sizeof(double) * n,
+                              hipMemcpyDeviceToDevice, ctx->stream)) != hipSuccess ||
+          (e = launch_zero_site_rows(ctx, rm0 + b * n, nbx.line_idx.as<const int>(), nbx.nline,
+                                     (int)kraw)) != hipSuccess)
+        return e;
+    }
+    raw_misfits = rm0;
+    rm_ld = n;
+  }
   for (size_t b0 = 0; b0 < nmodels;) {
     const size_t left = nmodels - b0;
     const int nbm = left >= 4 ? 4 : left >= 2 ? 2 : 1;
@@ -1549,6 +1613,23 @@ hipError_t launch_adjoint_mixed_batch(jgm_context* ctx, int kind, size_t nmodels
     }
     if (e != hipSuccess) return e;
     b0 += nbm;
+  }
+  if (route) {
+    const double* xyz = nbx.line_xyz.as<const double>();
+    const int F = nbx.nline;
+    const long N = (long)ncells(ctx);
+    for (size_t b = 0; b < nmodels; ++b) {
+      double* g = grads + b * grad_ld;
+      const double* rm = all_misfits + b * all_ld;
+      hipError_t e;
+      // the background gradient of every site, theirs included
+      if (!ctx->bgthick.empty() &&
+          (e = launch_background_gradient(ctx, kind, rm, g + N)) != hipSuccess)
+        return e;
+      if ((e = launch_adjoint_fp64_sites_add(ctx, kind, F, xyz, xyz + F, xyz + 2 * F,
+                                             nbx.line_idx.as<const int>(), rm, g)) != hipSuccess)
+        return e;
+    }
   }
   return hipSuccess;
 }

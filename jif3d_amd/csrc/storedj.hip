@@ -89,19 +89,20 @@ __global__ __launch_bounds__(kBlock) void k_fill_zero_f32(float* __restrict__ p,
 }
 
 // Build: one lane per site, the wave walks a chunk of columns (wave-uniform); FP64
-// closed form per cell (8 corners), stored as FP32 in the compact layout.  Column col
+// closed form per cell (8 corners), stored as T (FP32, or FP64 for the FP64 J) in the
+// compact layout.  Column col
 // is cell col, except for the magnetization kind, where it is component p = col / N of
 // cell col % N (J[3 s + i][p N + c] = mscale T_ip, OMPMagnetizationImp.cpp:25-117); the
 // vertical-gradient kind stores Bz at the site minus Bz 1 m below
 // (OMPMagneticGradImp.cpp:13-106), mscale = F 1e-7 / G.
-template <int MASK>
+template <int MASK, class T>
 __global__ __launch_bounds__(kBlock) void k_storedj_build(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     int S, const double* __restrict__ xlo, const double* __restrict__ ylo,
     const double* __restrict__ zlo, const double* __restrict__ xsz,
     const double* __restrict__ ysz, const double* __restrict__ zsz, int ny, int nz, long ncell,
     long cbase, long cend, long cells_per_chunk, int kind, FieldDir fd, double mscale, long ld,
-    float* __restrict__ J) {
+    T* __restrict__ J) {
   constexpr int NC = Ncomp<MASK>::value;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int sl = s < S ? s : S - 1;
@@ -126,48 +127,49 @@ __global__ __launch_bounds__(kBlock) void k_storedj_build(
       spread_components<MASK>(v, w);
     }
     if (s >= S) continue;
-    float* o = J + (col - cbase) * ld + (long)s * kJ;
+    T* o = J + (col - cbase) * ld + (long)s * kJ;
     if (kind == JGM_KIND_MAGNETIC_GRADIENT) {
       const double b0 = (u[5] * fd.bx + u[6] * fd.by + u[3] * fd.bz) * mscale;
       const double b1 = (w[5] * fd.bx + w[6] * fd.by + w[3] * fd.bz) * mscale;
-      o[0] = (float)(b0 - b1);
+      o[0] = (T)(b0 - b1);
     } else if (kind == JGM_KIND_MAGNETIZATION) {
       // column p of T = [[xx, xy, xz], [xy, yy, yz], [xz, yz, zz]]
       const double t0 = p == 0 ? u[1] : p == 1 ? u[4] : u[5];
       const double t1 = p == 0 ? u[4] : p == 1 ? u[2] : u[6];
       const double t2 = p == 0 ? u[5] : p == 1 ? u[6] : u[3];
-      o[0] = (float)(t0 * mscale);
-      o[1] = (float)(t1 * mscale);
-      o[2] = (float)(t2 * mscale);
+      o[0] = (T)(t0 * mscale);
+      o[1] = (T)(t1 * mscale);
+      o[2] = (T)(t2 * mscale);
     } else if (kind == JGM_KIND_SCALAR) {
-      o[0] = (float)u[0];
+      o[0] = (T)u[0];
     } else if (kind == JGM_KIND_MAGNETIC) {
-      o[0] = (float)((u[1] * fd.bx + u[4] * fd.by + u[5] * fd.bz) * fd.scale);
-      o[1] = (float)((u[4] * fd.bx + u[2] * fd.by + u[6] * fd.bz) * fd.scale);
-      o[2] = (float)((u[5] * fd.bx + u[6] * fd.by + u[3] * fd.bz) * fd.scale);
+      o[0] = (T)((u[1] * fd.bx + u[4] * fd.by + u[5] * fd.bz) * fd.scale);
+      o[1] = (T)((u[4] * fd.bx + u[2] * fd.by + u[6] * fd.bz) * fd.scale);
+      o[2] = (T)((u[5] * fd.bx + u[6] * fd.by + u[3] * fd.bz) * fd.scale);
     } else {
       int q = 0;
-      if (kind == JGM_KIND_SCALAR_TENSOR) o[q++] = (float)u[0];
-      o[q + 0] = (float)u[1];
-      o[q + 1] = (float)u[4];
-      o[q + 2] = (float)u[5];
-      o[q + 3] = (float)u[2];
-      o[q + 4] = (float)u[6];
-      o[q + 5] = (float)u[3];
+      if (kind == JGM_KIND_SCALAR_TENSOR) o[q++] = (T)u[0];
+      o[q + 0] = (T)u[1];
+      o[q + 1] = (T)u[4];
+      o[q + 2] = (T)u[5];
+      o[q + 3] = (T)u[2];
+      o[q + 4] = (T)u[6];
+      o[q + 5] = (T)u[3];
     }
   }
 }
 
 // Column-major X (n x nrhs) -> Xt[k][P] for k < npad (P = nrhs padded to 64); rows
 // k >= n and columns b >= nrhs are zero.
-__global__ __launch_bounds__(kBlock) void k_transpose_pad(const float* __restrict__ X, long n,
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_transpose_pad(const T* __restrict__ X, long n,
                                                           long npad, int nrhs, int P,
-                                                          float* __restrict__ Xt) {
+                                                          T* __restrict__ Xt) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= npad * P) return;
   const long k = idx / P;
   const int b = (int)(idx % P);
-  Xt[idx] = (k < n && b < nrhs) ? X[k + (long)b * n] : 0.0f;
+  Xt[idx] = (k < n && b < nrhs) ? X[k + (long)b * n] : T(0);
 }
 
 __device__ __forceinline__ f32x16 mfma32(float a, float b, f32x16 c) {
@@ -388,6 +390,121 @@ __global__ __launch_bounds__(kBlock) void k_pack_bf16_cells(const float* __restr
   }
 }
 
+// BF16 copy of J in J's layout (element (row, cell) at cell ld + row), eight rows per
+// 16 bytes, rounded to nearest even as mma_block rounds J
+__global__ __launch_bounds__(kBlock) void k_pack_bf16_rows(const float* __restrict__ J, long ngroups,
+                                                           bf16x8* __restrict__ Jc) {
+  for (long g = (long)blockIdx.x * blockDim.x + threadIdx.x; g < ngroups;
+       g += (long)gridDim.x * blockDim.x) {
+    const f32x4 lo = *(const f32x4*)(J + 8 * g), hi = *(const f32x4*)(J + 8 * g + 4);
+    bf16x8 v;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      v[e] = (__bf16)lo[e];
+      v[4 + e] = (__bf16)hi[e];
+    }
+    Jc[g] = v;
+  }
+}
+
+// Rt[ld][P] (FP32) -> Rb[ld / 8][P] (BF16, eight rows of one right-hand side per 16 bytes)
+__global__ __launch_bounds__(kBlock) void k_rows_to_bf16(const float* __restrict__ Rt, long ngroups,
+                                                         int P, bf16x8* __restrict__ Rb) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ngroups * P) return;
+  const long g = idx / P;
+  const int b = (int)(idx - g * P);
+  bf16x8 v;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = (__bf16)Rt[(8 * g + e) * P + b];
+  Rb[idx] = v;
+}
+
+// BF16 G^T = Rt^T J from the column-major BF16 copy of J: workgroup = 256 cells (wave: 64
+// cells) x 64 right-hand sides, no LDS.  For each 16-row k-step a lane's eight rows of its
+// cell (B operand, k = 8 (lane / 32) + e) and of its right-hand side (A operand) are one
+// 16-byte load each; 64 rows (four k-steps) of loads are in flight while the previous 64
+// are multiplied.  Same k order, rounding and flush interval as k_gemm_jtr<BF16>, so the
+// same results.
+__global__ __launch_bounds__(kBlock) void k_gemm_jtr_bf16c(const bf16x8* __restrict__ Jc, long ld,
+                                                           long N, long nblk,
+                                                           const bf16x8* __restrict__ Rb, int P,
+                                                           float* __restrict__ G, long ldg,
+                                                           int nrhs) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int li = lane & 31, lk = lane >> 5;
+  const int b0 = blockIdx.y * 64;
+  const long ldg8 = ld / 8;          // 8-row groups per cell
+  const long nblock64 = ld / 64;     // 64-row blocks (ld is a multiple of 64)
+  for (long blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const long cbase = blk * kColPad + 64 * wave;
+    const bf16x8* J0 = Jc + (cbase + li) * ldg8 + lk;
+    const bf16x8* J1 = J0 + 32 * ldg8;
+    const bf16x8* R0 = Rb + (long)lk * P + b0 + li;
+    const bf16x8* R1 = R0 + 32;
+    f32x16 acc[2][2];
+    float tot[2][2][16];
+#pragma unroll
+    for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+      for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[ta][tb][r] = tot[ta][tb][r] = 0.0f;
+    struct Q {
+      bf16x8 a[4][2], b[4][2];  // [k-step][tile]
+    } x, y;
+    auto load = [&](Q& q, long blk64) {
+      const long g0 = blk64 * 8;  // first 8-row group of the block
+#pragma unroll
+      for (int st = 0; st < 4; ++st) {
+        const long g = g0 + 2 * st;  // this lane's group: 16 st rows + 8 lk
+        q.a[st][0] = R0[g * P];
+        q.a[st][1] = R1[g * P];
+        q.b[st][0] = __builtin_nontemporal_load(J0 + g);
+        q.b[st][1] = __builtin_nontemporal_load(J1 + g);
+      }
+    };
+    auto mma = [&](const Q& q) {
+#pragma unroll
+      for (int st = 0; st < 4; ++st)
+#pragma unroll
+        for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+          for (int tb = 0; tb < 2; ++tb)
+            acc[ta][tb] = mfma_bf16(q.a[st][ta], q.b[st][tb], acc[ta][tb]);
+    };
+    load(x, 0);
+    for (long q = 0; q < nblock64; q += 2) {
+      load(y, q + 1 < nblock64 ? q + 1 : nblock64 - 1);  // past the end: unused reload
+      __builtin_amdgcn_sched_barrier(0);
+      mma(x);
+      if ((q & 3) == 3) flush(acc, tot);  // every 256 rows
+      __builtin_amdgcn_sched_barrier(0);
+      load(x, q + 2 < nblock64 ? q + 2 : nblock64 - 1);
+      __builtin_amdgcn_sched_barrier(0);
+      if (q + 1 < nblock64) {
+        mma(y);
+        if (((q + 1) & 3) == 3) flush(acc, tot);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    flush(acc, tot);
+#pragma unroll
+    for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+      for (int tb = 0; tb < 2; ++tb) {
+        const long c = cbase + 32 * tb + li;
+        if (c < N) {
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const int b = b0 + 32 * ta + 8 * (r >> 2) + 4 * lk + (r & 3);
+            if (b < nrhs) G[c + (long)b * ldg] = tot[ta][tb][r];
+          }
+        }
+      }
+  }
+}
+
 // Y[row + b*R] = sum over splits of Yp[split][row][b]
 __global__ __launch_bounds__(kBlock) void k_gemm_jm_reduce(const float* __restrict__ Yp,
                                                            int nsplit, long R, int P, int nrhs,
@@ -502,43 +619,45 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jtr(const float* __restrict__ J
 // operand Xt[npad][64]; the other 63 columns and the padding rows are zero.
 // FP64 model vectors (model b at v + b * v_ld, b < nb) -> right-hand sides of
 // Xt[npad][P] (FP32; columns nb.. and rows n.. zero).
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_vec_to_rhs(const double* __restrict__ v, long v_ld,
                                                        int nb, long n, long npad, int P,
-                                                       float* __restrict__ Xt) {
+                                                       T* __restrict__ Xt) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= npad * P) return;
   const long k = idx / P;
   const int b = (int)(idx - k * P);
-  Xt[idx] = (b < nb && k < n) ? (float)v[b * v_ld + k] : 0.0f;
+  Xt[idx] = (b < nb && k < n) ? (T)v[b * v_ld + k] : T(0);
 }
 
 // Raw misfits (raw layout per site; set b at rm + b * rm_ld, b < nb) -> compact
 // residual rows as right-hand sides of Xt[ld][P]: the symmetric tensor entries appear
 // twice in the raw layout, so their compact rows receive the sum of both residuals.
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_rm_to_compact(const double* __restrict__ rm0,
                                                           long rm_ld, int nb, int S, int kind,
                                                           long ld, int P,
-                                                          float* __restrict__ Xt) {
+                                                          T* __restrict__ Xt) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= ld * P) return;
   const long row = idx / P;
   const int b = (int)(idx - row * P);
   const int kJ = compact_rows(kind);
-  float v = 0.0f;
+  double v = 0.0;
   if (b < nb && row < (long)S * kJ) {
     const double* rm = rm0 + b * rm_ld;
     const long s = row / kJ;
     int q = (int)(row % kJ);
     if (kind == JGM_KIND_SCALAR || kind == JGM_KIND_MAGNETIC_GRADIENT) {
-      v = (float)rm[s];
+      v = rm[s];
     } else if (kind == JGM_KIND_MAGNETIC || kind == JGM_KIND_MAGNETIZATION) {
-      v = (float)rm[s * 3 + q];
+      v = rm[s * 3 + q];
     } else {
       const int kraw = kind == JGM_KIND_SCALAR_TENSOR ? 10 : 9;
       const double* r = rm + s * kraw;
       if (kind == JGM_KIND_SCALAR_TENSOR) {
         if (q == 0) {
-          v = (float)r[0];
+          v = r[0];
           q = -1;
         } else {
           r += 1;
@@ -546,29 +665,30 @@ __global__ __launch_bounds__(kBlock) void k_rm_to_compact(const double* __restri
         }
       }
       switch (q) {  // compact Uxx, Uxy, Uxz, Uyy, Uyz, Uzz
-        case 0: v = (float)r[0]; break;
-        case 1: v = (float)(r[1] + r[3]); break;
-        case 2: v = (float)(r[2] + r[6]); break;
-        case 3: v = (float)r[4]; break;
-        case 4: v = (float)(r[5] + r[7]); break;
-        case 5: v = (float)r[8]; break;
+        case 0: v = r[0]; break;
+        case 1: v = (r[1] + r[3]); break;
+        case 2: v = (r[2] + r[6]); break;
+        case 3: v = r[4]; break;
+        case 4: v = (r[5] + r[7]); break;
+        case 5: v = r[8]; break;
         default: break;
       }
     }
   }
-  Xt[idx] = v;
+  Xt[idx] = (T)v;
 }
 
 // Cached forward epilogue: compact J m (FP32) of site s plus the background layers,
 // assembled into the raw layout and transformed like the matrix-free path.
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_storedj_finish(
-    const float* __restrict__ Y, int S, int kind, const double* __restrict__ bgT,
+    const T* __restrict__ Y, int S, int kind, const double* __restrict__ bgT,
     const double* __restrict__ bgvals, int L, FieldDir fd, int tf, double tfbx, double tfby,
     double tfbz, double* __restrict__ out, double* __restrict__ raw_out) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S) return;
   const int kJ = compact_rows(kind);
-  const float* y = Y + (long)s * kJ;
+  const T* y = Y + (long)s * kJ;
   if (!has_background(kind)) {  // J rows are the data themselves
     double raw[3];
     for (int m = 0; m < kJ; ++m) raw[m] = y[m];
@@ -598,7 +718,8 @@ __global__ __launch_bounds__(kBlock) void k_storedj_finish(
   store_site(raw, kraw, s, tf, tfbx, tfby, tfbz, out, raw_out);
 }
 
-__global__ __launch_bounds__(kBlock) void k_scale_to_f64(const float* __restrict__ g, long n,
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_scale_to_f64(const T* __restrict__ g, long n,
                                                          double scale, double* __restrict__ out) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = scale * (double)g[i];
@@ -633,7 +754,8 @@ __global__ __launch_bounds__(kBlock) void k_f64_to_f32(const double* __restrict_
 }
 
 // Build the J columns of cells [c0, c1) into J (column c - c0 at J + (c - c0) * ld).
-hipError_t build_columns(jgm_context* ctx, int kind, long c0, long c1, long ld, float* J) {
+template <class T>
+hipError_t build_columns(jgm_context* ctx, int kind, long c0, long c1, long ld, T* J) {
   const int S = (int)ctx->S;
   const int site_blocks = (S + kBlock - 1) / kBlock;
   const long n = c1 - c0;
@@ -649,7 +771,7 @@ hipError_t build_columns(jgm_context* ctx, int kind, long c0, long c1, long ld, 
                                                        : ctx->fieldstrength * (1e-7 / kGravConst);
   ProfScope ps(ctx, "storedj_build_fp64");
 #define JGM_BUILD(M)                                                                            \
-  hipLaunchKernelGGL((k_storedj_build<M>), dim3(site_blocks, nchunks), dim3(kBlock), 0,        \
+  hipLaunchKernelGGL((k_storedj_build<M, T>), dim3(site_blocks, nchunks), dim3(kBlock), 0,     \
                      ctx->stream, ctx->d_sx.as<double>(), ctx->d_sy.as<double>(),               \
                      ctx->d_sz.as<double>(), S, ctx->d_xlo.as<double>(), ctx->d_ylo.as<double>(), \
                      ctx->d_zlo.as<double>(), ctx->d_xsz.as<double>(), ctx->d_ysz.as<double>(), \
@@ -737,8 +859,10 @@ hipError_t launch_storedj_build_resident(jgm_context* ctx, int kind, int precisi
   if (S == 0 || N == 0) return hipErrorInvalidValue;
   const long ld = round_up(R, kRowPad), colpad = round_up(N, kColPad);
   ctx->storedJ_kind = 0;
-  ++ctx->storedJ_gen;  // the packed BF16 copy no longer matches
+  ctx->storedJ_fp64 = false;
+  ++ctx->storedJ_gen;  // the BF16 copies no longer match
   ctx->storedJ_bf16.release();
+  ctx->storedJ_bf16c.release();
   release_host_storedj(ctx);
   hipError_t e;
   if (block_cells == 0) {
@@ -786,6 +910,35 @@ hipError_t launch_storedj_build_resident(jgm_context* ctx, int kind, int precisi
   return hipSuccess;
 }
 
+// FP64 J in HBM (the precision of the reference's FullSensitivity matrix), same compact
+// layout and padding as the FP32 J, for the FP64 matrix-core products.
+hipError_t launch_storedj_build_f64(jgm_context* ctx, int kind) {
+  const int S = (int)ctx->S;
+  const long N = storedj_columns(ctx, kind);
+  const long R = (long)S * compact_rows(kind);
+  if (S == 0 || N == 0) return hipErrorInvalidValue;
+  const long ld = round_up(R, kRowPad), colpad = round_up(N, kColPad);
+  ctx->storedJ_kind = 0;
+  ctx->storedJ_fp64 = false;
+  ++ctx->storedJ_gen;
+  ctx->storedJ_bf16.release();
+  ctx->storedJ_bf16c.release();
+  release_host_storedj(ctx);
+  hipError_t e;
+  if ((e = ctx->storedJ.ensure(sizeof(double) * (size_t)ld * colpad)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_fill_zero_f64, dim3(8 * ctx->cu_count), dim3(kBlock), 0, ctx->stream,
+                     ctx->storedJ.as<double>(), ld * colpad);
+  if ((e = build_columns(ctx, kind, 0, N, ld, ctx->storedJ.as<double>())) != hipSuccess) return e;
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  ctx->storedJ_kind = kind;
+  ctx->storedJ_fp64 = true;
+  ctx->storedJ_rows = (size_t)R;
+  ctx->storedJ_cols = (size_t)N;
+  ctx->storedJ_ld = (size_t)ld;
+  ctx->storedJ_colpad = (size_t)colpad;
+  return hipSuccess;
+}
+
 namespace {
 
 // Makes the packed BF16 copy of the HBM-resident J if it is missing or older than J.
@@ -805,6 +958,23 @@ bool ensure_bf16_copy(jgm_context* ctx) {
                      ctx->storedJ.as<const float>(), ld, ngroups,
                      ctx->storedJ_bf16.as<bf16x8>());
   ctx->storedJ_bf16_gen = ctx->storedJ_gen;
+  return true;
+}
+
+// The column-major BF16 copy for the BF16 J^T R, on the same terms.
+bool ensure_bf16c_copy(jgm_context* ctx) {
+  if (ctx->storedJ_bf16c_gen == ctx->storedJ_gen) return true;
+  if (!ctx->bf16_copy_enabled || ctx->storedJ_bf16c_failed_gen == ctx->storedJ_gen) return false;
+  const long ngroups = (long)ctx->storedJ_ld * (long)ctx->storedJ_colpad / 8;
+  if (ctx->storedJ_bf16c.ensure(sizeof(bf16x8) * (size_t)ngroups) != hipSuccess) {
+    (void)hipGetLastError();
+    ctx->storedJ_bf16c.release();
+    ctx->storedJ_bf16c_failed_gen = ctx->storedJ_gen;
+    return false;
+  }
+  hipLaunchKernelGGL(k_pack_bf16_rows, dim3(8 * ctx->cu_count), dim3(kBlock), 0, ctx->stream,
+                     ctx->storedJ.as<const float>(), ngroups, ctx->storedJ_bf16c.as<bf16x8>());
+  ctx->storedJ_bf16c_gen = ctx->storedJ_gen;
   return true;
 }
 
@@ -881,6 +1051,19 @@ hipError_t run_jtr(jgm_context* ctx, int gemm_precision, const float* Xt, int P,
   const long N = (long)ctx->storedJ_cols;
   const long ld = (long)ctx->storedJ_ld;
   hipStream_t st = ctx->stream;
+  // the BF16 product of an HBM-resident J reads the column-major BF16 copy, made once per J
+  if (gemm_precision == JGM_GEMM_BF16 && !ctx->storedJ_host && ensure_bf16c_copy(ctx)) {
+    const long nblk = (long)ctx->storedJ_colpad / kColPad;
+    hipError_t e = ctx->storedJ_rhs16.ensure(sizeof(__bf16) * (size_t)ld * P);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_rows_to_bf16, dim3(blocks_for(ld / 8 * P)), dim3(kBlock), 0, st, Xt,
+                       ld / 8, P, ctx->storedJ_rhs16.as<bf16x8>());
+    ProfScope ps(ctx, "storedj_jtr_bf16_packed");
+    hipLaunchKernelGGL(k_gemm_jtr_bf16c, dim3((unsigned)std::min<long>(nblk, 65535), (unsigned)(P / 64)),
+                       dim3(kBlock), 0, st, ctx->storedJ_bf16c.as<const bf16x8>(), ld, N, nblk,
+                       ctx->storedJ_rhs16.as<const bf16x8>(), P, out, N, nrhs);
+    return hipGetLastError();
+  }
   return for_each_block(ctx, [&](const float* J, long c0, long ncols) -> hipError_t {
     const long nblk = ncols / kColPad;
     const long nvalid = std::max<long>(0, std::min(ncols, N - c0));
@@ -902,7 +1085,228 @@ hipError_t run_jtr(jgm_context* ctx, int gemm_precision, const float* Xt, int P,
   });
 }
 
+// ---- FP64 J on the FP64 matrix cores -------------------------------------------------
+// v_mfma_f64_16x16x4_f64: A 16 x 4 (lane l holds A[l % 16][l / 16]), B 4 x 16 (lane l holds
+// B[l / 16][l % 16]), D 16 x 16 (lane l holds D[l / 16 + 4 r][l % 16] in element r).
+// The k index of one instruction may map to any four cells (rows) as long as A and B
+// agree: here k-step s of a 16-deep chunk takes cell (J M) or row (J^T R) cb + 4 g + s of
+// lane group g = l / 16, so that a lane's four J values of a chunk are contiguous in the
+// J^T R product.  A wave owns a 64 x 64 block of D as 4 x 4 tiles of 16 x 16 (FP64
+// accumulators, no flush needed).
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f64x4 mfma_f64(double a, double b, f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+struct Frag64 {
+  double a[4][4], b[4][4];  // [k-step][tile]
+};
+
+__device__ __forceinline__ void mma64(const Frag64& f, f64x4 (&acc)[4][4]) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[t][u] = mfma_f64(f.a[s][t], f.b[s][u], acc[t][u]);
+}
+
+// Y partials Yp[split][row][P] (FP64) of J M; grid and split layout as k_gemm_jm.
+__global__ __launch_bounds__(kBlock) void k_gemm_jm_f64(const double* __restrict__ J, long ld,
+                                                        long R, long colpad,
+                                                        const double* __restrict__ Mt, int P,
+                                                        long per, int row_tiles, int nsplit,
+                                                        double* __restrict__ Yp) {
+  const int w = blockIdx.x;
+  int split, rt;
+  if ((nsplit & 7) == 0) {
+    const int slot = w >> 3;
+    split = (w & 7) * (nsplit >> 3) + slot / row_tiles;
+    rt = slot % row_tiles;
+  } else {
+    split = w / row_tiles;
+    rt = w % row_tiles;
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int li = lane & 15, g = lane >> 4;
+  const long row0 = (long)rt * 256 + wave * 64;
+  if (row0 >= ld) return;  // whole wave outside the padded rows; no barriers follow
+  const int b0 = blockIdx.y * 64;
+  const long c0 = (long)split * per;
+  const long cend = c0 + per < colpad ? c0 + per : colpad;
+  const int nchunk = cend > c0 ? (int)((cend - c0) / kChunk) : 0;  // even
+  const double* Jw = J + row0 + li;
+  const double* Mw = Mt + b0 + li;
+  f64x4 acc[4][4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[t][u] = f64x4{0.0, 0.0, 0.0, 0.0};
+  Frag64 x, y;
+  auto load = [&](Frag64& f, long cb) {
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      const long cell = cb + 4 * g + st;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) f.a[st][t] = __builtin_nontemporal_load(Jw + cell * ld + 16 * t);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) f.b[st][u] = Mw[cell * P + 16 * u];
+    }
+  };
+  if (nchunk > 0) {
+    load(x, c0);
+    for (int q = 0; q < nchunk; q += 2) {
+      load(y, c0 + (long)(q + 1) * kChunk);
+      __builtin_amdgcn_sched_barrier(0);
+      mma64(x, acc);
+      __builtin_amdgcn_sched_barrier(0);
+      const int qn = q + 2 < nchunk ? q + 2 : nchunk - 1;  // the last reload is unused
+      load(x, c0 + (long)qn * kChunk);
+      __builtin_amdgcn_sched_barrier(0);
+      mma64(y, acc);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  double* out = Yp + (long)split * R * P;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const long row = row0 + 16 * t + g + 4 * r;
+      if (row < R) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) out[row * P + b0 + 16 * u + li] = acc[t][u][r];
+      }
+    }
+}
+
+// Y[row + b R] = sum over splits (fixed order) of Yp[split][row][b]
+__global__ __launch_bounds__(kBlock) void k_gemm_jm_reduce_f64(const double* __restrict__ Yp,
+                                                               int nsplit, long R, int P,
+                                                               int nrhs, double* __restrict__ Y) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= R * nrhs) return;
+  const long row = idx / nrhs;
+  const int b = (int)(idx % nrhs);
+  double a = 0.0;
+  for (int s = 0; s < nsplit; ++s) a += Yp[(long)s * R * P + row * P + b];
+  Y[row + (long)b * R] = a;
+}
+
+// G = J^T Rt over all rows: workgroup = 256 cells (wave: 64 cells) x 64 right-hand sides;
+// A = J^T (i = cell, k = row): a lane's four rows of a chunk are one 32-byte load from its
+// cell's column; B = Rt (k = row, j = right-hand side).  The next chunk's loads are in
+// flight while the current one is multiplied.
+__global__ __launch_bounds__(kBlock) void k_gemm_jtr_f64(const double* __restrict__ J, long ld,
+                                                         long N, long nblk,
+                                                         const double* __restrict__ Rt, int P,
+                                                         double* __restrict__ G, long ldg,
+                                                         int nrhs) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int li = lane & 15, g = lane >> 4;
+  const int b0 = blockIdx.y * 64;
+  const long nchunk = ld / kChunk;  // even: ld is a multiple of 64
+  for (long blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const long cbase = blk * kColPad + 64 * wave;
+    const double* Jw = J + (cbase + li) * ld + 4 * g;
+    const double* Rw = Rt + (long)(4 * g) * P + b0 + li;
+    f64x4 acc[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[t][u] = f64x4{0.0, 0.0, 0.0, 0.0};
+    Frag64 x, y;
+    auto load = [&](Frag64& f, long rb) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const f64x4 v = __builtin_nontemporal_load((const f64x4*)(Jw + (long)16 * t * ld + rb));
+#pragma unroll
+        for (int st = 0; st < 4; ++st) f.a[st][t] = v[st];
+      }
+#pragma unroll
+      for (int st = 0; st < 4; ++st)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) f.b[st][u] = Rw[(rb + st) * P + 16 * u];
+    };
+    load(x, 0);
+    for (long q = 0; q < nchunk; q += 2) {
+      load(y, (q + 1) * kChunk);
+      __builtin_amdgcn_sched_barrier(0);
+      mma64(x, acc);
+      __builtin_amdgcn_sched_barrier(0);
+      const long qn = q + 2 < nchunk ? q + 2 : nchunk - 1;  // the last reload is unused
+      load(x, qn * kChunk);
+      __builtin_amdgcn_sched_barrier(0);
+      mma64(y, acc);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long c = cbase + 16 * t + g + 4 * r;
+        if (c < N) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int b = b0 + 16 * u + li;
+            if (b < nrhs) G[c + (long)b * ldg] = acc[t][u][r];
+          }
+        }
+      }
+  }
+}
+
+// J M on Xt[colpad][P] (FP64 J in HBM) -> out (R x nrhs, column-major, FP64)
+hipError_t run_jm_f64(jgm_context* ctx, const double* Xt, int P, int nrhs, double* out) {
+  const long R = (long)ctx->storedJ_rows, ld = (long)ctx->storedJ_ld;
+  const long colpad = (long)ctx->storedJ_colpad;
+  const int rhs_tiles = P / 64;
+  const int row_tiles = (int)((ld + 255) / 256);
+  long nsplit = std::max<long>(1, 2L * ctx->cu_count / ((long)row_tiles * rhs_tiles));
+  nsplit = std::min<long>(nsplit, colpad / 32);
+  if (nsplit >= 16) nsplit = nsplit / 8 * 8;
+  const long per = round_up((colpad + nsplit - 1) / nsplit, 2 * kChunk);
+  hipError_t e = ctx->partial.ensure(sizeof(double) * (size_t)nsplit * R * P);
+  if (e != hipSuccess) return e;
+  {
+    ProfScope ps(ctx, "storedj_jm_fp64");
+    hipLaunchKernelGGL(k_gemm_jm_f64, dim3((unsigned)(nsplit * row_tiles), (unsigned)rhs_tiles),
+                       dim3(kBlock), 0, ctx->stream, ctx->storedJ.as<const double>(), ld, R,
+                       colpad, Xt, P, per, row_tiles, (int)nsplit, ctx->partial.as<double>());
+  }
+  hipLaunchKernelGGL(k_gemm_jm_reduce_f64, dim3(blocks_for(R * (long)nrhs)), dim3(kBlock), 0,
+                     ctx->stream, ctx->partial.as<const double>(), (int)nsplit, R, P, nrhs, out);
+  return hipGetLastError();
+}
+
+// J^T R on Xt[ld][P] (FP64 J in HBM) -> out (N x nrhs, column-major, FP64)
+hipError_t run_jtr_f64(jgm_context* ctx, const double* Xt, int P, int nrhs, double* out) {
+  const long N = (long)ctx->storedJ_cols, ld = (long)ctx->storedJ_ld;
+  const long nblk = (long)ctx->storedJ_colpad / kColPad;
+  ProfScope ps(ctx, "storedj_jtr_fp64");
+  hipLaunchKernelGGL(k_gemm_jtr_f64, dim3((unsigned)std::min<long>(nblk, 65535), (unsigned)(P / 64)),
+                     dim3(kBlock), 0, ctx->stream, ctx->storedJ.as<const double>(), ld, N, nblk,
+                     Xt, P, out, N, nrhs);
+  return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t launch_storedj_apply_f64(jgm_context* ctx, int transpose, const double* in,
+                                    size_t nrhs, double* out) {
+  const long R = (long)ctx->storedJ_rows, N = (long)ctx->storedJ_cols;
+  const long ld = (long)ctx->storedJ_ld, colpad = (long)ctx->storedJ_colpad;
+  const int P = (int)round_up((long)nrhs, 64);
+  const long K = transpose ? R : N, Kpad = transpose ? ld : colpad;
+  hipError_t e = ctx->coef32.ensure(sizeof(double) * (size_t)Kpad * P);
+  if (e != hipSuccess) return e;
+  double* Xt = ctx->coef32.as<double>();
+  hipLaunchKernelGGL(k_transpose_pad<double>, dim3(blocks_for(Kpad * P)), dim3(kBlock), 0,
+                     ctx->stream, in, K, Kpad, (int)nrhs, P, Xt);
+  return transpose ? run_jtr_f64(ctx, Xt, P, (int)nrhs, out)
+                   : run_jm_f64(ctx, Xt, P, (int)nrhs, out);
+}
 
 hipError_t launch_storedj_apply(jgm_context* ctx, int gemm_precision, int transpose,
                                 const float* in, size_t nrhs, float* out) {
@@ -913,8 +1317,8 @@ hipError_t launch_storedj_apply(jgm_context* ctx, int gemm_precision, int transp
   hipError_t e = ctx->coef32.ensure(sizeof(float) * (size_t)Kpad * P);
   if (e != hipSuccess) return e;
   float* Xt = ctx->coef32.as<float>();
-  hipLaunchKernelGGL(k_transpose_pad, dim3(blocks_for(Kpad * P)), dim3(kBlock), 0, ctx->stream,
-                     in, K, Kpad, (int)nrhs, P, Xt);
+  hipLaunchKernelGGL(k_transpose_pad<float>, dim3(blocks_for(Kpad * P)), dim3(kBlock), 0,
+                     ctx->stream, in, K, Kpad, (int)nrhs, P, Xt);
   return transpose ? run_jtr(ctx, gemm_precision, Xt, P, (int)nrhs, out)
                    : run_jm(ctx, gemm_precision, Xt, P, (int)nrhs, out);
 }
@@ -924,36 +1328,56 @@ hipError_t launch_storedj_forward(jgm_context* ctx, int gemm_precision, const do
   return launch_storedj_forward_batch(ctx, gemm_precision, 1, model, 0, out, 0, raw_out, 0);
 }
 
+namespace {
+
+// J M of P right-hand sides, FP32 modes on an FP32 J or FP64 on an FP64 J
+hipError_t run_jm_t(jgm_context* ctx, int gemm_precision, const float* Xt, int P, int nrhs,
+                    float* out) {
+  return run_jm(ctx, gemm_precision, Xt, P, nrhs, out);
+}
+hipError_t run_jm_t(jgm_context* ctx, int, const double* Xt, int P, int nrhs, double* out) {
+  return run_jm_f64(ctx, Xt, P, nrhs, out);
+}
+hipError_t run_jtr_t(jgm_context* ctx, int gemm_precision, const float* Xt, int P, int nrhs,
+                     float* out) {
+  return run_jtr(ctx, gemm_precision, Xt, P, nrhs, out);
+}
+hipError_t run_jtr_t(jgm_context* ctx, int, const double* Xt, int P, int nrhs, double* out) {
+  return run_jtr_f64(ctx, Xt, P, nrhs, out);
+}
+
 // Up to 64 models per J M product (the product's right-hand-side tile is 64 wide, so
-// a batch costs what one model costs); larger batches take several products.
-hipError_t launch_storedj_forward_batch(jgm_context* ctx, int gemm_precision, size_t nmodels,
-                                        const double* models, size_t model_ld, double* out,
-                                        size_t out_ld, double* raw_out, size_t raw_ld) {
+// a batch costs what one model costs); larger batches take several products.  T is the
+// element type of the stored J and of the products' operands.
+template <class T>
+hipError_t forward_batch_impl(jgm_context* ctx, int gemm_precision, size_t nmodels,
+                              const double* models, size_t model_ld, double* out, size_t out_ld,
+                              double* raw_out, size_t raw_ld) {
   const long R = (long)ctx->storedJ_rows, N = (long)ctx->storedJ_cols;
   const long colpad = (long)ctx->storedJ_colpad;
   const int kind = ctx->storedJ_kind;
   const int S = (int)ctx->S;
   const int L = has_background(kind) ? (int)ctx->bgthick.size() : 0;
   const int P = 64;
-  hipError_t e = ctx->coef32.ensure(sizeof(float) * (size_t)colpad * P);
+  hipError_t e = ctx->coef32.ensure(sizeof(T) * (size_t)colpad * P);
   if (e != hipSuccess) return e;
   const size_t nb_max = std::min<size_t>(nmodels, (size_t)P);
-  if ((e = ctx->storedJ_vec.ensure(sizeof(float) * (size_t)std::max(R * (long)nb_max, N))) !=
+  if ((e = ctx->storedJ_vec.ensure(sizeof(T) * (size_t)std::max(R * (long)nb_max, N))) !=
       hipSuccess)
     return e;
-  float* Xt = ctx->coef32.as<float>();
+  T* Xt = ctx->coef32.as<T>();
   const double tf_bx = std::cos(ctx->tf_inc) * std::cos(ctx->tf_dec);
   const double tf_by = std::cos(ctx->tf_inc) * std::sin(ctx->tf_dec);
   const double tf_bz = std::sin(ctx->tf_inc);
   for (size_t b0 = 0; b0 < nmodels; b0 += P) {
     const int nb = (int)std::min<size_t>(nmodels - b0, (size_t)P);
     const double* m = models + b0 * model_ld;
-    hipLaunchKernelGGL(k_vec_to_rhs, dim3(blocks_for(colpad * P)), dim3(kBlock), 0, ctx->stream,
-                       m, (long)model_ld, nb, N, colpad, P, Xt);
-    float* Y = ctx->storedJ_vec.as<float>();
-    if ((e = run_jm(ctx, gemm_precision, Xt, P, nb, Y)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_vec_to_rhs<T>, dim3(blocks_for(colpad * P)), dim3(kBlock), 0,
+                       ctx->stream, m, (long)model_ld, nb, N, colpad, P, Xt);
+    T* Y = ctx->storedJ_vec.as<T>();
+    if ((e = run_jm_t(ctx, gemm_precision, Xt, P, nb, Y)) != hipSuccess) return e;
     for (int b = 0; b < nb; ++b)
-      hipLaunchKernelGGL(k_storedj_finish, dim3(blocks_for(S)), dim3(kBlock), 0, ctx->stream,
+      hipLaunchKernelGGL(k_storedj_finish<T>, dim3(blocks_for(S)), dim3(kBlock), 0, ctx->stream,
                          Y + (size_t)b * R, S, kind, ctx->bgT.as<const double>(),
                          m + b * model_ld + (L > 0 ? N : 0), L, field_dir(ctx), ctx->transform,
                          tf_bx, tf_by, tf_bz, out + (b0 + b) * out_ld,
@@ -963,35 +1387,31 @@ hipError_t launch_storedj_forward_batch(jgm_context* ctx, int gemm_precision, si
   return hipSuccess;
 }
 
-hipError_t launch_storedj_gradient(jgm_context* ctx, int gemm_precision,
-                                   const double* raw_misfit, double* grad) {
-  return launch_storedj_gradient_batch(ctx, gemm_precision, 1, raw_misfit, 0, grad, 0);
-}
-
-hipError_t launch_storedj_gradient_batch(jgm_context* ctx, int gemm_precision, size_t nsets,
-                                         const double* raw_misfits, size_t misfit_ld,
-                                         double* grads, size_t grad_ld) {
+template <class T>
+hipError_t gradient_batch_impl(jgm_context* ctx, int gemm_precision, size_t nsets,
+                               const double* raw_misfits, size_t misfit_ld, double* grads,
+                               size_t grad_ld) {
   const long R = (long)ctx->storedJ_rows, N = (long)ctx->storedJ_cols;
   const long ld = (long)ctx->storedJ_ld;
   const int kind = ctx->storedJ_kind;
   const int P = 64;
-  hipError_t e = ctx->coef32.ensure(sizeof(float) * (size_t)ld * P);
+  hipError_t e = ctx->coef32.ensure(sizeof(T) * (size_t)ld * P);
   if (e != hipSuccess) return e;
   const size_t nb_max = std::min<size_t>(nsets, (size_t)P);
-  if ((e = ctx->storedJ_vec.ensure(sizeof(float) * (size_t)std::max(R, N * (long)nb_max))) !=
+  if ((e = ctx->storedJ_vec.ensure(sizeof(T) * (size_t)std::max(R, N * (long)nb_max))) !=
       hipSuccess)
     return e;
-  float* Xt = ctx->coef32.as<float>();
+  T* Xt = ctx->coef32.as<T>();
   for (size_t b0 = 0; b0 < nsets; b0 += P) {
     const int nb = (int)std::min<size_t>(nsets - b0, (size_t)P);
     const double* rm = raw_misfits + b0 * misfit_ld;
-    hipLaunchKernelGGL(k_rm_to_compact, dim3(blocks_for(ld * P)), dim3(kBlock), 0, ctx->stream,
-                       rm, (long)misfit_ld, nb, (int)ctx->S, kind, ld, P, Xt);
-    float* G = ctx->storedJ_vec.as<float>();
-    if ((e = run_jtr(ctx, gemm_precision, Xt, P, nb, G)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_rm_to_compact<T>, dim3(blocks_for(ld * P)), dim3(kBlock), 0,
+                       ctx->stream, rm, (long)misfit_ld, nb, (int)ctx->S, kind, ld, P, Xt);
+    T* G = ctx->storedJ_vec.as<T>();
+    if ((e = run_jtr_t(ctx, gemm_precision, Xt, P, nb, G)) != hipSuccess) return e;
     for (int b = 0; b < nb; ++b) {
       double* g = grads + (b0 + b) * grad_ld;
-      hipLaunchKernelGGL(k_scale_to_f64, dim3(blocks_for(N)), dim3(kBlock), 0, ctx->stream,
+      hipLaunchKernelGGL(k_scale_to_f64<T>, dim3(blocks_for(N)), dim3(kBlock), 0, ctx->stream,
                          G + (size_t)b * N, N, 2.0, g);
       if ((e = hipGetLastError()) != hipSuccess) return e;
       if (has_background(kind))
@@ -1002,6 +1422,33 @@ hipError_t launch_storedj_gradient_batch(jgm_context* ctx, int gemm_precision, s
     }
   }
   return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t launch_storedj_forward_batch(jgm_context* ctx, int gemm_precision, size_t nmodels,
+                                        const double* models, size_t model_ld, double* out,
+                                        size_t out_ld, double* raw_out, size_t raw_ld) {
+  if (ctx->storedJ_fp64)
+    return forward_batch_impl<double>(ctx, gemm_precision, nmodels, models, model_ld, out,
+                                      out_ld, raw_out, raw_ld);
+  return forward_batch_impl<float>(ctx, gemm_precision, nmodels, models, model_ld, out, out_ld,
+                                   raw_out, raw_ld);
+}
+
+hipError_t launch_storedj_gradient(jgm_context* ctx, int gemm_precision,
+                                   const double* raw_misfit, double* grad) {
+  return launch_storedj_gradient_batch(ctx, gemm_precision, 1, raw_misfit, 0, grad, 0);
+}
+
+hipError_t launch_storedj_gradient_batch(jgm_context* ctx, int gemm_precision, size_t nsets,
+                                         const double* raw_misfits, size_t misfit_ld,
+                                         double* grads, size_t grad_ld) {
+  if (ctx->storedJ_fp64)
+    return gradient_batch_impl<double>(ctx, gemm_precision, nsets, raw_misfits, misfit_ld,
+                                       grads, grad_ld);
+  return gradient_batch_impl<float>(ctx, gemm_precision, nsets, raw_misfits, misfit_ld, grads,
+                                    grad_ld);
 }
 
 }  // namespace jgm

@@ -287,9 +287,10 @@ class CachedGravMagCalculator(ThreeDGravMagCalculator):
 
 class FullSensitivityGravMagCalculator(CachedGravMagCalculator):
     """Stored sensitivities (jif3D/GravMag/FullSensitivityGravMagCalculator.h:99-221),
-    kept in HBM as FP32 in the compact row layout: the first Calculate for a geometry
-    builds J and returns the imp's direct result; later calls are J m (plus the
-    background layers) and 2 [J B]^T D^T misfit on the matrix cores in `gemm_precision`."""
+    kept in HBM in the compact row layout: the first Calculate for a geometry builds J
+    and returns the imp's direct result; later calls are J m (plus the background layers)
+    and 2 [J B]^T D^T misfit on the matrix cores in `gemm_precision`.  J is FP32, or FP64
+    (the reference's precision) with gemm_precision=GEMM_FP64."""
 
     host_block_cells = None
 
@@ -299,7 +300,8 @@ class FullSensitivityGravMagCalculator(CachedGravMagCalculator):
 
     def CalculateNewModel(self, model, data):
         ctx = self.Imp.context(model, data)
-        ctx.storedj_build(self.Imp.kind, self.Imp.precision, self.host_block_cells)
+        ctx.storedj_build(self.Imp.kind, self.Imp.precision, self.host_block_cells,
+                          fp64=self.gemm_precision == nat.GEMM_FP64)
         return self.Imp.Calculate(model, data)
 
     def CalculateCachedResult(self, model, data):

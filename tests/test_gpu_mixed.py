@@ -206,8 +206,9 @@ def test_mixed_sites_inside_the_mesh():
     """Borehole sites: inside cells, on cell faces, edges and corners.  The cells around
     such a site fall in its FP64 near box; the far field sees only distant cells.  At a
     site on a mesh line the reference's guarded log gives every cell along that line an
-    off-diagonal artifact (tests/test_oracle_guard.py) that the FP64 path reproduces and
-    the far-field expansion does not, so there only gz and the diagonal are compared."""
+    off-diagonal term (tests/test_oracle_guard.py) that the far-field expansion does not
+    have, so the mixed path evaluates such sites with the FP64 closed form throughout:
+    every component of every site, and the gradient of all residuals, agree with FP64."""
     m = cube_mesh(48, 24)
     rng = np.random.default_rng(21)
     m.SetDensities(2000.0 + 800.0 * rng.random(m.ncells))
@@ -226,14 +227,18 @@ def test_mixed_sites_inside_the_mesh():
     assert np.all(np.isfinite(f64)) and np.all(np.isfinite(mix))
     scale = np.abs(f64).max(axis=0)
     err = np.abs(mix - f64) / scale
-    assert err[:250].max() <= TOL  # sites off the mesh lines: every component
-    gz_diag = [0, 1, 5, 9]  # gz, Uxx, Uyy, Uzz
-    assert err[250:, gz_diag].max() <= TOL
+    assert err.max() <= TOL
+    # the five sites with two or three coordinates on corner planes took the FP64 path
+    assert np.max(np.abs(mix[[250, 251, 253, 254, 255]] - f64[[250, 251, 253, 254, 255]]) /
+                  scale) <= 1e-12
     r = rng.normal(size=f64.size)
-    r[2500:] = 0.0  # the gradient of the sites off the mesh lines
     g64 = ctx.adjoint(K, nat.FP64, mv, r)
     gmx = ctx.adjoint(K, nat.MIXED, mv, r)
     assert rel_inf(gmx, g64) <= TOL
+    # a mesh-line site alone: its gradient is the FP64 one
+    r1 = np.zeros_like(r)
+    r1[251 * 10:252 * 10] = r[251 * 10:252 * 10]
+    assert rel_inf(ctx.adjoint(K, nat.MIXED, mv, r1), ctx.adjoint(K, nat.FP64, mv, r1)) <= 1e-12
 
 
 def test_mixed_large_ragged_survey():
@@ -332,8 +337,9 @@ def test_mixed_adjoint_deep_mesh_borehole():
     m.SetDensities(2670 + rng.normal(0, 100, m.ncells))
     d = TensorGravityData()
     xs, ys, zs = [], [], []
-    for bx, by in ((35.0, 42.0), (81.0, 120.0), (150.0, 7.5)):
-        for z in np.linspace(-5.0, 900.0, 19):
+    # no two coordinates of a site on cell boundaries (mesh lines are tested apart)
+    for bx, by in ((35.0, 42.0), (81.0, 123.0), (152.5, 7.5)):
+        for z in np.linspace(-5.0, 895.0, 19):
             xs.append(bx)
             ys.append(by)
             zs.append(z)

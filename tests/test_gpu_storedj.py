@@ -291,12 +291,15 @@ def test_bf16_copy_follows_a_rebuilt_J():
     assert np.array_equal(ys[0], ys[2])
 
 
-def test_bf16_packed_copy_equals_rounding_in_registers(monkeypatch):
-    """The packed BF16 copy of J holds the BF16 roundings of the FP32 J in the k order of
-    the MFMA steps, so the BF16 J M that reads it equals, bit for bit, the one that rounds
-    the FP32 J in registers (the copy switched off with JIF3D_AMD_BF16_COPY=0 when the
-    context is created)."""
-    m, d = small_mesh()
+@pytest.mark.parametrize("shape", [(12, 10, 8, 4, 4), (20, 17, 13, 11, 9)])
+def test_bf16_packed_copy_equals_rounding_in_registers(monkeypatch, shape):
+    """The BF16 copies of J hold the BF16 roundings of the FP32 J (packed eight cells of a
+    row per 16 bytes for J M, in J's own column-major layout for J^T R) and the products
+    that read them use the same k order and flush interval, so they equal, bit for bit, the
+    products that round the FP32 J in registers (the copies switched off with
+    JIF3D_AMD_BF16_COPY=0 when the context is created)."""
+    nx, ny, nz, nsx, nsy = shape
+    m, d = small_mesh(nx, ny, nz, nsx, nsy)
     rng = np.random.default_rng(17)
     M = rng.normal(size=(6, m.ncells)).astype(np.float32)
     outs = []
@@ -305,11 +308,16 @@ def test_bf16_packed_copy_equals_rounding_in_registers(monkeypatch):
         ctx = make_ctx(m, d)
         ctx.storedj_build(nat.KIND_SCALAR_TENSOR, nat.FP64)
         R = ctx.storedj_rows()
+        Rr = rng.normal(size=(70, R)).astype(np.float32) if flag == "1" else Rr
         Y = ctx.alloc((6, R), np.float32)
+        G = ctx.alloc((70, m.ncells), np.float32)
         ctx.profile_enable(True)
         ctx.storedj_apply(nat.GEMM_BF16, ctx.to_device(M), Y)
+        ctx.storedj_apply_t(nat.GEMM_BF16, ctx.to_device(Rr), G)
         ctx.synchronize()
         names = set(ctx.profile())
         assert ("storedj_jm_bf16_packed" in names) == (flag == "1"), names
-        outs.append(Y.to_host())
-    assert np.array_equal(outs[0], outs[1])
+        assert ("storedj_jtr_bf16_packed" in names) == (flag == "1"), names
+        outs.append((Y.to_host(), G.to_host()))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])

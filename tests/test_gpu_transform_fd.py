@@ -6,9 +6,11 @@ jif3D/Gravity/test_transforms.cpp:124-177 for the FTG invariant).
 Setup as there: a random model of at most 5 cells per axis (MakeRandomModel,
 jif3D/Gravity/test_common.h:31-67, seeded here), 3 sites, observed data 1.1 times the
 synthetic data, unit errors; misfit sum (T(F(m)) - d_obs)^2, gradient LQDerivative(m, T(F(m))
-- d_obs), central differences with h = 1 % of each parameter, tolerance 0.01 % (the
-reference's).  Both precisions; the model is small, so the mixed path's near boxes hold
-every cell and it equals the FP64 closed form."""
+- d_obs), central differences, tolerance 0.01 % (the reference's).  The reference steps by
+1 % of each parameter; the total field's square root leaves an O(h^2) truncation of up to
+3e-4 there on some seeds, above that tolerance, so the steps here are 0.1 %.  Both
+precisions; the model is small, so the mixed path's near boxes hold every cell and it
+equals the FP64 closed form."""
 import numpy as np
 import pytest
 
@@ -73,7 +75,7 @@ def test_transformed_gradient_fd(orc, tf_name, seed, prec):
     grad = calc.LQDerivative(m, d, diff)
     assert grad.shape == mv.shape
     for i in range(mv.size):
-        h = 0.01 * mv[i]
+        h = 0.001 * mv[i]
         xp, xm = mv.copy(), mv.copy()
         xp[i] += h
         xm[i] -= h
