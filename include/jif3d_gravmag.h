@@ -144,6 +144,24 @@ jgm_status jgm_forward(jgm_context *ctx, jgm_kind kind, jgm_precision prec, cons
 jgm_status jgm_forward_host(jgm_context *ctx, jgm_kind kind, jgm_precision prec,
                             const double *model, double *out);
 
+/* Allocation-free form of jgm_forward, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  It calls no hipMalloc / hipFree and no allocating
+ * library routine: its working memory (background table, corner weights, partial sums)
+ * is one static device array of JGM_NOALLOC_SCRATCH_BYTES per GPU, reserved when the
+ * library is loaded and held until the process ends.  Same arguments and results as
+ * jgm_forward with JGM_FP64 (sums grouped into fewer chunks, so results agree to FP64
+ * rounding, not bit for bit); JGM_MIXED, JGM_KIND_MAGNETIC_GRADIENT and
+ * JGM_KIND_MAGNETIZATION return JGM_ERR_UNSUPPORTED, and so does a problem whose
+ * (mesh corners + 7 nsites nlayers + components nsites + nlayers) doubles exceed the
+ * scratch.  Unlike jgm_forward it keeps no raw data for a later jgm_adjoint.  The mesh,
+ * background and sites must have been set (those setters allocate).  All calls on one
+ * GPU share its scratch array: calls that overlap in time on one GPU (other threads,
+ * other streams, other contexts) race and must be serialised by the caller. */
+#define JGM_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+jgm_status jgm_forward_noalloc(jgm_context *ctx, jgm_kind kind, jgm_precision prec,
+                               const double *model, double *out);
+
 /* Least-squares derivative 2 J^T D^T misfit, matrix free (J recomputed, never stored):
  * ThreeDGravMagImplementation::LQDerivative (:177-231).  model is needed for the
  * transform Jacobian D.  misfit: nsites * data_per_site; grad: jgm_model_size(kind)

@@ -28,7 +28,8 @@ EXPORTS = [
     "jgm_set_stream", "jgm_synchronize", "jgm_set_mesh", "jgm_set_background", "jgm_set_sites",
     "jgm_set_field", "jgm_set_transform", "jgm_set_mixed_threshold", "jgm_set_mixed_far_radius",
     "jgm_data_per_site",
-    "jgm_n_params", "jgm_forward", "jgm_forward_host", "jgm_adjoint", "jgm_adjoint_host",
+    "jgm_n_params", "jgm_forward", "jgm_forward_host", "jgm_forward_noalloc", "jgm_adjoint",
+    "jgm_adjoint_host",
     "jgm_forward_batch", "jgm_adjoint_batch", "jgm_forward_batch_host", "jgm_adjoint_batch_host",
     "jgm_storedj_forward_batch", "jgm_storedj_gradient_batch", "jgm_storedj_forward_batch_host",
     "jgm_storedj_gradient_batch_host", "jgm_storedj_build_host", "jgm_storedj_host_resident",
@@ -81,6 +82,7 @@ def load():
         "jgm_set_mixed_far_radius": [vp, d],
         "jgm_forward": [vp, i, i, vp, vp],
         "jgm_forward_host": [vp, i, i, pd, pd],
+        "jgm_forward_noalloc": [vp, i, i, vp, vp],
         "jgm_adjoint": [vp, i, i, vp, vp, vp],
         "jgm_adjoint_host": [vp, i, i, pd, pd, pd],
         "jgm_forward_batch": [vp, i, i, sz, vp, sz, vp, sz],

@@ -169,6 +169,8 @@ jgm_status jgm_create(int device, void* stream, jgm_context** out) {
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
       cus > 0)
     ctx->cu_count = cus;
+  // the device address of the allocation-free forward's scratch array, once per device
+  jgm::noalloc_scratch_init(device);
   *out = ctx;
   return JGM_OK;
 }
@@ -412,6 +414,34 @@ jgm_status jgm_forward(jgm_context* ctx, jgm_kind kind, jgm_precision prec, cons
       return hip_fail(e, "forward");
     ctx->raw_cache = {true, (int)kind, (int)prec, ctx->config_gen, nmodel};
   }
+  return JGM_OK;
+}
+
+jgm_status jgm_forward_noalloc(jgm_context* ctx, jgm_kind kind, jgm_precision prec,
+                               const double* model, double* out) {
+  jgm_status st = check_ready(ctx, kind, prec);
+  if (st != JGM_OK) return st;
+  if (prec != JGM_FP64)
+    return fail(JGM_ERR_UNSUPPORTED, "jgm_forward_noalloc runs in JGM_FP64 only");
+  if (kind == JGM_KIND_MAGNETIC_GRADIENT || kind == JGM_KIND_MAGNETIZATION)
+    return fail(JGM_ERR_UNSUPPORTED,
+                "jgm_forward_noalloc: the scalar, tensor, magnetic and scalar+tensor kinds only");
+  if (!model || !out) return fail(JGM_ERR_INVALID_ARGUMENT, "null model or output pointer");
+  if (ctx->S == 0) return JGM_OK;
+  const size_t need = jgm::noalloc_forward_need(ctx, kind);
+  if (need > (size_t)JGM_NOALLOC_SCRATCH_BYTES)
+    return fail(JGM_ERR_UNSUPPORTED,
+                "jgm_forward_noalloc: this mesh and survey need %zu bytes of scratch "
+                "(corner weights, background table, one chunk of partial sums); the "
+                "allocation-free entry point has %zu",
+                need, (size_t)JGM_NOALLOC_SCRATCH_BYTES);
+  DeviceGuard g(ctx->device);
+  if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+  hipError_t e = jgm::launch_forward_fp64_noalloc(ctx, kind, model, out);
+  if (e == hipErrorOutOfMemory)
+    return fail(JGM_ERR_UNSUPPORTED, "jgm_forward_noalloc: scratch of %zu bytes too small",
+                (size_t)JGM_NOALLOC_SCRATCH_BYTES);
+  if (e != hipSuccess) return hip_fail(e, "jgm_forward_noalloc");
   return JGM_OK;
 }
 

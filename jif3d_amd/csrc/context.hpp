@@ -224,6 +224,10 @@ hipError_t launch_adjoint_mixed_batch(jgm_context* ctx, int kind, size_t nmodels
                                       const double* raw_misfits, size_t rm_ld, double* grads,
                                       size_t grad_ld);
 hipError_t build_near_boxes(jgm_context* ctx);
+// allocation-free FP64 forward (fp64_path.hip): the device scratch array's address is
+// looked up once per device (called by jgm_create with the device current);
+// noalloc_forward_need gives the bytes of scratch one call needs with a single chunk
+// of corner columns; the launch returns hipErrorOutOfMemory when even that does not fit
 // FP64 closed form for F of the sites (coordinates fx, fy, fz, survey indices idx):
 // the forward writes their data into out / raw_out at their own indices; the adjoint adds
 // their cell gradients (rm: raw misfits of the whole survey) into grad
@@ -234,6 +238,11 @@ hipError_t launch_adjoint_fp64_sites_add(jgm_context* ctx, int kind, int F, cons
                                          const double* fy, const double* fz, const int* idx,
                                          const double* rm, double* grad);
 hipError_t launch_zero_site_rows(jgm_context* ctx, double* a, const int* idx, int F, int k);
+void noalloc_scratch_init(int device);
+double* noalloc_scratch(int device, hipError_t* err);
+size_t noalloc_forward_need(const jgm_context* ctx, int kind);
+hipError_t launch_forward_fp64_noalloc(jgm_context* ctx, int kind, const double* model,
+                                       double* out);
 hipError_t launch_fill_zero(jgm_context* ctx, double* p, size_t n);
 // out[i] = sum over c < nchunks of part[c * n + i] (fixed order, two passes for many chunks)
 hipError_t launch_reduce_chunks(jgm_context* ctx, const double* part, int nchunks, long n,

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "context.hpp"
 #include "prism_device.hpp"
@@ -1147,6 +1148,155 @@ hipError_t launch_zero_site_rows(jgm_context* ctx, double* a, const int* idx, in
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free FP64 forward (jgm_forward_noalloc).  Its working memory -- layer
+// thicknesses, background table, corner weights, chunk partials and their sums -- lives
+// in `scratch`, one file-scope array per GPU of JGM_NOALLOC_SCRATCH_BYTES, held until the
+// process ends.  In host code `scratch` is a stand-in symbol, not a device address: the
+// kernels get the address hipGetSymbolAddress returned for the device, looked up once per
+// device when the first context on it is created.  Every region a kernel reads is written
+// earlier in the same call (weights, table and partials cover their whole extent), so
+// nothing needs clearing between calls; calls that overlap on one GPU race on it.
+namespace {
+constexpr size_t kNoallocDoubles = JGM_NOALLOC_SCRATCH_BYTES / sizeof(double);
+static __device__ __align__(256) double scratch[kNoallocDoubles];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+bool g_scratch_looked_up[kMaxScratchDevices] = {};
+double* g_scratch_ptr[kMaxScratchDevices] = {};
+hipError_t g_scratch_err[kMaxScratchDevices] = {};
+
+// offsets in doubles rounded up to 256 bytes
+inline size_t align32(size_t n) { return (n + 31) / 32 * 32; }
+
+// Scratch layout of one no-alloc forward: [thick L][bgT 7 S L][weights ncorners]
+// [partials nchunks NC S][partsum NC S][group sums]; 0 when even one chunk does not fit.
+struct NoallocPlan {
+  size_t thick = 0, bg = 0, w = 0, part = 0, psum = 0, gsum = 0, end = 0;
+  int nchunks = 0;
+  long cols_per_chunk = 0;
+};
+
+size_t noalloc_layout(NoallocPlan* p, size_t L, size_t S, size_t nq, size_t NC, int nchunks) {
+  p->thick = 0;
+  p->bg = align32(p->thick + L);
+  p->w = align32(p->bg + 7 * S * L);
+  p->part = align32(p->w + nq);
+  p->psum = align32(p->part + (size_t)nchunks * NC * S);
+  p->gsum = align32(p->psum + NC * S);
+  int per = 0;
+  const long groups = nchunks > 0 ? reduce_groups(nchunks, (long)(NC * S), &per) : 1;
+  p->end = p->gsum + (groups > 1 ? (size_t)groups * NC * S : 0);
+  p->nchunks = nchunks;
+  return p->end;
+}
+}  // namespace
+
+void noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) return;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_looked_up[device]) return;
+  void* ptr = nullptr;
+  // the caller has made `device` current
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  g_scratch_err[device] = e;
+  g_scratch_ptr[device] = e == hipSuccess ? static_cast<double*>(ptr) : nullptr;
+  g_scratch_looked_up[device] = true;
+}
+
+double* noalloc_scratch(int device, hipError_t* err) {
+  if (device < 0 || device >= kMaxScratchDevices) {
+    *err = hipErrorInvalidDevice;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  *err = g_scratch_looked_up[device] ? g_scratch_err[device] : hipErrorNotInitialized;
+  return g_scratch_ptr[device];
+}
+
+size_t noalloc_forward_need(const jgm_context* ctx, int kind) {
+  NoallocPlan p;
+  const size_t NC = mask_of(kind) == kMaskScalar ? Ncomp<kMaskScalar>::value
+                    : mask_of(kind) == kMaskAll  ? Ncomp<kMaskAll>::value
+                                                 : Ncomp<kMaskTensor>::value;
+  return sizeof(double) * noalloc_layout(&p, ctx->bgthick.size(), ctx->S,
+                                         ncells(ctx) > 0 ? ncorners(ctx) : 0, NC,
+                                         ncells(ctx) > 0 ? 1 : 0);
+}
+
+namespace {
+template <int MASK>
+hipError_t forward_noalloc_impl(jgm_context* ctx, int kind, double* base, const double* model,
+                                double* out) {
+  constexpr int NC = Ncomp<MASK>::value;
+  hipStream_t st = ctx->stream;
+  const int S = (int)ctx->S;
+  const long N = (long)ncells(ctx);
+  const int L = (int)ctx->bgthick.size();
+  const int ncx = (int)ctx->nx + 1, ncy = (int)ctx->ny + 1, ncz = (int)ctx->nz + 1;
+  const long ncols = (long)ncx * ncy;
+  const size_t nq = N > 0 ? (size_t)ncols * ncz : 0;
+  const int site_blocks = (S + kBlock - 1) / kBlock;
+  // as many chunks as forward_impl takes, or as many as fit
+  NoallocPlan plan;
+  int nchunks = 0;
+  if (N > 0) {
+    const long waves_per_chunk = (long)site_blocks * (kBlock / 64);
+    long want = std::min<long>((kTargetWaves + waves_per_chunk - 1) / waves_per_chunk, ncols);
+    want = std::max<long>(1, want);
+    while (want > 1 && noalloc_layout(&plan, L, S, nq, NC, (int)want) > kNoallocDoubles)
+      want = want * 3 / 4;
+    const long per = (ncols + want - 1) / want;
+    nchunks = (int)((ncols + per - 1) / per);
+    plan.cols_per_chunk = per;
+  }
+  if (noalloc_layout(&plan, L, S, nq, NC, nchunks) > kNoallocDoubles) return hipErrorOutOfMemory;
+  double* thick = base + plan.thick;
+  double* bgT = base + plan.bg;
+  if (L > 0) {
+    hipError_t e = hipMemcpyAsync(thick, ctx->bgthick.data(), sizeof(double) * L,
+                                  hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_background_table, dim3(blocks_for(S)), dim3(kBlock), 0, st,
+                       ctx->d_sx.as<double>(), ctx->d_sy.as<double>(), ctx->d_sz.as<double>(), S,
+                       static_cast<const double*>(thick), L, ctx->xc[0], ctx->yc[0], ctx->zc[0],
+                       ctx->xwidth, ctx->ywidth, ctx->zwidth, bgT);
+  }
+  if (nchunks > 0) {
+    double* w = base + plan.w;
+    double* part = base + plan.part;
+    hipLaunchKernelGGL(k_corner_weights, dim3(blocks_for(ncols * ncz)), dim3(kBlock), 0, st,
+                       model, 0L, 1, (int)ctx->nx, (int)ctx->ny, (int)ctx->nz, w);
+    launch_forward_corners<MASK>(1, dim3(site_blocks, nchunks), st, ctx->d_sx.as<const double>(),
+                                 ctx->d_sy.as<const double>(), ctx->d_sz.as<const double>(), S,
+                                 ctx->d_Xc.as<const double>(), ctx->d_Yc.as<const double>(),
+                                 ctx->d_Zc.as<const double>(), ncy, ncz,
+                                 static_cast<const double*>(w), ncols, plan.cols_per_chunk, part);
+    reduce_chunks_with(st, part, nchunks, (long)NC * S, base + plan.psum, base + plan.gsum);
+  }
+  const double tf_bx = std::cos(ctx->tf_inc) * std::cos(ctx->tf_dec);
+  const double tf_by = std::cos(ctx->tf_inc) * std::sin(ctx->tf_dec);
+  const double tf_bz = std::sin(ctx->tf_inc);
+  hipLaunchKernelGGL((k_forward_finish<MASK>), dim3(blocks_for(S)), dim3(kBlock), 0, st,
+                     static_cast<const double*>(base + plan.psum), nchunks > 0 ? 1 : 0, S,
+                     static_cast<const double*>(bgT), model + N, L, kind, field_dir(ctx),
+                     ctx->transform, tf_bx, tf_by, tf_bz, out, nullptr);
+  return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_forward_fp64_noalloc(jgm_context* ctx, int kind, const double* model,
+                                       double* out) {
+  hipError_t e = hipSuccess;
+  double* base = noalloc_scratch(ctx->device, &e);
+  if (!base) return e == hipSuccess ? hipErrorInvalidValue : e;
+  switch (mask_of(kind)) {
+    case kMaskScalar: return forward_noalloc_impl<kMaskScalar>(ctx, kind, base, model, out);
+    case kMaskAll: return forward_noalloc_impl<kMaskAll>(ctx, kind, base, model, out);
+    default: return forward_noalloc_impl<kMaskTensor>(ctx, kind, base, model, out);
+  }
+}
 
 namespace {
 __global__ __launch_bounds__(kBlock) void k_zero_flag(int* __restrict__ flag) {
