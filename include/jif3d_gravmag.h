@@ -125,6 +125,23 @@ jgm_status jgm_set_mixed_threshold(jgm_context *ctx, double q0);
  * hexadecapole for every far cell. */
 jgm_status jgm_set_mixed_far_radius(jgm_context *ctx, double q1);
 
+/* A-posteriori check of JGM_MIXED (on by default: 8 sites, 16 cells, tolerance 5e-7).
+ * The mixed path's error bound is relative to each component's largest magnitude over the
+ * survey (forward) and to the largest gradient entry (adjoint); it holds unless the cells'
+ * contributions cancel strongly (a nearly uniformly magnetised wide slab, whose external
+ * field is a small remainder of its cells' fields).  After a mixed forward of the scalar,
+ * tensor, magnetic or scalar+tensor kind the FP64 closed form is evaluated at nsites sample
+ * sites spread over the survey, after a mixed adjoint at ncells sample cells spread over the
+ * mesh, and when the largest sampled error, relative as above, exceeds `tolerance` the whole
+ * call is recomputed in FP64.  The check waits for the context's stream once per call.
+ * nsites = 0 turns it off (ncells = 0 turns off the adjoint part).  No counterpart in the
+ * reference, whose OMP imps are FP64 throughout. */
+jgm_status jgm_set_mixed_check(jgm_context *ctx, size_t nsites, size_t ncells, double tolerance);
+/* The sampled errors of the last checked mixed forward and adjoint, and the number of calls
+ * recomputed in FP64 since the context was created. */
+jgm_status jgm_mixed_check_report(const jgm_context *ctx, double *forward_error,
+                                  double *adjoint_error, size_t *fallbacks);
+
 /* Values per site after the transform, and the model vector length
  * (ncells + nlayers, ThreeDGravityModel::GetNModelParm). */
 size_t jgm_data_per_site(const jgm_context *ctx, jgm_kind kind);
@@ -313,6 +330,17 @@ jgm_status jgm_storedj_apply_t_f64(jgm_context *ctx, const double *R, size_t nrh
 jgm_status jgm_storedj_build_host(jgm_context *ctx, jgm_kind kind, jgm_precision prec,
                                   size_t block_cells);
 int jgm_storedj_host_resident(const jgm_context *ctx);
+/* Stored J larger than host memory too: as jgm_storedj_build_host, but J is written to a
+ * file in `directory` (the reference's DiskGravMagCalculator writes its J to a file in a
+ * temporary directory and reads it back for every cached product, DiskGravMagCalculator.h:
+ * 76-85, :127-200).  Every product reads the blocks back into two page-locked buffers, the
+ * read of block b + 1 overlapping the copy to HBM and the matrix-core work of block b;
+ * results equal the host-resident J's bit for bit.  The file is removed when J is released,
+ * rebuilt or the context destroyed.  Host memory used: 2 * block_cells * rows * 4 bytes.
+ * jgm_storedj_file_backed returns 1 when the current stored J is in a file. */
+jgm_status jgm_storedj_build_file(jgm_context *ctx, jgm_kind kind, jgm_precision prec,
+                                  size_t block_cells, const char *directory);
+int jgm_storedj_file_backed(const jgm_context *ctx);
 
 /* Device memory on the context's device, for callers without their own HIP
  * allocator (the CUDA host wrappers did this inside PrepareData/FreeData,

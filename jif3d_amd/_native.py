@@ -39,7 +39,8 @@ EXPORTS = [
     "jgm_storedj_forward_host", "jgm_storedj_gradient_host", "jgm_storedj_release", "jgm_device_alloc", "jgm_device_free",
     "jgm_copy_to_device", "jgm_copy_to_host", "jgm_copy_device", "jgm_profile_enable", "jgm_profile_reset",
     "jgm_profile_count", "jgm_profile_get", "jgm_storedj_build_fp64", "jgm_storedj_apply_f64",
-    "jgm_storedj_apply_t_f64",
+    "jgm_storedj_apply_t_f64", "jgm_set_mixed_check", "jgm_mixed_check_report",
+    "jgm_storedj_build_file", "jgm_storedj_file_backed",
 ]
 
 
@@ -104,6 +105,9 @@ def load():
         "jgm_storedj_gradient_batch_host": [vp, i, sz, pd, pd, pd],
         "jgm_storedj_release": [vp],
         "jgm_storedj_build_fp64": [vp, i],
+        "jgm_storedj_build_file": [vp, i, i, sz, ctypes.c_char_p],
+        "jgm_set_mixed_check": [vp, sz, sz, d],
+        "jgm_mixed_check_report": [vp, pd, pd, ctypes.POINTER(sz)],
         "jgm_storedj_apply_f64": [vp, vp, sz, vp],
         "jgm_storedj_apply_t_f64": [vp, vp, sz, vp],
         "jgm_device_alloc": [vp, sz, ctypes.POINTER(vp)],
@@ -128,6 +132,8 @@ def load():
     lib.jgm_model_size.argtypes = [vp, i]
     lib.jgm_storedj_host_resident.restype = i
     lib.jgm_storedj_host_resident.argtypes = [vp]
+    lib.jgm_storedj_file_backed.restype = i
+    lib.jgm_storedj_file_backed.argtypes = [vp]
     lib.jgm_storedj_rows.restype = sz
     lib.jgm_storedj_rows.argtypes = [vp]
     lib.jgm_profile_count.restype = sz
@@ -270,6 +276,18 @@ class Context:
     def set_mixed_threshold(self, q0):
         check(self.lib.jgm_set_mixed_threshold(self.handle, float(q0)))
 
+    def set_mixed_check(self, nsites, ncells, tolerance=5e-7):
+        """A-posteriori check of the mixed path (nsites = 0 turns it off)."""
+        check(self.lib.jgm_set_mixed_check(self.handle, int(nsites), int(ncells),
+                                           float(tolerance)))
+
+    def mixed_check_report(self):
+        """(sampled forward error, sampled adjoint error, calls recomputed in FP64)."""
+        fe, ae, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_size_t()
+        check(self.lib.jgm_mixed_check_report(self.handle, ctypes.byref(fe), ctypes.byref(ae),
+                                              ctypes.byref(n)))
+        return fe.value, ae.value, n.value
+
     def set_mixed_far_radius(self, q1):
         check(self.lib.jgm_set_mixed_far_radius(self.handle, float(q1)))
 
@@ -354,11 +372,16 @@ class Context:
         check(self.lib.jgm_sensitivity_rows(self.handle, kind, site0, nsites, _devptr(out)))
         return out
 
-    def storedj_build(self, kind, prec, host_block_cells=None, fp64=False):
+    def storedj_build(self, kind, prec, host_block_cells=None, fp64=False, directory=None):
         """Build the stored J in HBM, or with host_block_cells in page-locked host
-        memory, streamed to the device in blocks of that many cells; fp64 keeps J in FP64
-        (HBM only) for the GEMM_FP64 products."""
-        if fp64:
+        memory, streamed to the device in blocks of that many cells (with directory: in a
+        file there, read back block by block); fp64 keeps J in FP64 (HBM only) for the
+        GEMM_FP64 products."""
+        if directory is not None:
+            check(self.lib.jgm_storedj_build_file(self.handle, kind, prec,
+                                                  int(host_block_cells or 16384),
+                                                  os.fsencode(directory)))
+        elif fp64:
             if host_block_cells:
                 raise ValueError("the FP64 stored J is held in HBM only")
             check(self.lib.jgm_storedj_build_fp64(self.handle, kind))
@@ -370,6 +393,9 @@ class Context:
 
     def storedj_host_resident(self):
         return bool(self.lib.jgm_storedj_host_resident(self.handle))
+
+    def storedj_file_backed(self):
+        return bool(self.lib.jgm_storedj_file_backed(self.handle))
 
     def storedj_rows(self):
         return self.lib.jgm_storedj_rows(self.handle)

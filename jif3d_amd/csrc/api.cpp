@@ -2,6 +2,7 @@
 // dispatch to the HIP kernels.  See include/jif3d_gravmag.h for the contract.
 #include <hip/hip_runtime.h>
 
+#include <cerrno>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -28,6 +29,9 @@ jgm_status fail(jgm_status st, const char* fmt, ...) {
 }
 
 jgm_status hip_fail(hipError_t e, const char* where) {
+  // a failed runtime call also leaves its code as the thread's last error; consume it so
+  // the next launch check of an unrelated call does not report it again
+  (void)hipGetLastError();
   if (e == hipErrorOutOfMemory)
     return fail(JGM_ERR_OUT_OF_MEMORY, "%s: out of device memory", where);
   return fail(JGM_ERR_DEVICE, "%s: HIP error %d (%s)", where, (int)e, hipGetErrorString(e));
@@ -84,6 +88,39 @@ jgm_status upload(jgm::DeviceBuffer& b, const double* h, size_t n, hipStream_t s
   if (n) {
     e = hipMemcpyAsync(b.ptr, h, sizeof(double) * n, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(e, what);
+  }
+  return JGM_OK;
+}
+
+// After a mixed forward: FP64 at the sample sites against the mixed raw data; above the
+// tolerance the whole forward is recomputed in FP64.
+jgm_status mixed_forward_check(jgm_context* ctx, int kind, const double* model, double* out,
+                               double* raw_out) {
+  if (!jgm::mixed_check_applies(ctx, kind)) return JGM_OK;
+  double err = 0.0;
+  hipError_t e = jgm::launch_mixed_check_forward(ctx, kind, model, raw_out ? raw_out : out, &err);
+  if (e != hipSuccess) return hip_fail(e, "forward (mixed check)");
+  ctx->check_last_fwd = err;
+  if (err > ctx->check_tol) {
+    ++ctx->check_fallbacks;
+    if ((e = jgm::launch_forward_fp64(ctx, kind, model, out, raw_out)) != hipSuccess)
+      return hip_fail(e, "forward (FP64 after the mixed check)");
+  }
+  return JGM_OK;
+}
+
+// After a mixed adjoint (raw misfit rm): FP64 at the sample cells against the mixed
+// gradient; above the tolerance the gradient is recomputed in FP64.
+jgm_status mixed_adjoint_check(jgm_context* ctx, int kind, const double* rm, double* grad) {
+  if (!jgm::mixed_check_applies(ctx, kind)) return JGM_OK;
+  double err = 0.0;
+  hipError_t e = jgm::launch_mixed_check_adjoint(ctx, kind, rm, grad, &err);
+  if (e != hipSuccess) return hip_fail(e, "adjoint (mixed check)");
+  ctx->check_last_adj = err;
+  if (err > ctx->check_tol) {
+    ++ctx->check_fallbacks;
+    if ((e = jgm::launch_adjoint_fp64(ctx, kind, nullptr, rm, grad)) != hipSuccess)
+      return hip_fail(e, "adjoint (FP64 after the mixed check)");
   }
   return JGM_OK;
 }
@@ -373,6 +410,29 @@ jgm_status jgm_set_mixed_far_radius(jgm_context* ctx, double q1) {
   return JGM_OK;
 }
 
+jgm_status jgm_set_mixed_check(jgm_context* ctx, size_t nsites, size_t ncells,
+                               double tolerance) {
+  if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
+  if (!(tolerance > 0.0) || !std::isfinite(tolerance))
+    return fail(JGM_ERR_INVALID_ARGUMENT, "the check tolerance must be positive");
+  if (nsites > 4096 || ncells > 65536)
+    return fail(JGM_ERR_UNSUPPORTED, "at most 4096 sample sites and 65536 sample cells");
+  ctx->check_sites = nsites;
+  ctx->check_cells = ncells;
+  ctx->check_tol = tolerance;
+  ctx->check_gen = 0;  // new samples
+  return JGM_OK;
+}
+
+jgm_status jgm_mixed_check_report(const jgm_context* ctx, double* forward_error,
+                                  double* adjoint_error, size_t* fallbacks) {
+  if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
+  if (forward_error) *forward_error = ctx->check_last_fwd;
+  if (adjoint_error) *adjoint_error = ctx->check_last_adj;
+  if (fallbacks) *fallbacks = ctx->check_fallbacks;
+  return JGM_OK;
+}
+
 size_t jgm_data_per_site(const jgm_context* ctx, jgm_kind kind) {
   return ctx ? data_per_site(ctx, kind) : 0;
 }
@@ -408,6 +468,8 @@ jgm_status jgm_forward(jgm_context* ctx, jgm_kind kind, jgm_precision prec, cons
   else
     e = jgm::launch_forward_fp64(ctx, kind, model, out, raw_out);
   if (e != hipSuccess) return hip_fail(e, "forward");
+  if (prec == JGM_MIXED && (st = mixed_forward_check(ctx, kind, model, out, raw_out)) != JGM_OK)
+    return st;
   if (raw_out) {
     if (nmodel && (e = hipMemcpyAsync(ctx->raw_cache_model.ptr, model, sizeof(double) * nmodel,
                                       hipMemcpyDeviceToDevice, ctx->stream)) != hipSuccess)
@@ -507,6 +569,7 @@ jgm_status jgm_adjoint(jgm_context* ctx, jgm_kind kind, jgm_precision prec, cons
   else
     e = jgm::launch_adjoint_fp64(ctx, kind, model, rm, grad);
   if (e != hipSuccess) return hip_fail(e, "adjoint");
+  if (prec == JGM_MIXED) return mixed_adjoint_check(ctx, kind, rm, grad);
   return JGM_OK;
 }
 
@@ -592,12 +655,26 @@ jgm_status jgm_forward_batch(jgm_context* ctx, jgm_kind kind, jgm_precision prec
   if ((st = ensure_background(ctx)) != JGM_OK) return st;
   hipError_t e = hipSuccess;
   if (prec == JGM_MIXED) {
-    e = jgm::launch_forward_mixed_batch(ctx, kind, nmodels, models, model_ld, out, out_ld,
-                                        nullptr, 0);
-  } else {
-    e = jgm::launch_forward_fp64_batch(ctx, kind, nmodels, models, model_ld, out, out_ld, nullptr,
-                                       0);
+    // with a transform the check compares raw data, kept here per model
+    const bool check = jgm::mixed_check_applies(ctx, kind) && ctx->transform != JGM_TF_NONE;
+    const size_t nraw = ctx->S * jgm::raw_per_site(kind);
+    double* raw = nullptr;
+    if (check) {
+      if ((e = ctx->raw.ensure(sizeof(double) * nmodels * nraw)) != hipSuccess)
+        return hip_fail(e, "forward_batch");
+      raw = ctx->raw.as<double>();
+    }
+    e = jgm::launch_forward_mixed_batch(ctx, kind, nmodels, models, model_ld, out, out_ld, raw,
+                                        nraw);
+    if (e != hipSuccess) return hip_fail(e, "forward_batch");
+    for (size_t b = 0; b < nmodels; ++b)
+      if ((st = mixed_forward_check(ctx, kind, models + b * model_ld, out + b * out_ld,
+                                    raw ? raw + b * nraw : nullptr)) != JGM_OK)
+        return st;
+    return JGM_OK;
   }
+  e = jgm::launch_forward_fp64_batch(ctx, kind, nmodels, models, model_ld, out, out_ld, nullptr,
+                                     0);
   if (e != hipSuccess) return hip_fail(e, "forward_batch");
   return JGM_OK;
 }
@@ -654,9 +731,13 @@ jgm_status jgm_adjoint_batch(jgm_context* ctx, jgm_kind kind, jgm_precision prec
   }
   if (prec == JGM_MIXED) {
     e = jgm::launch_adjoint_mixed_batch(ctx, kind, nmodels, rm, rm_ld, grads, grad_ld);
-  } else {
-    e = jgm::launch_adjoint_fp64_batch(ctx, kind, nmodels, rm, rm_ld, grads, grad_ld);
+    if (e != hipSuccess) return hip_fail(e, "adjoint_batch");
+    for (size_t b = 0; b < nmodels; ++b)
+      if ((st = mixed_adjoint_check(ctx, kind, rm + b * rm_ld, grads + b * grad_ld)) != JGM_OK)
+        return st;
+    return JGM_OK;
   }
+  e = jgm::launch_adjoint_fp64_batch(ctx, kind, nmodels, rm, rm_ld, grads, grad_ld);
   if (e != hipSuccess) return hip_fail(e, "adjoint_batch");
   return JGM_OK;
 }
@@ -723,9 +804,10 @@ jgm_status jgm_storedj_build(jgm_context* ctx, jgm_kind kind, jgm_precision prec
 jgm_status jgm_storedj_build_fp64(jgm_context* ctx, jgm_kind kind) {
   jgm_status st = check_ready(ctx, kind, JGM_FP64);
   if (st != JGM_OK) return st;
+  if (ctx->S == 0 || jgm::ncells(ctx) == 0)
+    return fail(JGM_ERR_INVALID_ARGUMENT, "no sites or no cells");
   DeviceGuard g(ctx->device);
   hipError_t e = jgm::launch_storedj_build_f64(ctx, kind);
-  if (e == hipErrorInvalidValue) return fail(JGM_ERR_INVALID_ARGUMENT, "no sites or no cells");
   if (e != hipSuccess) return hip_fail(e, "FP64 stored J build");
   return JGM_OK;
 }
@@ -743,6 +825,24 @@ jgm_status jgm_storedj_build_host(jgm_context* ctx, jgm_kind kind, jgm_precision
 
 int jgm_storedj_host_resident(const jgm_context* ctx) {
   return ctx && ctx->storedJ_kind && ctx->storedJ_host ? 1 : 0;
+}
+
+jgm_status jgm_storedj_build_file(jgm_context* ctx, jgm_kind kind, jgm_precision prec,
+                                  size_t block_cells, const char* directory) {
+  jgm_status st = check_ready(ctx, kind, prec);
+  if (st != JGM_OK) return st;
+  if (block_cells == 0) return fail(JGM_ERR_INVALID_ARGUMENT, "block_cells must be positive");
+  if (!directory || !*directory) return fail(JGM_ERR_INVALID_ARGUMENT, "no directory");
+  DeviceGuard g(ctx->device);
+  hipError_t e = jgm::launch_storedj_build_file(ctx, kind, block_cells, directory);
+  if (e == hipErrorFileNotFound)
+    return fail(JGM_ERR_DEVICE, "stored J file in %s: %s", directory, std::strerror(errno));
+  if (e != hipSuccess) return hip_fail(e, "file-backed stored J build");
+  return JGM_OK;
+}
+
+int jgm_storedj_file_backed(const jgm_context* ctx) {
+  return ctx && ctx->storedJ_kind && ctx->storedJ_fd >= 0 ? 1 : 0;
 }
 
 size_t jgm_storedj_rows(const jgm_context* ctx) {

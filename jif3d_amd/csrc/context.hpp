@@ -133,6 +133,18 @@ struct jgm_context {
   double mixed_q0 = 8.0;
   double mixed_q1 = 26.0;
 
+  // a-posteriori check of JGM_MIXED: FP64 at check_sites sample sites after a forward and
+  // at check_cells sample cells after an adjoint; a sampled error above check_tol (relative
+  // to the survey maximum of each component, or to the largest gradient entry) recomputes
+  // the call in FP64.  Sample indices are uploaded per geometry (check_gen).
+  size_t check_sites = 8, check_cells = 16;
+  double check_tol = 5e-7;
+  size_t check_F = 0, check_FC = 0;
+  unsigned long check_gen = 0;
+  double check_last_fwd = 0.0, check_last_adj = 0.0;
+  size_t check_fallbacks = 0;
+  jgm::DeviceBuffer check_sidx, check_cidx, check_w, check_part;
+
   // incremented by every setter: caches record the generation they were made for
   unsigned long config_gen = 1;
   // raw (untransformed) data of the last jgm_forward made with a data transform, and a
@@ -174,6 +186,11 @@ struct jgm_context {
   // the staging buffers); null when J is in HBM
   float* storedJ_host = nullptr;
   size_t storedJ_block = 0;
+  // file-backed stored J: the same layout in a file (storedJ_fd open, removed on release),
+  // read block by block into two page-locked buffers and streamed like the host-resident J
+  int storedJ_fd = -1;
+  std::string storedJ_file;
+  float* storedJ_pin[2] = {nullptr, nullptr};
   hipStream_t copy_stream = nullptr;
   hipEvent_t ev_copied[2] = {nullptr, nullptr}, ev_used[2] = {nullptr, nullptr};
   hipEvent_t ev_start = nullptr;
@@ -238,6 +255,13 @@ hipError_t launch_adjoint_fp64_sites_add(jgm_context* ctx, int kind, int F, cons
                                          const double* fy, const double* fz, const int* idx,
                                          const double* rm, double* grad);
 hipError_t launch_zero_site_rows(jgm_context* ctx, double* a, const int* idx, int F, int k);
+// a-posteriori check of a mixed forward (raw data of every site in mixed_raw) or adjoint
+// (its gradient): the largest sampled error, relative as stated, in *err_host (waits)
+bool mixed_check_applies(const jgm_context* ctx, int kind);
+hipError_t launch_mixed_check_forward(jgm_context* ctx, int kind, const double* model,
+                                      const double* mixed_raw, double* err_host);
+hipError_t launch_mixed_check_adjoint(jgm_context* ctx, int kind, const double* raw_misfit,
+                                      const double* grad, double* err_host);
 void noalloc_scratch_init(int device);
 double* noalloc_scratch(int device, hipError_t* err);
 size_t noalloc_forward_need(const jgm_context* ctx, int kind);
@@ -256,6 +280,8 @@ hipError_t launch_storedj_build(jgm_context* ctx, int kind, int precision);
 hipError_t launch_storedj_build_resident(jgm_context* ctx, int kind, int precision,
                                          size_t block_cells);
 void release_host_storedj(jgm_context* ctx);
+hipError_t launch_storedj_build_file(jgm_context* ctx, int kind, size_t block_cells,
+                                     const char* dir);
 hipError_t launch_storedj_apply(jgm_context* ctx, int gemm_precision, int transpose,
                                 const float* in, size_t nrhs, float* out);
 hipError_t launch_storedj_build_f64(jgm_context* ctx, int kind);

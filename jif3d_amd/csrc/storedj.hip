@@ -28,8 +28,14 @@
 #include <type_traits>
 #include <hip/hip_runtime.h>
 
+#include <fcntl.h>
+#include <unistd.h>
+
 #include <algorithm>
+#include <atomic>
+#include <cerrno>
 #include <cmath>
+#include <string>
 #include <vector>
 
 #include "context.hpp"
@@ -68,6 +74,11 @@ __host__ __device__ inline int compact_rows(int kind) {
 inline long storedj_columns(const jgm_context* ctx, int kind) {
   const long n = (long)ncells(ctx);
   return kind == JGM_KIND_MAGNETIZATION ? 3 * n : n;
+}
+
+// J streamed block by block from host memory or a file (not resident in HBM)
+inline bool storedj_streamed(const jgm_context* ctx) {
+  return ctx->storedJ_host != nullptr || ctx->storedJ_fd >= 0;
 }
 
 // the vertical-gradient and magnetization kinds have no background columns
@@ -460,8 +471,10 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jtr_bf16c(const bf16x8* __restr
         const long g = g0 + 2 * st;  // this lane's group: 16 st rows + 8 lk
         q.a[st][0] = R0[g * P];
         q.a[st][1] = R1[g * P];
-        q.b[st][0] = __builtin_nontemporal_load(J0 + g);
-        q.b[st][1] = __builtin_nontemporal_load(J1 + g);
+        // plain loads: the four k-steps of a 64-row block read the four 32-byte quarters of
+        // one 128-byte line per cell, so the line is fetched once and served from L1
+        q.b[st][0] = J0[g];
+        q.b[st][1] = J1[g];
       }
     };
     auto mma = [&](const Q& q) {
@@ -804,12 +817,66 @@ hipError_t ensure_streaming(jgm_context* ctx) {
 // staging buffers (copy stream), so that the copy of block b + 1 overlaps the products
 // on block b (context stream).  A staging buffer is refilled only after the products
 // that read it have finished.
+// pread / pwrite of a whole byte range (short transfers repeated); false on an I/O error
+bool file_io(int fd, void* buf, size_t bytes, off_t off, bool write) {
+  char* p = static_cast<char*>(buf);
+  while (bytes > 0) {
+    const ssize_t n = write ? ::pwrite(fd, p, bytes, off) : ::pread(fd, p, bytes, off);
+    if (n < 0 && errno == EINTR) continue;
+    if (n <= 0) return false;
+    p += n;
+    bytes -= (size_t)n;
+    off += n;
+  }
+  return true;
+}
+
+// The file-backed J: block b read from the file into one of two page-locked buffers
+// while the GPU works on block b - 1; each buffer is reread only after its copy to HBM
+// has completed.
+template <class F>
+hipError_t for_each_file_block(jgm_context* ctx, F&& fn) {
+  const long ld = (long)ctx->storedJ_ld, colpad = (long)ctx->storedJ_colpad;
+  const long blk = (long)ctx->storedJ_block;
+  float* stage[2] = {ctx->storedJ.as<float>(), ctx->storedJ.as<float>() + blk * ld};
+  hipError_t e;
+  if ((e = hipEventRecord(ctx->ev_start, ctx->stream)) != hipSuccess) return e;
+  if ((e = hipStreamWaitEvent(ctx->copy_stream, ctx->ev_start, 0)) != hipSuccess) return e;
+  auto read_block = [&](long c0, float* dst) -> hipError_t {
+    const long nc = std::min(blk, colpad - c0);
+    return file_io(ctx->storedJ_fd, dst, sizeof(float) * (size_t)(nc * ld),
+                   (off_t)(sizeof(float) * (size_t)(c0 * ld)), false)
+               ? hipSuccess
+               : hipErrorFileNotFound;
+  };
+  if ((e = read_block(0, ctx->storedJ_pin[0])) != hipSuccess) return e;
+  for (long c0 = 0, b = 0; c0 < colpad; c0 += blk, ++b) {
+    const int i = (int)(b & 1);
+    const long nc = std::min(blk, colpad - c0);
+    if (b >= 2 && (e = hipStreamWaitEvent(ctx->copy_stream, ctx->ev_used[i], 0)) != hipSuccess)
+      return e;
+    if ((e = hipMemcpyAsync(stage[i], ctx->storedJ_pin[i], sizeof(float) * nc * ld,
+                            hipMemcpyHostToDevice, ctx->copy_stream)) != hipSuccess)
+      return e;
+    if ((e = hipEventRecord(ctx->ev_copied[i], ctx->copy_stream)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(ctx->stream, ctx->ev_copied[i], 0)) != hipSuccess) return e;
+    if ((e = fn(static_cast<const float*>(stage[i]), c0, nc)) != hipSuccess) return e;
+    if ((e = hipEventRecord(ctx->ev_used[i], ctx->stream)) != hipSuccess) return e;
+    if (c0 + blk < colpad) {  // the next block into the other buffer, once its copy is done
+      if (b >= 1 && (e = hipEventSynchronize(ctx->ev_copied[1 - i])) != hipSuccess) return e;
+      if ((e = read_block(c0 + blk, ctx->storedJ_pin[1 - i])) != hipSuccess) return e;
+    }
+  }
+  return hipSuccess;
+}
+
 template <class F>
 hipError_t for_each_block(jgm_context* ctx, F&& fn) {
   const long ld = (long)ctx->storedJ_ld, colpad = (long)ctx->storedJ_colpad;
-  if (!ctx->storedJ_host) return fn(ctx->storedJ.as<const float>(), 0L, colpad);
+  if (!storedj_streamed(ctx)) return fn(ctx->storedJ.as<const float>(), 0L, colpad);
   hipError_t e = ensure_streaming(ctx);
   if (e != hipSuccess) return e;
+  if (ctx->storedJ_fd >= 0) return for_each_file_block(ctx, fn);
   const long blk = (long)ctx->storedJ_block;
   float* stage[2] = {ctx->storedJ.as<float>(), ctx->storedJ.as<float>() + blk * ld};
   // earlier work on the context stream may still read the staging buffers
@@ -834,13 +901,90 @@ hipError_t for_each_block(jgm_context* ctx, F&& fn) {
 }  // namespace
 
 void release_host_storedj(jgm_context* ctx) {
-  if (ctx->storedJ_host) {
+  if (ctx->storedJ_host || ctx->storedJ_fd >= 0) {
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
-    (void)hipHostFree(ctx->storedJ_host);
   }
+  if (ctx->storedJ_host) (void)hipHostFree(ctx->storedJ_host);
+  for (float*& p : ctx->storedJ_pin) {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+  }
+  if (ctx->storedJ_fd >= 0) {
+    ::close(ctx->storedJ_fd);
+    ::unlink(ctx->storedJ_file.c_str());
+  }
+  ctx->storedJ_fd = -1;
+  ctx->storedJ_file.clear();
   ctx->storedJ_host = nullptr;
   ctx->storedJ_block = 0;
+}
+
+// J in a file in `dir`, built in blocks of block_cells cells through an HBM staging
+// buffer and a page-locked buffer; the file is removed when J is released or rebuilt.
+hipError_t launch_storedj_build_file(jgm_context* ctx, int kind, size_t block_cells,
+                                     const char* dir) {
+  static std::atomic<unsigned long> serial{0};
+  const int S = (int)ctx->S;
+  const long N = storedj_columns(ctx, kind);
+  const long R = (long)S * compact_rows(kind);
+  if (S == 0 || N == 0) return hipErrorInvalidValue;
+  const long ld = round_up(R, kRowPad), colpad = round_up(N, kColPad);
+  ctx->storedJ_kind = 0;
+  ctx->storedJ_fp64 = false;
+  ++ctx->storedJ_gen;
+  ctx->storedJ_bf16.release();
+  ctx->storedJ_bf16c.release();
+  release_host_storedj(ctx);
+  const long blk = std::min<long>(round_up((long)block_cells, kColPad), colpad);
+  hipError_t e;
+  ctx->storedJ.release();
+  if ((e = ctx->storedJ.ensure(sizeof(float) * (size_t)(2 * blk * ld))) != hipSuccess) return e;
+  for (float*& p : ctx->storedJ_pin) {
+    void* h = nullptr;
+    if ((e = hipHostMalloc(&h, sizeof(float) * (size_t)(blk * ld), hipHostMallocDefault)) !=
+        hipSuccess) {
+      release_host_storedj(ctx);
+      return e;
+    }
+    p = static_cast<float*>(h);
+  }
+  ctx->storedJ_file = std::string(dir) + "/jgm_J_" + std::to_string((long)::getpid()) + "_" +
+                      std::to_string(serial.fetch_add(1)) + ".bin";
+  ctx->storedJ_fd = ::open(ctx->storedJ_file.c_str(), O_RDWR | O_CREAT | O_TRUNC, 0600);
+  if (ctx->storedJ_fd < 0) {
+    ctx->storedJ_file.clear();
+    release_host_storedj(ctx);
+    return hipErrorFileNotFound;
+  }
+  ctx->storedJ_block = (size_t)blk;
+  float* stage = ctx->storedJ.as<float>();
+  for (long c0 = 0; c0 < colpad && e == hipSuccess; c0 += blk) {
+    const long nc = std::min(blk, colpad - c0);
+    hipLaunchKernelGGL(k_fill_zero_f32, dim3(8 * ctx->cu_count), dim3(kBlock), 0, ctx->stream,
+                       stage, nc * ld);
+    if (c0 < N && (e = build_columns(ctx, kind, c0, std::min(c0 + nc, N), ld, stage)) !=
+                      hipSuccess)
+      break;
+    if ((e = hipMemcpyAsync(ctx->storedJ_pin[0], stage, sizeof(float) * nc * ld,
+                            hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+      break;
+    if (!file_io(ctx->storedJ_fd, ctx->storedJ_pin[0], sizeof(float) * (size_t)(nc * ld),
+                 (off_t)(sizeof(float) * (size_t)(c0 * ld)), true))
+      e = hipErrorFileNotFound;
+  }
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e != hipSuccess) {
+    release_host_storedj(ctx);
+    return e;
+  }
+  ctx->storedJ_kind = kind;
+  ctx->storedJ_rows = (size_t)R;
+  ctx->storedJ_cols = (size_t)N;
+  ctx->storedJ_ld = (size_t)ld;
+  ctx->storedJ_colpad = (size_t)colpad;
+  return hipSuccess;
 }
 
 hipError_t launch_storedj_build(jgm_context* ctx, int kind, int precision) {
@@ -986,7 +1130,7 @@ hipError_t run_jm(jgm_context* ctx, int gemm_precision, const float* Xt, int P, 
   hipStream_t st = ctx->stream;
   const int rhs_tiles = P / 64;
   const int row_tiles = (int)((ld + 255) / 256);
-  const bool streamed = ctx->storedJ_host != nullptr;
+  const bool streamed = storedj_streamed(ctx);
   hipError_t e;
   // the BF16 product of an HBM-resident J reads the packed BF16 copy, made once per J
   const bool packed = gemm_precision == JGM_GEMM_BF16 && !streamed && ensure_bf16_copy(ctx);
@@ -1052,7 +1196,7 @@ hipError_t run_jtr(jgm_context* ctx, int gemm_precision, const float* Xt, int P,
   const long ld = (long)ctx->storedJ_ld;
   hipStream_t st = ctx->stream;
   // the BF16 product of an HBM-resident J reads the column-major BF16 copy, made once per J
-  if (gemm_precision == JGM_GEMM_BF16 && !ctx->storedJ_host && ensure_bf16c_copy(ctx)) {
+  if (gemm_precision == JGM_GEMM_BF16 && !storedj_streamed(ctx) && ensure_bf16c_copy(ctx)) {
     const long nblk = (long)ctx->storedJ_colpad / kColPad;
     hipError_t e = ctx->storedJ_rhs16.ensure(sizeof(__bf16) * (size_t)ld * P);
     if (e != hipSuccess) return e;

@@ -332,10 +332,21 @@ class FullSensitivityGravMagCalculator(CachedGravMagCalculator):
 class DiskGravMagCalculator(FullSensitivityGravMagCalculator):
     """The role of jif3D/GravMag/DiskGravMagCalculator.h:100-200 (a J larger than the
     memory that computes with it, kept elsewhere and read back for every cached product):
-    J is kept in page-locked host memory and streamed to HBM in blocks of
-    `block_cells` cells, each block's copy overlapping the products on the previous one.
-    The cached gradient keeps the factor 2 that the reference's Disk calculator omits."""
+    J is kept in page-locked host memory, or with `TempDir` in a file in that directory as
+    the reference keeps it (DiskGravMagCalculator(TheImp, TDir), :64-66), and streamed to
+    HBM in blocks of `block_cells` cells, each block's copy (and file read) overlapping the
+    products on the previous one.  The cached gradient keeps the factor 2 that the
+    reference's Disk calculator omits."""
 
-    def __init__(self, imp, gemm_precision=nat.GEMM_FP32, block_cells=16384):
+    def __init__(self, imp, gemm_precision=nat.GEMM_FP32, block_cells=16384, TempDir=None):
         super().__init__(imp, gemm_precision)
         self.host_block_cells = int(block_cells)
+        self.TempDir = TempDir
+
+    def CalculateNewModel(self, model, data):
+        if self.TempDir is None:
+            return super().CalculateNewModel(model, data)
+        ctx = self.Imp.context(model, data)
+        ctx.storedj_build(self.Imp.kind, self.Imp.precision, self.host_block_cells,
+                          directory=self.TempDir)
+        return self.Imp.Calculate(model, data)

@@ -89,7 +89,10 @@ def test_tensor_caching(tf):
     second = calc.Calculate(m, d)
     assert close_pct(first, second)
     r = np.random.default_rng(8).normal(size=first.size)
-    assert close_pct(calc.LQDerivative(m, d, r), direct.LQDerivative(m, d, r), 1e-10)
+    g, gd = calc.LQDerivative(m, d, r), direct.LQDerivative(m, d, r)
+    # cached (per-cell J entries) and direct (corner-first) gradients sum differently
+    # rounded terms: compared against the largest entry
+    assert np.max(np.abs(g - gd)) <= 1e-12 * np.max(np.abs(gd))
 
 
 def test_magnetic_tfa_caching():
@@ -198,4 +201,4 @@ def test_background_change_uses_the_cached_J():
     r = np.random.default_rng(14).normal(size=d.nsites)
     g_cached = calc.LQDerivative(m, d, r)
     g_direct = MinMemGravMagCalculator(ScalarHipGravityImp()).LQDerivative(m, d, r)
-    assert close_pct(g_cached, g_direct, 1e-10)
+    assert np.max(np.abs(g_cached - g_direct)) <= 1e-12 * np.max(np.abs(g_direct))

@@ -97,3 +97,58 @@ def test_disk_calculator_matches_full_sensitivity():
     assert np.max(np.abs(cf - cd)) <= 1e-6 * np.max(np.abs(cf))
     r = rng.normal(size=cf.size)
     assert np.array_equal(full.LQDerivative(m, d, r), disk.LQDerivative(m, d, r))
+
+
+@pytest.mark.parametrize("block", [256, 768, 100000])
+def test_file_backed_J_equals_host_resident(tmp_path, block):
+    """J in a file (the reference's Disk calculator keeps it in a file in a temporary
+    directory, DiskGravMagCalculator.h:64-85): read back block by block, the products equal
+    those of the host-resident J with the same blocks bit for bit; the file exists while J
+    does and is removed on release."""
+    m, d, rng = problem(7)
+    kind = nat.KIND_SCALAR_TENSOR
+    a, b = nat.Context(0), nat.Context(0)
+    for c in (a, b):
+        c.configure(m, d)
+    a.storedj_build(kind, nat.FP64, host_block_cells=block)
+    b.storedj_build(kind, nat.FP64, host_block_cells=block, directory=str(tmp_path))
+    assert b.storedj_file_backed() and not b.storedj_host_resident()
+    files = list(tmp_path.iterdir())
+    assert len(files) == 1 and files[0].stat().st_size >= 4 * b.storedj_rows() * m.ncells
+    R, N = a.storedj_rows(), a.storedj_cols()
+    M = rng.normal(size=(5, N)).astype(np.float32)
+    Rr = rng.normal(size=(5, R)).astype(np.float32)
+    outs = []
+    for c in (a, b):
+        Y, G = c.alloc((5, R), np.float32), c.alloc((5, N), np.float32)
+        c.storedj_apply(nat.GEMM_FP32, c.to_device(M), Y)
+        c.storedj_apply_t(nat.GEMM_FP32, c.to_device(Rr), G)
+        outs.append((Y.to_host(), G.to_host()))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+    mv = m.GetModelParameters()
+    r = rng.normal(size=d.nsites * 10)
+    assert np.array_equal(a.storedj_forward(nat.GEMM_FP32, mv), b.storedj_forward(nat.GEMM_FP32, mv))
+    assert np.array_equal(a.storedj_gradient(nat.GEMM_FP32, mv, r),
+                          b.storedj_gradient(nat.GEMM_FP32, mv, r))
+    b.storedj_release()
+    assert not list(tmp_path.iterdir())
+
+
+def test_disk_calculator_with_a_directory(tmp_path):
+    """DiskGravMagCalculator(imp, TempDir=...): direct first result, cached results through
+    the file-backed J, against the direct imp."""
+    m, d, rng = problem(8)
+    calc = DiskGravMagCalculator(ScalarTensorHipGravityImp(), block_cells=512,
+                                 TempDir=str(tmp_path))
+    direct = ScalarTensorHipGravityImp()
+    first = calc.Calculate(m, d)
+    assert np.array_equal(first, direct.Calculate(m, d))
+    assert len(list(tmp_path.iterdir())) == 1
+    mv = m.GetModelParameters()
+    m.SetModelParameters(mv * (1 + 0.01 * rng.standard_normal(mv.size)))
+    got, want = calc.Calculate(m, d), direct.Calculate(m, d)
+    assert np.max(np.abs(got - want)) <= 1e-5 * np.max(np.abs(want))
+    r = rng.normal(size=got.size)
+    g, gw = calc.LQDerivative(m, d, r), direct.LQDerivative(m, d, r)
+    assert np.max(np.abs(g - gw)) <= 1e-5 * np.max(np.abs(gw))
