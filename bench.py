@@ -21,7 +21,8 @@ with HBM and matrix-core fractions.
 
 `--workload configs` prints instead one table row per benchmark configuration (gz
 forward; FTG forward; gz+FTG adjoint with a finite-difference check; magnetic total-field
-forward + adjoint), each with the oracle's CPU rate beside it.
+forward + adjoint; magnetic vertical gradient forward + adjoint), each with the oracle's CPU
+rate beside it.
 """
 import argparse
 import json
@@ -261,7 +262,9 @@ def run_storedj(args):
 
 
 FE_FWD = {"gz": (55.0, 10.0), "ftg": (106.0, 60.0), "gz+ftg": (133.0, 70.0),
-          "magnetic": (106.0, 72.0)}  # flop-equivalents per pair: a * rho_c + b
+          "magnetic": (106.0, 72.0),
+          "maggrad": (212.0, 122.0)}  # flop-equivalents per pair: a * rho_c + b (maggrad:
+#                                       two tensor box terms, the site and 1 m below)
 
 
 def run_configs(args):
@@ -431,6 +434,27 @@ def run_configs(args):
                          transform=orc.TF_TOTAL_FIELD_ANOMALY, tf_angles=field[:2])
     cpu = cpu_rate(cpu_mag, d.nsites, budget=4.0)
     row(4, "magnetic TFA forward + adjoint", m, d, times, "magnetic", errs, cpu, passes=2)
+
+    # config 4 geometry, vertical magnetic gradient (two-point FP64; slab form in mixed)
+    ctx.set_transform(nat.TF_NONE, 0.0, 0.0)
+    kind = nat.KIND_MAGNETIC_GRADIENT
+    res, times = {}, {}
+    for name, prec in (("fp64", nat.FP64), ("mixed", nat.MIXED)):
+        def step(prec=prec):
+            ctx.forward(kind, prec, dm, out)
+            ctx.adjoint(kind, prec, dm, dr, grad)
+        times[name] = timed(ctx, step)
+        res[name] = (out.to_host(), grad.to_host())
+    errs = {"mixed_vs_fp64": rel(res["mixed"][0], res["fp64"][0]),
+            "mixed_vs_fp64_gradient": rel(res["mixed"][1], res["fp64"][1])}
+
+    def cpu_grad(n):
+        xn = tuple(c[:n] for c in x)
+        orc.calculate(orc.MAGGRAD, m, mv, xn, field=field)
+        orc.lqderivative(orc.MAGGRAD, m, mv, xn, rtf[:n], field=field)
+    cpu = cpu_rate(cpu_grad, d.nsites, budget=4.0)
+    row(4, "magnetic vertical gradient forward + adjoint", m, d, times, "maggrad", errs, cpu,
+        passes=2)
     return rows
 
 
@@ -516,7 +540,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards "
-              "by sites (forward) and cells (adjoint), see DESIGN.md", file=sys.stderr)
+              "by sites (forward) and cells (adjoint)", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "storedj":

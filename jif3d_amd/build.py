@@ -22,7 +22,7 @@ LIB = os.path.join(HERE, "libjif3d_amd.so")
 ARCH = "gfx950"
 
 SOURCES = ["api.cpp", "profiler.cpp", "fp64_path.hip", "mixed_path.hip", "storedj.hip",
-           "mixed_check.hip"]
+           "mixed_check.hip", "maggrad_mixed.hip"]
 
 # Packed FP32 instructions are disabled for all device code.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

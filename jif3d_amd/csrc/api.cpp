@@ -57,16 +57,11 @@ bool kind_ok(int kind) {
          kind == JGM_KIND_MAGNETIZATION;
 }
 
-// the vertical-gradient kind has FP64 kernels only
-bool fp64_only(int kind) { return kind == JGM_KIND_MAGNETIC_GRADIENT; }
-
 jgm_status check_ready(const jgm_context* ctx, int kind, int prec) {
   if (!ctx) return fail(JGM_ERR_INVALID_ARGUMENT, "null context");
   if (!kind_ok(kind)) return fail(JGM_ERR_INVALID_ARGUMENT, "unknown kind %d", kind);
   if (prec != JGM_FP64 && prec != JGM_MIXED)
     return fail(JGM_ERR_INVALID_ARGUMENT, "unknown precision %d", prec);
-  if (prec == JGM_MIXED && fp64_only(kind))
-    return fail(JGM_ERR_UNSUPPORTED, "kind %d runs in JGM_FP64 only", kind);
   if (!ctx->mesh_set) return fail(JGM_ERR_NOT_READY, "mesh not set (jgm_set_mesh)");
   if (!ctx->sites_set) return fail(JGM_ERR_NOT_READY, "sites not set (jgm_set_sites)");
   const int tf = ctx->transform;
@@ -233,6 +228,7 @@ jgm_status jgm_destroy(jgm_context* ctx) {
                                &ctx->raw_cache_model, &ctx->raw_cache_raw, &ctx->raw_cache_flag,
                                &ctx->partsum, &ctx->chunk_scratch, &ctx->mag_part};
   for (auto* b : bufs) b->release();
+  jgm::release_slab_mesh(ctx);
   jgm::release_host_storedj(ctx);
   if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
   for (hipEvent_t ev : {ctx->ev_copied[0], ctx->ev_copied[1], ctx->ev_used[0], ctx->ev_used[1],
@@ -314,6 +310,7 @@ jgm_status jgm_set_mesh(jgm_context* ctx, size_t nx, size_t ny, size_t nz, const
   hipError_t e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return hip_fail(e, "mesh upload");
   ++ctx->config_gen;
+  ++ctx->geom_gen;
   ctx->mesh_set = true;
   ctx->bg_valid = false;
   ctx->near.valid = false;
@@ -353,6 +350,7 @@ jgm_status jgm_set_sites(jgm_context* ctx, size_t nsites, const double* x, const
   hipError_t e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return hip_fail(e, "site upload");
   ++ctx->config_gen;
+  ++ctx->geom_gen;
   ctx->sites_set = true;
   ctx->bg_valid = false;
   ctx->near.valid = false;

@@ -76,6 +76,20 @@ struct NearBoxes {
   std::vector<int> line_sites;  // host copy of the indices
 };
 
+// Mixed-precision vertical magnetic gradient: the slab mesh (1 m slabs just above every
+// mesh plane, zero-weight gaps between them) held by a child context, made per geometry
+// (gen); plane_of[kv]: the mesh plane of slab layer kv (-1 for a gap), layer_of[K]: the
+// layer of plane K's slab; nline sites evaluated with the FP64 two-point form.
+struct SlabMesh {
+  jgm_context* child = nullptr;
+  unsigned long gen = 0;
+  bool ok = false;
+  size_t nzv = 0;
+  int nline = 0;
+  DeviceBuffer plane_of, layer_of, line_idx, line_xyz;
+  DeviceBuffer rho, sums, rmt, gv, rm0;
+};
+
 // Optional per-kernel timing: HIP events recorded on the context's stream around
 // each launch, harvested lazily.
 struct Profiler {
@@ -147,6 +161,12 @@ struct jgm_context {
 
   // incremented by every setter: caches record the generation they were made for
   unsigned long config_gen = 1;
+  // incremented by the mesh and site setters only
+  unsigned long geom_gen = 1;
+  // the vertical gradient's slab mesh (mixed precision)
+  jgm::SlabMesh slab;
+  // a child context records its kernel timings into this one's profiler
+  jgm_context* prof_parent = nullptr;
   // raw (untransformed) data of the last jgm_forward made with a data transform, and a
   // copy of its model: an adjoint at the same model takes the transform's Jacobian from
   // it instead of a second forward
@@ -255,6 +275,25 @@ hipError_t launch_adjoint_fp64_sites_add(jgm_context* ctx, int kind, int F, cons
                                          const double* fy, const double* fz, const int* idx,
                                          const double* rm, double* grad);
 hipError_t launch_zero_site_rows(jgm_context* ctx, double* a, const int* idx, int F, int k);
+// the vertical-gradient kind's FP64 two-point form at F listed sites (coordinates fx, fy,
+// fz; survey indices idx): rows idx of out / raw_out, or gradients added into grad
+hipError_t launch_maggrad_fp64_sites(jgm_context* ctx, const double* model, int F,
+                                     const double* fx, const double* fy, const double* fz,
+                                     const int* idx, double* out, double* raw_out);
+hipError_t launch_maggrad_fp64_sites_add(jgm_context* ctx, int F, const double* fx,
+                                         const double* fy, const double* fz, const int* idx,
+                                         const double* rm, double* grad);
+// mixed tensor sums (S x 6: xx, yy, zz, xy, xz, yz, carrying G and the signs) of one
+// density model, and the mixed tensor gradient of a raw tensor misfit (9 per site)
+hipError_t launch_tensor_sums_mixed(jgm_context* ctx, const double* rho, double* sums);
+hipError_t launch_tensor_adjoint_mixed(jgm_context* ctx, const double* rmt, double* grad);
+// the vertical-gradient kind in mixed precision through its slab mesh (maggrad_mixed.hip);
+// FP64 when a cell is thinner than 1 m (maggrad_mixed_slabs false)
+hipError_t launch_forward_maggrad_mixed(jgm_context* ctx, const double* model, double* out,
+                                        double* raw_out);
+hipError_t launch_adjoint_maggrad_mixed(jgm_context* ctx, const double* rm, double* grad);
+bool maggrad_mixed_slabs(jgm_context* ctx);
+void release_slab_mesh(jgm_context* ctx);
 // a-posteriori check of a mixed forward (raw data of every site in mixed_raw) or adjoint
 // (its gradient): the largest sampled error, relative as stated, in *err_host (waits)
 bool mixed_check_applies(const jgm_context* ctx, int kind);

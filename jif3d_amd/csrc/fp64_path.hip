@@ -611,9 +611,18 @@ __global__ __launch_bounds__(kBlock) void k_pair_sites(const double* __restrict_
 
 // Bz projection of the tensor kernel's components (xx, yy, zz, xy, xz, yz):
 // (T b)_z = Uxz bx + Uyz by + Uzz bz, times F mu0/(4 pi) / G = F 1e-7 / G.
+// dst[i] += src[i]
+__global__ __launch_bounds__(kBlock) void k_add_f64(double* __restrict__ dst,
+                                                    const double* __restrict__ src, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] += src[i];
+}
+
+// Site s of the list goes to row idx[s] of the output (row s without idx).
 __global__ __launch_bounds__(kBlock) void k_maggrad_finish(const double* __restrict__ partial,
                                                            int nchunks, int S, FieldDir fd,
-                                                           double fs, double* __restrict__ out,
+                                                           double fs, const int* __restrict__ idx,
+                                                           double* __restrict__ out,
                                                            double* __restrict__ raw_out) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S) return;
@@ -632,17 +641,22 @@ __global__ __launch_bounds__(kBlock) void k_maggrad_finish(const double* __restr
             fs * (1e-7 / kGravConst);
   }
   const double v = bz[0] - bz[1];
-  out[s] = v;
-  if (raw_out) raw_out[s] = v;
+  const int o = idx ? idx[s] : s;
+  out[o] = v;
+  if (raw_out) raw_out[o] = v;
 }
 
 // Adjoint coefficients of the 2S virtual sites: +r at the site, -r 1 m below.
+// Misfit of listed site s at rm[idx[s]] (rm[s] without idx).
 __global__ __launch_bounds__(kBlock) void k_maggrad_coef(const double* __restrict__ rm, int S,
                                                          FieldDir fd, double fs,
+                                                         const int* __restrict__ idx,
                                                          double* __restrict__ coef) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= 2 * S) return;
-  const double r = (v < S ? rm[v] : -rm[v - S]) * fs * (1e-7 / kGravConst);
+  const int s = v < S ? v : v - S;
+  const double rs = rm[idx ? idx[s] : s];
+  const double r = (v < S ? rs : -rs) * fs * (1e-7 / kGravConst);
   double* c = coef + (long)v * 6;  // xx, yy, zz, xy, xz, yz with their sign * G
   c[0] = 0.0;
   c[1] = 0.0;
@@ -761,10 +775,12 @@ double magnetization_scale() {
   return 1.0e9 * mu0 / (4 * kPi * kGravConst) * kGravConst;  // factor times the G of T
 }
 
-hipError_t maggrad_forward(jgm_context* ctx, const double* model, double* out,
-                           double* raw_out) {
+// Vertical gradient at S listed sites (sx, sy, sz), written to rows idx[s] of out and
+// raw_out (rows s without idx): the tensor corner sums at the site and 1 m below.
+hipError_t maggrad_forward_at(jgm_context* ctx, const double* model, int S, const double* sx,
+                              const double* sy, const double* sz, const int* idx, double* out,
+                              double* raw_out) {
   hipStream_t st = ctx->stream;
-  const int S = (int)ctx->S;
   const int S2 = 2 * S;
   const long N = (long)ncells(ctx);
   const int ncy = (int)ctx->ny + 1, ncz = (int)ctx->nz + 1;
@@ -772,9 +788,8 @@ hipError_t maggrad_forward(jgm_context* ctx, const double* model, double* out,
   long nchunks = 0, per = 1;
   hipError_t e;
   if ((e = ctx->vsites.ensure(sizeof(double) * 6 * (size_t)S)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_pair_sites, dim3(blocks_for(S2)), dim3(kBlock), 0, st,
-                     ctx->d_sx.as<double>(), ctx->d_sy.as<double>(), ctx->d_sz.as<double>(), S,
-                     1.0, ctx->vsites.as<double>());
+  hipLaunchKernelGGL(k_pair_sites, dim3(blocks_for(S2)), dim3(kBlock), 0, st, sx, sy, sz, S, 1.0,
+                     ctx->vsites.as<double>());
   if (N > 0) {
     site_chunks(S2, ncols, &nchunks, &per);
     if ((e = ctx->weights.ensure(sizeof(double) * (size_t)ncols * ncz)) != hipSuccess) return e;
@@ -798,30 +813,40 @@ hipError_t maggrad_forward(jgm_context* ctx, const double* model, double* out,
   }
   hipLaunchKernelGGL(k_maggrad_finish, dim3(blocks_for(S)), dim3(kBlock), 0, st,
                      ctx->partsum.as<const double>(), nchunks > 0 ? 1 : 0, S, field_dir(ctx),
-                     ctx->fieldstrength, out, raw_out);
+                     ctx->fieldstrength, idx, out, raw_out);
   return hipGetLastError();
 }
 
-hipError_t maggrad_adjoint(jgm_context* ctx, const double* rm, double* grad) {
+hipError_t maggrad_forward(jgm_context* ctx, const double* model, double* out,
+                           double* raw_out) {
+  return maggrad_forward_at(ctx, model, (int)ctx->S, ctx->d_sx.as<const double>(),
+                            ctx->d_sy.as<const double>(), ctx->d_sz.as<const double>(), nullptr,
+                            out, raw_out);
+}
+
+// Gradient of the vertical gradient's misfit at S listed sites (misfit of site s at
+// rm[idx[s]]): stored into grad[0, N + L) (background columns zero, as
+// OMPMagneticGradImp::CalcBackground), or added to grad[0, N) when add is set.
+hipError_t maggrad_adjoint_at(jgm_context* ctx, int S, const double* sx, const double* sy,
+                              const double* sz, const int* idx, const double* rm, double* grad,
+                              bool add) {
   hipStream_t st = ctx->stream;
-  const int S = (int)ctx->S;
   const int S2 = 2 * S;
   const long N = (long)ncells(ctx);
   const int L = (int)ctx->bgthick.size();
   const int ncy = (int)ctx->ny + 1, ncz = (int)ctx->nz + 1;
   const long nq = (long)ncorners(ctx);
   hipError_t e;
-  if (L > 0) {  // OMPMagneticGradImp::CalcBackground returns zeros
+  if (L > 0 && !add) {
     if ((e = launch_fill_zero(ctx, grad + N, (size_t)L)) != hipSuccess) return e;
   }
-  if (N == 0) return hipSuccess;
+  if (N == 0 || S == 0) return hipSuccess;
   if ((e = ctx->vsites.ensure(sizeof(double) * 6 * (size_t)S)) != hipSuccess) return e;
   if ((e = ctx->coef.ensure(sizeof(double) * 6 * (size_t)S2)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_pair_sites, dim3(blocks_for(S2)), dim3(kBlock), 0, st,
-                     ctx->d_sx.as<double>(), ctx->d_sy.as<double>(), ctx->d_sz.as<double>(), S,
-                     1.0, ctx->vsites.as<double>());
+  hipLaunchKernelGGL(k_pair_sites, dim3(blocks_for(S2)), dim3(kBlock), 0, st, sx, sy, sz, S, 1.0,
+                     ctx->vsites.as<double>());
   hipLaunchKernelGGL(k_maggrad_coef, dim3(blocks_for(S2)), dim3(kBlock), 0, st, rm, S,
-                     field_dir(ctx), ctx->fieldstrength, ctx->coef.as<double>());
+                     field_dir(ctx), ctx->fieldstrength, idx, ctx->coef.as<double>());
   int nchunks, per;
   corner_chunks(S2, nq, &nchunks, &per);
   if ((e = ctx->hpart.ensure(sizeof(double) * (size_t)nchunks * nq)) != hipSuccess) return e;
@@ -840,9 +865,23 @@ hipError_t maggrad_adjoint(jgm_context* ctx, const double* rm, double* grad) {
                        ctx->hsum.as<double>());
     H = ctx->hsum.as<const double>();
   }
+  if (!add) {
+    hipLaunchKernelGGL(k_corner_difference, dim3(blocks_for(N)), dim3(kBlock), 0, st, H,
+                       (int)ctx->nx, (int)ctx->ny, (int)ctx->nz, grad);
+    return hipGetLastError();
+  }
+  if ((e = ctx->rt_tmp.ensure(sizeof(double) * (size_t)N)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_corner_difference, dim3(blocks_for(N)), dim3(kBlock), 0, st, H,
-                     (int)ctx->nx, (int)ctx->ny, (int)ctx->nz, grad);
+                     (int)ctx->nx, (int)ctx->ny, (int)ctx->nz, ctx->rt_tmp.as<double>());
+  hipLaunchKernelGGL(k_add_f64, dim3(blocks_for(N)), dim3(kBlock), 0, st, grad,
+                     ctx->rt_tmp.as<const double>(), N);
   return hipGetLastError();
+}
+
+hipError_t maggrad_adjoint(jgm_context* ctx, const double* rm, double* grad) {
+  return maggrad_adjoint_at(ctx, (int)ctx->S, ctx->d_sx.as<const double>(),
+                            ctx->d_sy.as<const double>(), ctx->d_sz.as<const double>(), nullptr,
+                            rm, grad, false);
 }
 
 hipError_t magnetization_forward(jgm_context* ctx, const double* model, double* out,
@@ -1016,12 +1055,6 @@ __global__ __launch_bounds__(kBlock) void k_zero_site_rows(double* __restrict__ 
   a[(long)idx[t / k] * k + t % k] = 0.0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_add_f64(double* __restrict__ dst,
-                                                    const double* __restrict__ src, long n) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] += src[i];
-}
-
 template <int MASK>
 hipError_t forward_sites_impl(jgm_context* ctx, int kind, const double* model, int F,
                               const double* fx, const double* fy, const double* fz,
@@ -1139,6 +1172,20 @@ hipError_t launch_adjoint_fp64_sites_add(jgm_context* ctx, int kind, int F, cons
     case kMaskAll: return adjoint_sites_impl<kMaskAll>(ctx, kind, F, fx, fy, fz, idx, rm, grad);
     default: return adjoint_sites_impl<kMaskTensor>(ctx, kind, F, fx, fy, fz, idx, rm, grad);
   }
+}
+
+hipError_t launch_maggrad_fp64_sites(jgm_context* ctx, const double* model, int F,
+                                     const double* fx, const double* fy, const double* fz,
+                                     const int* idx, double* out, double* raw_out) {
+  if (F <= 0 || ncells(ctx) == 0) return hipSuccess;
+  return maggrad_forward_at(ctx, model, F, fx, fy, fz, idx, out, raw_out);
+}
+
+hipError_t launch_maggrad_fp64_sites_add(jgm_context* ctx, int F, const double* fx,
+                                         const double* fy, const double* fz, const int* idx,
+                                         const double* rm, double* grad) {
+  if (F <= 0 || ncells(ctx) == 0) return hipSuccess;
+  return maggrad_adjoint_at(ctx, F, fx, fy, fz, idx, rm, grad, true);
 }
 
 hipError_t launch_zero_site_rows(jgm_context* ctx, double* a, const int* idx, int F, int k) {

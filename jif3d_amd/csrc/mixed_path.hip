@@ -1511,6 +1511,24 @@ hipError_t route_line_sites_forward(jgm_context* ctx, int kind, size_t nmodels,
 }
 }  // namespace
 
+hipError_t launch_tensor_sums_mixed(jgm_context* ctx, const double* rho, double* sums) {
+  if (!ctx->near.valid) {
+    hipError_t e = build_near_boxes(ctx);
+    if (e != hipSuccess) return e;
+  }
+  return forward_dispatch<kMaskTensor>(ctx, JGM_KIND_TENSOR, 1, rho, ncells(ctx), nullptr, 0,
+                                       nullptr, 0, sums);
+}
+
+hipError_t launch_tensor_adjoint_mixed(jgm_context* ctx, const double* rmt, double* grad) {
+  if (!ctx->near.valid) {
+    hipError_t e = build_near_boxes(ctx);
+    if (e != hipSuccess) return e;
+  }
+  return adjoint_dispatch<kMaskTensor>(ctx, JGM_KIND_TENSOR, 1, rmt, 9 * ctx->S, grad,
+                                       ncells(ctx), false);
+}
+
 hipError_t launch_forward_mixed(jgm_context* ctx, int kind, const double* model, double* out,
                                 double* raw_out) {
   return launch_forward_mixed_batch(ctx, kind, 1, model, 0, out, 0, raw_out, 0);
@@ -1519,6 +1537,14 @@ hipError_t launch_forward_mixed(jgm_context* ctx, int kind, const double* model,
 hipError_t launch_forward_mixed_batch(jgm_context* ctx, int kind, size_t nmodels,
                                       const double* models, size_t model_ld, double* out,
                                       size_t out_ld, double* raw_out, size_t raw_ld) {
+  if (kind == JGM_KIND_MAGNETIC_GRADIENT) {  // through the slab mesh's child context
+    for (size_t b = 0; b < nmodels; ++b) {
+      hipError_t e = launch_forward_maggrad_mixed(ctx, models + b * model_ld, out + b * out_ld,
+                                                  raw_out ? raw_out + b * raw_ld : nullptr);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
   if (!ctx->near.valid) {
     hipError_t e = build_near_boxes(ctx);
     if (e != hipSuccess) return e;
@@ -1565,6 +1591,14 @@ hipError_t launch_adjoint_mixed(jgm_context* ctx, int kind, const double* /*mode
 hipError_t launch_adjoint_mixed_batch(jgm_context* ctx, int kind, size_t nmodels,
                                       const double* raw_misfits, size_t rm_ld, double* grads,
                                       size_t grad_ld) {
+  if (kind == JGM_KIND_MAGNETIC_GRADIENT) {
+    for (size_t b = 0; b < nmodels; ++b) {
+      hipError_t e =
+          launch_adjoint_maggrad_mixed(ctx, raw_misfits + b * rm_ld, grads + b * grad_ld);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
   if (!ctx->near.valid) {
     hipError_t e = build_near_boxes(ctx);
     if (e != hipSuccess) return e;

@@ -6,6 +6,7 @@
 namespace jgm {
 
 long prof_begin(jgm_context* ctx, const char* name) {
+  if (ctx->prof_parent) ctx = ctx->prof_parent;
   Profiler& p = ctx->prof;
   if (!p.enabled) return -1;
   size_t idx = p.entries.size();
@@ -35,6 +36,7 @@ long prof_begin(jgm_context* ctx, const char* name) {
 }
 
 void prof_end(jgm_context* ctx, long token) {
+  if (ctx->prof_parent) ctx = ctx->prof_parent;
   if (token < 0 || (size_t)token >= ctx->prof.pending.size()) return;
   (void)hipEventRecord(ctx->prof.pending[token].stop, ctx->stream);
 }

@@ -166,11 +166,13 @@ class HipMagneticSusceptibilityImp(HipGravMagImplementation):
 
 class HipMagneticGradImp(HipGravMagImplementation):
     """OMPMagneticGradImp(Inc, Dec, Fs) (jif3D/Magnetics/OMPMagneticGradImp.cpp:13-118):
-    one value per site, Bz at the site minus Bz 1 m below, no background; FP64."""
+    one value per site, Bz at the site minus Bz 1 m below, no background.  FP64 by
+    default; nat.MIXED evaluates the difference as the field of 1 m slabs at the mesh
+    planes (FP64 when a cell is thinner than 1 m)."""
     kind = nat.KIND_MAGNETIC_GRADIENT
 
-    def __init__(self, Inc=0.0, Dec=0.0, Fs=1.0, device=0, stream=None):
-        super().__init__(nat.FP64, device, stream)
+    def __init__(self, Inc=0.0, Dec=0.0, Fs=1.0, precision=nat.FP64, device=0, stream=None):
+        super().__init__(precision, device, stream)
         self.Inclination, self.Declination, self.FieldStrength = float(Inc), float(Dec), float(Fs)
 
     def _field(self):

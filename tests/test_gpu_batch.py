@@ -61,8 +61,6 @@ def context(kind, seed=1):
 @pytest.mark.parametrize("kind", KINDS)
 @pytest.mark.parametrize("nb", [3, 8, 13])
 def test_batch_equals_single(kind, prec, nb):
-    if prec == nat.MIXED and kind == nat.KIND_MAGNETIC_GRADIENT:
-        pytest.skip("the vertical-gradient kind has no mixed-precision path")
     ctx, rng = context(kind, seed=nb)
     M = models_for(ctx, kind, rng, nb)
     F = ctx.forward_batch(kind, prec, M)

@@ -128,16 +128,9 @@ def test_magnetization_dot_product():
     assert abs(2 * np.dot(y, r) - np.dot(mv, g)) <= 1e-12 * np.abs(mv).dot(np.abs(g))
 
 
-def test_vertical_gradient_rejects_mixed_precision():
+def test_model_sizes_of_the_magnetic_extensions():
     m, d = susceptibility_model(8)
     ctx = nat.Context(0)
     ctx.configure(m, d, FIELD)
-    kind = nat.KIND_MAGNETIC_GRADIENT
-    with pytest.raises(nat.JgmError) as e:
-        ctx.forward(kind, nat.MIXED, np.zeros(ctx.model_size(kind)))
-    assert e.value.status == nat.JGM_ERR_UNSUPPORTED
-    with pytest.raises(nat.JgmError) as e:  # the stored J is built from the FP64 path
-        ctx.storedj_build(kind, nat.MIXED)
-    assert e.value.status == nat.JGM_ERR_UNSUPPORTED
     assert ctx.model_size(nat.KIND_MAGNETIZATION) == 3 * m.ncells
     assert ctx.model_size(nat.KIND_MAGNETIC_GRADIENT) == m.ncells + 2
