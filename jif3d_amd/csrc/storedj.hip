@@ -237,19 +237,6 @@ __device__ __forceinline__ void mma_block(const float (&a)[2][8], const float (&
   }
 }
 
-// BF16 J M on the packed BF16 copy of J: A arrives as the lane's eight BF16 k values
-// (one 16-byte load), B is rounded in registers as in mma_block.
-__device__ __forceinline__ void mma_block_packed(const bf16x8 (&a)[2], const float (&b)[2][8],
-                                                 f32x16 (&acc)[2][2]) {
-  bf16x8 bh[2], bl[2];
-#pragma unroll
-  for (int t = 0; t < 2; ++t) to_bf16<false>(b[t], bh[t], bl[t]);
-#pragma unroll
-  for (int ta = 0; ta < 2; ++ta)
-#pragma unroll
-    for (int tb = 0; tb < 2; ++tb) acc[ta][tb] = mfma_bf16(a[ta], bh[tb], acc[ta][tb]);
-}
-
 // Two-level accumulation: the MFMA accumulators are added into FP32 totals and reset
 // every 256 k, so no FP32 running sum grows longer than 256 k (the error of one long
 // FP32 sum grows with its length).
@@ -274,25 +261,16 @@ __device__ __forceinline__ int chunk_k(int s, int lk) {
 // Y partials: Yp[split][row][P].  Grid: x = nsplit * row_tiles workgroups (XCD-grouped
 // when nsplit is a multiple of 8 so that the workgroups of one split share an L2), y =
 // right-hand-side tiles of 64.  Each split covers `per` cells (a multiple of 32).
-// PACKED (BF16 only): J is the packed BF16 copy (k_pack_bf16_cells), read with one
-// 16-byte load per lane and 32-row tile instead of eight 4-byte loads: half the bytes of
-// the FP32 J, the same rounded values in the same k order, so the same results.
 struct JmFrag {
   float a[2][8], b[2][8];
 };
-struct JmFragPacked {
-  bf16x8 a[2];
-  float b[2][8];
-};
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
-template <int PREC, bool PACKED>
-__global__ __launch_bounds__(kBlock) void k_gemm_jm(const void* __restrict__ Jv, long ld, long R,
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_gemm_jm(const float* __restrict__ J, long ld, long R,
                                                     long colpad, const float* __restrict__ Mt,
                                                     int P, long per, int row_tiles, int nsplit,
                                                     float* __restrict__ Yp) {
-  static_assert(!PACKED || PREC == JGM_GEMM_BF16, "the packed copy holds BF16 J only");
-  using Frag = typename std::conditional<PACKED, JmFragPacked, JmFrag>::type;
   const int w = blockIdx.x;
   int split, rt;
   if ((nsplit & 7) == 0) {
@@ -312,8 +290,7 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm(const void* __restrict__ Jv,
   const long c0 = (long)split * per;
   const long cend = c0 + per < colpad ? c0 + per : colpad;
   const int nchunk = cend > c0 ? (int)((cend - c0) / kChunk) : 0;  // even
-  const float* Jw = static_cast<const float*>(Jv) + row0 + li;
-  const u32x4* Jp = static_cast<const u32x4*>(Jv) + row0 + li;
+  const float* Jw = J + row0 + li;
   const float* Mw = Mt + b0 + li;
   f32x16 acc[2][2];
 #pragma unroll
@@ -329,31 +306,18 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm(const void* __restrict__ Jv,
     for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
       for (int r = 0; r < 16; ++r) tot[ta][tb][r] = 0.0f;
-  Frag x, y;
-  auto load = [&](Frag& f, long cb) {
-    if constexpr (PACKED) {
-      // this lane's eight cells cb + 8 lk .. cb + 8 lk + 7 are one packed group
-      const long grp = (cb >> 3) + lk;
-      f.a[0] = __builtin_bit_cast(bf16x8, __builtin_nontemporal_load(Jp + grp * ld));
-      f.a[1] = __builtin_bit_cast(bf16x8, __builtin_nontemporal_load(Jp + grp * ld + 32));
-    }
+  JmFrag x, y;
+  auto load = [&](JmFrag& f, long cb) {
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
       const long cell = cb + chunk_k<PREC>(s, lk);
-      if constexpr (!PACKED) {
-        f.a[0][s] = __builtin_nontemporal_load(Jw + cell * ld);
-        f.a[1][s] = __builtin_nontemporal_load(Jw + cell * ld + 32);
-      }
+      f.a[0][s] = __builtin_nontemporal_load(Jw + cell * ld);
+      f.a[1][s] = __builtin_nontemporal_load(Jw + cell * ld + 32);
       f.b[0][s] = Mw[cell * P];
       f.b[1][s] = Mw[cell * P + 32];
     }
   };
-  auto mma = [&](const Frag& f) {
-    if constexpr (PACKED)
-      mma_block_packed(f.a, f.b, acc);
-    else
-      mma_block<PREC>(f.a, f.b, acc);
-  };
+  auto mma = [&](const JmFrag& f) { mma_block<PREC>(f.a, f.b, acc); };
   // sched_barrier keeps the scheduler from sinking the next chunk's loads down to
   // their uses, which would expose the full memory latency once per chunk
   if (nchunk > 0) {
@@ -386,6 +350,105 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm(const void* __restrict__ Jv,
     }
 }
 
+// BF16 J M from the packed BF16 copy of J (k_pack_bf16_cells: eight cells of a row per 16
+// bytes) and a BF16 copy of M (k_rows_to_bf16: eight cells of a right-hand side per 16
+// bytes).  Per 16-cell k-step a lane's A operand (row li or li + 32 of its 64, cells
+// 8 lk .. 8 lk + 7) and B operand (right-hand side li or li + 32, the same cells) are one
+// 16-byte load each, and 64 cells (four k-steps) of loads are in flight while the previous
+// 64 are multiplied.  Same split layout, k order, rounding and flush interval as
+// k_gemm_jm<BF16>, so the same results; per must be a multiple of 64.
+__global__ __launch_bounds__(kBlock) void k_gemm_jm_bf16p(const bf16x8* __restrict__ Jp, long ld,
+                                                          long R, long colpad,
+                                                          const bf16x8* __restrict__ Mb, int P,
+                                                          long per, int row_tiles, int nsplit,
+                                                          float* __restrict__ Yp) {
+  const int w = blockIdx.x;
+  int split, rt;
+  if ((nsplit & 7) == 0) {
+    const int slot = w >> 3;
+    split = (w & 7) * (nsplit >> 3) + slot / row_tiles;
+    rt = slot % row_tiles;
+  } else {
+    split = w / row_tiles;
+    rt = w % row_tiles;
+  }
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int li = lane & 31, lk = lane >> 5;
+  const long row0 = (long)rt * 256 + wave * 64;
+  if (row0 >= ld) return;  // whole wave outside the padded rows; no barriers follow
+  const int b0 = blockIdx.y * 64;
+  const long c0 = (long)split * per;
+  const long cend = c0 + per < colpad ? c0 + per : colpad;
+  const long nblk = cend > c0 ? (cend - c0) / 64 : 0;
+  const bf16x8* A0 = Jp + row0 + li;
+  const bf16x8* A1 = A0 + 32;
+  const bf16x8* B0 = Mb + b0 + li;
+  const bf16x8* B1 = B0 + 32;
+  f32x16 acc[2][2];
+  float tot[2][2][16];
+#pragma unroll
+  for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[ta][tb][r] = tot[ta][tb][r] = 0.0f;
+  struct Q {
+    bf16x8 a[4][2], b[4][2];  // [k-step][tile]
+  } x, y;
+  auto load = [&](Q& q, long blk) {
+    const long g0 = (c0 + 64 * blk) / 8;  // first 8-cell group of the block
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      const long g = g0 + 2 * st + lk;  // this lane's group: cells 16 st + 8 lk ..
+      q.a[st][0] = __builtin_bit_cast(bf16x8, __builtin_nontemporal_load(
+                                                  reinterpret_cast<const u32x4*>(A0 + g * ld)));
+      q.a[st][1] = __builtin_bit_cast(bf16x8, __builtin_nontemporal_load(
+                                                  reinterpret_cast<const u32x4*>(A1 + g * ld)));
+      q.b[st][0] = B0[g * P];
+      q.b[st][1] = B1[g * P];
+    }
+  };
+  auto mma = [&](const Q& q) {
+#pragma unroll
+    for (int st = 0; st < 4; ++st)
+#pragma unroll
+      for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb)
+          acc[ta][tb] = mfma_bf16(q.a[st][ta], q.b[st][tb], acc[ta][tb]);
+  };
+  if (nblk > 0) {
+    load(x, 0);
+    for (long q = 0; q < nblk; q += 2) {
+      load(y, q + 1 < nblk ? q + 1 : nblk - 1);  // past the end: unused reload
+      __builtin_amdgcn_sched_barrier(0);
+      mma(x);
+      if ((q & 3) == 3) flush(acc, tot);  // every 256 cells from c0
+      __builtin_amdgcn_sched_barrier(0);
+      load(x, q + 2 < nblk ? q + 2 : nblk - 1);
+      __builtin_amdgcn_sched_barrier(0);
+      if (q + 1 < nblk) {
+        mma(y);
+        if (((q + 1) & 3) == 3) flush(acc, tot);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  flush(acc, tot);
+  float* out = Yp + (long)split * R * P;
+#pragma unroll
+  for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const long row = row0 + 32 * ta + 8 * (r >> 2) + 4 * lk + (r & 3);
+      if (row < R) {
+        out[row * P + b0 + li] = tot[ta][0][r];
+        out[row * P + b0 + 32 + li] = tot[ta][1][r];
+      }
+    }
+}
+
 // BF16 copy of J for the BF16 J M: element (row, cell) at
 // Jp[((cell / 8) ld + row) 8 + cell % 8], rounded to nearest even as mma_block rounds it.
 __global__ __launch_bounds__(kBlock) void k_pack_bf16_cells(const float* __restrict__ J, long ld,
@@ -401,20 +464,27 @@ __global__ __launch_bounds__(kBlock) void k_pack_bf16_cells(const float* __restr
   }
 }
 
-// BF16 copy of J in J's layout (element (row, cell) at cell ld + row), eight rows per
-// 16 bytes, rounded to nearest even as mma_block rounds J
-__global__ __launch_bounds__(kBlock) void k_pack_bf16_rows(const float* __restrict__ J, long ngroups,
-                                                           bf16x8* __restrict__ Jc) {
-  for (long g = (long)blockIdx.x * blockDim.x + threadIdx.x; g < ngroups;
-       g += (long)gridDim.x * blockDim.x) {
-    const f32x4 lo = *(const f32x4*)(J + 8 * g), hi = *(const f32x4*)(J + 8 * g + 4);
+// BF16 copy of J for the BF16 J^T R, tiled for the matrix cores: eight rows of one cell per
+// 16 bytes, and the 32 cells of a tile side by side for each 8-row group, so that group g
+// of cell c is element ((c / 32) ngr + g) 32 + c % 32 (ngr = ld / 8) and one load
+// instruction of a wave reads two contiguous 512-byte runs.  Rounded to nearest even as
+// mma_block rounds J.
+__global__ __launch_bounds__(kBlock) void k_pack_bf16_tiles(const float* __restrict__ J, long ld,
+                                                            long colpad, bf16x8* __restrict__ Jt) {
+  const long ngr = ld / 8, n = colpad * ngr;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (long)gridDim.x * blockDim.x) {
+    const long li = idx & 31, t = idx >> 5;
+    const long g = t % ngr, tile = t / ngr;
+    const float* src = J + (tile * 32 + li) * ld + 8 * g;
+    const f32x4 lo = *(const f32x4*)src, hi = *(const f32x4*)(src + 4);
     bf16x8 v;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       v[e] = (__bf16)lo[e];
       v[4 + e] = (__bf16)hi[e];
     }
-    Jc[g] = v;
+    Jt[idx] = v;
   }
 }
 
@@ -431,7 +501,7 @@ __global__ __launch_bounds__(kBlock) void k_rows_to_bf16(const float* __restrict
   Rb[idx] = v;
 }
 
-// BF16 G^T = Rt^T J from the column-major BF16 copy of J: workgroup = 256 cells (wave: 64
+// BF16 G^T = Rt^T J from the tiled BF16 copy of J (k_pack_bf16_tiles): workgroup = 256 cells (wave: 64
 // cells) x 64 right-hand sides, no LDS.  For each 16-row k-step a lane's eight rows of its
 // cell (B operand, k = 8 (lane / 32) + e) and of its right-hand side (A operand) are one
 // 16-byte load each; 64 rows (four k-steps) of loads are in flight while the previous 64
@@ -445,12 +515,12 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jtr_bf16c(const bf16x8* __restr
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int li = lane & 31, lk = lane >> 5;
   const int b0 = blockIdx.y * 64;
-  const long ldg8 = ld / 8;          // 8-row groups per cell
+  const long ngr = ld / 8;           // 8-row groups per cell
   const long nblock64 = ld / 64;     // 64-row blocks (ld is a multiple of 64)
   for (long blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
     const long cbase = blk * kColPad + 64 * wave;
-    const bf16x8* J0 = Jc + (cbase + li) * ldg8 + lk;
-    const bf16x8* J1 = J0 + 32 * ldg8;
+    const bf16x8* J0 = Jc + (cbase / 32) * ngr * 32 + lk * 32 + li;  // tile of cells cbase..
+    const bf16x8* J1 = J0 + ngr * 32;                                // .. and cbase + 32..
     const bf16x8* R0 = Rb + (long)lk * P + b0 + li;
     const bf16x8* R1 = R0 + 32;
     f32x16 acc[2][2];
@@ -471,10 +541,10 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jtr_bf16c(const bf16x8* __restr
         const long g = g0 + 2 * st;  // this lane's group: 16 st rows + 8 lk
         q.a[st][0] = R0[g * P];
         q.a[st][1] = R1[g * P];
-        // plain loads: the four k-steps of a 64-row block read the four 32-byte quarters of
-        // one 128-byte line per cell, so the line is fetched once and served from L1
-        q.b[st][0] = J0[g];
-        q.b[st][1] = J1[g];
+        q.b[st][0] = __builtin_bit_cast(bf16x8, __builtin_nontemporal_load(
+                                                    reinterpret_cast<const u32x4*>(J0 + g * 32)));
+        q.b[st][1] = __builtin_bit_cast(bf16x8, __builtin_nontemporal_load(
+                                                    reinterpret_cast<const u32x4*>(J1 + g * 32)));
       }
     };
     auto mma = [&](const Q& q) {
@@ -1105,7 +1175,7 @@ bool ensure_bf16_copy(jgm_context* ctx) {
   return true;
 }
 
-// The column-major BF16 copy for the BF16 J^T R, on the same terms.
+// The tiled BF16 copy for the BF16 J^T R, on the same terms.
 bool ensure_bf16c_copy(jgm_context* ctx) {
   if (ctx->storedJ_bf16c_gen == ctx->storedJ_gen) return true;
   if (!ctx->bf16_copy_enabled || ctx->storedJ_bf16c_failed_gen == ctx->storedJ_gen) return false;
@@ -1116,8 +1186,9 @@ bool ensure_bf16c_copy(jgm_context* ctx) {
     ctx->storedJ_bf16c_failed_gen = ctx->storedJ_gen;
     return false;
   }
-  hipLaunchKernelGGL(k_pack_bf16_rows, dim3(8 * ctx->cu_count), dim3(kBlock), 0, ctx->stream,
-                     ctx->storedJ.as<const float>(), ngroups, ctx->storedJ_bf16c.as<bf16x8>());
+  hipLaunchKernelGGL(k_pack_bf16_tiles, dim3(8 * ctx->cu_count), dim3(kBlock), 0, ctx->stream,
+                     ctx->storedJ.as<const float>(), (long)ctx->storedJ_ld,
+                     (long)ctx->storedJ_colpad, ctx->storedJ_bf16c.as<bf16x8>());
   ctx->storedJ_bf16c_gen = ctx->storedJ_gen;
   return true;
 }
@@ -1146,11 +1217,18 @@ hipError_t run_jm(jgm_context* ctx, int gemm_precision, const float* Xt, int P, 
     long nsplit = std::max<long>(1, (long)ctx->cu_count / ((long)row_tiles * rhs_tiles));
     nsplit = std::min<long>(nsplit, ncols / 32);
     if (nsplit >= 16) nsplit = nsplit / 8 * 8;
-    const long per = round_up((ncols + nsplit - 1) / nsplit, 2 * kChunk);
+    // splits of whole 64-cell blocks (the packed kernel's stage)
+    const long per = round_up((ncols + nsplit - 1) / nsplit, 4 * kChunk);
     hipError_t e2 = ctx->partial.ensure(sizeof(float) * (size_t)nsplit * R * P);
     if (e2 != hipSuccess) return e2;
     dim3 grid((unsigned)(nsplit * row_tiles), (unsigned)rhs_tiles);
     const float* Mt = Xt + c0 * P;
+    if (packed) {  // M rounded to BF16 once, eight cells of a right-hand side per 16 bytes
+      if ((e2 = ctx->storedJ_rhs16.ensure(sizeof(__bf16) * (size_t)ncols * P)) != hipSuccess)
+        return e2;
+      hipLaunchKernelGGL(k_rows_to_bf16, dim3(blocks_for(ncols / 8 * P)), dim3(kBlock), 0, st, Mt,
+                         ncols / 8, P, ctx->storedJ_rhs16.as<bf16x8>());
+    }
     {
       // the packed BF16 product is profiled under its own name, so that timings know
       // whether 2 or 4 bytes of J were read per element
@@ -1158,18 +1236,21 @@ hipError_t run_jm(jgm_context* ctx, int gemm_precision, const float* Xt, int P, 
                         : gemm_precision == JGM_GEMM_BF16 ? (packed ? "storedj_jm_bf16_packed"
                                                                     : "storedj_jm_bf16")
                                                           : "storedj_jm_bf16_refined");
-#define JGM_JM(PR, PK, JV)                                                                      \
-  hipLaunchKernelGGL((k_gemm_jm<PR, PK>), grid, dim3(kBlock), 0, st, JV, ld, R, ncols, Mt, P, per, \
+#define JGM_JM(PR)                                                                        \
+  hipLaunchKernelGGL((k_gemm_jm<PR>), grid, dim3(kBlock), 0, st, J, ld, R, ncols, Mt, P, per, \
                      row_tiles, (int)nsplit, ctx->partial.as<float>())
       switch (gemm_precision) {
-        case JGM_GEMM_FP32: JGM_JM(JGM_GEMM_FP32, false, J); break;
+        case JGM_GEMM_FP32: JGM_JM(JGM_GEMM_FP32); break;
         case JGM_GEMM_BF16:
           if (packed)
-            JGM_JM(JGM_GEMM_BF16, true, ctx->storedJ_bf16.as<const void>());
+            hipLaunchKernelGGL(k_gemm_jm_bf16p, grid, dim3(kBlock), 0, st,
+                               ctx->storedJ_bf16.as<const bf16x8>(), ld, R, ncols,
+                               ctx->storedJ_rhs16.as<const bf16x8>(), P, per, row_tiles,
+                               (int)nsplit, ctx->partial.as<float>());
           else
-            JGM_JM(JGM_GEMM_BF16, false, J);
+            JGM_JM(JGM_GEMM_BF16);
           break;
-        default: JGM_JM(JGM_GEMM_BF16_REFINED, false, J); break;
+        default: JGM_JM(JGM_GEMM_BF16_REFINED); break;
       }
 #undef JGM_JM
     }
