@@ -354,9 +354,9 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm(const float* __restrict__ J,
 // bytes) and a BF16 copy of M (k_rows_to_bf16: eight cells of a right-hand side per 16
 // bytes).  Per 16-cell k-step a lane's A operand (row li or li + 32 of its 64, cells
 // 8 lk .. 8 lk + 7) and B operand (right-hand side li or li + 32, the same cells) are one
-// 16-byte load each, and 64 cells (four k-steps) of loads are in flight while the previous
-// 64 are multiplied.  Same split layout, k order, rounding and flush interval as
-// k_gemm_jm<BF16>, so the same results; per must be a multiple of 64.
+// 16-byte load each; J is loaded in 64-cell blocks (four k-steps) two blocks ahead of the
+// multiplication, M one block ahead.  Same split layout, k order, rounding and flush
+// interval as k_gemm_jm<BF16>, so the same results; per must be a multiple of 64.
 __global__ __launch_bounds__(kBlock) void k_gemm_jm_bf16p(const bf16x8* __restrict__ Jp, long ld,
                                                           long R, long colpad,
                                                           const bf16x8* __restrict__ Mb, int P,
@@ -393,10 +393,15 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm_bf16p(const bf16x8* __restri
     for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[ta][tb][r] = tot[ta][tb][r] = 0.0f;
-  struct Q {
-    bf16x8 a[4][2], b[4][2];  // [k-step][tile]
-  } x, y;
-  auto load = [&](Q& q, long blk) {
+  // J fragments of three 64-cell blocks and M fragments of two are live: J is loaded two
+  // blocks ahead (HBM latency), M one block ahead (L2)
+  struct JQ {
+    bf16x8 a[4][2];  // [k-step][row tile]
+  } j0, j1, j2;
+  struct MQ {
+    bf16x8 b[4][2];  // [k-step][right-hand-side tile]
+  } m0, m1;
+  auto loadJ = [&](JQ& q, long blk) {
     const long g0 = (c0 + 64 * blk) / 8;  // first 8-cell group of the block
 #pragma unroll
     for (int st = 0; st < 4; ++st) {
@@ -405,34 +410,47 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm_bf16p(const bf16x8* __restri
                                                   reinterpret_cast<const u32x4*>(A0 + g * ld)));
       q.a[st][1] = __builtin_bit_cast(bf16x8, __builtin_nontemporal_load(
                                                   reinterpret_cast<const u32x4*>(A1 + g * ld)));
+    }
+  };
+  auto loadM = [&](MQ& q, long blk) {
+    const long g0 = (c0 + 64 * blk) / 8;
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      const long g = g0 + 2 * st + lk;
       q.b[st][0] = B0[g * P];
       q.b[st][1] = B1[g * P];
     }
   };
-  auto mma = [&](const Q& q) {
+  // block b: multiply (jc, mc) while block b + 2 of J and b + 1 of M load (clamped to the
+  // last block past the end: unused reloads of valid memory)
+  auto step = [&](const JQ& jc, const MQ& mc, JQ& jl, MQ& ml, long b) {
+    loadJ(jl, b + 2 < nblk ? b + 2 : nblk - 1);
+    loadM(ml, b + 1 < nblk ? b + 1 : nblk - 1);
+    __builtin_amdgcn_sched_barrier(0);
+    if (b < nblk) {
 #pragma unroll
-    for (int st = 0; st < 4; ++st)
+      for (int st = 0; st < 4; ++st)
 #pragma unroll
-      for (int ta = 0; ta < 2; ++ta)
+        for (int ta = 0; ta < 2; ++ta)
 #pragma unroll
-        for (int tb = 0; tb < 2; ++tb)
-          acc[ta][tb] = mfma_bf16(q.a[st][ta], q.b[st][tb], acc[ta][tb]);
+          for (int tb = 0; tb < 2; ++tb)
+            acc[ta][tb] = mfma_bf16(jc.a[st][ta], mc.b[st][tb], acc[ta][tb]);
+      if ((b & 3) == 3) flush(acc, tot);  // every 256 cells from c0
+    }
+    __builtin_amdgcn_sched_barrier(0);
   };
   if (nblk > 0) {
-    load(x, 0);
-    for (long q = 0; q < nblk; q += 2) {
-      load(y, q + 1 < nblk ? q + 1 : nblk - 1);  // past the end: unused reload
-      __builtin_amdgcn_sched_barrier(0);
-      mma(x);
-      if ((q & 3) == 3) flush(acc, tot);  // every 256 cells from c0
-      __builtin_amdgcn_sched_barrier(0);
-      load(x, q + 2 < nblk ? q + 2 : nblk - 1);
-      __builtin_amdgcn_sched_barrier(0);
-      if (q + 1 < nblk) {
-        mma(y);
-        if (((q + 1) & 3) == 3) flush(acc, tot);
-      }
-      __builtin_amdgcn_sched_barrier(0);
+    loadJ(j0, 0);
+    loadJ(j1, nblk > 1 ? 1 : 0);
+    loadM(m0, 0);
+    // block b uses J set b % 3 and M set b % 2
+    for (long q = 0; q < nblk; q += 6) {
+      step(j0, m0, j2, m1, q);
+      step(j1, m1, j0, m0, q + 1);
+      step(j2, m0, j1, m1, q + 2);
+      step(j0, m1, j2, m0, q + 3);
+      step(j1, m0, j0, m1, q + 4);
+      step(j2, m1, j1, m0, q + 5);
     }
   }
   flush(acc, tot);
@@ -504,8 +522,8 @@ __global__ __launch_bounds__(kBlock) void k_rows_to_bf16(const float* __restrict
 // BF16 G^T = Rt^T J from the tiled BF16 copy of J (k_pack_bf16_tiles): workgroup = 256 cells (wave: 64
 // cells) x 64 right-hand sides, no LDS.  For each 16-row k-step a lane's eight rows of its
 // cell (B operand, k = 8 (lane / 32) + e) and of its right-hand side (A operand) are one
-// 16-byte load each; 64 rows (four k-steps) of loads are in flight while the previous 64
-// are multiplied.  Same k order, rounding and flush interval as k_gemm_jtr<BF16>, so the
+// 16-byte load each; J is loaded in 64-row blocks (four k-steps) two blocks ahead of the
+// multiplication, R one block ahead.  Same k order, rounding and flush interval as k_gemm_jtr<BF16>, so the
 // same results.
 __global__ __launch_bounds__(kBlock) void k_gemm_jtr_bf16c(const bf16x8* __restrict__ Jc, long ld,
                                                            long N, long nblk,
@@ -531,45 +549,62 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jtr_bf16c(const bf16x8* __restr
       for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[ta][tb][r] = tot[ta][tb][r] = 0.0f;
-    struct Q {
-      bf16x8 a[4][2], b[4][2];  // [k-step][tile]
-    } x, y;
-    auto load = [&](Q& q, long blk64) {
+    // J fragments of three 64-row blocks and R fragments of two are live: J is loaded two
+    // blocks ahead (HBM latency), R one block ahead (L2)
+    struct JQ {
+      bf16x8 b[4][2];  // [k-step][cell tile]
+    } j0, j1, j2;
+    struct RQ {
+      bf16x8 a[4][2];  // [k-step][right-hand-side tile]
+    } r0, r1;
+    auto loadJ = [&](JQ& q, long blk64) {
       const long g0 = blk64 * 8;  // first 8-row group of the block
 #pragma unroll
       for (int st = 0; st < 4; ++st) {
         const long g = g0 + 2 * st;  // this lane's group: 16 st rows + 8 lk
-        q.a[st][0] = R0[g * P];
-        q.a[st][1] = R1[g * P];
         q.b[st][0] = __builtin_bit_cast(bf16x8, __builtin_nontemporal_load(
                                                     reinterpret_cast<const u32x4*>(J0 + g * 32)));
         q.b[st][1] = __builtin_bit_cast(bf16x8, __builtin_nontemporal_load(
                                                     reinterpret_cast<const u32x4*>(J1 + g * 32)));
       }
     };
-    auto mma = [&](const Q& q) {
+    auto loadR = [&](RQ& q, long blk64) {
+      const long g0 = blk64 * 8;
 #pragma unroll
-      for (int st = 0; st < 4; ++st)
-#pragma unroll
-        for (int ta = 0; ta < 2; ++ta)
-#pragma unroll
-          for (int tb = 0; tb < 2; ++tb)
-            acc[ta][tb] = mfma_bf16(q.a[st][ta], q.b[st][tb], acc[ta][tb]);
+      for (int st = 0; st < 4; ++st) {
+        const long g = g0 + 2 * st;
+        q.a[st][0] = R0[g * P];
+        q.a[st][1] = R1[g * P];
+      }
     };
-    load(x, 0);
-    for (long q = 0; q < nblock64; q += 2) {
-      load(y, q + 1 < nblock64 ? q + 1 : nblock64 - 1);  // past the end: unused reload
+    const long nb = nblock64;
+    auto step = [&](const JQ& jc, const RQ& rc, JQ& jl, RQ& rl, long b) {
+      loadJ(jl, b + 2 < nb ? b + 2 : nb - 1);  // past the end: unused reloads
+      loadR(rl, b + 1 < nb ? b + 1 : nb - 1);
       __builtin_amdgcn_sched_barrier(0);
-      mma(x);
-      if ((q & 3) == 3) flush(acc, tot);  // every 256 rows
-      __builtin_amdgcn_sched_barrier(0);
-      load(x, q + 2 < nblock64 ? q + 2 : nblock64 - 1);
-      __builtin_amdgcn_sched_barrier(0);
-      if (q + 1 < nblock64) {
-        mma(y);
-        if (((q + 1) & 3) == 3) flush(acc, tot);
+      if (b < nb) {
+#pragma unroll
+        for (int st = 0; st < 4; ++st)
+#pragma unroll
+          for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+            for (int tb = 0; tb < 2; ++tb)
+              acc[ta][tb] = mfma_bf16(rc.a[st][ta], jc.b[st][tb], acc[ta][tb]);
+        if ((b & 3) == 3) flush(acc, tot);  // every 256 rows
       }
       __builtin_amdgcn_sched_barrier(0);
+    };
+    loadJ(j0, 0);
+    loadJ(j1, nb > 1 ? 1 : 0);
+    loadR(r0, 0);
+    // block b uses J set b % 3 and R set b % 2
+    for (long q = 0; q < nb; q += 6) {
+      step(j0, r0, j2, r1, q);
+      step(j1, r1, j0, r0, q + 1);
+      step(j2, r0, j1, r1, q + 2);
+      step(j0, r1, j2, r0, q + 3);
+      step(j1, r0, j0, r1, q + 4);
+      step(j2, r1, j1, r0, q + 5);
     }
     flush(acc, tot);
 #pragma unroll
