@@ -119,13 +119,13 @@ jgm_status jgm_set_transform(jgm_context *ctx, jgm_transform tf, double inclinat
                              double declination);
 /* Near-field radius of JGM_MIXED in cell half-diagonals (default 8). */
 jgm_status jgm_set_mixed_threshold(jgm_context *ctx, double q0);
-/* Second radius of JGM_MIXED in cell half-diagonals (default 26): cells farther than
+/* Second radius of JGM_MIXED in cell half-diagonals (default 32): cells farther than
  * q1 from a site keep the monopole (and, for non-cubic cells, quadrupole) terms only;
  * the hexadecapole term dropped there is O((h/d)^4) of each cell's field.  0 keeps the
  * hexadecapole for every far cell. */
 jgm_status jgm_set_mixed_far_radius(jgm_context *ctx, double q1);
 
-/* A-posteriori check of JGM_MIXED (on by default: 8 sites, 16 cells, tolerance 5e-7).
+/* A-posteriori check of JGM_MIXED (on by default: 4 sites, 16 cells, tolerance 5e-7).
  * The mixed path's error bound is relative to each component's largest magnitude over the
  * survey (forward) and to the largest gradient entry (adjoint); it holds unless the cells'
  * contributions cancel strongly (a nearly uniformly magnetised wide slab, whose external

@@ -145,13 +145,13 @@ struct jgm_context {
   // in half-diagonals) beyond which the far field keeps only monopole (and quadrupole)
   // terms; 0 keeps the hexadecapole everywhere
   double mixed_q0 = 8.0;
-  double mixed_q1 = 26.0;
+  double mixed_q1 = 32.0;
 
   // a-posteriori check of JGM_MIXED: FP64 at check_sites sample sites after a forward and
   // at check_cells sample cells after an adjoint; a sampled error above check_tol (relative
   // to the survey maximum of each component, or to the largest gradient entry) recomputes
   // the call in FP64.  Sample indices are uploaded per geometry (check_gen).
-  size_t check_sites = 8, check_cells = 16;
+  size_t check_sites = 4, check_cells = 16;
   double check_tol = 5e-7;
   size_t check_F = 0, check_FC = 0;
   unsigned long check_gen = 0;

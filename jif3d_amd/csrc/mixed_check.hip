@@ -23,14 +23,14 @@ namespace {
 constexpr int kBlock = 256;
 inline unsigned blocks_for(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
+// Sum over the block (256 threads = 4 waves): within each wave by lane shuffles, then
+// the four wave sums in a fixed order.  Every thread gets the result.
 __device__ inline double block_sum(double v, double* red) {
-  red[threadIdx.x] = v;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  const double r = red[0];
+  const double r = (red[0] + red[1]) + (red[2] + red[3]);
   __syncthreads();
   return r;
 }
@@ -58,8 +58,8 @@ __global__ __launch_bounds__(kBlock) void k_check_weights(const double* __restri
   w[q] = acc;
 }
 
-// Forward at F sample sites: each block sums w(q) F_m(s, q) over a strided share of the
-// corners for every sample site; partial[(block F + f) NC + m].
+// Forward at F sample sites: block (x, f) sums w(q) F_m(s_f, q) over a strided share of
+// the corners; partial[(x F + f) NC + m].
 template <int MASK>
 __global__ __launch_bounds__(kBlock) void k_check_forward(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
@@ -67,29 +67,29 @@ __global__ __launch_bounds__(kBlock) void k_check_forward(
     const double* __restrict__ Yc, const double* __restrict__ Zc, int ncy, int ncz, long nq,
     const double* __restrict__ w, double* __restrict__ partial) {
   constexpr int NC = Ncomp<MASK>::value;
-  __shared__ double red[kBlock];
+  __shared__ double red[kBlock / 64];
   const long stride = (long)gridDim.x * blockDim.x;
-  for (int f = 0; f < F; ++f) {
-    const int s = sidx[f];
-    const double px = sx[s], py = sy[s], pz = sz[s];
-    double acc[NC];
+  const int f = blockIdx.y;
+  const int s = sidx[f];
+  const double px = sx[s], py = sy[s], pz = sz[s];
+  double acc[NC];
 #pragma unroll
-    for (int m = 0; m < NC; ++m) acc[m] = 0.0;
-    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += stride) {
-      const int K = (int)(q % ncz);
-      const long t = q / ncz;
-      const int J = (int)(t % ncy);
-      const int I = (int)(t / ncy);
-      double tv[NC];
-      corner_terms_f64<MASK>(px - Xc[I], py - Yc[J], pz - Zc[K], tv);
-      const double wq = w[q];
+  for (int m = 0; m < NC; ++m) acc[m] = 0.0;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += stride) {
+    const int K = (int)(q % ncz);
+    const long t = q / ncz;
+    const int J = (int)(t % ncy);
+    const int I = (int)(t / ncy);
+    double tv[NC];
+    corner_terms_f64<MASK>(px - Xc[I], py - Yc[J], pz - Zc[K], tv);
+    const double wq = w[q];
 #pragma unroll
-      for (int m = 0; m < NC; ++m) acc[m] = fma(wq, tv[m], acc[m]);
-    }
-    for (int m = 0; m < NC; ++m) {
-      const double v = block_sum(acc[m], red);
-      if (threadIdx.x == 0) partial[((long)blockIdx.x * F + f) * NC + m] = v;
-    }
+    for (int m = 0; m < NC; ++m) acc[m] = fma(wq, tv[m], acc[m]);
+  }
+#pragma unroll
+  for (int m = 0; m < NC; ++m) {
+    const double v = block_sum(acc[m], red);
+    if (threadIdx.x == 0) partial[((long)blockIdx.x * F + f) * NC + m] = v;
   }
 }
 
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(kBlock) void k_check_forward_sites(
     FieldDir fd, const double* __restrict__ mixed_raw, const double* __restrict__ smax,
     double* __restrict__ errs) {
   constexpr int NC = Ncomp<MASK>::value;
-  __shared__ double red[kBlock];
+  __shared__ double red[kBlock / 64];
   const int f = blockIdx.x;
   double acc[NC], sc[NC];
   comp_scales<MASK>(sc);
@@ -284,8 +284,10 @@ hipError_t check_forward_impl(jgm_context* ctx, int kind, const double* model,
   const int kraw = (int)raw_per_site(kind);
   const int ncy = (int)ctx->ny + 1, ncz = (int)ctx->nz + 1;
   const long nq = (long)ncorners(ctx);
-  const int nblocks = (int)std::max<long>(1, std::min<long>(2L * ctx->cu_count,
-                                                            (nq + 8L * kBlock - 1) / (8L * kBlock)));
+  // blocks per sample site: about 2 CUs' worth over all sites, each thread taking at least
+  // 32 corners so that the block sums stay a small part of the work
+  const long want = std::max<long>(1, 2L * ctx->cu_count / std::max(1, F));
+  const int nblocks = (int)std::max<long>(1, std::min<long>(want, (nq + 32L * kBlock - 1) / (32L * kBlock)));
   hipError_t e;
   // partials (nblocks F NC), survey maxima (kraw), per-site errors (F), the error (1)
   if ((e = ctx->check_w.ensure(sizeof(double) * (size_t)nq)) != hipSuccess ||
@@ -300,7 +302,7 @@ hipError_t check_forward_impl(jgm_context* ctx, int kind, const double* model,
     ProfScope ps(ctx, "mixed_check_forward_fp64");
     hipLaunchKernelGGL(k_check_weights, dim3(blocks_for(nq)), dim3(kBlock), 0, st, model,
                        (int)ctx->nx, (int)ctx->ny, (int)ctx->nz, ctx->check_w.as<double>());
-    hipLaunchKernelGGL((k_check_forward<MASK>), dim3(nblocks), dim3(kBlock), 0, st,
+    hipLaunchKernelGGL((k_check_forward<MASK>), dim3(nblocks, F), dim3(kBlock), 0, st,
                        ctx->d_sx.as<const double>(), ctx->d_sy.as<const double>(),
                        ctx->d_sz.as<const double>(), ctx->check_sidx.as<const int>(), F,
                        ctx->d_Xc.as<const double>(), ctx->d_Yc.as<const double>(),

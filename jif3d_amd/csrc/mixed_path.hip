@@ -14,7 +14,9 @@
 // U_ij = G rho V d2Phi/dx_i dx_j.  Odd orders vanish by symmetry, so the neglected
 // term is O((h/d)^6) relative.  Cubes have B = 0.  The formulas are evaluated in
 // direction cosines u = x/r so that no power of 1/r below r^-7 is formed (FP32
-// range).  Beyond a second radius (q1 half-diagonals, default 26, tested per tile
+// range); 1/r is the hardware estimate, refined by one Newton step in the multipole
+// path and used as it is, with its mean bias taken out in FP64, in the monopole paths.
+// Beyond a second radius (q1 half-diagonals, default 32, tested per tile
 // against the distance from the site to the tile's nearest cell centre) the
 // hexadecapole, O((h/d)^4) of a cell's field there, is dropped; for cubes the remaining
 // monopole is evaluated in a
@@ -61,6 +63,17 @@ constexpr int kTile = 4;  // far-field tile edge (cells)
 constexpr int kFarForwardWaves = 65536;
 constexpr int kFarAdjointWaves = 98304;
 constexpr int kTableWidth = 12;  // floats per cell of a non-uniform mesh
+
+// Mean relative error of the hardware reciprocal square root (v_rsq_f32) on gfx950,
+// measured over 4M values of r^2 log-uniform in [1e2, 1e7]: -7.4e-9 (-8.6e-9 for even
+// binary exponents of r^2, -6.3e-9 for odd), rms 2.8e-8 -- about the rms that one
+// Newton step leaves, but biased.  The monopole paths (cells beyond the second radius,
+// most pairs) use the estimate without a Newton step and take the bias out of their
+// FP32 sums in FP64: s^3 and s^5 are low by 3 and 5 times it, coherently over millions
+// of cells.  The multipole path, whose terms are larger, keeps the Newton step.
+constexpr double kRsqBias = 7.416e-9;
+constexpr double kFixS3 = 1.0 + 3.0 * kRsqBias;
+constexpr double kFixS5 = 1.0 + 5.0 * kRsqBias;
 
 // Mesh classes of the far-field kernels.
 enum : int { kMeshCubes = 0, kMeshUniform = 1, kMeshGeneral = 2 };
@@ -349,9 +362,7 @@ __device__ __forceinline__ void far_tile_mono(const FarGeom& g, const float* __r
         const float Z = Zc[kk];
         const float ZZ = Z * Z;
         const float r2 = XXYY + ZZ;
-        float sv = __builtin_amdgcn_rsqf(r2);
-        const float e = fmaf(-(r2 * sv), sv, 1.0f);  // one Newton step
-        sv = fmaf(0.5f * sv, e, sv);
+        const float sv = __builtin_amdgcn_rsqf(r2);  // bias removed at the row's flush
         const float s2 = sv * sv;
         const float s3 = s2 * sv;
 #pragma unroll
@@ -383,17 +394,22 @@ __device__ __forceinline__ void far_tile_mono(const FarGeom& g, const float* __r
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       int k = 0;
-      if constexpr (GZ) acc64[b][k++] += (double)(-AZ3[b]);
+      // explicit FMAs: the same rounding in every instantiation (batches equal singles)
+      auto put = [&](double fix, float v) {
+        acc64[b][k] = fma(fix, (double)v, acc64[b][k]);
+        ++k;
+      };
+      if constexpr (GZ) put(kFixS3, -AZ3[b]);
       if constexpr (DIAG) {
         const float P = (X * X) * A5[b];
-        acc64[b][k++] += (double)fmaf(2.0f, P, -(AYY5[b] + AZZ5[b]));
-        acc64[b][k++] += (double)fmaf(2.0f, AYY5[b], -(P + AZZ5[b]));
-        acc64[b][k++] += (double)fmaf(2.0f, AZZ5[b], -(P + AYY5[b]));
+        put(kFixS5, fmaf(2.0f, P, -(AYY5[b] + AZZ5[b])));
+        put(kFixS5, fmaf(2.0f, AYY5[b], -(P + AZZ5[b])));
+        put(kFixS5, fmaf(2.0f, AZZ5[b], -(P + AYY5[b])));
       }
       if constexpr (OFF) {
-        acc64[b][k++] += (double)(X3 * AY5[b]);
-        acc64[b][k++] += (double)(X3 * AZ5[b]);
-        acc64[b][k++] += (double)(3.0f * AYZ5[b]);
+        put(kFixS5, X3 * AY5[b]);
+        put(kFixS5, X3 * AZ5[b]);
+        put(kFixS5, 3.0f * AYZ5[b]);
       }
     }
   }
@@ -630,12 +646,14 @@ __global__ __launch_bounds__(kBlock) void k_mixed_coef(const double* __restrict_
   //   sum_m c_m T_m = s^5 (Q(X, Y, Z) - c_gz Z r^2),
   //   Q = 3 ((c_xx - t/3) X^2 + (c_yy - t/3) Y^2 + (c_zz - t/3) Z^2 + c_xy XY + c_xz XZ
   //       + c_yz YZ);
-  // stored: -c_gz, the six factors of Q, 0
+  // stored: -c_gz, the six factors of Q, 0; scaled by kFixS5 for the bias of the raw
+  // s^5 of the monopole path (rounded to FP32 per site, the scaling holds on average over
+  // the sites a cell's sum runs over)
   float* o = mono + (long)s * 8;
   const double t3 = (c7[1] + c7[2] + c7[3]) / 3.0;
-  o[0] = (float)(-c7[0]);
-  for (int m = 1; m < 4; ++m) o[m] = (float)(3.0 * (c7[m] - t3));
-  for (int m = 4; m < 7; ++m) o[m] = (float)(3.0 * c7[m]);
+  o[0] = (float)(-c7[0] * kFixS5);
+  for (int m = 1; m < 4; ++m) o[m] = (float)(3.0 * (c7[m] - t3) * kFixS5);
+  for (int m = 4; m < 7; ++m) o[m] = (float)(3.0 * c7[m] * kFixS5);
   o[7] = 0.0f;
 }
 
@@ -672,9 +690,7 @@ __device__ __forceinline__ void far_adjoint_site(
     for (int q = 0; q < kQ; ++q) {
       const float Z = Zq[q];
       const float r2 = fmaf(Z, Z, XXYY);
-      float sv = __builtin_amdgcn_rsqf(r2);
-      const float e = fmaf(-(r2 * sv), sv, 1.0f);  // one Newton step
-      sv = fmaf(0.5f * sv, e, sv);
+      const float sv = __builtin_amdgcn_rsqf(r2);  // bias removed in the coefficients
       const float s2 = sv * sv;
       const float s5 = (s2 * sv) * s2;
 #pragma unroll
