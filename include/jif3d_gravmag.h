@@ -119,7 +119,7 @@ jgm_status jgm_set_transform(jgm_context *ctx, jgm_transform tf, double inclinat
                              double declination);
 /* Near-field radius of JGM_MIXED in cell half-diagonals (default 8). */
 jgm_status jgm_set_mixed_threshold(jgm_context *ctx, double q0);
-/* Second radius of JGM_MIXED in cell half-diagonals (default 32): cells farther than
+/* Second radius of JGM_MIXED in cell half-diagonals (default 28): cells farther than
  * q1 from a site keep the monopole (and, for non-cubic cells, quadrupole) terms only;
  * the hexadecapole term dropped there is O((h/d)^4) of each cell's field.  0 keeps the
  * hexadecapole for every far cell. */

@@ -145,7 +145,7 @@ struct jgm_context {
   // in half-diagonals) beyond which the far field keeps only monopole (and quadrupole)
   // terms; 0 keeps the hexadecapole everywhere
   double mixed_q0 = 8.0;
-  double mixed_q1 = 32.0;
+  double mixed_q1 = 28.0;
 
   // a-posteriori check of JGM_MIXED: FP64 at check_sites sample sites after a forward and
   // at check_cells sample cells after an adjoint; a sampled error above check_tol (relative

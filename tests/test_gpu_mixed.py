@@ -173,23 +173,23 @@ def test_headline_accuracy_full_size():
 
 def test_far_radius_settings():
     """q1 = 0 keeps the hexadecapole for every far cell; the default second radius
-    (monopole only beyond 32 half-diagonals) stays within the stated tolerance; the
+    (monopole only beyond 28 half-diagonals) stays within the stated tolerance; the
     regrouped monopole paths for cubes match the generic expansion."""
     m, d = small_headline()
     ref = ScalarTensorHipGravityImp(nat.FP64).Calculate(m, d)
     r = np.random.default_rng(6).normal(size=d.nsites * 10) * 1e-9
     g64 = ScalarTensorHipGravityImp(nat.FP64).LQDerivative(m, d, r)
     out = {}
-    for q1 in (0.0, 32.0, 4.0):
+    for q1 in (0.0, 28.0, 4.0):
         imp = ScalarTensorHipGravityImp(nat.MIXED)
         imp.context(m, d).set_mixed_far_radius(q1)
         imp.context(m, d).set_mixed_check(0, 0, 5e-7)  # the expansion itself, no fallback
         out[q1] = imp.Calculate(m, d)
         assert rel_inf(imp.LQDerivative(m, d, r), g64) <= TOL
     assert comp_err(out[0.0], ref, 10) <= TOL
-    assert comp_err(out[32.0], ref, 10) <= TOL
+    assert comp_err(out[28.0], ref, 10) <= TOL
     # a second radius inside the near box drops the hexadecapole for every far cell
-    assert comp_err(out[4.0], ref, 10) > comp_err(out[32.0], ref, 10)
+    assert comp_err(out[4.0], ref, 10) > comp_err(out[28.0], ref, 10)
     ctx = nat.Context(0)
     with pytest.raises(nat.JgmError):
         ctx.set_mixed_far_radius(1.0)
