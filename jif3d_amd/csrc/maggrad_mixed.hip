@@ -200,8 +200,10 @@ hipError_t ensure_slab_mesh(jgm_context* ctx) {
 
 hipError_t launch_forward_maggrad_mixed(jgm_context* ctx, const double* model, double* out,
                                         double* raw_out) {
-  hipError_t e = ensure_slab_mesh(ctx);
-  if (e == hipErrorNotSupported) return launch_forward_fp64(ctx, JGM_KIND_MAGNETIC_GRADIENT, model, out, raw_out);
+  // no cells: the FP64 path's zeros (nothing to expand)
+  hipError_t e = ncells(ctx) == 0 ? hipErrorNotSupported : ensure_slab_mesh(ctx);
+  if (e == hipErrorNotSupported)
+    return launch_forward_fp64(ctx, JGM_KIND_MAGNETIC_GRADIENT, model, out, raw_out);
   if (e != hipSuccess) return e;
   SlabMesh& sm = ctx->slab;
   jgm_context* c = sm.child;
@@ -237,7 +239,7 @@ hipError_t launch_forward_maggrad_mixed(jgm_context* ctx, const double* model, d
 }
 
 hipError_t launch_adjoint_maggrad_mixed(jgm_context* ctx, const double* rm, double* grad) {
-  hipError_t e = ensure_slab_mesh(ctx);
+  hipError_t e = ncells(ctx) == 0 ? hipErrorNotSupported : ensure_slab_mesh(ctx);
   if (e == hipErrorNotSupported)
     return launch_adjoint_fp64(ctx, JGM_KIND_MAGNETIC_GRADIENT, nullptr, rm, grad);
   if (e != hipSuccess) return e;

@@ -122,3 +122,37 @@ def test_imp_mixed_precision_and_geometry_change():
     mixed2 = MinMemGravMagCalculator(HipMagneticGradImp(*f2, precision=nat.MIXED))
     assert rel_inf(mixed2.Calculate(m, d2), ref2.Calculate(m, d2)) <= TOL
     assert rel_inf(mixed.Calculate(m, d2), ref.Calculate(m, d2)) <= TOL
+
+
+def test_edge_cases():
+    """No cells (background only: the vertical gradient has no background terms), one cell
+    and one site (inside every near box: FP64 throughout), no sites."""
+    field = (0.7, -0.4, 52000.0)
+    m = ThreeDSusceptibilityModel()
+    m.SetMeshSize(0, 0, 0)
+    m.SetBackgroundThicknesses([200.0, 3500.0])
+    m.SetBackgroundSusceptibilities([0.01, 0.02])
+    d = TotalFieldMagneticData()
+    d.SetMeasurementPoints([3e4, 5e4], [2e4, 6e4], [0.0, -10.0])
+    ctx = nat.Context(0)
+    ctx.configure(m, d, field)
+    mv = m.GetModelParameters()
+    assert np.array_equal(ctx.forward(K, nat.MIXED, mv), ctx.forward(K, nat.FP64, mv))
+    r = np.array([1.0, -2.0])
+    assert np.array_equal(ctx.adjoint(K, nat.MIXED, mv, r), ctx.adjoint(K, nat.FP64, mv, r))
+    m = ThreeDSusceptibilityModel()
+    m.SetMeshSize(1, 1, 1)
+    for f in (m.SetXCellSizes, m.SetYCellSizes, m.SetZCellSizes):
+        f([20.0])
+    m.SetSusceptibilities([0.05])
+    d = TotalFieldMagneticData()
+    d.AddMeasurementPoint(5.0, 7.0, -3.0)
+    ctx = nat.Context(0)
+    ctx.configure(m, d, field)
+    mv = m.GetModelParameters()
+    ref = ctx.forward(K, nat.FP64, mv)
+    assert rel_inf(ctx.forward(K, nat.MIXED, mv), ref) <= 1e-12
+    g64 = ctx.adjoint(K, nat.FP64, mv, np.array([1.5]))
+    assert rel_inf(ctx.adjoint(K, nat.MIXED, mv, np.array([1.5])), g64) <= 1e-12
+    imp = HipMagneticGradImp(*field, precision=nat.MIXED)
+    assert imp.Calculate(m, TotalFieldMagneticData()).size == 0
