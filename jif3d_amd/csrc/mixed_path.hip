@@ -576,18 +576,22 @@ __global__ __launch_bounds__(kBlock) void k_near_forward(
       for (int m = 0; m < NC; ++m) acc[m] = fma(w, t[m], acc[m]);
     }
   }
-  __shared__ double red[kBlock];
+  // block sums: within each wave by lane shuffles, then the four wave sums in fixed order
+  __shared__ double red[NC][kBlock / 64];
   double sc[NC];
   comp_scales<MASK>(sc);
+#pragma unroll
   for (int m = 0; m < NC; ++m) {
-    red[threadIdx.x] = acc[m];
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) nearout[(long)s * NC + m] = red[0] * sc[m];
-    __syncthreads();
+    double v = acc[m];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[m][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int m = 0; m < NC; ++m)
+      nearout[(long)s * NC + m] = ((red[m][0] + red[m][1]) + (red[m][2] + red[m][3])) * sc[m];
   }
 }
 
