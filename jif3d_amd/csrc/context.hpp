@@ -63,6 +63,9 @@ struct NearBoxes {
   // FP32 offset of each cell centre from the centre of the first cell of its 4-cell
   // tile run along the axis (exact for uniform meshes)
   DeviceBuffer xoff, yoff, zoff;
+  // squared largest half-size of each 4-run of cells per axis (z also per 16-run): the
+  // tiles' half-diagonals that scale the second radius
+  DeviceBuffer hx2, hy2, hz2, hz2_16;
   int mesh_class = 2;  // 0: cubes of one size, 1: one cell shape, 2: general
   DeviceBuffer adj_table;  // FP32 per-cell table with unit density (G V, B...)
   bool adj_table_valid = false;

@@ -16,8 +16,8 @@
 // direction cosines u = x/r so that no power of 1/r below r^-7 is formed (FP32
 // range); 1/r is the hardware estimate, refined by one Newton step in the multipole
 // path and used as it is, with its mean bias taken out in FP64, in the monopole paths.
-// Beyond a second radius (q1 half-diagonals, default 28, tested per tile
-// against the distance from the site to the tile's nearest cell centre) the
+// Beyond a second radius (q1 half-diagonals of the tile's largest cell, default 28,
+// tested per tile against the distance from the site to the tile's nearest cell centre) the
 // hexadecapole, O((h/d)^4) of a cell's field there, is dropped; for cubes the remaining
 // monopole is evaluated in a
 // regrouped form (running sums of w s^3 and w s^5 along a column of cells in the
@@ -304,6 +304,12 @@ struct FarGeom {
   const float* xoff;  // centre minus the centre of the first cell of its 4-run (FP32)
   const float* yoff;
   const float* zoff;
+  // squared largest half-size per 4-run of cells along each axis (z also per 16-run, the
+  // adjoint's tile depth): a tile's squared largest half-diagonal is the sum of its three
+  const float* hx2;
+  const float* hy2;
+  const float* hz2;
+  const float* hz2_16;
   int nx, ny, nz;
   int ntx, nty, ntz;
 };
@@ -479,7 +485,7 @@ template <int MASK, int MESH, int NB>
 __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(far_fwd_waves(NB)))) void k_far_forward(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
-    int S, const int* __restrict__ order, const int* __restrict__ box, float r1sq, FarGeom g,
+    int S, const int* __restrict__ order, const int* __restrict__ box, float q1sq, FarGeom g,
     const float* __restrict__ shape, const float* __restrict__ wt, MpConst uc, long ntiles,
     long tiles_per_chunk, double* __restrict__ farpart) {
   constexpr int NC = Ncomp<MASK>::value;
@@ -507,11 +513,13 @@ __attribute__((amdgpu_waves_per_eu(far_fwd_waves(NB)))) void k_far_forward(
                 Zt = (float)(pz - g.zcen[k0]);
     const bool hit = !(bx[1] < i0 || bx[0] > i0 + ni - 1 || bx[3] < j0 || bx[2] > j0 + nj - 1 ||
                        bx[5] < k0 || bx[4] > k0 + nk - 1);
-    // some cell of the tile closer than the second radius: keep the hexadecapole (the
-    // offsets to the tile's cell centres span [Xt - extent, Xt] along each axis)
+    // some cell of the tile closer than the second radius, q1 half-diagonals of the tile's
+    // largest cell: keep the hexadecapole (the offsets to the tile's cell centres span
+    // [Xt - extent, Xt] along each axis)
     const float dx = fmaxf(0.0f, fmaxf(-Xt, Xt - g.xoff[i0 + ni - 1]));
     const float dy = fmaxf(0.0f, fmaxf(-Yt, Yt - g.yoff[j0 + nj - 1]));
     const float dz = fmaxf(0.0f, fmaxf(-Zt, Zt - g.zoff[k0 + nk - 1]));
+    const float r1sq = q1sq * (g.hx2[ti] + g.hy2[tj] + g.hz2[tk]);
     const bool need = fmaf(dx, dx, fmaf(dy, dy, dz * dz)) < r1sq;
     if (__any(hit))
       far_tile<MASK, MESH, true, true, NB>(g, shape, wt, uc, i0, j0, k0, ni, nj, nk, Xt, Yt, Zt,
@@ -786,7 +794,7 @@ __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(far_adj_waves(NB)))) void k_far_adjoint(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     const float* __restrict__ coef, const float* __restrict__ mono, const int* __restrict__ box,
-    float r1sq, int S, int sites_per_chunk, FarGeom g, const float* __restrict__ table,
+    float q1sq, int S, int sites_per_chunk, FarGeom g, const float* __restrict__ table,
     MpConst uc, long ntiles, double* __restrict__ gpart) {
   constexpr int TW = TableWidth<MESH>::value;
   constexpr int KT = kQ * kTile;  // tile depth in k
@@ -813,6 +821,8 @@ __attribute__((amdgpu_waves_per_eu(far_adj_waves(NB)))) void k_far_adjoint(
   // extent of the tile's cell centres from its first cell, per axis
   const float ex = (float)(g.xcen[i0 + ni - 1] - x0), ey = (float)(g.ycen[j0 + nj - 1] - y0),
               ez = (float)(g.zcen[k0 + nk - 1] - z0);
+  // the second radius: q1 half-diagonals of the tile's largest cell
+  const float r1sq = q1sq * (g.hx2[ti] + g.hy2[tj] + g.hz2_16[tk]);
   bool vq[kQ];
   int kq[kQ];
   long cq[kQ];
@@ -1035,6 +1045,10 @@ FarGeom far_geom(const jgm_context* ctx) {
   g.xoff = ctx->near.xoff.as<const float>();
   g.yoff = ctx->near.yoff.as<const float>();
   g.zoff = ctx->near.zoff.as<const float>();
+  g.hx2 = ctx->near.hx2.as<const float>();
+  g.hy2 = ctx->near.hy2.as<const float>();
+  g.hz2 = ctx->near.hz2.as<const float>();
+  g.hz2_16 = ctx->near.hz2_16.as<const float>();
   g.nx = (int)ctx->nx;
   g.ny = (int)ctx->ny;
   g.nz = (int)ctx->nz;
@@ -1044,12 +1058,11 @@ FarGeom far_geom(const jgm_context* ctx) {
   return g;
 }
 
-// Square of the second radius in m^2 (cells closer than it keep the hexadecapole);
-// every cell when the radius is 0.
-float second_radius_sq(const jgm_context* ctx) {
+// Square of the second radius in tile half-diagonals (q1^2; a tile's cells closer than q1
+// of its largest cell's half-diagonals keep the hexadecapole); every cell when q1 is 0.
+float second_radius_q2(const jgm_context* ctx) {
   if (!(ctx->mixed_q1 > 0.0)) return FLT_MAX;
-  const double r1 = ctx->mixed_q1 * ctx->near.half_diag;
-  return (float)(r1 * r1);
+  return (float)(ctx->mixed_q1 * ctx->mixed_q1);
 }
 
 MpConst uniform_const(const jgm_context* ctx) {
@@ -1193,7 +1206,7 @@ hipError_t forward_mixed_impl(jgm_context* ctx, int kind, int nbm, const double*
                                      ctx->d_sx.as<const double>(), ctx->d_sy.as<const double>(),
                                      ctx->d_sz.as<const double>(), S,
                                      ctx->near.site_order.as<const int>(),
-                                     ctx->near.box.as<const int>(), second_radius_sq(ctx), g,
+                                     ctx->near.box.as<const int>(), second_radius_q2(ctx), g,
                                      ctx->near.adj_table.as<const float>(),
                                      ctx->weights.as<const float>(), uniform_const(ctx), ntiles,
                                      per, ctx->farpart.as<double>());
@@ -1296,7 +1309,7 @@ hipError_t adjoint_mixed_impl(jgm_context* ctx, int kind, int nbm, const double*
                                      ctx->d_sx.as<const double>(), ctx->d_sy.as<const double>(),
                                      ctx->d_sz.as<const double>(), ctx->coef32.as<const float>(),
                                      ctx->coef_mono.as<const float>(), nb.box.as<const int>(),
-                                     second_radius_sq(ctx), S, per, g,
+                                     second_radius_q2(ctx), S, per, g,
                                      nb.adj_table.as<const float>(), uniform_const(ctx), ntiles8,
                                      ctx->hpart.as<double>());
     }
@@ -1524,6 +1537,18 @@ hipError_t build_near_boxes(jgm_context* ctx) {
   };
   const std::vector<float> xoff = run_offsets(xcen), yoff = run_offsets(ycen),
                            zoff = run_offsets(zcen);
+  // squared largest half-size of each run of `len` cells along an axis
+  auto run_h2 = [](const std::vector<double>& sz, int len) {
+    std::vector<float> h((sz.size() + len - 1) / len + 1, 0.0f);
+    for (size_t i = 0; i < sz.size(); ++i)
+      h[i / len] = std::max(h[i / len], (float)(0.25 * sz[i] * sz[i]));
+    return h;
+  };
+  if ((e = upload_vec(nb.hx2, run_h2(ctx->xs, kTile), st)) != hipSuccess ||
+      (e = upload_vec(nb.hy2, run_h2(ctx->ys, kTile), st)) != hipSuccess ||
+      (e = upload_vec(nb.hz2, run_h2(ctx->zs, kTile), st)) != hipSuccess ||
+      (e = upload_vec(nb.hz2_16, run_h2(ctx->zs, kQ * kTile), st)) != hipSuccess)
+    return e;
   if ((e = upload_vec(nb.xoff, xoff, st)) != hipSuccess) return e;
   if ((e = upload_vec(nb.yoff, yoff, st)) != hipSuccess) return e;
   if ((e = upload_vec(nb.zoff, zoff, st)) != hipSuccess) return e;
