@@ -1058,8 +1058,10 @@ hipError_t launch_storedj_build_file(jgm_context* ctx, int kind, size_t block_ce
                       std::to_string(serial.fetch_add(1)) + ".bin";
   ctx->storedJ_fd = ::open(ctx->storedJ_file.c_str(), O_RDWR | O_CREAT | O_TRUNC, 0600);
   if (ctx->storedJ_fd < 0) {
+    const int err = errno;  // kept for the caller's message past the frees below
     ctx->storedJ_file.clear();
     release_host_storedj(ctx);
+    errno = err;
     return hipErrorFileNotFound;
   }
   ctx->storedJ_block = (size_t)blk;
@@ -1081,7 +1083,9 @@ hipError_t launch_storedj_build_file(jgm_context* ctx, int kind, size_t block_ce
   }
   if (e == hipSuccess) e = hipGetLastError();
   if (e != hipSuccess) {
+    const int err = errno;
     release_host_storedj(ctx);
+    errno = err;
     return e;
   }
   ctx->storedJ_kind = kind;

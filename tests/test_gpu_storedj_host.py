@@ -152,3 +152,25 @@ def test_disk_calculator_with_a_directory(tmp_path):
     r = rng.normal(size=got.size)
     g, gw = calc.LQDerivative(m, d, r), direct.LQDerivative(m, d, r)
     assert np.max(np.abs(g - gw)) <= 1e-5 * np.max(np.abs(gw))
+
+
+def test_file_backed_J_errors(tmp_path):
+    """A directory that does not exist is reported (no J, no file left); bad block sizes
+    are rejected; a later build in a good directory works."""
+    m, d, rng = problem(9)
+    kind = nat.KIND_SCALAR
+    ctx = nat.Context(0)
+    ctx.configure(m, d)
+    with pytest.raises(nat.JgmError) as e:
+        ctx.storedj_build(kind, nat.FP64, host_block_cells=256, directory=str(tmp_path / "nope"))
+    assert "nope" in str(e.value)
+    assert ctx.storedj_rows() == 0 and not ctx.storedj_file_backed()
+    st = ctx.lib.jgm_storedj_build_file(ctx.handle, kind, nat.FP64, 0, str(tmp_path).encode())
+    assert st == nat.JGM_ERR_INVALID_ARGUMENT
+    assert not list(tmp_path.iterdir())
+    ctx.storedj_build(kind, nat.FP64, host_block_cells=256, directory=str(tmp_path))
+    assert ctx.storedj_file_backed() and ctx.storedj_rows() == d.nsites
+    mv = m.GetModelParameters()
+    want = ctx.forward(kind, nat.FP64, mv)
+    got = ctx.storedj_forward(nat.GEMM_FP32, mv)
+    assert np.max(np.abs(got - want)) <= 1e-5 * np.max(np.abs(want))
