@@ -70,7 +70,6 @@ struct NearBoxes {
   DeviceBuffer adj_table;  // FP32 per-cell table with unit density (G V, B...)
   bool adj_table_valid = false;
   DeviceBuffer site_order;  // int32 [S]: sites in Morton order of (x, y), for compact waves
-  double half_diag = 0.0;   // largest cell half-diagonal (m)
   // sites on mesh lines (two coordinates equal to corner coordinates of the mesh): the
   // mixed path evaluates them with the FP64 closed form; nline of them, their survey
   // indices (int32) and coordinates (FP64 x[nline], y[nline], z[nline])

@@ -1453,7 +1453,6 @@ hipError_t build_near_boxes(jgm_context* ctx) {
   // any cell outside the box is >= R away along some axis, and R >= q0 * its half-diagonal
   const double R = ctx->mixed_q0 * 0.5 * std::sqrt(mx * mx + my * my + mz * mz);
   nb.radius = R;
-  nb.half_diag = 0.5 * std::sqrt(mx * mx + my * my + mz * mz);
   bool uniform = nx > 0 && ny > 0 && nz > 0;
   for (double v : ctx->xs) uniform = uniform && v == ctx->xs[0];
   for (double v : ctx->ys) uniform = uniform && v == ctx->ys[0];

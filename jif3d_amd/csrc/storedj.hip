@@ -350,6 +350,28 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm(const float* __restrict__ J,
     }
 }
 
+// Operand blocks shared by the four waves of a workgroup: 64 k (eight groups of eight) x 64
+// right-hand sides of a BF16 operand in the [k / 8][P] layout of k_rows_to_bf16, staged in
+// LDS as [group][right-hand side].  Thread t carries entries t and t + 256 from global memory
+// to LDS; a lane's MFMA operand for k-step st and tile t is entry (2 st + lk) 64 + 32 t + li.
+struct OpStage {
+  bf16x8 v[2];
+};
+
+__device__ __forceinline__ void op_load(OpStage& o, const bf16x8* __restrict__ base, long g0,
+                                        int P) {
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int e = (int)threadIdx.x + 256 * h;
+    o.v[h] = base[(g0 + (e >> 6)) * P + (e & 63)];
+  }
+}
+
+__device__ __forceinline__ void op_store(const OpStage& o, bf16x8* lds) {
+  lds[threadIdx.x] = o.v[0];
+  lds[threadIdx.x + 256] = o.v[1];
+}
+
 // BF16 J M from the packed BF16 copy of J (k_pack_bf16_cells: eight cells of a row per 16
 // bytes) and a BF16 copy of M (k_rows_to_bf16: eight cells of a right-hand side per 16
 // bytes).  Per 16-cell k-step a lane's A operand (row li or li + 32 of its 64, cells
@@ -362,6 +384,7 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm_bf16p(const bf16x8* __restri
                                                           const bf16x8* __restrict__ Mb, int P,
                                                           long per, int row_tiles, int nsplit,
                                                           float* __restrict__ Yp) {
+  __shared__ bf16x8 sM[2][512];
   const int w = blockIdx.x;
   int split, rt;
   if ((nsplit & 7) == 0) {
@@ -376,15 +399,15 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm_bf16p(const bf16x8* __restri
   const int lane = threadIdx.x & 63;
   const int li = lane & 31, lk = lane >> 5;
   const long row0 = (long)rt * 256 + wave * 64;
-  if (row0 >= ld) return;  // whole wave outside the padded rows; no barriers follow
+  // a wave past the padded rows only helps stage M (every wave reaches every barrier)
+  const bool active = row0 < ld;
   const int b0 = blockIdx.y * 64;
   const long c0 = (long)split * per;
   const long cend = c0 + per < colpad ? c0 + per : colpad;
   const long nblk = cend > c0 ? (cend - c0) / 64 : 0;
   const bf16x8* A0 = Jp + row0 + li;
   const bf16x8* A1 = A0 + 32;
-  const bf16x8* B0 = Mb + b0 + li;
-  const bf16x8* B1 = B0 + 32;
+  const bf16x8* Mbase = Mb + b0;
   f32x16 acc[2][2];
   float tot[2][2][16];
 #pragma unroll
@@ -393,15 +416,15 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm_bf16p(const bf16x8* __restri
     for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[ta][tb][r] = tot[ta][tb][r] = 0.0f;
-  // J fragments of three 64-cell blocks and M fragments of two are live: J is loaded two
-  // blocks ahead (HBM latency), M one block ahead (L2)
+  // J fragments of four 64-cell blocks are live (loaded three blocks ahead); M is staged in
+  // LDS for the four waves, block b + 1 stored while block b is multiplied, block b + 2 in
+  // registers on its way
   struct JQ {
     bf16x8 a[4][2];  // [k-step][row tile]
-  } j0, j1, j2;
-  struct MQ {
-    bf16x8 b[4][2];  // [k-step][right-hand-side tile]
-  } m0, m1;
+  } j0, j1, j2, j3;
+  OpStage m0, m1;
   auto loadJ = [&](JQ& q, long blk) {
+    if (!active) return;
     const long g0 = (c0 + 64 * blk) / 8;  // first 8-cell group of the block
 #pragma unroll
     for (int st = 0; st < 4; ++st) {
@@ -412,48 +435,47 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm_bf16p(const bf16x8* __restri
                                                   reinterpret_cast<const u32x4*>(A1 + g * ld)));
     }
   };
-  auto loadM = [&](MQ& q, long blk) {
-    const long g0 = (c0 + 64 * blk) / 8;
-#pragma unroll
-    for (int st = 0; st < 4; ++st) {
-      const long g = g0 + 2 * st + lk;
-      q.b[st][0] = B0[g * P];
-      q.b[st][1] = B1[g * P];
-    }
-  };
-  // block b: multiply (jc, mc) while block b + 2 of J and b + 1 of M load (clamped to the
-  // last block past the end: unused reloads of valid memory)
-  auto step = [&](const JQ& jc, const MQ& mc, JQ& jl, MQ& ml, long b) {
-    loadJ(jl, b + 2 < nblk ? b + 2 : nblk - 1);
-    loadM(ml, b + 1 < nblk ? b + 1 : nblk - 1);
+  auto clampb = [&](long b) { return b < nblk ? b : nblk - 1; };
+  // block b (J set jc, M in LDS buffer buf): load J block b + 3 into jl and M block b + 2
+  // into mload, multiply, store M block b + 1 (mstore) into the other buffer
+  auto step = [&](const JQ& jc, JQ& jl, OpStage& mload, const OpStage& mstore, int buf, long b) {
+    loadJ(jl, clampb(b + 3));
+    op_load(mload, Mbase, (c0 + 64 * clampb(b + 2)) / 8, P);
     __builtin_amdgcn_sched_barrier(0);
-    if (b < nblk) {
+    if (b < nblk && active) {
+      const bf16x8* sb = sM[buf];
 #pragma unroll
-      for (int st = 0; st < 4; ++st)
+      for (int st = 0; st < 4; ++st) {
+        const bf16x8 bv0 = sb[(2 * st + lk) * 64 + li], bv1 = sb[(2 * st + lk) * 64 + 32 + li];
 #pragma unroll
-        for (int ta = 0; ta < 2; ++ta)
-#pragma unroll
-          for (int tb = 0; tb < 2; ++tb)
-            acc[ta][tb] = mfma_bf16(jc.a[st][ta], mc.b[st][tb], acc[ta][tb]);
+        for (int ta = 0; ta < 2; ++ta) {
+          acc[ta][0] = mfma_bf16(jc.a[st][ta], bv0, acc[ta][0]);
+          acc[ta][1] = mfma_bf16(jc.a[st][ta], bv1, acc[ta][1]);
+        }
+      }
       if ((b & 3) == 3) flush(acc, tot);  // every 256 cells from c0
     }
-    __builtin_amdgcn_sched_barrier(0);
+    op_store(mstore, sM[buf ^ 1]);
+    __syncthreads();
   };
   if (nblk > 0) {
     loadJ(j0, 0);
-    loadJ(j1, nblk > 1 ? 1 : 0);
-    loadM(m0, 0);
-    // block b uses J set b % 3 and M set b % 2
-    for (long q = 0; q < nblk; q += 6) {
-      step(j0, m0, j2, m1, q);
-      step(j1, m1, j0, m0, q + 1);
-      step(j2, m0, j1, m1, q + 2);
-      step(j0, m1, j2, m0, q + 3);
-      step(j1, m0, j0, m1, q + 4);
-      step(j2, m1, j1, m0, q + 5);
+    loadJ(j1, clampb(1));
+    loadJ(j2, clampb(2));
+    op_load(m0, Mbase, c0 / 8, P);
+    op_store(m0, sM[0]);
+    op_load(m1, Mbase, (c0 + 64 * clampb(1)) / 8, P);
+    __syncthreads();
+    // block b: J set b % 4, M register stage b % 2, LDS buffer b % 2
+    for (long q = 0; q < nblk; q += 4) {
+      step(j0, j3, m0, m1, 0, q);
+      step(j1, j0, m1, m0, 1, q + 1);
+      step(j2, j1, m0, m1, 0, q + 2);
+      step(j3, j2, m1, m0, 1, q + 3);
     }
   }
   flush(acc, tot);
+  if (!active) return;
   float* out = Yp + (long)split * R * P;
 #pragma unroll
   for (int ta = 0; ta < 2; ++ta)
@@ -530,17 +552,17 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jtr_bf16c(const bf16x8* __restr
                                                            const bf16x8* __restrict__ Rb, int P,
                                                            float* __restrict__ G, long ldg,
                                                            int nrhs) {
+  __shared__ bf16x8 sR[2][512];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int li = lane & 31, lk = lane >> 5;
   const int b0 = blockIdx.y * 64;
   const long ngr = ld / 8;           // 8-row groups per cell
-  const long nblock64 = ld / 64;     // 64-row blocks (ld is a multiple of 64)
+  const long nb = ld / 64;           // 64-row blocks (ld is a multiple of 64)
+  const bf16x8* Rbase = Rb + b0;
   for (long blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
     const long cbase = blk * kColPad + 64 * wave;
     const bf16x8* J0 = Jc + (cbase / 32) * ngr * 32 + lk * 32 + li;  // tile of cells cbase..
     const bf16x8* J1 = J0 + ngr * 32;                                // .. and cbase + 32..
-    const bf16x8* R0 = Rb + (long)lk * P + b0 + li;
-    const bf16x8* R1 = R0 + 32;
     f32x16 acc[2][2];
     float tot[2][2][16];
 #pragma unroll
@@ -549,14 +571,13 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jtr_bf16c(const bf16x8* __restr
       for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[ta][tb][r] = tot[ta][tb][r] = 0.0f;
-    // J fragments of three 64-row blocks and R fragments of two are live: J is loaded two
-    // blocks ahead (HBM latency), R one block ahead (L2)
+    // J fragments of four 64-row blocks are live (loaded three blocks ahead); R is staged in
+    // LDS for the four waves, block b + 1 stored while block b is multiplied, block b + 2
+    // in registers on its way
     struct JQ {
       bf16x8 b[4][2];  // [k-step][cell tile]
-    } j0, j1, j2;
-    struct RQ {
-      bf16x8 a[4][2];  // [k-step][right-hand-side tile]
-    } r0, r1;
+    } j0, j1, j2, j3;
+    OpStage r0, r1;
     auto loadJ = [&](JQ& q, long blk64) {
       const long g0 = blk64 * 8;  // first 8-row group of the block
 #pragma unroll
@@ -568,43 +589,41 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jtr_bf16c(const bf16x8* __restr
                                                     reinterpret_cast<const u32x4*>(J1 + g * 32)));
       }
     };
-    auto loadR = [&](RQ& q, long blk64) {
-      const long g0 = blk64 * 8;
-#pragma unroll
-      for (int st = 0; st < 4; ++st) {
-        const long g = g0 + 2 * st;
-        q.a[st][0] = R0[g * P];
-        q.a[st][1] = R1[g * P];
-      }
-    };
-    const long nb = nblock64;
-    auto step = [&](const JQ& jc, const RQ& rc, JQ& jl, RQ& rl, long b) {
-      loadJ(jl, b + 2 < nb ? b + 2 : nb - 1);  // past the end: unused reloads
-      loadR(rl, b + 1 < nb ? b + 1 : nb - 1);
+    auto clampb = [&](long b) { return b < nb ? b : nb - 1; };
+    auto step = [&](const JQ& jc, JQ& jl, OpStage& rload, const OpStage& rstore, int buf,
+                    long b) {
+      loadJ(jl, clampb(b + 3));  // past the end: unused reloads of valid memory
+      op_load(rload, Rbase, 8 * clampb(b + 2), P);
       __builtin_amdgcn_sched_barrier(0);
       if (b < nb) {
+        const bf16x8* sa = sR[buf];
 #pragma unroll
-        for (int st = 0; st < 4; ++st)
+        for (int st = 0; st < 4; ++st) {
+          const bf16x8 av0 = sa[(2 * st + lk) * 64 + li], av1 = sa[(2 * st + lk) * 64 + 32 + li];
 #pragma unroll
-          for (int ta = 0; ta < 2; ++ta)
-#pragma unroll
-            for (int tb = 0; tb < 2; ++tb)
-              acc[ta][tb] = mfma_bf16(rc.a[st][ta], jc.b[st][tb], acc[ta][tb]);
+          for (int tb = 0; tb < 2; ++tb) {
+            acc[0][tb] = mfma_bf16(av0, jc.b[st][tb], acc[0][tb]);
+            acc[1][tb] = mfma_bf16(av1, jc.b[st][tb], acc[1][tb]);
+          }
+        }
         if ((b & 3) == 3) flush(acc, tot);  // every 256 rows
       }
-      __builtin_amdgcn_sched_barrier(0);
+      op_store(rstore, sR[buf ^ 1]);
+      __syncthreads();
     };
     loadJ(j0, 0);
-    loadJ(j1, nb > 1 ? 1 : 0);
-    loadR(r0, 0);
-    // block b uses J set b % 3 and R set b % 2
-    for (long q = 0; q < nb; q += 6) {
-      step(j0, r0, j2, r1, q);
-      step(j1, r1, j0, r0, q + 1);
-      step(j2, r0, j1, r1, q + 2);
-      step(j0, r1, j2, r0, q + 3);
-      step(j1, r0, j0, r1, q + 4);
-      step(j2, r1, j1, r0, q + 5);
+    loadJ(j1, clampb(1));
+    loadJ(j2, clampb(2));
+    op_load(r0, Rbase, 0, P);
+    op_store(r0, sR[0]);
+    op_load(r1, Rbase, 8 * clampb(1), P);
+    __syncthreads();
+    // block b: J set b % 4, R register stage b % 2, LDS buffer b % 2
+    for (long q = 0; q < nb; q += 4) {
+      step(j0, j3, r0, r1, 0, q);
+      step(j1, j0, r1, r0, 1, q + 1);
+      step(j2, j1, r0, r1, 0, q + 2);
+      step(j3, j2, r1, r0, 1, q + 3);
     }
     flush(acc, tot);
 #pragma unroll
