@@ -70,6 +70,16 @@ struct NearBoxes {
   DeviceBuffer adj_table;  // FP32 per-cell table with unit density (G V, B...)
   bool adj_table_valid = false;
   DeviceBuffer site_order;  // int32 [S]: sites in Morton order of (x, y), for compact waves
+  // cost-balanced partition of the far adjoint's work (geometry, q1 and the wave target
+  // only): per workgroup of four adjoint tiles, adj_chunks + 1 site boundaries (int32)
+  DeviceBuffer adj_ranges;
+  int adj_chunks = 0;
+  double plan_q1 = -1.0;
+  bool plan_valid = false;
+  // host copies the plan reads: cell centres, boxes, run half-sizes
+  std::vector<double> h_xcen, h_ycen, h_zcen;
+  std::vector<int> h_box;
+  std::vector<float> h_hx2, h_hy2, h_hz2_16;
   // sites on mesh lines (two coordinates equal to corner coordinates of the mesh): the
   // mixed path evaluates them with the FP64 closed form; nline of them, their survey
   // indices (int32) and coordinates (FP64 x[nline], y[nline], z[nline])
@@ -148,6 +158,9 @@ struct jgm_context {
   // terms; 0 keeps the hexadecapole everywhere
   double mixed_q0 = 8.0;
   double mixed_q1 = 28.0;
+  // waves per far adjoint launch its work is partitioned for (environment variable
+  // JIF3D_AMD_FAR_ADJOINT_WAVES at creation)
+  long far_adj_waves = 49152;
 
   // a-posteriori check of JGM_MIXED: FP64 at check_sites sample sites after a forward and
   // at check_cells sample cells after an adjoint; a sampled error above check_tol (relative

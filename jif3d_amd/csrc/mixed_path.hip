@@ -55,13 +55,14 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kTile = 4;  // far-field tile edge (cells)
-// Waves per far-field launch (the sites or tiles are cut into chunks to reach about this
-// many).  The tiles' costs differ (monopole, hexadecapole, masked near-box tiles), and many
+// Waves per far forward launch: the tiles are cut into equal chunks to reach about this
+// many.  The tiles' costs differ (monopole, hexadecapole, masked near-box tiles), and many
 // small chunks balance the CUs: at 128x128x64 cells and 4096 sites, measured against 16384,
-// the forward is fastest at 65536 (2.40 -> 2.04 ms) and the adjoint at 98304 (2.41 -> 2.18
-// ms).  The partial sums this leaves (~0.3 GB per model) are reduced in fixed order.
+// the forward is fastest at 65536 (2.40 -> 2.04 ms); chunks of equal estimated cost (the
+// adjoint's partition below) were slower here at every count tried (6144-65536 waves:
+// 1.84-2.39 ms against 1.71).  The partial sums this leaves (~0.3 GB per model) are
+// reduced in fixed order.
 constexpr int kFarForwardWaves = 65536;
-constexpr int kFarAdjointWaves = 98304;
 constexpr int kTableWidth = 12;  // floats per cell of a non-uniform mesh
 
 // Mean relative error of the hardware reciprocal square root (v_rsq_f32) on gfx950,
@@ -794,8 +795,8 @@ __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(far_adj_waves(NB)))) void k_far_adjoint(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     const float* __restrict__ coef, const float* __restrict__ mono, const int* __restrict__ box,
-    float q1sq, int S, int sites_per_chunk, FarGeom g, const float* __restrict__ table,
-    MpConst uc, long ntiles, double* __restrict__ gpart) {
+    float q1sq, int S, int nchunks, const int* __restrict__ ranges, FarGeom g,
+    const float* __restrict__ table, MpConst uc, long ntiles, double* __restrict__ gpart) {
   constexpr int TW = TableWidth<MESH>::value;
   constexpr int KT = kQ * kTile;  // tile depth in k
   // per-site values of the wave's run of 64 sites: offsets and flags, and the NB sets of
@@ -836,8 +837,10 @@ __attribute__((amdgpu_waves_per_eu(far_adj_waves(NB)))) void k_far_adjoint(
     ozq[q] = (float)(g.zcen[kq[q]] - z0);
     ccq[q] = cell_const<MESH>(table, cq[q], uc);
   }
-  const int s0 = blockIdx.y * sites_per_chunk;
-  const int s1 = s0 + sites_per_chunk < S ? s0 + sites_per_chunk : S;
+  // this chunk's sites: a range of the planned partition (equal estimated cost for every
+  // chunk of the workgroup's four tiles)
+  const int s0 = ranges[blockIdx.x * (nchunks + 1) + blockIdx.y];
+  const int s1 = ranges[blockIdx.x * (nchunks + 1) + blockIdx.y + 1];
   double a64[NB][kQ];
   float acc[NB][kQ];
 #pragma unroll
@@ -1126,6 +1129,96 @@ const char* far_name(int mask, bool adjoint) {
                              : "far_forward_fp32_ftg";
 }
 
+// Relative cost of one site-cell pair of the far field by path: the monopole beyond the
+// second radius (regrouped for cubes), the multipole through the hexadecapole, and the same
+// with the near-box mask (from the kernels' VALU instruction counts per pair).
+struct PairCost {
+  float mono, hexa, hit;
+};
+inline PairCost pair_cost(int mesh_class) {
+  return mesh_class == kMeshCubes ? PairCost{1.0f, 6.0f, 6.5f} : PairCost{1.0f, 2.2f, 2.5f};
+}
+
+// Boundaries of nch pieces of equal cost of a sequence with prefix sums pre[0..n]
+// (pre[0] = 0): out[j] = the first i with pre[i] >= j pre[n] / nch, out[nch] = n.
+void split_equal(const std::vector<double>& pre, int nch, int* out) {
+  const size_t n = pre.size() - 1;
+  out[0] = 0;
+  size_t i = 0;
+  for (int j = 1; j < nch; ++j) {
+    const double target = pre[n] * j / nch;
+    while (i < n && pre[i] < target) ++i;
+    out[j] = (int)i;
+  }
+  out[nch] = (int)n;
+}
+
+// Distance from [lo, hi] to [a, b] along one axis (0 when they overlap).
+inline double gap(double lo, double hi, double a, double b) {
+  return std::max(0.0, std::max(lo - b, a - hi));
+}
+
+// The far adjoint's work partition (geometry, the second radius and the wave target only,
+// so the same for every kind and number of misfit vectors: batches equal single calls bit
+// for bit): the sites of each workgroup of four adjoint tiles are cut into adj_chunks
+// ranges of equal estimated cost (each site's path per tile as the kernel takes it).  At
+// 128x128x64 cells and 4096 sites, 12 such chunks (49152 waves) ran the far adjoint in
+// 1.83 ms against 1.84 ms for 24 equal ones, with half the FP64 partial sums (0.10 GB).
+hipError_t ensure_far_plan(jgm_context* ctx) {
+  NearBoxes& nb = ctx->near;
+  if (nb.plan_valid && nb.plan_q1 == ctx->mixed_q1) return hipSuccess;
+  const int nx = (int)ctx->nx, ny = (int)ctx->ny, nz = (int)ctx->nz;
+  const int S = (int)ctx->S;
+  const PairCost pc = pair_cost(nb.mesh_class);
+  const double q1sq = (double)second_radius_q2(ctx);
+  const std::vector<double>&xc = nb.h_xcen, &yc = nb.h_ycen, &zc = nb.h_zcen;
+  const int ntx = (nx + kTile - 1) / kTile, nty = (ny + kTile - 1) / kTile;
+  hipError_t e;
+  std::vector<int> ar;  // uploaded asynchronously: alive until the synchronisation below
+  // adjoint: workgroups of four tiles of 4 x 4 x (4 kQ) cells, tk fastest
+  {
+    constexpr int KT = kQ * kTile;
+    const int ntz = (nz + KT - 1) / KT;
+    const long ntiles = (long)ntx * nty * ntz, nblocks = (ntiles + 3) / 4;
+    long C = (ctx->far_adj_waves + 4 * nblocks - 1) / std::max(1L, 4 * nblocks);
+    C = std::max<long>(1, std::min<long>(C, std::max(1, S)));
+    ar.assign((size_t)std::max(1L, nblocks) * (C + 1), 0);
+    std::vector<double> cost(S), pre(S + 1);
+    for (long tb = 0; tb < nblocks; ++tb) {
+      std::fill(cost.begin(), cost.end(), 0.0);
+      for (long t = 4 * tb; t < std::min(ntiles, 4 * tb + 4); ++t) {
+        const int tk = (int)(t % ntz), tj = (int)((t / ntz) % nty), ti = (int)(t / ((long)ntz * nty));
+        const int i0 = ti * kTile, j0 = tj * kTile, k0 = tk * KT;
+        const int i1 = std::min(nx, i0 + kTile) - 1, j1 = std::min(ny, j0 + kTile) - 1,
+                  k1 = std::min(nz, k0 + KT) - 1;
+        const double r1sq = q1sq * ((double)nb.h_hx2[ti] + nb.h_hy2[tj] + nb.h_hz2_16[tk]);
+        const double ncell = (double)((i1 - i0 + 1) * (j1 - j0 + 1) * (k1 - k0 + 1));
+        const double xa = xc[i0], xb = xc[i1], ya = yc[j0], yb = yc[j1], za = zc[k0], zb = zc[k1];
+        for (int s = 0; s < S; ++s) {
+          const int* bx = &nb.h_box[(size_t)s * 6];
+          const bool hit = bx[0] <= i1 && bx[1] >= i0 && bx[2] <= j1 && bx[3] >= j0 &&
+                           bx[4] <= k1 && bx[5] >= k0;
+          const double dx = gap(xa, xb, ctx->hsx[s], ctx->hsx[s]),
+                       dy = gap(ya, yb, ctx->hsy[s], ctx->hsy[s]),
+                       dz = gap(za, zb, ctx->hsz[s], ctx->hsz[s]);
+          const bool need = dx * dx + dy * dy + dz * dz < r1sq;
+          cost[s] += (hit ? pc.hit : need ? pc.hexa : pc.mono) * ncell;
+        }
+      }
+      pre[0] = 0.0;
+      for (int s = 0; s < S; ++s) pre[s + 1] = pre[s] + cost[s];
+      split_equal(pre, (int)C, &ar[(size_t)tb * (C + 1)]);
+    }
+    if ((e = upload_vec(nb.adj_ranges, ar, ctx->stream)) != hipSuccess) return e;
+    nb.adj_chunks = (int)C;
+  }
+  // the host vectors above are read by the uploads
+  if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return e;
+  nb.plan_q1 = ctx->mixed_q1;
+  nb.plan_valid = true;
+  return hipSuccess;
+}
+
 // Unit-density cell table of the mesh (G V and, for general meshes, the shape
 // constants), built once per geometry; the adjoint's weights and the forward's shapes.
 template <int MESH>
@@ -1296,10 +1389,8 @@ hipError_t adjoint_mixed_impl(jgm_context* ctx, int kind, int nbm, const double*
     if ((e = ensure_shape_table<MESH>(ctx)) != hipSuccess) return e;
     // the adjoint walks 4x4x(4 kQ) tiles, kQ cells per lane
     const long ntiles8 = (long)g.ntx * g.nty * ((g.nz + kQ * kTile - 1) / (kQ * kTile));
-    int nchunks = (int)std::min<long>(
-        S, std::max<long>(1, (kFarAdjointWaves + ntiles8 - 1) / ntiles8));
-    const int per = (S + nchunks - 1) / nchunks;
-    nchunks = (S + per - 1) / per;
+    if ((e = ensure_far_plan(ctx)) != hipSuccess) return e;
+    const int nchunks = nb.adj_chunks;
     if ((e = ctx->hpart.ensure(sizeof(double) * (size_t)nbm * nchunks * N)) != hipSuccess)
       return e;
     if ((e = ctx->hsum.ensure(sizeof(double) * (size_t)N)) != hipSuccess) return e;
@@ -1309,7 +1400,8 @@ hipError_t adjoint_mixed_impl(jgm_context* ctx, int kind, int nbm, const double*
                                      ctx->d_sx.as<const double>(), ctx->d_sy.as<const double>(),
                                      ctx->d_sz.as<const double>(), ctx->coef32.as<const float>(),
                                      ctx->coef_mono.as<const float>(), nb.box.as<const int>(),
-                                     second_radius_q2(ctx), S, per, g,
+                                     second_radius_q2(ctx), S, nchunks,
+                                     nb.adj_ranges.as<const int>(), g,
                                      nb.adj_table.as<const float>(), uniform_const(ctx), ntiles8,
                                      ctx->hpart.as<double>());
     }
@@ -1450,9 +1542,14 @@ hipError_t build_near_boxes(jgm_context* ctx) {
   for (double v : ctx->xs) mx = std::max(mx, std::fabs(v));
   for (double v : ctx->ys) my = std::max(my, std::fabs(v));
   for (double v : ctx->zs) mz = std::max(mz, std::fabs(v));
-  // any cell outside the box is >= R away along some axis, and R >= q0 * its half-diagonal
-  const double R = ctx->mixed_q0 * 0.5 * std::sqrt(mx * mx + my * my + mz * mz);
-  nb.radius = R;
+  // reach of layer k: q0 half-diagonals of the largest cell shape of the layer.  A site's
+  // box takes the layers whose centre is within their reach of it, and along x and y the
+  // largest reach of those layers, so every cell outside the box is at least q0 of its own
+  // half-diagonals away (along z if its layer was not taken, else along x or y)
+  std::vector<double> reach(nz);
+  for (int k = 0; k < nz; ++k)
+    reach[k] = ctx->mixed_q0 * 0.5 * std::sqrt(mx * mx + my * my + ctx->zs[k] * ctx->zs[k]);
+  nb.radius = ctx->mixed_q0 * 0.5 * std::sqrt(mx * mx + my * my + mz * mz);
   bool uniform = nx > 0 && ny > 0 && nz > 0;
   for (double v : ctx->xs) uniform = uniform && v == ctx->xs[0];
   for (double v : ctx->ys) uniform = uniform && v == ctx->ys[0];
@@ -1471,9 +1568,17 @@ hipError_t build_near_boxes(jgm_context* ctx) {
   size_t total_cells = 0, total_corners = 0;
   for (size_t s = 0; s < S; ++s) {
     int lo[3], hi[3];
-    axis_range(xcen, ctx->hsx[s], R, &lo[0], &hi[0]);
-    axis_range(ycen, ctx->hsy[s], R, &lo[1], &hi[1]);
-    axis_range(zcen, ctx->hsz[s], R, &lo[2], &hi[2]);
+    double Rh = 0.0;
+    lo[2] = nz;
+    hi[2] = -1;
+    for (int k = 0; k < nz; ++k)
+      if (std::fabs(ctx->hsz[s] - zcen[k]) < reach[k]) {
+        lo[2] = std::min(lo[2], k);
+        hi[2] = k;
+        Rh = std::max(Rh, reach[k]);
+      }
+    axis_range(xcen, ctx->hsx[s], Rh, &lo[0], &hi[0]);
+    axis_range(ycen, ctx->hsy[s], Rh, &lo[1], &hi[1]);
     bool empty = false;
     for (int a = 0; a < 3; ++a) empty = empty || lo[a] > hi[a];
     corner_off[s] = (long)total_corners;
@@ -1553,6 +1658,15 @@ hipError_t build_near_boxes(jgm_context* ctx) {
   if ((e = upload_vec(nb.zoff, zoff, st)) != hipSuccess) return e;
   const std::vector<int> order = morton_order(ctx);
   if ((e = upload_vec(nb.site_order, order, st)) != hipSuccess) return e;
+  // host copies for the far-field work plan (made on the first mixed call)
+  nb.h_xcen = xcen;
+  nb.h_ycen = ycen;
+  nb.h_zcen = zcen;
+  nb.h_box = box;
+  nb.h_hx2 = run_h2(ctx->xs, kTile);
+  nb.h_hy2 = run_h2(ctx->ys, kTile);
+  nb.h_hz2_16 = run_h2(ctx->zs, kQ * kTile);
+  nb.plan_valid = false;
   // sites with two coordinates on corner planes of the mesh lie on a mesh line: there the
   // reference's guarded logarithms (log(max(u + r, eps)), BasicGravElements.cpp:53-59)
   // give every cell with an edge on that line a term that the far-field expansion does not

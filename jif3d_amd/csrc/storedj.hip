@@ -372,29 +372,28 @@ __device__ __forceinline__ void op_store(const OpStage& o, bf16x8* lds) {
   lds[threadIdx.x + 256] = o.v[1];
 }
 
-// BF16 J M from the packed BF16 copy of J (k_pack_bf16_cells: eight cells of a row per 16
-// bytes) and a BF16 copy of M (k_rows_to_bf16: eight cells of a right-hand side per 16
-// bytes).  Per 16-cell k-step a lane's A operand (row li or li + 32 of its 64, cells
-// 8 lk .. 8 lk + 7) and B operand (right-hand side li or li + 32, the same cells) are one
-// 16-byte load each; J is loaded in 64-cell blocks (four k-steps) two blocks ahead of the
-// multiplication, M one block ahead.  Same split layout, k order, rounding and flush
+// BF16 J M from the tiled BF16 copy of J (k_pack_bf16_rowtiles: per 256-row tile and 64-cell
+// block the 2048 16-byte groups in the order the four waves' MFMA operands take them) and a
+// BF16 copy of M (k_rows_to_bf16: eight cells of a right-hand side per 16 bytes).  Per
+// 16-cell k-step a lane's A operand (row li or li + 32 of its 64, cells 8 lk .. 8 lk + 7) and
+// B operand (right-hand side li or li + 32, the same cells) are one 16-byte load each, and
+// each load instruction of a wave reads one contiguous 1 KB run (a workgroup's whole share
+// of J is one contiguous range); J is loaded in 64-cell blocks (four k-steps) three blocks
+// ahead of the multiplication, M two.  Same split layout, k order, rounding and flush
 // interval as k_gemm_jm<BF16>, so the same results; per must be a multiple of 64.
-__global__ __launch_bounds__(kBlock) void k_gemm_jm_bf16p(const bf16x8* __restrict__ Jp, long ld,
+__global__ __launch_bounds__(kBlock) void k_gemm_jm_bf16p(const bf16x8* __restrict__ Jt, long ld,
                                                           long R, long colpad,
                                                           const bf16x8* __restrict__ Mb, int P,
                                                           long per, int row_tiles, int nsplit,
                                                           float* __restrict__ Yp) {
   __shared__ bf16x8 sM[2][512];
+  // workgroup w runs on XCD w % 8: each XCD takes a contiguous range of the (split, row
+  // tile) pairs in split-major order, so that the row tiles of a split, which read the same
+  // cells of M, share one L2
   const int w = blockIdx.x;
-  int split, rt;
-  if ((nsplit & 7) == 0) {
-    const int slot = w >> 3;
-    split = (w & 7) * (nsplit >> 3) + slot / row_tiles;
-    rt = slot % row_tiles;
-  } else {
-    split = w / row_tiles;
-    rt = w % row_tiles;
-  }
+  const int nw = nsplit * row_tiles, x = w & 7;
+  const int item = x * (nw / 8) + min(x, nw % 8) + (w >> 3);
+  const int split = item / row_tiles, rt = item % row_tiles;
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int li = lane & 31, lk = lane >> 5;
@@ -405,8 +404,8 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm_bf16p(const bf16x8* __restri
   const long c0 = (long)split * per;
   const long cend = c0 + per < colpad ? c0 + per : colpad;
   const long nblk = cend > c0 ? (cend - c0) / 64 : 0;
-  const bf16x8* A0 = Jp + row0 + li;
-  const bf16x8* A1 = A0 + 32;
+  // this wave's groups of its row tile's first block of the split (tiled_group_index)
+  const bf16x8* A0 = Jt + ((long)rt * (colpad / 64) + c0 / 64) * 2048 + wave * 512 + lane;
   const bf16x8* Mbase = Mb + b0;
   f32x16 acc[2][2];
   float tot[2][2][16];
@@ -425,15 +424,13 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm_bf16p(const bf16x8* __restri
   OpStage m0, m1;
   auto loadJ = [&](JQ& q, long blk) {
     if (!active) return;
-    const long g0 = (c0 + 64 * blk) / 8;  // first 8-cell group of the block
+    const bf16x8* p = A0 + blk * 2048;
 #pragma unroll
-    for (int st = 0; st < 4; ++st) {
-      const long g = g0 + 2 * st + lk;  // this lane's group: cells 16 st + 8 lk ..
-      q.a[st][0] = __builtin_bit_cast(bf16x8, __builtin_nontemporal_load(
-                                                  reinterpret_cast<const u32x4*>(A0 + g * ld)));
-      q.a[st][1] = __builtin_bit_cast(bf16x8, __builtin_nontemporal_load(
-                                                  reinterpret_cast<const u32x4*>(A1 + g * ld)));
-    }
+    for (int st = 0; st < 4; ++st)
+#pragma unroll
+      for (int ta = 0; ta < 2; ++ta)
+        q.a[st][ta] = __builtin_bit_cast(
+            bf16x8, __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p + st * 128 + ta * 64)));
   };
   auto clampb = [&](long b) { return b < nblk ? b : nblk - 1; };
   // block b (J set jc, M in LDS buffer buf): load J block b + 3 into jl and M block b + 2
@@ -489,18 +486,30 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm_bf16p(const bf16x8* __restri
     }
 }
 
-// BF16 copy of J for the BF16 J M: element (row, cell) at
-// Jp[((cell / 8) ld + row) 8 + cell % 8], rounded to nearest even as mma_block rounds it.
-__global__ __launch_bounds__(kBlock) void k_pack_bf16_cells(const float* __restrict__ J, long ld,
-                                                            long ngroups, bf16x8* __restrict__ Jp) {
-  const long n = ngroups * ld;
+// BF16 copy of J for the BF16 J M, tiled for the matrix cores: rows in tiles of 256 (a J M
+// workgroup), cells in blocks of 64 (a pipeline stage), and within a tile's block the 2048
+// groups of eight cells of one row in the order [wave][k-step][row half][lane / 32][lane % 32]:
+// row = 256 tile + 64 wave + 32 half + lane % 32, cells 64 block + 16 k-step + 8 (lane / 32)
+// .. + 7.  Rows past ld are zero.  Rounded to nearest even as mma_block rounds J.
+__global__ __launch_bounds__(kBlock) void k_pack_bf16_rowtiles(const float* __restrict__ J, long ld,
+                                                               long colpad,
+                                                               bf16x8* __restrict__ Jt) {
+  const long nblk = colpad / 64, n = (ld + 255) / 256 * 256 * colpad / 8;
   for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
        idx += (long)gridDim.x * blockDim.x) {
-    const long grp = idx / ld, row = idx - grp * ld;
+    long t = idx;
+    const int li = (int)(t & 31); t >>= 5;
+    const int lk = (int)(t & 1); t >>= 1;
+    const int ta = (int)(t & 1); t >>= 1;
+    const int st = (int)(t & 3); t >>= 2;
+    const int wave = (int)(t & 3); t >>= 2;
+    const long blk = t % nblk, rt = t / nblk;
+    const long row = rt * 256 + wave * 64 + ta * 32 + li;
+    const long c0 = blk * 64 + st * 16 + lk * 8;
     bf16x8 v;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) v[e] = (__bf16)J[(8 * grp + e) * ld + row];
-    Jp[idx] = v;
+    for (int e = 0; e < 8; ++e) v[e] = (__bf16)(row < ld ? J[(c0 + e) * ld + row] : 0.0f);
+    Jt[idx] = v;
   }
 }
 
@@ -1219,16 +1228,17 @@ namespace {
 bool ensure_bf16_copy(jgm_context* ctx) {
   if (ctx->storedJ_bf16_gen == ctx->storedJ_gen) return true;
   if (!ctx->bf16_copy_enabled || ctx->storedJ_bf16_failed_gen == ctx->storedJ_gen) return false;
-  const long ld = (long)ctx->storedJ_ld, ngroups = (long)ctx->storedJ_colpad / 8;
-  if (ctx->storedJ_bf16.ensure(sizeof(__bf16) * (size_t)ld * ngroups * 8) != hipSuccess) {
+  const long ld = (long)ctx->storedJ_ld, colpad = (long)ctx->storedJ_colpad;
+  // rows padded to whole 256-row tiles
+  if (ctx->storedJ_bf16.ensure(sizeof(__bf16) * (size_t)round_up(ld, 256) * colpad) !=
+      hipSuccess) {
     (void)hipGetLastError();
     ctx->storedJ_bf16.release();
     ctx->storedJ_bf16_failed_gen = ctx->storedJ_gen;  // not retried for this J
     return false;
   }
-  hipLaunchKernelGGL(k_pack_bf16_cells, dim3(8 * ctx->cu_count), dim3(kBlock), 0, ctx->stream,
-                     ctx->storedJ.as<const float>(), ld, ngroups,
-                     ctx->storedJ_bf16.as<bf16x8>());
+  hipLaunchKernelGGL(k_pack_bf16_rowtiles, dim3(8 * ctx->cu_count), dim3(kBlock), 0, ctx->stream,
+                     ctx->storedJ.as<const float>(), ld, colpad, ctx->storedJ_bf16.as<bf16x8>());
   ctx->storedJ_bf16_gen = ctx->storedJ_gen;
   return true;
 }

@@ -293,8 +293,9 @@ def test_bf16_copy_follows_a_rebuilt_J():
 
 @pytest.mark.parametrize("shape", [(12, 10, 8, 4, 4), (20, 17, 13, 11, 9)])
 def test_bf16_packed_copy_equals_rounding_in_registers(monkeypatch, shape):
-    """The BF16 copies of J hold the BF16 roundings of the FP32 J (packed eight cells of a
-    row per 16 bytes for J M, in J's own column-major layout for J^T R) and the products
+    """The BF16 copies of J hold the BF16 roundings of the FP32 J (in 256-row tiles of
+    64-cell blocks for J M, rows past the last one zero; eight rows of a cell per 16 bytes in
+    32-cell tiles for J^T R) and the products
     that read them use the same k order and flush interval, so they equal, bit for bit, the
     products that round the FP32 J in registers (the copies switched off with
     JIF3D_AMD_BF16_COPY=0 when the context is created)."""
