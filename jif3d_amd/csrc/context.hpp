@@ -76,10 +76,6 @@ struct NearBoxes {
   int adj_chunks = 0;
   double plan_q1 = -1.0;
   bool plan_valid = false;
-  // host copies the plan reads: cell centres, boxes, run half-sizes
-  std::vector<double> h_xcen, h_ycen, h_zcen;
-  std::vector<int> h_box;
-  std::vector<float> h_hx2, h_hy2, h_hz2_16;
   // sites on mesh lines (two coordinates equal to corner coordinates of the mesh): the
   // mixed path evaluates them with the FP64 closed form; nline of them, their survey
   // indices (int32) and coordinates (FP64 x[nline], y[nline], z[nline])

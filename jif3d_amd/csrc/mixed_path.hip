@@ -1129,91 +1129,119 @@ const char* far_name(int mask, bool adjoint) {
                              : "far_forward_fp32_ftg";
 }
 
-// Relative cost of one site-cell pair of the far field by path: the monopole beyond the
-// second radius (regrouped for cubes), the multipole through the hexadecapole, and the same
-// with the near-box mask (from the kernels' VALU instruction counts per pair).
-struct PairCost {
-  float mono, hexa, hit;
-};
-inline PairCost pair_cost(int mesh_class) {
-  return mesh_class == kMeshCubes ? PairCost{1.0f, 6.0f, 6.5f} : PairCost{1.0f, 2.2f, 2.5f};
-}
-
-// Boundaries of nch pieces of equal cost of a sequence with prefix sums pre[0..n]
-// (pre[0] = 0): out[j] = the first i with pre[i] >= j pre[n] / nch, out[nch] = n.
-void split_equal(const std::vector<double>& pre, int nch, int* out) {
-  const size_t n = pre.size() - 1;
-  out[0] = 0;
-  size_t i = 0;
-  for (int j = 1; j < nch; ++j) {
-    const double target = pre[n] * j / nch;
-    while (i < n && pre[i] < target) ++i;
-    out[j] = (int)i;
+// The far adjoint's work partition: the sites of each workgroup of four adjoint tiles are cut
+// into nchunks ranges of equal estimated cost, each site's cost being, per tile, its path as
+// k_far_adjoint takes it (masked near-box tile, hexadecapole within the second radius,
+// monopole beyond) times the tile's cells, in integer units (cubes 2 / 12 / 13, other
+// shapes 10 / 22 / 25: the kernels' instruction counts per pair).  Block tb writes
+// ranges[tb (nchunks + 1) + j], the first site at which the running cost reaches j / nchunks
+// of the block's total, from an exact integer prefix sum (so the same for every launch).
+template <int MESH>
+__device__ __forceinline__ long long adjoint_site_cost(const double* __restrict__ sx,
+                                                       const double* __restrict__ sy,
+                                                       const double* __restrict__ sz,
+                                                       const int* __restrict__ box, float q1sq,
+                                                       const FarGeom& g, long ntiles, long tb,
+                                                       int s) {
+  constexpr int KT = kQ * kTile;
+  const int ntz = (g.nz + KT - 1) / KT;
+  const long long cm = MESH == kMeshCubes ? 2 : 10, ch = MESH == kMeshCubes ? 12 : 22,
+                  ck = MESH == kMeshCubes ? 13 : 25;
+  long long c = 0;
+  for (long t = 4 * tb; t < 4 * tb + 4 && t < ntiles; ++t) {
+    // the tile's geometry and the site's tests as in k_far_adjoint
+    const int tk = (int)(t % ntz);
+    const long tt = t / ntz;
+    const int tj = (int)(tt % g.nty), ti = (int)(tt / g.nty);
+    const int i0 = ti * kTile, j0 = tj * kTile, k0 = tk * KT;
+    const int ni = min(kTile, g.nx - i0), nj = min(kTile, g.ny - j0), nk = min(KT, g.nz - k0);
+    const double x0 = g.xcen[i0], y0 = g.ycen[j0], z0 = g.zcen[k0];
+    const float ex = (float)(g.xcen[i0 + ni - 1] - x0), ey = (float)(g.ycen[j0 + nj - 1] - y0),
+                ez = (float)(g.zcen[k0 + nk - 1] - z0);
+    const float r1sq = q1sq * (g.hx2[ti] + g.hy2[tj] + g.hz2_16[tk]);
+    const float Xl = (float)(sx[s] - x0), Yl = (float)(sy[s] - y0), Zl = (float)(sz[s] - z0);
+    const float dx = fmaxf(0.0f, fmaxf(-Xl, Xl - ex)), dy = fmaxf(0.0f, fmaxf(-Yl, Yl - ey)),
+                dz = fmaxf(0.0f, fmaxf(-Zl, Zl - ez));
+    const bool need = fmaf(dx, dx, fmaf(dy, dy, dz * dz)) < r1sq;
+    const int* bl = box + s * 6;
+    const bool hit = !(bl[1] < i0 || bl[0] > i0 + ni - 1 || bl[3] < j0 || bl[2] > j0 + nj - 1 ||
+                       bl[5] < k0 || bl[4] > k0 + nk - 1);
+    c += (hit ? ck : need ? ch : cm) * (long long)(ni * nj * nk);
   }
-  out[nch] = (int)n;
+  return c;
 }
 
-// Distance from [lo, hi] to [a, b] along one axis (0 when they overlap).
-inline double gap(double lo, double hi, double a, double b) {
-  return std::max(0.0, std::max(lo - b, a - hi));
+template <int MESH>
+__global__ __launch_bounds__(kBlock) void k_far_adjoint_plan(
+    const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
+    int S, const int* __restrict__ box, float q1sq, FarGeom g, long ntiles, int nchunks,
+    int* __restrict__ ranges) {
+  __shared__ long long scan[kBlock];
+  const long tb = blockIdx.x;
+  const int tid = threadIdx.x;
+  int* out = ranges + tb * (nchunks + 1);
+  // total cost of the block's sites
+  long long part = 0;
+  for (int s = tid; s < S; s += kBlock) part += adjoint_site_cost<MESH>(sx, sy, sz, box, q1sq, g, ntiles, tb, s);
+  scan[tid] = part;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (tid < w) scan[tid] += scan[tid + w];
+    __syncthreads();
+  }
+  const long long total = scan[0];
+  __syncthreads();
+  if (tid == 0) {
+    out[0] = 0;
+    out[nchunks] = S;
+  }
+  // running prefix in runs of kBlock sites (inclusive scan in LDS); split j goes to the site
+  // whose cost interval (run + excl, run + incl] holds j total / nchunks
+  long long run = 0;
+  for (int s0 = 0; s0 < S; s0 += kBlock) {
+    const int s = s0 + tid;
+    const long long c = s < S ? adjoint_site_cost<MESH>(sx, sy, sz, box, q1sq, g, ntiles, tb, s) : 0;
+    scan[tid] = c;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+      const long long v = tid >= off ? scan[tid - off] : 0;
+      __syncthreads();
+      scan[tid] += v;
+      __syncthreads();
+    }
+    const long long incl = run + scan[tid], excl = incl - c;
+    if (c > 0 && total > 0) {
+      const long long jlo = excl * nchunks / total + 1, jhi = incl * nchunks / total;
+      for (long long j = jlo; j <= jhi && j < nchunks; ++j)
+        if (j >= 1) out[j] = s + 1;
+    }
+    run += scan[kBlock - 1];
+    __syncthreads();
+  }
 }
 
-// The far adjoint's work partition (geometry, the second radius and the wave target only,
-// so the same for every kind and number of misfit vectors: batches equal single calls bit
-// for bit): the sites of each workgroup of four adjoint tiles are cut into adj_chunks
-// ranges of equal estimated cost (each site's path per tile as the kernel takes it).  At
-// 128x128x64 cells and 4096 sites, 12 such chunks (49152 waves) ran the far adjoint in
+// Makes the far adjoint's partition if the geometry, the second radius or the wave target
+// changed (stream-ordered before the adjoint that reads it; no host work beyond the launch).
+// At 128x128x64 cells and 4096 sites, 12 planned chunks (49152 waves) ran the far adjoint in
 // 1.83 ms against 1.84 ms for 24 equal ones, with half the FP64 partial sums (0.10 GB).
+template <int MESH>
 hipError_t ensure_far_plan(jgm_context* ctx) {
   NearBoxes& nb = ctx->near;
   if (nb.plan_valid && nb.plan_q1 == ctx->mixed_q1) return hipSuccess;
-  const int nx = (int)ctx->nx, ny = (int)ctx->ny, nz = (int)ctx->nz;
   const int S = (int)ctx->S;
-  const PairCost pc = pair_cost(nb.mesh_class);
-  const double q1sq = (double)second_radius_q2(ctx);
-  const std::vector<double>&xc = nb.h_xcen, &yc = nb.h_ycen, &zc = nb.h_zcen;
-  const int ntx = (nx + kTile - 1) / kTile, nty = (ny + kTile - 1) / kTile;
-  hipError_t e;
-  std::vector<int> ar;  // uploaded asynchronously: alive until the synchronisation below
-  // adjoint: workgroups of four tiles of 4 x 4 x (4 kQ) cells, tk fastest
-  {
-    constexpr int KT = kQ * kTile;
-    const int ntz = (nz + KT - 1) / KT;
-    const long ntiles = (long)ntx * nty * ntz, nblocks = (ntiles + 3) / 4;
-    long C = (ctx->far_adj_waves + 4 * nblocks - 1) / std::max(1L, 4 * nblocks);
-    C = std::max<long>(1, std::min<long>(C, std::max(1, S)));
-    ar.assign((size_t)std::max(1L, nblocks) * (C + 1), 0);
-    std::vector<double> cost(S), pre(S + 1);
-    for (long tb = 0; tb < nblocks; ++tb) {
-      std::fill(cost.begin(), cost.end(), 0.0);
-      for (long t = 4 * tb; t < std::min(ntiles, 4 * tb + 4); ++t) {
-        const int tk = (int)(t % ntz), tj = (int)((t / ntz) % nty), ti = (int)(t / ((long)ntz * nty));
-        const int i0 = ti * kTile, j0 = tj * kTile, k0 = tk * KT;
-        const int i1 = std::min(nx, i0 + kTile) - 1, j1 = std::min(ny, j0 + kTile) - 1,
-                  k1 = std::min(nz, k0 + KT) - 1;
-        const double r1sq = q1sq * ((double)nb.h_hx2[ti] + nb.h_hy2[tj] + nb.h_hz2_16[tk]);
-        const double ncell = (double)((i1 - i0 + 1) * (j1 - j0 + 1) * (k1 - k0 + 1));
-        const double xa = xc[i0], xb = xc[i1], ya = yc[j0], yb = yc[j1], za = zc[k0], zb = zc[k1];
-        for (int s = 0; s < S; ++s) {
-          const int* bx = &nb.h_box[(size_t)s * 6];
-          const bool hit = bx[0] <= i1 && bx[1] >= i0 && bx[2] <= j1 && bx[3] >= j0 &&
-                           bx[4] <= k1 && bx[5] >= k0;
-          const double dx = gap(xa, xb, ctx->hsx[s], ctx->hsx[s]),
-                       dy = gap(ya, yb, ctx->hsy[s], ctx->hsy[s]),
-                       dz = gap(za, zb, ctx->hsz[s], ctx->hsz[s]);
-          const bool need = dx * dx + dy * dy + dz * dz < r1sq;
-          cost[s] += (hit ? pc.hit : need ? pc.hexa : pc.mono) * ncell;
-        }
-      }
-      pre[0] = 0.0;
-      for (int s = 0; s < S; ++s) pre[s + 1] = pre[s] + cost[s];
-      split_equal(pre, (int)C, &ar[(size_t)tb * (C + 1)]);
-    }
-    if ((e = upload_vec(nb.adj_ranges, ar, ctx->stream)) != hipSuccess) return e;
-    nb.adj_chunks = (int)C;
-  }
-  // the host vectors above are read by the uploads
-  if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return e;
+  const FarGeom g = far_geom(ctx);
+  const long ntiles = (long)g.ntx * g.nty * ((g.nz + kQ * kTile - 1) / (kQ * kTile));
+  const long nblocks = (ntiles + 3) / 4;
+  long C = (ctx->far_adj_waves + 4 * nblocks - 1) / std::max(1L, 4 * nblocks);
+  C = std::max<long>(1, std::min<long>(C, std::max(1, S)));
+  hipError_t e = nb.adj_ranges.ensure(sizeof(int) * (size_t)std::max(1L, nblocks) * (C + 1));
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_far_adjoint_plan<MESH>), dim3((unsigned)nblocks), dim3(kBlock), 0,
+                     ctx->stream, ctx->d_sx.as<const double>(), ctx->d_sy.as<const double>(),
+                     ctx->d_sz.as<const double>(), S, nb.box.as<const int>(),
+                     second_radius_q2(ctx), g, ntiles, (int)C, nb.adj_ranges.as<int>());
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  nb.adj_chunks = (int)C;
   nb.plan_q1 = ctx->mixed_q1;
   nb.plan_valid = true;
   return hipSuccess;
@@ -1389,7 +1417,7 @@ hipError_t adjoint_mixed_impl(jgm_context* ctx, int kind, int nbm, const double*
     if ((e = ensure_shape_table<MESH>(ctx)) != hipSuccess) return e;
     // the adjoint walks 4x4x(4 kQ) tiles, kQ cells per lane
     const long ntiles8 = (long)g.ntx * g.nty * ((g.nz + kQ * kTile - 1) / (kQ * kTile));
-    if ((e = ensure_far_plan(ctx)) != hipSuccess) return e;
+    if ((e = ensure_far_plan<MESH>(ctx)) != hipSuccess) return e;
     const int nchunks = nb.adj_chunks;
     if ((e = ctx->hpart.ensure(sizeof(double) * (size_t)nbm * nchunks * N)) != hipSuccess)
       return e;
@@ -1658,15 +1686,7 @@ hipError_t build_near_boxes(jgm_context* ctx) {
   if ((e = upload_vec(nb.zoff, zoff, st)) != hipSuccess) return e;
   const std::vector<int> order = morton_order(ctx);
   if ((e = upload_vec(nb.site_order, order, st)) != hipSuccess) return e;
-  // host copies for the far-field work plan (made on the first mixed call)
-  nb.h_xcen = xcen;
-  nb.h_ycen = ycen;
-  nb.h_zcen = zcen;
-  nb.h_box = box;
-  nb.h_hx2 = run_h2(ctx->xs, kTile);
-  nb.h_hy2 = run_h2(ctx->ys, kTile);
-  nb.h_hz2_16 = run_h2(ctx->zs, kQ * kTile);
-  nb.plan_valid = false;
+  nb.plan_valid = false;  // the far adjoint's partition is made again on its next call
   // sites with two coordinates on corner planes of the mesh lie on a mesh line: there the
   // reference's guarded logarithms (log(max(u + r, eps)), BasicGravElements.cpp:53-59)
   // give every cell with an edge on that line a term that the far-field expansion does not
