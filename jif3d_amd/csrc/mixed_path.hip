@@ -58,10 +58,11 @@ constexpr int kTile = 4;  // far-field tile edge (cells)
 // Waves per far forward launch: the tiles are cut into equal chunks to reach about this
 // many.  The tiles' costs differ (monopole, hexadecapole, masked near-box tiles), and many
 // small chunks balance the CUs: at 128x128x64 cells and 4096 sites, measured against 16384,
-// the forward is fastest at 65536 (2.40 -> 2.04 ms); chunks of equal estimated cost (the
-// adjoint's partition below) were slower here at every count tried (6144-65536 waves:
-// 1.84-2.39 ms against 1.71).  The partial sums this leaves (~0.3 GB per model) are
-// reduced in fixed order.
+// the forward is fastest at 65536 (2.40 -> 2.04 ms); chunks of equal estimated cost per
+// block of 256 sites (as the adjoint's partition below, from the paths each wave takes per
+// tile) were slower at every count tried (8192-65536 waves: 1.73-2.24 ms against 1.69 ms for
+// these equal chunks, which hold whole columns of tiles).  The partial sums this leaves
+// (~0.3 GB per model) are reduced in fixed order.
 constexpr int kFarForwardWaves = 65536;
 constexpr int kTableWidth = 12;  // floats per cell of a non-uniform mesh
 
@@ -1171,18 +1172,14 @@ __device__ __forceinline__ long long adjoint_site_cost(const double* __restrict_
   return c;
 }
 
-template <int MESH>
-__global__ __launch_bounds__(kBlock) void k_far_adjoint_plan(
-    const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
-    int S, const int* __restrict__ box, float q1sq, FarGeom g, long ntiles, int nchunks,
-    int* __restrict__ ranges) {
-  __shared__ long long scan[kBlock];
-  const long tb = blockIdx.x;
+// Split points of a block's sequence of n costs cost(i) >= 0 (one workgroup of kBlock threads):
+// out[j] = the first index at which the running cost reaches j / nchunks of the total, from an
+// exact integer prefix sum; out[0] = 0, out[nchunks] = n.
+template <class F>
+__device__ void split_by_cost(long n, int nchunks, F cost, long long* scan, int* out) {
   const int tid = threadIdx.x;
-  int* out = ranges + tb * (nchunks + 1);
-  // total cost of the block's sites
   long long part = 0;
-  for (int s = tid; s < S; s += kBlock) part += adjoint_site_cost<MESH>(sx, sy, sz, box, q1sq, g, ntiles, tb, s);
+  for (long i = tid; i < n; i += kBlock) part += cost(i);
   scan[tid] = part;
   __syncthreads();
   for (int w = kBlock / 2; w > 0; w >>= 1) {
@@ -1193,14 +1190,14 @@ __global__ __launch_bounds__(kBlock) void k_far_adjoint_plan(
   __syncthreads();
   if (tid == 0) {
     out[0] = 0;
-    out[nchunks] = S;
+    out[nchunks] = (int)n;
   }
-  // running prefix in runs of kBlock sites (inclusive scan in LDS); split j goes to the site
-  // whose cost interval (run + excl, run + incl] holds j total / nchunks
+  // running prefix in runs of kBlock (inclusive scan in LDS); split j goes to the index whose
+  // cost interval (run + excl, run + incl] holds j total / nchunks
   long long run = 0;
-  for (int s0 = 0; s0 < S; s0 += kBlock) {
-    const int s = s0 + tid;
-    const long long c = s < S ? adjoint_site_cost<MESH>(sx, sy, sz, box, q1sq, g, ntiles, tb, s) : 0;
+  for (long i0 = 0; i0 < n; i0 += kBlock) {
+    const long i = i0 + tid;
+    const long long c = i < n ? cost(i) : 0;
     scan[tid] = c;
     __syncthreads();
     for (int off = 1; off < kBlock; off <<= 1) {
@@ -1213,11 +1210,26 @@ __global__ __launch_bounds__(kBlock) void k_far_adjoint_plan(
     if (c > 0 && total > 0) {
       const long long jlo = excl * nchunks / total + 1, jhi = incl * nchunks / total;
       for (long long j = jlo; j <= jhi && j < nchunks; ++j)
-        if (j >= 1) out[j] = s + 1;
+        if (j >= 1) out[j] = (int)(i + 1);
     }
     run += scan[kBlock - 1];
     __syncthreads();
   }
+}
+
+template <int MESH>
+__global__ __launch_bounds__(kBlock) void k_far_adjoint_plan(
+    const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
+    int S, const int* __restrict__ box, float q1sq, FarGeom g, long ntiles, int nchunks,
+    int* __restrict__ ranges) {
+  __shared__ long long scan[kBlock];
+  const long tb = blockIdx.x;
+  split_by_cost(
+      S, nchunks,
+      [&](long s) {
+        return adjoint_site_cost<MESH>(sx, sy, sz, box, q1sq, g, ntiles, tb, (int)s);
+      },
+      scan, ranges + tb * (nchunks + 1));
 }
 
 // Makes the far adjoint's partition if the geometry, the second radius or the wave target
