@@ -197,6 +197,8 @@ jgm_status jgm_create(int device, void* stream, jgm_context** out) {
   ctx->device = device;
   ctx->stream = static_cast<hipStream_t>(stream);
   if (const char* v = std::getenv("JIF3D_AMD_BF16_COPY")) ctx->bf16_copy_enabled = v[0] != '0';
+  if (const char* v = std::getenv("JIF3D_AMD_FAR_FORWARD_WAVES"))
+    ctx->far_fwd_waves = std::max(1L, std::atol(v));
   if (const char* v = std::getenv("JIF3D_AMD_FAR_ADJOINT_WAVES"))
     ctx->far_adj_waves = std::max(1L, std::atol(v));
   int cus = 0;

@@ -154,8 +154,10 @@ struct jgm_context {
   // terms; 0 keeps the hexadecapole everywhere
   double mixed_q0 = 8.0;
   double mixed_q1 = 28.0;
-  // waves per far adjoint launch its work is partitioned for (environment variable
-  // JIF3D_AMD_FAR_ADJOINT_WAVES at creation)
+  // waves per far-field launch the work is cut for: forward (equal chunks of tiles) and
+  // adjoint (planned chunks of sites); environment variables JIF3D_AMD_FAR_FORWARD_WAVES and
+  // JIF3D_AMD_FAR_ADJOINT_WAVES at creation
+  long far_fwd_waves = 65536;
   long far_adj_waves = 49152;
 
   // a-posteriori check of JGM_MIXED: FP64 at check_sites sample sites after a forward and

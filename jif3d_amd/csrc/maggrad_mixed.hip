@@ -193,6 +193,7 @@ hipError_t ensure_slab_mesh(jgm_context* ctx) {
     c->near.valid = false;
   }
   c->mixed_q1 = ctx->mixed_q1;
+  c->far_fwd_waves = ctx->far_fwd_waves;
   c->far_adj_waves = ctx->far_adj_waves;
   return hipSuccess;
 }

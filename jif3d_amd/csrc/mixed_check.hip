@@ -285,9 +285,10 @@ hipError_t check_forward_impl(jgm_context* ctx, int kind, const double* model,
   const int ncy = (int)ctx->ny + 1, ncz = (int)ctx->nz + 1;
   const long nq = (long)ncorners(ctx);
   // blocks per sample site: about 2 CUs' worth over all sites, each thread taking at least
-  // 32 corners so that the block sums stay a small part of the work
+  // 8 corners so that the block sums stay a small part of the work (on small meshes a floor
+  // of 32 left most CUs idle: config 1's 139K corners gave 17 blocks per site)
   const long want = std::max<long>(1, 2L * ctx->cu_count / std::max(1, F));
-  const int nblocks = (int)std::max<long>(1, std::min<long>(want, (nq + 32L * kBlock - 1) / (32L * kBlock)));
+  const int nblocks = (int)std::max<long>(1, std::min<long>(want, (nq + 8L * kBlock - 1) / (8L * kBlock)));
   hipError_t e;
   // partials (nblocks F NC), survey maxima (kraw), per-site errors (F), the error (1)
   if ((e = ctx->check_w.ensure(sizeof(double) * (size_t)nq)) != hipSuccess ||

@@ -32,7 +32,7 @@
 //
 // Layouts in HBM
 //   cell table   FP32: uniform mesh [c] = {w}; otherwise [c][12] =
-//                {w, Bx, By, Bz, a1, a2, a3, b12, b13, b23, 0, 0}
+//                {w, Bx, By, Bz, 4 a1, 4 a2, 4 a3, 2 b12, 2 b13, 2 b23, 0, 0}
 //                w = G rho V (forward) or G V (adjoint)
 //   far partials forward [chunk][NC][S] FP64; adjoint [chunk][N] FP64
 //   near boxes   int32 [S][6]; per-column CSR of sites; per-site offsets of the
@@ -55,15 +55,14 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kTile = 4;  // far-field tile edge (cells)
-// Waves per far forward launch: the tiles are cut into equal chunks to reach about this
-// many.  The tiles' costs differ (monopole, hexadecapole, masked near-box tiles), and many
+// Waves per far forward launch (jgm_context::far_fwd_waves, 65536 by default): the tiles are
+// cut into equal chunks to reach about this many.  The tiles' costs differ (monopole, hexadecapole, masked near-box tiles), and many
 // small chunks balance the CUs: at 128x128x64 cells and 4096 sites, measured against 16384,
 // the forward is fastest at 65536 (2.40 -> 2.04 ms); chunks of equal estimated cost per
 // block of 256 sites (as the adjoint's partition below, from the paths each wave takes per
 // tile) were slower at every count tried (8192-65536 waves: 1.73-2.24 ms against 1.69 ms for
 // these equal chunks, which hold whole columns of tiles).  The partial sums this leaves
 // (~0.3 GB per model) are reduced in fixed order.
-constexpr int kFarForwardWaves = 65536;
 constexpr int kTableWidth = 12;  // floats per cell of a non-uniform mesh
 
 // Mean relative error of the hardware reciprocal square root (v_rsq_f32) on gfx950,
@@ -87,9 +86,10 @@ __global__ void k_fill_zero(double* __restrict__ p, size_t n) {
   if (i < n) p[i] = 0.0;
 }
 
-// Multipole constants of one prism shape (see the file comment).
+// Multipole constants of one prism shape (see the file comment), stored as the products the
+// field formulas use: B_m, 4 a_m and 2 b_mn.
 struct MpConst {
-  float Bx, By, Bz, a1, a2, a3, b12, b13, b23;
+  float Bx, By, Bz, a4x, a4y, a4z, b2xy, b2xz, b2yz;
 };
 
 __host__ __device__ inline MpConst mp_const(double dx, double dy, double dz) {
@@ -106,18 +106,21 @@ __host__ __device__ inline MpConst mp_const(double dx, double dy, double dz) {
   m.Bx = (float)(0.5 * (hx2 - hb));
   m.By = (float)(0.5 * (hy2 - hb));
   m.Bz = (float)(0.5 * (hz2 - hb));
-  m.a1 = (float)(c * (A - p / 14 + w / 280));
-  m.a2 = (float)(c * (B - q / 14 + w / 280));
-  m.a3 = (float)(c * (C - t / 14 + w / 280));
-  m.b12 = (float)(c * (D - (p + q) / 14 + w / 140));
-  m.b13 = (float)(c * (E - (p + t) / 14 + w / 140));
-  m.b23 = (float)(c * (F - (q + t) / 14 + w / 140));
+  m.a4x = (float)(4 * c * (A - p / 14 + w / 280));
+  m.a4y = (float)(4 * c * (B - q / 14 + w / 280));
+  m.a4z = (float)(4 * c * (C - t / 14 + w / 280));
+  m.b2xy = (float)(2 * c * (D - (p + q) / 14 + w / 140));
+  m.b2xz = (float)(2 * c * (E - (p + t) / 14 + w / 140));
+  m.b2yz = (float)(2 * c * (F - (q + t) / 14 + w / 140));
   return m;
 }
 
 // Field of a prism of unit G*rho*V at offset (X, Y, Z) = site - centre, in the
 // kernel component order of MASK.  QUAD = false drops the quadrupole (B = 0 for
 // cubes); HEXA = false drops the hexadecapole (used beyond the second radius).
+// With v the squared direction cosines, h_x = dH/dv_x = 4 a1 vx + 2 b12 vy + 2 b13 vz (and
+// likewise h_y, h_z), and H is homogeneous of degree 2 in v, so 4 H = vx h_x + vy h_y + vz h_z:
+// the hexadecapole value comes from its gradient in three operations.
 template <int MASK, bool QUAD, bool HEXA = true>
 __device__ __forceinline__ void multipole_f32(float X, float Y, float Z, const MpConst& c,
                                               float* T) {
@@ -133,42 +136,43 @@ __device__ __forceinline__ void multipole_f32(float X, float Y, float Z, const M
   const float vx = ux * ux, vy = uy * uy, vz = uz * uz;
   float P = 3.0f * s3;  // coefficient of u_i u_j
   float base = -s3;     // isotropic diagonal part
-  float hx = 0.0f, hy = 0.0f, hz = 0.0f;
+  float hx = 0.0f, hy = 0.0f, hz = 0.0f, tx = 0.0f, ty = 0.0f, tz = 0.0f;
   if constexpr (HEXA) {
-    const float hh = fmaf(c.a1 * vx, vx,
-                          fmaf(c.a2 * vy, vy,
-                               fmaf(c.a3 * vz, vz,
-                                    fmaf(c.b12 * vx, vy, fmaf(c.b13 * vx, vz, c.b23 * vy * vz)))));
-    P = fmaf(99.0f * hh, s7, P);
-    base = fmaf(-9.0f * hh, s7, base);
-    const float t2b = 2.0f * c.b12, t2c = 2.0f * c.b13, t2d = 2.0f * c.b23;
-    hx = fmaf(4.0f * c.a1, vx, fmaf(t2b, vy, t2c * vz));
-    hy = fmaf(4.0f * c.a2, vy, fmaf(t2b, vx, t2d * vz));
-    hz = fmaf(4.0f * c.a3, vz, fmaf(t2c, vx, t2d * vy));
+    tx = c.a4x * vx;
+    ty = c.a4y * vy;
+    tz = c.a4z * vz;
+    hx = fmaf(c.b2xy, vy, fmaf(c.b2xz, vz, tx));
+    hy = fmaf(c.b2xy, vx, fmaf(c.b2yz, vz, ty));
+    hz = fmaf(c.b2xz, vx, fmaf(c.b2yz, vy, tz));
+    const float hh4 = fmaf(vx, hx, fmaf(vy, hy, vz * hz));  // 4 H
+    P = fmaf(24.75f * hh4, s7, P);                            // + 99 H s^7
+    base = fmaf(-2.25f * hh4, s7, base);                      // - 9 H s^7
   }
   if constexpr (QUAD) {
     const float q = fmaf(c.Bx, vx, fmaf(c.By, vy, c.Bz * vz));
     P = fmaf(35.0f * q, s5, P);
     base = fmaf(-5.0f * q, s5, base);
   }
+  const float t2 = 2.0f * s5;
   int k = 0;
   if constexpr ((MASK & kGz) != 0) {
     float g = base;
     if constexpr (HEXA) g = fmaf(hz, s7, g);
-    if constexpr (QUAD) g = fmaf(2.0f * c.Bz, s5, g);
+    if constexpr (QUAD) g = fmaf(c.Bz, t2, g);
     T[k++] = Z * g;
   }
   if constexpr ((MASK & kDiag) != 0) {
     float px = P, py = P, pz = P, ox = base, oy = base, oz = base;
     if constexpr (HEXA) {
+      // h_xx = h_x + 8 a1 vx = h_x + 2 (4 a1 vx), likewise for y and z
       const float m18 = -18.0f * s7;
-      const float hxx = fmaf(8.0f * c.a1, vx, hx), hyy = fmaf(8.0f * c.a2, vy, hy),
-                  hzz = fmaf(8.0f * c.a3, vz, hz);
       px = fmaf(hx, m18, px); py = fmaf(hy, m18, py); pz = fmaf(hz, m18, pz);
-      ox = fmaf(hxx, s7, ox); oy = fmaf(hyy, s7, oy); oz = fmaf(hzz, s7, oz);
+      ox = fmaf(fmaf(2.0f, tx, hx), s7, ox);
+      oy = fmaf(fmaf(2.0f, ty, hy), s7, oy);
+      oz = fmaf(fmaf(2.0f, tz, hz), s7, oz);
     }
     if constexpr (QUAD) {
-      const float m20 = -20.0f * s5, t2 = 2.0f * s5;
+      const float m20 = -20.0f * s5;
       px = fmaf(c.Bx, m20, px); py = fmaf(c.By, m20, py); pz = fmaf(c.Bz, m20, pz);
       ox = fmaf(c.Bx, t2, ox); oy = fmaf(c.By, t2, oy); oz = fmaf(c.Bz, t2, oz);
     }
@@ -179,10 +183,10 @@ __device__ __forceinline__ void multipole_f32(float X, float Y, float Z, const M
   if constexpr ((MASK & kOff) != 0) {
     float cxy = P, cxz = P, cyz = P;
     if constexpr (HEXA) {
-      const float m9 = -9.0f * s7, f4 = 4.0f * s7;
-      cxy = fmaf(hx + hy, m9, fmaf(c.b12, f4, P));
-      cxz = fmaf(hx + hz, m9, fmaf(c.b13, f4, P));
-      cyz = fmaf(hy + hz, m9, fmaf(c.b23, f4, P));
+      const float m9 = -9.0f * s7, f2 = 2.0f * s7;  // 4 b s^7 = (2 b) (2 s^7)
+      cxy = fmaf(hx + hy, m9, fmaf(c.b2xy, f2, P));
+      cxz = fmaf(hx + hz, m9, fmaf(c.b2xz, f2, P));
+      cyz = fmaf(hy + hz, m9, fmaf(c.b2yz, f2, P));
     }
     if constexpr (QUAD) {
       const float m10 = -10.0f * s5;
@@ -245,8 +249,8 @@ __device__ __forceinline__ MpConst cell_const(const float* __restrict__ table, l
     const float* t = table + c * kTableWidth + 1;
     MpConst m;
     m.Bx = t[0]; m.By = t[1]; m.Bz = t[2];
-    m.a1 = t[3]; m.a2 = t[4]; m.a3 = t[5];
-    m.b12 = t[6]; m.b13 = t[7]; m.b23 = t[8];
+    m.a4x = t[3]; m.a4y = t[4]; m.a4z = t[5];
+    m.b2xy = t[6]; m.b2xz = t[7]; m.b2yz = t[8];
     return m;
   }
 }
@@ -275,8 +279,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_table(const double* __restrict_
     float* o = table + c * kTableWidth;
     o[0] = (float)w;
     o[1] = m.Bx; o[2] = m.By; o[3] = m.Bz;
-    o[4] = m.a1; o[5] = m.a2; o[6] = m.a3;
-    o[7] = m.b12; o[8] = m.b13; o[9] = m.b23;
+    o[4] = m.a4x; o[5] = m.a4y; o[6] = m.a4z;
+    o[7] = m.b2xy; o[8] = m.b2xz; o[9] = m.b2yz;
     o[10] = 0.0f; o[11] = 0.0f;
   }
 }
@@ -352,7 +356,7 @@ __device__ __forceinline__ void far_tile(const FarGeom& g, const float* __restri
         const MpConst cc = cell_const<MESH>(shape, cb + kk, uc);
         float T[NC];
         if constexpr (MESH == kMeshCubes && HEXA)
-          multipole_cube_f32<MASK>(X, Y, Z, cc.a1, T);
+          multipole_cube_f32<MASK>(X, Y, Z, 0.25f * cc.a4x, T);
         else
           multipole_f32<MASK, QUAD, HEXA>(X, Y, Z, cc, T);
 #pragma unroll
@@ -475,14 +479,19 @@ __device__ __forceinline__ void far_tile_mono(const FarGeom& g, const float* __r
 // every pair's multipole terms; model b's sums go to
 // farpart[((b * nchunks + chunk) * NC + m) * S + s].
 // Minimum waves per SIMD the far-field kernels are built for, by models per launch (1:
-// the register count decides).  One model: six (at most 80 VGPRs; the few spills stay
-// outside the cell loops), measured faster than the five that 93 VGPRs allow and than
-// seven or eight, whose spills reach the loops.  Two models in the forward and four in
-// the adjoint: four waves (2.12 -> 1.96 and 1.35 -> 1.28 ms per model at 128x128x64
-// cells, 4096 sites); more waves for the others were no faster or spilled in the loops.
+// the register count decides).  One model: six (at most 80 VGPRs; no scratch with the cube
+// forms), measured faster than the five that 93 VGPRs allow and than seven or eight, whose
+// spills reach the loops.  Two models in the forward and four in the adjoint: four waves
+// (2.12 -> 1.96 and 1.35 -> 1.28 ms per model at 128x128x64 cells, 4096 sites); more waves
+// for the others were no faster or spilled in the loops.  The single-model adjoint of other
+// cell shapes holds four cells' multipole constants per lane and spilled 100-350 bytes per
+// lane at six waves: four (99-128 VGPRs) halved it (config 1's 64x64x32 layered mesh
+// 0.82 -> 0.42 ms, the vertical gradient's slab mesh 32.2 -> 21.1 ms per forward + adjoint;
+// three waves, no scratch at all, measured within 3 % either way).
 constexpr int far_fwd_waves(int nb) { return nb == 1 ? 6 : nb == 2 ? 4 : 1; }
-constexpr int far_adj_waves(int nb) { return nb == 1 ? 6 : nb == 4 ? 4 : 1; }
-
+constexpr int far_adj_waves(int nb, int mesh) {
+  return nb == 1 ? (mesh == kMeshCubes ? 6 : 4) : nb == 4 ? 4 : 1;
+}
 template <int MASK, int MESH, int NB>
 __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(far_fwd_waves(NB)))) void k_far_forward(
@@ -759,7 +768,7 @@ __device__ __forceinline__ void far_adjoint_site(
     for (int q = 0; q < kQ; ++q) {
       float T[NC];
       if (MESH == kMeshCubes && need)
-        multipole_cube_f32<MASK>(X, Y, Zq[q], ccq[q].a1, T);
+        multipole_cube_f32<MASK>(X, Y, Zq[q], 0.25f * ccq[q].a4x, T);
       else if (need)
         multipole_f32<MASK, QUAD, true>(X, Y, Zq[q], ccq[q], T);
       else
@@ -793,7 +802,7 @@ __device__ __forceinline__ void far_adjoint_site(
 // sums at gpart[(b nchunks + chunk) ncell + c].
 template <int MASK, int MESH, int NB>
 __global__ __launch_bounds__(kBlock)
-__attribute__((amdgpu_waves_per_eu(far_adj_waves(NB)))) void k_far_adjoint(
+__attribute__((amdgpu_waves_per_eu(far_adj_waves(NB, MESH)))) void k_far_adjoint(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     const float* __restrict__ coef, const float* __restrict__ mono, const int* __restrict__ box,
     float q1sq, int S, int nchunks, const int* __restrict__ ranges, FarGeom g,
@@ -1319,7 +1328,7 @@ hipError_t forward_mixed_impl(jgm_context* ctx, int kind, int nbm, const double*
   if (N > 0) {
     const int site_blocks = (S + kBlock - 1) / kBlock;
     const long waves_per_chunk = (long)site_blocks * (kBlock / 64);
-    long want = (kFarForwardWaves + waves_per_chunk - 1) / waves_per_chunk;
+    long want = (ctx->far_fwd_waves + waves_per_chunk - 1) / waves_per_chunk;
     want = std::max<long>(1, std::min<long>(want, ntiles));
     const long per = (ntiles + want - 1) / want;
     nchunks = (int)((ntiles + per - 1) / per);
