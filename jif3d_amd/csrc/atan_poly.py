@@ -2,7 +2,7 @@
 
 atan(v) = v + v^3 g(v^2) on |v| <= tan(pi/8); g is the degree-10 polynomial in x = v^2 that
 minimises the largest relative error of atan (Remez exchange, 60 significant digits), rounded to
-doubles.  Run `python -m jif3d_amd.tools.atan_poly` to print them and the error they leave.
+doubles.  Run `python jif3d_amd/csrc/atan_poly.py` to print them and the error they leave.
 """
 import mpmath as mp
 

@@ -91,7 +91,7 @@ __device__ __forceinline__ double log_pos_f64(double a) {
 // corner_terms_f64 sends other offsets to the library atan2), written for this use: the octant reduction
 // u = min / max of |p|, |q| and v = u, or (u - 1) / (u + 1) when u > tan(pi/8), from one
 // division (refined hardware reciprocal and its residual), atan(v) = v + v^3 g(v^2) with the
-// degree-10 minimax polynomial g of jif3d_amd/tools/atan_poly.py (relative error 1.3e-18), and
+// degree-10 minimax polynomial g of jif3d_amd/csrc/atan_poly.py (relative error 1.3e-18), and
 // the octant and quadrant offset m pi/4 added in two parts (m hi exact for m <= 4).  Within
 // 2 ulp of the exact value over 2e5 random arguments of every octant; ~45 VALU operations
 // against ~70 for the library atan2, which also handles zero, infinite and NaN arguments.
