@@ -38,8 +38,9 @@ struct Ncomp {
 
 // Natural logarithm of a finite a >= DBL_EPSILON (the guarded log arguments of the
 // corner function), written for this use: a = 2^e m with m in [1/sqrt2, sqrt2),
-// log m = 2 atanh(f) = 2 (f + f^3/3 + ... + f^23/23), f = (m - 1)/(m + 1), |f| <= 0.1716
-// (the series' remainder is below 2^-57).  f comes from the refined hardware reciprocal
+// log m = 2 atanh(f) = 2 f + 2 f f^2 q(f^2), f = (m - 1)/(m + 1), |f| <= 0.1716, q the
+// degree-7 minimax polynomial of jif3d_amd/csrc/corner_poly.py (q ~ 1/3 + f^2/5 + ...;
+// relative error 3e-18, below 1e-19 in the logarithm).  f comes from the refined hardware reciprocal
 // and is corrected by its exact residual (m + 1 split into a sum of two doubles), and the
 // result is assembled in double-double, so the error stays near half an ulp; about 45
 // VALU operations against ~95 for the library log, which also handles zero, negative,
@@ -55,25 +56,22 @@ __device__ __forceinline__ double log_pos_f64(double a) {
   const double den = m + 1.0;
   const double bb0 = den - m;
   const double dlo = (m - (den - bb0)) + (1.0 - bb0);  // den + dlo = m + 1 exactly
-  // hardware reciprocal estimate, refined twice (each Newton step squares its error)
+  // hardware reciprocal estimate (relative error below 2^-24.4, measured on gfx950) and one
+  // Newton step (2.2e-15); f's remaining error is carried by flo, formed from the exact residual
   double rc = __builtin_amdgcn_rcp(den);
-  rc = fma(rc, fma(-den, rc, 1.0), rc);
   rc = fma(rc, fma(-den, rc, 1.0), rc);
   const double f = num * rc;
   const double flo = (fma(-f, den, num) - f * dlo) * rc;  // f + flo = num / (m + 1)
   const double f2 = f * f;
-  double q = 1.0 / 23.0;
-  q = fma(q, f2, 1.0 / 21.0);
-  q = fma(q, f2, 1.0 / 19.0);
-  q = fma(q, f2, 1.0 / 17.0);
-  q = fma(q, f2, 1.0 / 15.0);
-  q = fma(q, f2, 1.0 / 13.0);
-  q = fma(q, f2, 1.0 / 11.0);
-  q = fma(q, f2, 1.0 / 9.0);
-  q = fma(q, f2, 1.0 / 7.0);
-  q = fma(q, f2, 1.0 / 5.0);
-  q = fma(q, f2, 1.0 / 3.0);
-  q *= f2;  // f^2/3 + f^4/5 + ...
+  double q = 0.0654368673815417;
+  q = fma(q, f2, 0.06634346805939063);
+  q = fma(q, f2, 0.07693121530786205);
+  q = fma(q, f2, 0.09090897784329731);
+  q = fma(q, f2, 0.11111111196258087);
+  q = fma(q, f2, 0.14285714285399548);
+  q = fma(q, f2, 0.20000000000000445);
+  q = fma(q, f2, 0.3333333333333333);
+  q *= f2;  // ~ f^2/3 + f^4/5 + ...
   const double ln2_hi = 6.93147180369123816490e-01;  // 32 significant bits: e ln2_hi exact
   const double ln2_lo = 1.90821492927058770002e-10;
   const double de = (double)e;
@@ -90,8 +88,8 @@ __device__ __forceinline__ double log_pos_f64(double a) {
 // [1e-280, 1e280] (the corner terms' products of site-corner offsets in [1e-140, 1e140];
 // corner_terms_f64 sends other offsets to the library atan2), written for this use: the octant reduction
 // u = min / max of |p|, |q| and v = u, or (u - 1) / (u + 1) when u > tan(pi/8), from one
-// division (refined hardware reciprocal and its residual), atan(v) = v + v^3 g(v^2) with the
-// degree-10 minimax polynomial g of jif3d_amd/csrc/atan_poly.py (relative error 1.3e-18), and
+// division (hardware reciprocal, one Newton step, the quotient's residual), atan(v) = v + v^3 g(v^2) with the
+// degree-10 minimax polynomial g of jif3d_amd/csrc/corner_poly.py (relative error 1.3e-18), and
 // the octant and quadrant offset m pi/4 added in two parts (m hi exact for m <= 4).  Within
 // 2 ulp of the exact value over 2e5 random arguments of every octant; ~45 VALU operations
 // against ~70 for the library atan2, which also handles zero, infinite and NaN arguments.
@@ -102,8 +100,7 @@ __device__ __forceinline__ double atan2_f64(double p, double q) {
   const bool big = num > 0.41421356237309503 * den;  // tan(pi/8)
   const double n2 = big ? num - den : num, d2 = big ? num + den : den;
   double r = __builtin_amdgcn_rcp(d2);
-  r = fma(r, fma(-d2, r, 1.0), r);
-  r = fma(r, fma(-d2, r, 1.0), r);
+  r = fma(r, fma(-d2, r, 1.0), r);  // one Newton step: 2.2e-15 relative (log_pos_f64)
   double v = n2 * r;
   v = fma(r, fma(-d2, v, n2), v);  // the quotient corrected by its residual
   const double x = v * v;

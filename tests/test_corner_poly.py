@@ -1,8 +1,11 @@
-"""The FP64 arctangent of the corner terms (atan2_f64 in prism_device.hpp): its polynomial
-coefficients, read from the header, give atan(v) on the reduced range |v| <= tan(pi/8) within
-an ulp of numpy's arctan in double arithmetic, and the octant/quadrant assembly reproduces
-arctan2 in every octant.  The kernels' results with it are compared with the oracle on the GPU
-(test_gpu_parity.py, test_gpu_configs.py)."""
+"""The FP64 elementary functions of the corner terms (prism_device.hpp).  atan2_f64: its
+polynomial coefficients, read from the header, give atan(v) on the reduced range
+|v| <= tan(pi/8) within an ulp of numpy's arctan in double arithmetic, and the octant/quadrant
+assembly reproduces arctan2 in every octant.  log_pos_f64: its polynomial q matches
+(atanh(f)/f - 1)/f^2 = sum f^2k / (2k + 3), in extended precision, to 5e-16 relative over the
+reduced range.  The
+kernels' results with both are compared with the oracle on the GPU (test_gpu_parity.py,
+test_gpu_configs.py)."""
 import os
 import re
 
@@ -66,3 +69,27 @@ def test_every_octant_within_two_ulp():
     assert np.max(np.abs(f - ref) / np.spacing(np.abs(ref))) <= 2.0
     # a wrong octant or quadrant offset would be off by a multiple of pi/4
     assert np.all(np.sign(f) == np.sign(ref))
+
+
+def log_coefficients():
+    src = open(HDR).read()
+    body = src[src.index("double log_pos_f64("):]
+    body = body[:body.index("\n}\n")]
+    first = float(re.search(r"double q = (\S+);", body).group(1))
+    rest = [float(v) for v in re.findall(r"q = fma\(q, f2, (\S+)\);", body)]
+    return [first] + rest
+
+
+def test_log_polynomial_on_the_reduced_range():
+    coef = log_coefficients()
+    assert len(coef) == 8
+    fmax = (np.sqrt(2.0) - 1) / (np.sqrt(2.0) + 1)
+    f = np.linspace(0.0, fmax, 100001).astype(np.longdouble)
+    x = f * f
+    # q(x) = sum_k x^k / (2k + 3), summed in extended precision (x <= 0.0295: 30 terms)
+    exact = sum(x ** k / np.longdouble(2 * k + 3) for k in range(30))
+    q = np.full_like(x, coef[0])
+    for c in coef[1:]:
+        q = q * x + np.longdouble(c)
+    # q's relative error enters the logarithm scaled by x q <= 0.01: 5e-16 here is 0.05 ulp
+    assert float(np.max(np.abs(q - exact) / exact)) <= 5e-16
