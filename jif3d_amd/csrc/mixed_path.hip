@@ -487,10 +487,13 @@ __device__ __forceinline__ void far_tile_mono(const FarGeom& g, const float* __r
 // cell shapes holds four cells' multipole constants per lane and spilled 100-350 bytes per
 // lane at six waves: four (99-128 VGPRs) halved it (config 1's 64x64x32 layered mesh
 // 0.82 -> 0.42 ms, the vertical gradient's slab mesh 32.2 -> 21.1 ms per forward + adjoint;
-// three waves, no scratch at all, measured within 3 % either way).
+// three waves, no scratch at all, measured within 3 % either way).  Four misfit vectors on
+// a general mesh spilled ~300 bytes per lane at four waves: two (~190 VGPRs, no scratch) took
+// config 1's batch of four from 1.21 to 1.03 ms (on config 2's uniform mesh two waves were
+// 4.5 % slower than four, which spill 40 bytes).
 constexpr int far_fwd_waves(int nb) { return nb == 1 ? 6 : nb == 2 ? 4 : 1; }
 constexpr int far_adj_waves(int nb, int mesh) {
-  return nb == 1 ? (mesh == kMeshCubes ? 6 : 4) : nb == 4 ? 4 : 1;
+  return nb == 1 ? (mesh == kMeshCubes ? 6 : 4) : nb == 4 ? (mesh == kMeshGeneral ? 2 : 4) : 1;
 }
 template <int MASK, int MESH, int NB>
 __global__ __launch_bounds__(kBlock)
