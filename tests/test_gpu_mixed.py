@@ -242,6 +242,36 @@ def test_mixed_sites_inside_the_mesh():
     assert rel_inf(ctx.adjoint(K, nat.MIXED, mv, r1), ctx.adjoint(K, nat.FP64, mv, r1)) <= 1e-12
 
 
+def test_mixed_strongly_graded_layers():
+    """Near boxes from each layer's own reach: on layers that grow 15 % per layer (5 m at the
+    top, ~110 m at the bottom) a surface site's box holds only the thin top layers within q0
+    of their own half-diagonals, and a buried site's box the layers around it; every cell
+    outside is still q0 of its own half-diagonals away, so the stated error holds for data
+    and gradient, above the mesh and inside it."""
+    m = ThreeDGravityModel()
+    m.SetMeshSize(32, 32, 24)
+    m.SetXCellSizes(np.full(32, 25.0))
+    m.SetYCellSizes(np.full(32, 25.0))
+    m.SetZCellSizes(5.0 * 1.15 ** np.arange(24))
+    rng = np.random.default_rng(23)
+    m.SetDensities(2670.0 + 300.0 * rng.standard_normal(m.ncells))
+    d = TensorGravityData()
+    for _ in range(300):
+        d.AddMeasurementPoint(rng.uniform(10, 790), rng.uniform(10, 790), -rng.uniform(1, 40))
+    for _ in range(60):
+        d.AddMeasurementPoint(rng.uniform(10, 790), rng.uniform(10, 790), rng.uniform(3, 900))
+    ctx = nat.Context(0)
+    ctx.configure(m, d)
+    ctx.set_mixed_check(0, 0, 5e-7)  # the expansion itself, no fallback
+    K = nat.KIND_SCALAR_TENSOR
+    mv = m.GetModelParameters()
+    f64 = ctx.forward(K, nat.FP64, mv)
+    mix = ctx.forward(K, nat.MIXED, mv)
+    assert comp_err(mix, f64, 10) <= TOL
+    r = rng.normal(size=f64.size)
+    assert rel_inf(ctx.adjoint(K, nat.MIXED, mv, r), ctx.adjoint(K, nat.FP64, mv, r)) <= TOL
+
+
 def test_mixed_large_ragged_survey():
     """20011 sites (not a multiple of the 64-lane wave) over 64x64x16 cubes: forward and
     gradient against FP64, and a batch of two equal to single calls."""
