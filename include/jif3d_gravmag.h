@@ -63,8 +63,9 @@ typedef enum jgm_kind {
 } jgm_kind;
 
 /* JGM_FP64: closed-form box terms in FP64 for every site-cell pair.
- * JGM_MIXED: FP64 closed form for cells inside each site's near box (all cells
- * closer than q0 cell half-diagonals), FP32 multipole of the prism about its centre
+ * JGM_MIXED: FP64 closed form for cells inside each site's near box (every cell closer
+ * than q0 of its own half-diagonals: the layers within their reach of the site, and along
+ * x and y the largest reach of those layers), FP32 multipole of the prism about its centre
  * through the hexadecapole for the rest, FP64 accumulation across tiles. */
 typedef enum jgm_precision { JGM_FP64 = 0, JGM_MIXED = 1 } jgm_precision;
 
@@ -117,7 +118,8 @@ jgm_status jgm_set_field(jgm_context *ctx, double inclination, double declinatio
 /* Data transform (ThreeDGravMagImplementation::SetDataTransform, :67-70). */
 jgm_status jgm_set_transform(jgm_context *ctx, jgm_transform tf, double inclination,
                              double declination);
-/* Near-field radius of JGM_MIXED in cell half-diagonals (default 8). */
+/* Near-field radius of JGM_MIXED in half-diagonals of each layer's largest cell shape
+ * (default 8). */
 jgm_status jgm_set_mixed_threshold(jgm_context *ctx, double q0);
 /* Second radius of JGM_MIXED in cell half-diagonals (default 28): cells farther than
  * q1 from a site keep the monopole (and, for non-cubic cells, quadrupole) terms only;
@@ -249,9 +251,9 @@ size_t jgm_storedj_rows(const jgm_context *ctx);
  * to BF16 (FP32 accumulation); JGM_GEMM_BF16_REFINED adds the products of the BF16
  * remainders (hi*hi + hi*lo + lo*hi), close to FP32 accuracy at BF16 matrix-core rate.
  * The first JGM_GEMM_BF16 product of an HBM-resident J makes a BF16 copy of J (2 bytes
- * per element of device memory, kept until J is rebuilt or released) that the J M
- * products read instead of the FP32 J, with the same results; when that memory cannot
- * be allocated they read the FP32 J. */
+ * per element of device memory, rows padded to a multiple of 256, kept until J is
+ * rebuilt or released) that the J M products read instead of the FP32 J, with the same
+ * results; when that memory cannot be allocated they read the FP32 J. */
 jgm_status jgm_storedj_apply(jgm_context *ctx, jgm_gemm_precision prec, const float *M,
                              size_t nrhs, float *Y);
 /* G = J^T R for nrhs residual vectors (R: rows x nrhs, G: cols x nrhs), the batched
