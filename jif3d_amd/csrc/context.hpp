@@ -44,7 +44,7 @@ struct DeviceBuffer {
 
 // Near-field description for the mixed-precision kernels: per site an index box
 // of cells [lo, hi] (inclusive) evaluated with the FP64 closed form; every cell
-// outside it is at least q0 half-diagonals away from the site.
+// outside it is at least q0 of its own layer's largest half-diagonals away from the site.
 struct NearBoxes {
   DeviceBuffer box;       // int32 [S][6]: i0, i1, j0, j1, k0, k1 (empty: i0 > i1)
   DeviceBuffer col_ptr;   // int32 [ncols+1]: CSR of sites whose box covers column (i, j)
@@ -56,7 +56,7 @@ struct NearBoxes {
   size_t ncols = 0;
   size_t total_near_cells = 0;    // sum over sites of box cells
   size_t total_near_corners = 0;  // sum over sites of box corners
-  double radius = 0.0;          // q0 * largest cell half-diagonal (m)
+  double radius = 0.0;          // q0 * largest cell half-diagonal of the mesh (m)
   bool valid = false;
   // far-field geometry: cell centres (FP64)
   DeviceBuffer xcen, ycen, zcen;
