@@ -459,16 +459,27 @@ def run_configs(args):
 
 
 def cpu_model():
-    """The host CPU's model name (first processor in /proc/cpuinfo)."""
+    """The host CPU's model name, family/model/stepping and clock (first processor in
+    /proc/cpuinfo; virtual machines may report only a short model name), and the logical
+    CPUs the process sees."""
+    fields = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if not line.strip():
+                    break  # the first processor only
+                key, _, val = line.partition(":")
+                fields[key.strip()] = val.strip()
     except OSError:
         pass
-    import platform
-    return platform.processor() or "unknown"
+    if "model name" not in fields:
+        import platform
+        return platform.processor() or "unknown"
+    extra = [f"{k} {fields[k]}" for k in ("cpu family", "model", "stepping") if k in fields]
+    if "cpu MHz" in fields:
+        extra.append(f"{float(fields['cpu MHz']):.0f} MHz")
+    extra.append(f"{os.cpu_count()} logical CPUs visible")
+    return f"{fields['model name']} ({', '.join(extra)})"
 
 
 def cpu_baseline(model, data, budget_s, runs=5):
