@@ -258,12 +258,27 @@ __device__ __forceinline__ int chunk_k(int s, int lk) {
   return PREC == JGM_GEMM_FP32 ? 2 * s + lk : 8 * lk + s;
 }
 
-// Y partials: Yp[split][row][P].  Grid: x = nsplit * row_tiles workgroups (XCD-grouped
-// when nsplit is a multiple of 8 so that the workgroups of one split share an L2), y =
-// right-hand-side tiles of 64.  Each split covers `per` cells (a multiple of 32).
+// Workgroup w of nw runs on XCD w % 8: each XCD takes a contiguous range of the (split,
+// row tile) items in split-major order, so that the row tiles of a split, which read the
+// same cells of M, share one L2 (XCD x holds nw / 8 items, one more when x < nw % 8, which
+// is the number of w < nw with w % 8 == x, so the map is one to one).
+__device__ __forceinline__ void xcd_item(int w, int nw, int row_tiles, int& split, int& rt) {
+  const int x = w & 7;
+  const int item = x * (nw / 8) + min(x, nw % 8) + (w >> 3);
+  split = item / row_tiles;
+  rt = item % row_tiles;
+}
+
+// Y partials: Yp[split][row][P].  Grid: x = nsplit * row_tiles workgroups in XCD order
+// (xcd_item), y = right-hand-side tiles of 64.  Each split covers `per` cells (a
+// multiple of 64).
 struct JmFrag {
   float a[2][8], b[2][8];
 };
+// register stages of k_gemm_jm: 4 (234 VGPRs for FP32, no spills) reads FP32 J at 113
+// TFLOP/s on 7168 x 131072 with the clocks up; 2 reaches 84, 3 97; 5 and 6 need AGPR
+// copies and are slower
+constexpr int kJmStages = 4;
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 template <int PREC>
@@ -273,14 +288,7 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm(const float* __restrict__ J,
                                                     float* __restrict__ Yp) {
   const int w = blockIdx.x;
   int split, rt;
-  if ((nsplit & 7) == 0) {
-    const int slot = w >> 3;
-    split = (w & 7) * (nsplit >> 3) + slot / row_tiles;
-    rt = slot % row_tiles;
-  } else {
-    split = w / row_tiles;
-    rt = w % row_tiles;
-  }
+  xcd_item(w, nsplit * row_tiles, row_tiles, split, rt);
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int li = lane & 31, lk = lane >> 5;
@@ -306,33 +314,36 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm(const float* __restrict__ J,
     for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
       for (int r = 0; r < 16; ++r) tot[ta][tb][r] = 0.0f;
-  JmFrag x, y;
-  auto load = [&](JmFrag& f, long cb) {
+  // a ring of kJmStages chunks in registers: chunk q is loaded while q - kJmStages + 1 is
+  // multiplied, so kJmStages - 1 chunks of loads are in flight behind each MFMA block
+  JmFrag f[kJmStages];
+  auto load = [&](JmFrag& g, int q) {
+    const long cb = c0 + (long)(q < nchunk ? q : nchunk - 1) * kChunk;  // past the end: unused
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
       const long cell = cb + chunk_k<PREC>(s, lk);
-      f.a[0][s] = __builtin_nontemporal_load(Jw + cell * ld);
-      f.a[1][s] = __builtin_nontemporal_load(Jw + cell * ld + 32);
-      f.b[0][s] = Mw[cell * P];
-      f.b[1][s] = Mw[cell * P + 32];
+      g.a[0][s] = __builtin_nontemporal_load(Jw + cell * ld);
+      g.a[1][s] = __builtin_nontemporal_load(Jw + cell * ld + 32);
+      g.b[0][s] = Mw[cell * P];
+      g.b[1][s] = Mw[cell * P + 32];
     }
   };
-  auto mma = [&](const JmFrag& f) { mma_block<PREC>(f.a, f.b, acc); };
-  // sched_barrier keeps the scheduler from sinking the next chunk's loads down to
-  // their uses, which would expose the full memory latency once per chunk
+  // sched_barrier keeps the scheduler from sinking the loads ahead down to their uses,
+  // which would expose the full memory latency once per chunk
   if (nchunk > 0) {
-    load(x, c0);
-    for (int q = 0; q < nchunk; q += 2) {
-      load(y, c0 + (long)(q + 1) * kChunk);
-      __builtin_amdgcn_sched_barrier(0);
-      mma(x);
-      __builtin_amdgcn_sched_barrier(0);
-      const int qn = q + 2 < nchunk ? q + 2 : nchunk - 1;  // the last reload is unused
-      load(x, c0 + (long)qn * kChunk);
-      __builtin_amdgcn_sched_barrier(0);
-      mma(y);
-      __builtin_amdgcn_sched_barrier(0);
-      if (((q + 2) & 15) == 0) flush(acc, tot);  // every 256 cells
+#pragma unroll
+    for (int d = 0; d < kJmStages - 1; ++d) load(f[d], d);
+    for (int q = 0; q < nchunk; q += kJmStages) {
+#pragma unroll
+      for (int d = 0; d < kJmStages; ++d) {
+        load(f[(d + kJmStages - 1) % kJmStages], q + d + kJmStages - 1);
+        __builtin_amdgcn_sched_barrier(0);
+        if (q + d < nchunk) {
+          mma_block<PREC>(f[d].a, f[d].b, acc);
+          if (((q + d + 1) & 15) == 0) flush(acc, tot);  // every 256 cells
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
   }
   flush(acc, tot);
@@ -387,13 +398,8 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm_bf16p(const bf16x8* __restri
                                                           long per, int row_tiles, int nsplit,
                                                           float* __restrict__ Yp) {
   __shared__ bf16x8 sM[2][512];
-  // workgroup w runs on XCD w % 8: each XCD takes a contiguous range of the (split, row
-  // tile) pairs in split-major order, so that the row tiles of a split, which read the same
-  // cells of M, share one L2
-  const int w = blockIdx.x;
-  const int nw = nsplit * row_tiles, x = w & 7;
-  const int item = x * (nw / 8) + min(x, nw % 8) + (w >> 3);
-  const int split = item / row_tiles, rt = item % row_tiles;
+  int split, rt;
+  xcd_item(blockIdx.x, nsplit * row_tiles, row_tiles, split, rt);
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int li = lane & 31, lk = lane >> 5;
@@ -1280,11 +1286,9 @@ hipError_t run_jm(jgm_context* ctx, int gemm_precision, const float* Xt, int P, 
   }
   e = for_each_block(ctx, [&](const float* J, long c0, long ncols) -> hipError_t {
     // one workgroup per CU: a second co-resident workgroup slows both down (the CU's
-    // memory pipeline is the limit); splits in multiples of 8 when there are enough of
-    // them to group by XCD
+    // memory pipeline is the limit)
     long nsplit = std::max<long>(1, (long)ctx->cu_count / ((long)row_tiles * rhs_tiles));
     nsplit = std::min<long>(nsplit, ncols / 32);
-    if (nsplit >= 16) nsplit = nsplit / 8 * 8;
     // splits of whole 64-cell blocks (the packed kernel's stage)
     const long per = round_up((ncols + nsplit - 1) / nsplit, 4 * kChunk);
     hipError_t e2 = ctx->partial.ensure(sizeof(float) * (size_t)nsplit * R * P);
@@ -1395,6 +1399,9 @@ __device__ __forceinline__ f64x4 mfma_f64(double a, double b, f64x4 c) {
 struct Frag64 {
   double a[4][4], b[4][4];  // [k-step][tile]
 };
+// register stages of k_gemm_jm_f64: 3 (206 VGPRs with the 128 accumulator AGPRs still 1
+// wave per SIMD) reads FP64 J 13 % faster than 2; 4 would spill
+constexpr int kJm64Stages = 3;
 
 __device__ __forceinline__ void mma64(const Frag64& f, f64x4 (&acc)[4][4]) {
 #pragma unroll
@@ -1413,14 +1420,7 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm_f64(const double* __restrict
                                                         double* __restrict__ Yp) {
   const int w = blockIdx.x;
   int split, rt;
-  if ((nsplit & 7) == 0) {
-    const int slot = w >> 3;
-    split = (w & 7) * (nsplit >> 3) + slot / row_tiles;
-    rt = slot % row_tiles;
-  } else {
-    split = w / row_tiles;
-    rt = w % row_tiles;
-  }
+  xcd_item(w, nsplit * row_tiles, row_tiles, split, rt);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int li = lane & 15, g = lane >> 4;
   const long row0 = (long)rt * 256 + wave * 64;
@@ -1436,29 +1436,30 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm_f64(const double* __restrict
   for (int t = 0; t < 4; ++t)
 #pragma unroll
     for (int u = 0; u < 4; ++u) acc[t][u] = f64x4{0.0, 0.0, 0.0, 0.0};
-  Frag64 x, y;
-  auto load = [&](Frag64& f, long cb) {
+  // a ring of kJm64Stages chunks in registers, as in k_gemm_jm
+  Frag64 f[kJm64Stages];
+  auto load = [&](Frag64& h, int q) {
+    const long cb = c0 + (long)(q < nchunk ? q : nchunk - 1) * kChunk;  // past the end: unused
 #pragma unroll
     for (int st = 0; st < 4; ++st) {
       const long cell = cb + 4 * g + st;
 #pragma unroll
-      for (int t = 0; t < 4; ++t) f.a[st][t] = __builtin_nontemporal_load(Jw + cell * ld + 16 * t);
+      for (int t = 0; t < 4; ++t) h.a[st][t] = __builtin_nontemporal_load(Jw + cell * ld + 16 * t);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) f.b[st][u] = Mw[cell * P + 16 * u];
+      for (int u = 0; u < 4; ++u) h.b[st][u] = Mw[cell * P + 16 * u];
     }
   };
   if (nchunk > 0) {
-    load(x, c0);
-    for (int q = 0; q < nchunk; q += 2) {
-      load(y, c0 + (long)(q + 1) * kChunk);
-      __builtin_amdgcn_sched_barrier(0);
-      mma64(x, acc);
-      __builtin_amdgcn_sched_barrier(0);
-      const int qn = q + 2 < nchunk ? q + 2 : nchunk - 1;  // the last reload is unused
-      load(x, c0 + (long)qn * kChunk);
-      __builtin_amdgcn_sched_barrier(0);
-      mma64(y, acc);
-      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int d = 0; d < kJm64Stages - 1; ++d) load(f[d], d);
+    for (int q = 0; q < nchunk; q += kJm64Stages) {
+#pragma unroll
+      for (int d = 0; d < kJm64Stages; ++d) {
+        load(f[(d + kJm64Stages - 1) % kJm64Stages], q + d + kJm64Stages - 1);
+        __builtin_amdgcn_sched_barrier(0);
+        if (q + d < nchunk) mma64(f[d], acc);
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
   }
   double* out = Yp + (long)split * R * P;
@@ -1556,9 +1557,10 @@ hipError_t run_jm_f64(jgm_context* ctx, const double* Xt, int P, int nrhs, doubl
   const long colpad = (long)ctx->storedJ_colpad;
   const int rhs_tiles = P / 64;
   const int row_tiles = (int)((ld + 255) / 256);
-  long nsplit = std::max<long>(1, 2L * ctx->cu_count / ((long)row_tiles * rhs_tiles));
+  // one round of workgroups (one per CU at 1 wave per SIMD): on 7168 x 65536, 9 splits
+  // run 1.02 ms against 1.12 ms for 16 (448 workgroups, a second round 3/4 full)
+  long nsplit = std::max<long>(1, (long)ctx->cu_count / ((long)row_tiles * rhs_tiles));
   nsplit = std::min<long>(nsplit, colpad / 32);
-  if (nsplit >= 16) nsplit = nsplit / 8 * 8;
   const long per = round_up((colpad + nsplit - 1) / nsplit, 2 * kChunk);
   hipError_t e = ctx->partial.ensure(sizeof(double) * (size_t)nsplit * R * P);
   if (e != hipSuccess) return e;
