@@ -228,8 +228,8 @@ jgm_status jgm_destroy(jgm_context* ctx) {
                                &ctx->near.zcen, &ctx->near.adj_table, &ctx->near.corner_off,
                                &ctx->near.cell_off, &ctx->near_h, &ctx->near_v, &ctx->vsites, &ctx->coef_mono,
                                &ctx->near.site_order, &ctx->near.xoff,
-                               &ctx->near.yoff, &ctx->near.zoff, &ctx->acc64,
-                               &ctx->raw_cache_model, &ctx->raw_cache_raw, &ctx->raw_cache_flag,
+                               &ctx->near.yoff, &ctx->near.zoff, &ctx->near.kmap,
+                               &ctx->near.listed, &ctx->acc64, &ctx->raw_cache_model, &ctx->raw_cache_raw, &ctx->raw_cache_flag,
                                &ctx->partsum, &ctx->chunk_scratch, &ctx->mag_part};
   for (auto* b : bufs) b->release();
   jgm::release_slab_mesh(ctx);

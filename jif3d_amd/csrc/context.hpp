@@ -67,6 +67,14 @@ struct NearBoxes {
   // tiles' half-diagonals that scale the second radius
   DeviceBuffer hx2, hy2, hz2, hz2_16;
   int mesh_class = 2;  // 0: cubes of one size, 1: one cell shape, 2: general
+  // the layers the far field runs over (jgm_context::far_layers; all when that is empty):
+  // nzf of them, kmap (int32 [nzf]) their mesh layers in increasing order, listed (int32
+  // [nz]) 1 for a listed layer, far_dz the cell height of the first of them.  The z arrays
+  // above (zcen, zoff, hz2, hz2_16) run over the listed layers.
+  DeviceBuffer kmap, listed;
+  int nzf = 0;
+  bool far_subset = false;
+  double far_dz = 0.0;
   DeviceBuffer adj_table;  // FP32 per-cell table with unit density (G V, B...)
   bool adj_table_valid = false;
   DeviceBuffer site_order;  // int32 [S]: sites in Morton order of (x, y), for compact waves
@@ -159,6 +167,11 @@ struct jgm_context {
   // JIF3D_AMD_FAR_ADJOINT_WAVES at creation
   long far_fwd_waves = 65536;
   long far_adj_waves = 49152;
+  // mesh layers the mixed far field runs over, increasing (empty: every layer).  Every other
+  // layer must carry zero weight in every model: the far field skips it, and the mixed
+  // adjoint returns zero for its cells.  The vertical gradient's slab mesh lists its slabs
+  // (its gap layers are empty).  Changing it needs the near boxes made again.
+  std::vector<int> far_layers;
 
   // a-posteriori check of JGM_MIXED: FP64 at check_sites sample sites after a forward and
   // at check_cells sample cells after an adjoint; a sampled error above check_tol (relative

@@ -10,8 +10,9 @@
 // starting there.  The vertical gradient of the mesh is therefore the Bz of a second mesh
 // of nz + 1 slab layers, separated by zero-weight gap layers (Z_K, Z_{K+1} - 1), and that
 // mesh goes through the ordinary mixed tensor path (FP64 near boxes, FP32 multipole far
-// field) of a child context holding its geometry.  Nothing cancels beyond the slabs'
-// separation, so the far field's FP32 error stays at the magnetic kind's level.
+// field) of a child context holding its geometry, whose far field runs over the slab layers
+// only.  Nothing cancels beyond the slabs' separation, so the far field's FP32 error stays
+// at the magnetic kind's level.
 //
 // The slabs need cells at least 1 m thick; on thinner cells the call runs the FP64 path.
 // Sites whose site or 1 m-lower point lies on a mesh line (two coordinates on corner
@@ -155,6 +156,11 @@ hipError_t ensure_slab_mesh(jgm_context* ctx) {
                      ctx->xs.data(), ctx->ys.data(), zsz.data()) != JGM_OK ||
         jgm_set_sites(c, ctx->S, ctx->hsx.data(), ctx->hsy.data(), ctx->hsz.data()) != JGM_OK)
       return hipErrorInvalidValue;
+    // the far field runs over the slabs only: the gap layers are empty in every model and
+    // their gradient entries are not used (half the far-field pairs; the slabs all have one
+    // shape on a mesh of uniform columns)
+    c->far_layers = layer_of;
+    c->near.valid = false;
     // sites with the site or its 1 m-lower point on a mesh line
     std::vector<int> line;
     for (size_t s = 0; s < ctx->S; ++s) {

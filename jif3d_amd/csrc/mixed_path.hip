@@ -303,6 +303,10 @@ __global__ __launch_bounds__(kBlock) void k_cell_weights(const double* __restric
   for (int b = 0; b < nb; ++b) wt[c * nb + b] = (float)(kGravConst * rho[b * model_ld + c] * vol);
 }
 
+// The far field's geometry.  Along z it runs over nzf far layers: every mesh layer, or
+// a listed subset (kmap[f] the mesh layer of far layer f, increasing); zcen, zoff, hz2 and
+// hz2_16 are indexed by far layer, tiles group runs of far layers, and cell indices and
+// near-box tests use mesh layers (nz of them).
 struct FarGeom {
   const double* xcen;
   const double* ycen;
@@ -316,9 +320,20 @@ struct FarGeom {
   const float* hy2;
   const float* hz2;
   const float* hz2_16;
-  int nx, ny, nz;
+  const int* kmap;  // null: far layer f is mesh layer f
+  int nx, ny, nz, nzf;
   int ntx, nty, ntz;
 };
+
+// Mesh layer of far layer f (never listed on meshes of cubes, whose far field keeps every
+// layer)
+template <int MESH>
+__device__ __forceinline__ int mesh_layer(const FarGeom& g, int f) {
+  if constexpr (MESH == kMeshCubes)
+    return f;
+  else
+    return g.kmap ? g.kmap[f] : f;
+}
 
 template <int MESH>
 struct TableWidth {
@@ -344,16 +359,17 @@ __device__ __forceinline__ void far_tile(const FarGeom& g, const float* __restri
     for (int jj = 0; jj < nj; ++jj) {
       const float Y = Yt - g.yoff[j0 + jj];
       const bool inxy = inx && (j0 + jj >= bx[2]) && (j0 + jj <= bx[3]);
-      const long cb = ((long)(i0 + ii) * g.ny + (j0 + jj)) * g.nz + k0;
+      const long cb = ((long)(i0 + ii) * g.ny + (j0 + jj)) * g.nz;  // the column's layer 0
       float acc[NB][NC];
 #pragma unroll
       for (int b = 0; b < NB; ++b)
 #pragma unroll
         for (int m = 0; m < NC; ++m) acc[b][m] = 0.0f;
       for (int kk = 0; kk < nk; ++kk) {
-        const bool near = inxy && (k0 + kk >= bx[4]) && (k0 + kk <= bx[5]);
+        const int km = mesh_layer<MESH>(g, k0 + kk);
+        const bool near = inxy && (km >= bx[4]) && (km <= bx[5]);
         const float Z = near ? 1.0e6f : Zt - g.zoff[k0 + kk];
-        const MpConst cc = cell_const<MESH>(shape, cb + kk, uc);
+        const MpConst cc = cell_const<MESH>(shape, cb + km, uc);
         float T[NC];
         if constexpr (MESH == kMeshCubes && HEXA)
           multipole_cube_f32<MASK>(X, Y, Z, 0.25f * cc.a4x, T);
@@ -361,7 +377,7 @@ __device__ __forceinline__ void far_tile(const FarGeom& g, const float* __restri
           multipole_f32<MASK, QUAD, HEXA>(X, Y, Z, cc, T);
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-          const float w = near ? 0.0f : wt[(cb + kk) * NB + b];
+          const float w = near ? 0.0f : wt[(cb + km) * NB + b];
 #pragma unroll
           for (int m = 0; m < NC; ++m) acc[b][m] = fmaf(w, T[m], acc[b][m]);
         }
@@ -522,11 +538,12 @@ __attribute__((amdgpu_waves_per_eu(far_fwd_waves(NB)))) void k_far_forward(
     const int tj = (int)(tt % g.nty);
     const int ti = (int)(tt / g.nty);
     const int i0 = ti * kTile, j0 = tj * kTile, k0 = tk * kTile;
-    const int ni = min(kTile, g.nx - i0), nj = min(kTile, g.ny - j0), nk = min(kTile, g.nz - k0);
+    const int ni = min(kTile, g.nx - i0), nj = min(kTile, g.ny - j0), nk = min(kTile, g.nzf - k0);
     const float Xt = (float)(px - g.xcen[i0]), Yt = (float)(py - g.ycen[j0]),
                 Zt = (float)(pz - g.zcen[k0]);
+    // near box against the mesh layers the tile's far layers span
     const bool hit = !(bx[1] < i0 || bx[0] > i0 + ni - 1 || bx[3] < j0 || bx[2] > j0 + nj - 1 ||
-                       bx[5] < k0 || bx[4] > k0 + nk - 1);
+                       bx[5] < mesh_layer<MESH>(g, k0) || bx[4] > mesh_layer<MESH>(g, k0 + nk - 1));
     // some cell of the tile closer than the second radius, q1 half-diagonals of the tile's
     // largest cell: keep the hexadecapole (the offsets to the tile's cell centres span
     // [Xt - extent, Xt] along each axis)
@@ -820,13 +837,15 @@ __attribute__((amdgpu_waves_per_eu(far_adj_waves(NB, MESH)))) void k_far_adjoint
   const int lane = threadIdx.x & 63;
   const int run = threadIdx.x - lane;
   if (t >= ntiles) return;
-  const int ntz = (g.nz + KT - 1) / KT;
+  const int ntz = (g.nzf + KT - 1) / KT;
   const int tk = (int)(t % ntz);
   const long tt = t / ntz;
   const int tj = (int)(tt % g.nty);
   const int ti = (int)(tt / g.nty);
   const int i0 = ti * kTile, j0 = tj * kTile, k0 = tk * KT;
-  const int ni = min(kTile, g.nx - i0), nj = min(kTile, g.ny - j0), nk = min(KT, g.nz - k0);
+  const int ni = min(kTile, g.nx - i0), nj = min(kTile, g.ny - j0), nk = min(KT, g.nzf - k0);
+  // mesh layers spanned by the tile's far layers (near-box tests)
+  const int kml = mesh_layer<MESH>(g, k0), kmh = mesh_layer<MESH>(g, k0 + nk - 1);
   const int ii = lane >> 4, jj = (lane >> 2) & 3, kk = lane & 3;
   const bool vij = ii < ni && jj < nj;
   const int i = i0 + (vij ? ii : 0), j = j0 + (vij ? jj : 0);
@@ -838,16 +857,17 @@ __attribute__((amdgpu_waves_per_eu(far_adj_waves(NB, MESH)))) void k_far_adjoint
   // the second radius: q1 half-diagonals of the tile's largest cell
   const float r1sq = q1sq * (g.hx2[ti] + g.hy2[tj] + g.hz2_16[tk]);
   bool vq[kQ];
-  int kq[kQ];
+  int kq[kQ];  // mesh layer of each of the lane's cells
   long cq[kQ];
   float ozq[kQ];
   MpConst ccq[kQ];
 #pragma unroll
   for (int q = 0; q < kQ; ++q) {
     vq[q] = vij && kk + q * kTile < nk;
-    kq[q] = k0 + (vq[q] ? kk + q * kTile : 0);
+    const int f = k0 + (vq[q] ? kk + q * kTile : 0);  // far layer
+    kq[q] = mesh_layer<MESH>(g, f);
     cq[q] = ((long)i * g.ny + j) * g.nz + kq[q];
-    ozq[q] = (float)(g.zcen[kq[q]] - z0);
+    ozq[q] = (float)(g.zcen[f] - z0);
     ccq[q] = cell_const<MESH>(table, cq[q], uc);
   }
   // this chunk's sites: a range of the planned partition (equal estimated cost for every
@@ -877,7 +897,7 @@ __attribute__((amdgpu_waves_per_eu(far_adj_waves(NB, MESH)))) void k_far_adjoint
       const bool nl = fmaf(dx, dx, fmaf(dy, dy, dz * dz)) < r1sq;
       const int* bl = box + sl * 6;
       const bool hl = !(bl[1] < i0 || bl[0] > i0 + ni - 1 || bl[3] < j0 ||
-                        bl[2] > j0 + nj - 1 || bl[5] < k0 || bl[4] > k0 + nk - 1);
+                        bl[2] > j0 + nj - 1 || bl[5] < kml || bl[4] > kmh);
       sitev[threadIdx.x] = make_float4(Xl, Yl, Zl, (float)((nl ? 1 : 0) | (hl ? 2 : 0)));
       if (MESH == kMeshCubes) {
 #pragma unroll
@@ -1033,13 +1053,21 @@ __global__ __launch_bounds__(64) void k_near_adjoint_gather(
   }
 }
 
+// grad = near + far chunk sums per cell; cells of layers outside a far-layer list (listed
+// non-null, listed[k] == 0) get zero, their far sums never having been formed
 __global__ __launch_bounds__(kBlock) void k_adjoint_mixed_finish(const double* __restrict__ gpart,
                                                                  int nchunks,
                                                                  const double* __restrict__ gnear,
                                                                  long n,
+                                                                 const int* __restrict__ listed,
+                                                                 int nz,
                                                                  double* __restrict__ grad) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
+  if (listed && !listed[c % nz]) {
+    grad[c] = 0.0;
+    return;
+  }
   double a = gnear[c];
   for (int q = 0; q < nchunks; ++q) a += gpart[(long)q * n + c];
   grad[c] = a;
@@ -1065,12 +1093,14 @@ FarGeom far_geom(const jgm_context* ctx) {
   g.hy2 = ctx->near.hy2.as<const float>();
   g.hz2 = ctx->near.hz2.as<const float>();
   g.hz2_16 = ctx->near.hz2_16.as<const float>();
+  g.kmap = ctx->near.far_subset ? ctx->near.kmap.as<const int>() : nullptr;
   g.nx = (int)ctx->nx;
   g.ny = (int)ctx->ny;
   g.nz = (int)ctx->nz;
+  g.nzf = ctx->near.far_subset ? ctx->near.nzf : g.nz;
   g.ntx = (g.nx + kTile - 1) / kTile;
   g.nty = (g.ny + kTile - 1) / kTile;
-  g.ntz = (g.nz + kTile - 1) / kTile;
+  g.ntz = (g.nzf + kTile - 1) / kTile;
   return g;
 }
 
@@ -1083,7 +1113,7 @@ float second_radius_q2(const jgm_context* ctx) {
 
 MpConst uniform_const(const jgm_context* ctx) {
   if (ctx->nx == 0 || ctx->ny == 0 || ctx->nz == 0) return MpConst{};
-  return mp_const(ctx->xs[0], ctx->ys[0], ctx->zs[0]);
+  return mp_const(ctx->xs[0], ctx->ys[0], ctx->near.far_subset ? ctx->near.far_dz : ctx->zs[0]);
 }
 
 // Sites sorted by the Morton code of their (x, y) position in the survey's bounding
@@ -1157,7 +1187,7 @@ __device__ __forceinline__ long long adjoint_site_cost(const double* __restrict_
                                                        const FarGeom& g, long ntiles, long tb,
                                                        int s) {
   constexpr int KT = kQ * kTile;
-  const int ntz = (g.nz + KT - 1) / KT;
+  const int ntz = (g.nzf + KT - 1) / KT;
   const long long cm = MESH == kMeshCubes ? 2 : 10, ch = MESH == kMeshCubes ? 12 : 22,
                   ck = MESH == kMeshCubes ? 13 : 25;
   long long c = 0;
@@ -1167,7 +1197,7 @@ __device__ __forceinline__ long long adjoint_site_cost(const double* __restrict_
     const long tt = t / ntz;
     const int tj = (int)(tt % g.nty), ti = (int)(tt / g.nty);
     const int i0 = ti * kTile, j0 = tj * kTile, k0 = tk * KT;
-    const int ni = min(kTile, g.nx - i0), nj = min(kTile, g.ny - j0), nk = min(KT, g.nz - k0);
+    const int ni = min(kTile, g.nx - i0), nj = min(kTile, g.ny - j0), nk = min(KT, g.nzf - k0);
     const double x0 = g.xcen[i0], y0 = g.ycen[j0], z0 = g.zcen[k0];
     const float ex = (float)(g.xcen[i0 + ni - 1] - x0), ey = (float)(g.ycen[j0 + nj - 1] - y0),
                 ez = (float)(g.zcen[k0 + nk - 1] - z0);
@@ -1178,7 +1208,7 @@ __device__ __forceinline__ long long adjoint_site_cost(const double* __restrict_
     const bool need = fmaf(dx, dx, fmaf(dy, dy, dz * dz)) < r1sq;
     const int* bl = box + s * 6;
     const bool hit = !(bl[1] < i0 || bl[0] > i0 + ni - 1 || bl[3] < j0 || bl[2] > j0 + nj - 1 ||
-                       bl[5] < k0 || bl[4] > k0 + nk - 1);
+                       bl[5] < mesh_layer<MESH>(g, k0) || bl[4] > mesh_layer<MESH>(g, k0 + nk - 1));
     c += (hit ? ck : need ? ch : cm) * (long long)(ni * nj * nk);
   }
   return c;
@@ -1254,7 +1284,7 @@ hipError_t ensure_far_plan(jgm_context* ctx) {
   if (nb.plan_valid && nb.plan_q1 == ctx->mixed_q1) return hipSuccess;
   const int S = (int)ctx->S;
   const FarGeom g = far_geom(ctx);
-  const long ntiles = (long)g.ntx * g.nty * ((g.nz + kQ * kTile - 1) / (kQ * kTile));
+  const long ntiles = (long)g.ntx * g.nty * ((g.nzf + kQ * kTile - 1) / (kQ * kTile));
   const long nblocks = (ntiles + 3) / 4;
   long C = (ctx->far_adj_waves + 4 * nblocks - 1) / std::max(1L, 4 * nblocks);
   C = std::max<long>(1, std::min<long>(C, std::max(1, S)));
@@ -1440,7 +1470,7 @@ hipError_t adjoint_mixed_impl(jgm_context* ctx, int kind, int nbm, const double*
                          ctx->coef_mono.as<float>() + (size_t)b * S * 8);
     if ((e = ensure_shape_table<MESH>(ctx)) != hipSuccess) return e;
     // the adjoint walks 4x4x(4 kQ) tiles, kQ cells per lane
-    const long ntiles8 = (long)g.ntx * g.nty * ((g.nz + kQ * kTile - 1) / (kQ * kTile));
+    const long ntiles8 = (long)g.ntx * g.nty * ((g.nzf + kQ * kTile - 1) / (kQ * kTile));
     if ((e = ensure_far_plan<MESH>(ctx)) != hipSuccess) return e;
     const int nchunks = nb.adj_chunks;
     if ((e = ctx->hpart.ensure(sizeof(double) * (size_t)nbm * nchunks * N)) != hipSuccess)
@@ -1486,7 +1516,9 @@ hipError_t adjoint_mixed_impl(jgm_context* ctx, int kind, int nbm, const double*
       }
       hipLaunchKernelGGL(k_adjoint_mixed_finish, dim3(blocks_for(N)), dim3(kBlock), 0, st,
                          ctx->hpart.as<const double>() + (size_t)b * nchunks * N, nchunks,
-                         ctx->hsum.as<const double>(), N, grad + b * grad_ld);
+                         ctx->hsum.as<const double>(), N,
+                         nb.far_subset ? nb.listed.as<const int>() : nullptr, g.nz,
+                         grad + b * grad_ld);
     }
   }
   if (with_bg && !ctx->bgthick.empty()) {
@@ -1602,12 +1634,33 @@ hipError_t build_near_boxes(jgm_context* ctx) {
   for (int k = 0; k < nz; ++k)
     reach[k] = ctx->mixed_q0 * 0.5 * std::sqrt(mx * mx + my * my + ctx->zs[k] * ctx->zs[k]);
   nb.radius = ctx->mixed_q0 * 0.5 * std::sqrt(mx * mx + my * my + mz * mz);
-  bool uniform = nx > 0 && ny > 0 && nz > 0;
+  // the far field's layers: the listed ones (increasing mesh layers), else every layer
+  const bool subset = !ctx->far_layers.empty();
+  std::vector<int> fl;
+  if (subset) {
+    fl = ctx->far_layers;
+    for (size_t f = 0; f < fl.size(); ++f)
+      if (fl[f] < 0 || fl[f] >= nz || (f > 0 && fl[f] <= fl[f - 1])) return hipErrorInvalidValue;
+  } else {
+    for (int k = 0; k < nz; ++k) fl.push_back(k);
+  }
+  const int nzf = (int)fl.size();
+  std::vector<double> zcen_f(nzf), zs_f(nzf);
+  for (int f = 0; f < nzf; ++f) {
+    zcen_f[f] = zcen[fl[f]];
+    zs_f[f] = ctx->zs[fl[f]];
+  }
+  // the cell shapes the far field sees; a listed subset never takes the cube paths, which
+  // run over every layer
+  bool uniform = nx > 0 && ny > 0 && nzf > 0;
   for (double v : ctx->xs) uniform = uniform && v == ctx->xs[0];
   for (double v : ctx->ys) uniform = uniform && v == ctx->ys[0];
-  for (double v : ctx->zs) uniform = uniform && v == ctx->zs[0];
-  const bool cubes = uniform && ctx->xs[0] == ctx->ys[0] && ctx->xs[0] == ctx->zs[0];
+  for (double v : zs_f) uniform = uniform && v == zs_f[0];
+  const bool cubes = !subset && uniform && ctx->xs[0] == ctx->ys[0] && ctx->xs[0] == zs_f[0];
   nb.mesh_class = cubes ? kMeshCubes : uniform ? kMeshUniform : kMeshGeneral;
+  nb.far_subset = subset;
+  nb.nzf = nzf;
+  nb.far_dz = nzf > 0 ? zs_f[0] : 0.0;
   // monotone centres are required by the binary searches
   bool monotone = true;
   for (int i = 1; i < nx; ++i) monotone = monotone && xcen[i] > xcen[i - 1];
@@ -1685,14 +1738,21 @@ hipError_t build_near_boxes(jgm_context* ctx) {
   if ((e = upload_vec(nb.cell_off, cell_off, st)) != hipSuccess) return e;
   if ((e = upload_vec(nb.xcen, xcen, st)) != hipSuccess) return e;
   if ((e = upload_vec(nb.ycen, ycen, st)) != hipSuccess) return e;
-  if ((e = upload_vec(nb.zcen, zcen, st)) != hipSuccess) return e;
+  if ((e = upload_vec(nb.zcen, zcen_f, st)) != hipSuccess) return e;
+  if (subset) {
+    std::vector<int> listed(nz, 0);
+    for (int k : fl) listed[k] = 1;
+    if ((e = upload_vec(nb.kmap, fl, st)) != hipSuccess ||
+        (e = upload_vec(nb.listed, listed, st)) != hipSuccess)
+      return e;
+  }
   auto run_offsets = [](const std::vector<double>& cen) {
     std::vector<float> off(cen.size());
     for (size_t i = 0; i < cen.size(); ++i) off[i] = (float)(cen[i] - cen[i / kTile * kTile]);
     return off;
   };
   const std::vector<float> xoff = run_offsets(xcen), yoff = run_offsets(ycen),
-                           zoff = run_offsets(zcen);
+                           zoff = run_offsets(zcen_f);
   // squared largest half-size of each run of `len` cells along an axis
   auto run_h2 = [](const std::vector<double>& sz, int len) {
     std::vector<float> h((sz.size() + len - 1) / len + 1, 0.0f);
@@ -1702,8 +1762,8 @@ hipError_t build_near_boxes(jgm_context* ctx) {
   };
   if ((e = upload_vec(nb.hx2, run_h2(ctx->xs, kTile), st)) != hipSuccess ||
       (e = upload_vec(nb.hy2, run_h2(ctx->ys, kTile), st)) != hipSuccess ||
-      (e = upload_vec(nb.hz2, run_h2(ctx->zs, kTile), st)) != hipSuccess ||
-      (e = upload_vec(nb.hz2_16, run_h2(ctx->zs, kQ * kTile), st)) != hipSuccess)
+      (e = upload_vec(nb.hz2, run_h2(zs_f, kTile), st)) != hipSuccess ||
+      (e = upload_vec(nb.hz2_16, run_h2(zs_f, kQ * kTile), st)) != hipSuccess)
     return e;
   if ((e = upload_vec(nb.xoff, xoff, st)) != hipSuccess) return e;
   if ((e = upload_vec(nb.yoff, yoff, st)) != hipSuccess) return e;

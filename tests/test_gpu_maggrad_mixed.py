@@ -93,6 +93,22 @@ def test_ragged_mesh_lines_and_inside(seed):
         assert np.array_equal(Gb[b], ctx.adjoint(K, nat.MIXED, M[b], R[b]))
 
 
+def test_one_metre_cells():
+    """Cells exactly 1 m thick leave no gap layer: their two slabs are adjacent layers of the
+    slab mesh, both in the far field's layer list."""
+    m, d, _ = ragged(11)
+    dz = m.GetZCellSizes().copy()
+    dz[2] = dz[3] = dz[9] = 1.0
+    m.SetZCellSizes(dz)
+    ctx = nat.Context(0)
+    ctx.configure(m, d, (0.2, 1.1, 49000.0))
+    mv = m.GetModelParameters()
+    ref = ctx.forward(K, nat.FP64, mv)
+    assert rel_inf(ctx.forward(K, nat.MIXED, mv), ref) <= TOL
+    r = np.random.default_rng(12).normal(size=ref.size)
+    assert rel_inf(ctx.adjoint(K, nat.MIXED, mv, r), ctx.adjoint(K, nat.FP64, mv, r)) <= TOL
+
+
 def test_thin_cells_run_fp64():
     m, d, _ = ragged(7, dz_min=2)
     dz = m.GetZCellSizes().copy()
