@@ -858,17 +858,18 @@ __attribute__((amdgpu_waves_per_eu(far_adj_waves(NB, MESH)))) void k_far_adjoint
   const float r1sq = q1sq * (g.hx2[ti] + g.hy2[tj] + g.hz2_16[tk]);
   bool vq[kQ];
   int kq[kQ];  // mesh layer of each of the lane's cells
-  long cq[kQ];
   float ozq[kQ];
   MpConst ccq[kQ];
+  // cell index of the lane's cell q (formed again for the output rather than held through
+  // the site loop)
+  const long cij = ((long)i * g.ny + j) * g.nz;
 #pragma unroll
   for (int q = 0; q < kQ; ++q) {
     vq[q] = vij && kk + q * kTile < nk;
     const int f = k0 + (vq[q] ? kk + q * kTile : 0);  // far layer
     kq[q] = mesh_layer<MESH>(g, f);
-    cq[q] = ((long)i * g.ny + j) * g.nz + kq[q];
     ozq[q] = (float)(g.zcen[f] - z0);
-    ccq[q] = cell_const<MESH>(table, cq[q], uc);
+    ccq[q] = cell_const<MESH>(table, cij + kq[q], uc);
   }
   // this chunk's sites: a range of the planned partition (equal estimated cost for every
   // chunk of the workgroup's four tiles)
@@ -923,7 +924,13 @@ __attribute__((amdgpu_waves_per_eu(far_adj_waves(NB, MESH)))) void k_far_adjoint
         const int* bx = box + s * 6;
         const bool inxy = i >= bx[0] && i <= bx[1] && j >= bx[2] && j <= bx[3];
 #pragma unroll
-        for (int q = 0; q < kQ; ++q) nq[q] = inxy && kq[q] >= bx[4] && kq[q] <= bx[5];
+        for (int q = 0; q < kQ; ++q) {
+          // for one misfit vector on cubes formed here instead of held through the loop
+          // (spilled there at six waves); past the mesh (vq false) the test decides nothing
+          // that is stored
+          const int km = MESH == kMeshCubes && NB == 1 ? k0 + kk + q * kTile : kq[q];
+          nq[q] = inxy && km >= bx[4] && km <= bx[5];
+        }
         far_adjoint_site<MASK, MESH, NB, true>(coef, m4, S, s, v.x, v.y, v.z, need, ox, oy, ozq,
                                                ccq, nq, acc);
       } else {
@@ -950,11 +957,13 @@ __attribute__((amdgpu_waves_per_eu(far_adj_waves(NB, MESH)))) void k_far_adjoint
   const long ncell = (long)g.nx * g.ny * g.nz;
 #pragma unroll
   for (int q = 0; q < kQ; ++q) {
-    const double gv = 2.0 * (double)table[cq[q] * TW];  // G V
+    const int km = MESH == kMeshCubes && NB == 1 ? k0 + (vq[q] ? kk + q * kTile : 0) : kq[q];
+    const long cq = ((long)i * g.ny + j) * g.nz + km;
+    const double gv = 2.0 * (double)table[cq * TW];  // G V
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       a64[b][q] += (double)acc[b][q];
-      if (vq[q]) gpart[((long)b * gridDim.y + blockIdx.y) * ncell + cq[q]] = gv * a64[b][q];
+      if (vq[q]) gpart[((long)b * gridDim.y + blockIdx.y) * ncell + cq] = gv * a64[b][q];
     }
   }
 }
