@@ -297,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm(const float* __restrict__ J,
   const int b0 = blockIdx.y * 64;
   const long c0 = (long)split * per;
   const long cend = c0 + per < colpad ? c0 + per : colpad;
-  const int nchunk = cend > c0 ? (int)((cend - c0) / kChunk) : 0;  // even
+  const int nchunk = cend > c0 ? (int)((cend - c0) / kChunk) : 0;
   const float* Jw = J + row0 + li;
   const float* Mw = Mt + b0 + li;
   f32x16 acc[2][2];
@@ -1428,7 +1428,7 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm_f64(const double* __restrict
   const int b0 = blockIdx.y * 64;
   const long c0 = (long)split * per;
   const long cend = c0 + per < colpad ? c0 + per : colpad;
-  const int nchunk = cend > c0 ? (int)((cend - c0) / kChunk) : 0;  // even
+  const int nchunk = cend > c0 ? (int)((cend - c0) / kChunk) : 0;
   const double* Jw = J + row0 + li;
   const double* Mw = Mt + b0 + li;
   f64x4 acc[4][4];
