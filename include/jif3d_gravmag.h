@@ -52,7 +52,9 @@ typedef enum jgm_status {
  * three values per cell, [Mx; My; Mz], B = 1e9 mu0/(4 pi G) T M (OMPMagnetizationImp,
  * jif3D/Magnetics/OMPMagnetizationImp.cpp:25-117; no background, model length
  * 3 ncells; JGM_MIXED evaluates the tensor far field once per site-cell pair for the
- * three components).  The vertical-gradient kind runs in JGM_FP64 only. */
+ * three components).  JGM_MIXED evaluates the vertical gradient as the Bz of 1 m slabs
+ * just above the mesh planes (the difference of the two fields, exactly), with FP64 for
+ * sites on mesh lines; a mesh with a cell thinner than 1 m runs it in JGM_FP64. */
 typedef enum jgm_kind {
   JGM_KIND_SCALAR = 1,
   JGM_KIND_TENSOR = 2,
