@@ -309,10 +309,11 @@ hipError_t launch_maggrad_fp64_sites(jgm_context* ctx, const double* model, int 
 hipError_t launch_maggrad_fp64_sites_add(jgm_context* ctx, int F, const double* fx,
                                          const double* fy, const double* fz, const int* idx,
                                          const double* rm, double* grad);
-// mixed tensor sums (S x 6: xx, yy, zz, xy, xz, yz, carrying G and the signs) of one
-// density model, and the mixed tensor gradient of a raw tensor misfit (9 per site)
-hipError_t launch_tensor_sums_mixed(jgm_context* ctx, const double* rho, double* sums);
-hipError_t launch_tensor_adjoint_mixed(jgm_context* ctx, const double* rmt, double* grad);
+// mixed sums of the tensor's z column (S x 3: zz, xz, yz, carrying G and the signs) of one
+// density model, and their mixed gradient for a raw tensor misfit (9 per site, only its
+// zx, zy, zz entries reaching them); meshes of cubes excepted (hipErrorNotSupported)
+hipError_t launch_zcol_sums_mixed(jgm_context* ctx, const double* rho, double* sums);
+hipError_t launch_zcol_adjoint_mixed(jgm_context* ctx, const double* rmt, double* grad);
 // the vertical-gradient kind in mixed precision through its slab mesh (maggrad_mixed.hip);
 // FP64 when a cell is thinner than 1 m (maggrad_mixed_slabs false)
 hipError_t launch_forward_maggrad_mixed(jgm_context* ctx, const double* model, double* out,

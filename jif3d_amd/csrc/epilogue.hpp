@@ -118,6 +118,7 @@ __device__ inline void site_coefficients(const double* __restrict__ rm, int s, i
 template <int MASK>
 __host__ __device__ inline void select_components(const double* c7, double* out) {
   int k = 0;
+  if (MASK & kZcol) { out[k++] = c7[3]; out[k++] = c7[5]; out[k++] = c7[6]; }
   if (MASK & kGz) out[k++] = c7[0];
   if (MASK & kDiag) { out[k++] = c7[1]; out[k++] = c7[2]; out[k++] = c7[3]; }
   if (MASK & kOff) { out[k++] = c7[4]; out[k++] = c7[5]; out[k++] = c7[6]; }
@@ -128,6 +129,7 @@ template <int MASK>
 __host__ __device__ inline void spread_components(const double* in, double* u7) {
   int k = 0;
   for (int m = 0; m < 7; ++m) u7[m] = 0.0;
+  if (MASK & kZcol) { u7[3] = in[k++]; u7[5] = in[k++]; u7[6] = in[k++]; }
   if (MASK & kGz) u7[0] = in[k++];
   if (MASK & kDiag) { u7[1] = in[k++]; u7[2] = in[k++]; u7[3] = in[k++]; }
   if (MASK & kOff) { u7[4] = in[k++]; u7[5] = in[k++]; u7[6] = in[k++]; }

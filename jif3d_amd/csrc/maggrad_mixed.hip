@@ -9,9 +9,9 @@
 // (z = Z_K) carries the susceptibility of the cell ending there minus that of the cell
 // starting there.  The vertical gradient of the mesh is therefore the Bz of a second mesh
 // of nz + 1 slab layers, separated by zero-weight gap layers (Z_K, Z_{K+1} - 1), and that
-// mesh goes through the ordinary mixed tensor path (FP64 near boxes, FP32 multipole far
-// field) of a child context holding its geometry, whose far field runs over the slab layers
-// only.  Nothing cancels beyond the slabs' separation, so the far field's FP32 error stays
+// mesh goes through the mixed path (FP64 near boxes, FP32 multipole far field) for the
+// tensor's z column (Uzz, Uxz, Uyz, the terms Bz needs) of a child context holding its
+// geometry, whose far field runs over the slab layers only.  Nothing cancels beyond the slabs' separation, so the far field's FP32 error stays
 // at the magnetic kind's level.
 //
 // The slabs need cells at least 1 m thick; on thinner cells the call runs the FP64 path.
@@ -53,16 +53,16 @@ __global__ __launch_bounds__(kBlock) void k_slab_weights(const double* __restric
   rho_v[v] = w;
 }
 
-// out[s] = F 1e-7 / G (Uxz bx + Uyz by + Uzz bz) of the slab mesh's tensor sums (tensor
-// order xx, yy, zz, xy, xz, yz; the sums carry G and the component signs)
+// out[s] = F 1e-7 / G (Uxz bx + Uyz by + Uzz bz) of the slab mesh's z-column sums (order
+// zz, xz, yz; the sums carry G and the component signs)
 __global__ __launch_bounds__(kBlock) void k_slab_finish(const double* __restrict__ sums, int S,
                                                         double bx, double by, double bz,
                                                         double scale, double* __restrict__ out,
                                                         double* __restrict__ raw_out) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S) return;
-  const double* u = sums + (long)s * 6;
-  const double v = scale * (u[4] * bx + u[5] * by + u[2] * bz);
+  const double* u = sums + (long)s * 3;
+  const double v = scale * (u[1] * bx + u[2] * by + u[0] * bz);
   out[s] = v;
   if (raw_out) raw_out[s] = v;
 }
@@ -219,7 +219,7 @@ hipError_t launch_forward_maggrad_mixed(jgm_context* ctx, const double* model, d
   const long ncol = (long)ctx->nx * ctx->ny;
   const long Nv = ncol * (long)sm.nzv;
   if ((e = sm.rho.ensure(sizeof(double) * (size_t)std::max<long>(1, Nv))) != hipSuccess ||
-      (e = sm.sums.ensure(sizeof(double) * 6 * (size_t)std::max(1, S))) != hipSuccess)
+      (e = sm.sums.ensure(sizeof(double) * 3 * (size_t)std::max(1, S))) != hipSuccess)
     return e;
   double bx, by, bz;
   field_unit(ctx, &bx, &by, &bz);
@@ -229,7 +229,7 @@ hipError_t launch_forward_maggrad_mixed(jgm_context* ctx, const double* model, d
                        sm.plane_of.as<const int>(), ncol, (int)ctx->nz, (int)sm.nzv,
                        sm.rho.as<double>());
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = launch_tensor_sums_mixed(c, sm.rho.as<const double>(), sm.sums.as<double>())) !=
+    if ((e = launch_zcol_sums_mixed(c, sm.rho.as<const double>(), sm.sums.as<double>())) !=
         hipSuccess)
       return e;
     hipLaunchKernelGGL(k_slab_finish, dim3(blocks_for(S)), dim3(kBlock), 0, ctx->stream,
@@ -281,7 +281,7 @@ hipError_t launch_adjoint_maggrad_mixed(jgm_context* ctx, const double* rm, doub
     hipLaunchKernelGGL(k_slab_tensor_misfit, dim3(blocks_for(9L * S)), dim3(kBlock), 0,
                        ctx->stream, r, S, bx, by, bz, slab_scale(ctx), sm.rmt.as<double>());
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = launch_tensor_adjoint_mixed(c, sm.rmt.as<const double>(), sm.gv.as<double>())) !=
+    if ((e = launch_zcol_adjoint_mixed(c, sm.rmt.as<const double>(), sm.gv.as<double>())) !=
         hipSuccess)
       return e;
     hipLaunchKernelGGL(k_slab_gradient, dim3(blocks_for(N)), dim3(kBlock), 0, ctx->stream,

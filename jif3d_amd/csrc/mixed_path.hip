@@ -198,6 +198,27 @@ __device__ __forceinline__ void multipole_f32(float X, float Y, float Z, const M
     T[k++] = (ux * uz) * cxz;
     T[k++] = (uy * uz) * cyz;
   }
+  if constexpr ((MASK & kZcol) != 0) {
+    // Uzz, Uxz, Uyz by the same operations as the two branches above
+    float pz = P, oz = base, cxz = P, cyz = P;
+    if constexpr (HEXA) {
+      pz = fmaf(hz, -18.0f * s7, pz);
+      oz = fmaf(fmaf(2.0f, tz, hz), s7, oz);
+      const float m9 = -9.0f * s7, f2 = 2.0f * s7;
+      cxz = fmaf(hx + hz, m9, fmaf(c.b2xz, f2, P));
+      cyz = fmaf(hy + hz, m9, fmaf(c.b2yz, f2, P));
+    }
+    if constexpr (QUAD) {
+      pz = fmaf(c.Bz, -20.0f * s5, pz);
+      oz = fmaf(c.Bz, t2, oz);
+      const float m10 = -10.0f * s5;
+      cxz = fmaf(c.Bx + c.Bz, m10, cxz);
+      cyz = fmaf(c.By + c.Bz, m10, cyz);
+    }
+    T[k++] = fmaf(vz, pz, oz);
+    T[k++] = (ux * uz) * cxz;
+    T[k++] = (uy * uz) * cyz;
+  }
 }
 
 // multipole_f32 for a cube (no quadrupole) with the hexadecapole.  The cube's fourth-moment
@@ -506,14 +527,20 @@ __device__ __forceinline__ void far_tile_mono(const FarGeom& g, const float* __r
 // three waves, no scratch at all, measured within 3 % either way).  Four misfit vectors on
 // a general mesh spilled ~300 bytes per lane at four waves: two (~190 VGPRs, no scratch) took
 // config 1's batch of four from 1.21 to 1.03 ms (on config 2's uniform mesh two waves were
-// 4.5 % slower than four, which spill 40 bytes).
-constexpr int far_fwd_waves(int nb) { return nb == 1 ? 6 : nb == 2 ? 4 : 1; }
-constexpr int far_adj_waves(int nb, int mesh) {
-  return nb == 1 ? (mesh == kMeshCubes ? 6 : 4) : nb == 4 ? (mesh == kMeshGeneral ? 2 : 4) : 1;
+// 4.5 % slower than four, which spill 40 bytes).  The z column alone on a uniform mesh (the
+// vertical gradient's slabs) needs fewer registers: eight waves forward (59 VGPRs; 3.88 ->
+// 3.68 ms at 128x128 columns of 65 slabs, 4096 sites), six adjoint (80; 3.91 -> 3.87 ms).
+constexpr int far_fwd_waves(int nb, int mesh, int mask) {
+  return nb == 1 ? (mask == kMaskZcol && mesh == kMeshUniform ? 8 : 6) : nb == 2 ? 4 : 1;
+}
+constexpr int far_adj_waves(int nb, int mesh, int mask) {
+  return nb == 1 ? (mesh == kMeshCubes || (mask == kMaskZcol && mesh == kMeshUniform) ? 6 : 4)
+         : nb == 4 ? (mesh == kMeshGeneral ? 2 : 4)
+                   : 1;
 }
 template <int MASK, int MESH, int NB>
 __global__ __launch_bounds__(kBlock)
-__attribute__((amdgpu_waves_per_eu(far_fwd_waves(NB)))) void k_far_forward(
+__attribute__((amdgpu_waves_per_eu(far_fwd_waves(NB, MESH, MASK)))) void k_far_forward(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     int S, const int* __restrict__ order, const int* __restrict__ box, float q1sq, FarGeom g,
     const float* __restrict__ shape, const float* __restrict__ wt, MpConst uc, long ntiles,
@@ -787,12 +814,16 @@ __device__ __forceinline__ void far_adjoint_site(
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {
       float T[NC];
-      if (MESH == kMeshCubes && need)
-        multipole_cube_f32<MASK>(X, Y, Zq[q], 0.25f * ccq[q].a4x, T);
-      else if (need)
+      if constexpr (MESH == kMeshCubes) {
+        if (need)
+          multipole_cube_f32<MASK>(X, Y, Zq[q], 0.25f * ccq[q].a4x, T);
+        else
+          multipole_f32<MASK, QUAD, false>(X, Y, Zq[q], ccq[q], T);
+      } else if (need) {
         multipole_f32<MASK, QUAD, true>(X, Y, Zq[q], ccq[q], T);
-      else
+      } else {
         multipole_f32<MASK, QUAD, false>(X, Y, Zq[q], ccq[q], T);
+      }
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
         const float* cs = coef + ((long)b * S + s) * NC;
@@ -822,7 +853,7 @@ __device__ __forceinline__ void far_adjoint_site(
 // sums at gpart[(b nchunks + chunk) ncell + c].
 template <int MASK, int MESH, int NB>
 __global__ __launch_bounds__(kBlock)
-__attribute__((amdgpu_waves_per_eu(far_adj_waves(NB, MESH)))) void k_far_adjoint(
+__attribute__((amdgpu_waves_per_eu(far_adj_waves(NB, MESH, MASK)))) void k_far_adjoint(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     const float* __restrict__ coef, const float* __restrict__ mono, const int* __restrict__ box,
     float q1sq, int S, int nchunks, const int* __restrict__ ranges, FarGeom g,
@@ -1175,9 +1206,11 @@ const char* far_name(int mask, bool adjoint) {
   if (adjoint)
     return mask == kMaskScalar ? "far_adjoint_fp32_gz"
            : mask == kMaskAll  ? "far_adjoint_fp32_gz_ftg"
+           : mask == kMaskZcol ? "far_adjoint_fp32_ftg_zcol"
                                : "far_adjoint_fp32_ftg";
   return mask == kMaskScalar ? "far_forward_fp32_gz"
          : mask == kMaskAll  ? "far_forward_fp32_gz_ftg"
+         : mask == kMaskZcol ? "far_forward_fp32_ftg_zcol"
                              : "far_forward_fp32_ftg";
 }
 
@@ -1539,14 +1572,18 @@ hipError_t adjoint_mixed_impl(jgm_context* ctx, int kind, int nbm, const double*
   return hipGetLastError();
 }
 
+// The z-column mask serves far-layer subsets only, which never take the cube paths.
 template <int MASK>
 hipError_t forward_dispatch(jgm_context* ctx, int kind, int nbm, const double* models,
                             size_t model_ld, double* out, size_t out_ld, double* raw_out,
                             size_t raw_ld, double* sums = nullptr) {
   switch (ctx->near.mesh_class) {
     case kMeshCubes:
-      return forward_mixed_impl<MASK, kMeshCubes>(ctx, kind, nbm, models, model_ld, out, out_ld,
-                                                  raw_out, raw_ld, sums);
+      if constexpr (MASK == kMaskZcol)
+        return hipErrorNotSupported;
+      else
+        return forward_mixed_impl<MASK, kMeshCubes>(ctx, kind, nbm, models, model_ld, out,
+                                                    out_ld, raw_out, raw_ld, sums);
     case kMeshUniform:
       return forward_mixed_impl<MASK, kMeshUniform>(ctx, kind, nbm, models, model_ld, out, out_ld,
                                                     raw_out, raw_ld, sums);
@@ -1561,8 +1598,11 @@ hipError_t adjoint_dispatch(jgm_context* ctx, int kind, int nbm, const double* r
                             double* grad, size_t grad_ld, bool with_bg = true) {
   switch (ctx->near.mesh_class) {
     case kMeshCubes:
-      return adjoint_mixed_impl<MASK, kMeshCubes>(ctx, kind, nbm, rm, rm_ld, grad, grad_ld,
-                                                  with_bg);
+      if constexpr (MASK == kMaskZcol)
+        return hipErrorNotSupported;
+      else
+        return adjoint_mixed_impl<MASK, kMeshCubes>(ctx, kind, nbm, rm, rm_ld, grad, grad_ld,
+                                                    with_bg);
     case kMeshUniform:
       return adjoint_mixed_impl<MASK, kMeshUniform>(ctx, kind, nbm, rm, rm_ld, grad, grad_ld,
                                                     with_bg);
@@ -1827,22 +1867,22 @@ hipError_t route_line_sites_forward(jgm_context* ctx, int kind, size_t nmodels,
 }
 }  // namespace
 
-hipError_t launch_tensor_sums_mixed(jgm_context* ctx, const double* rho, double* sums) {
+hipError_t launch_zcol_sums_mixed(jgm_context* ctx, const double* rho, double* sums) {
   if (!ctx->near.valid) {
     hipError_t e = build_near_boxes(ctx);
     if (e != hipSuccess) return e;
   }
-  return forward_dispatch<kMaskTensor>(ctx, JGM_KIND_TENSOR, 1, rho, ncells(ctx), nullptr, 0,
-                                       nullptr, 0, sums);
+  return forward_dispatch<kMaskZcol>(ctx, JGM_KIND_TENSOR, 1, rho, ncells(ctx), nullptr, 0,
+                                     nullptr, 0, sums);
 }
 
-hipError_t launch_tensor_adjoint_mixed(jgm_context* ctx, const double* rmt, double* grad) {
+hipError_t launch_zcol_adjoint_mixed(jgm_context* ctx, const double* rmt, double* grad) {
   if (!ctx->near.valid) {
     hipError_t e = build_near_boxes(ctx);
     if (e != hipSuccess) return e;
   }
-  return adjoint_dispatch<kMaskTensor>(ctx, JGM_KIND_TENSOR, 1, rmt, 9 * ctx->S, grad,
-                                       ncells(ctx), false);
+  return adjoint_dispatch<kMaskZcol>(ctx, JGM_KIND_TENSOR, 1, rmt, 9 * ctx->S, grad,
+                                     ncells(ctx), false);
 }
 
 hipError_t launch_forward_mixed(jgm_context* ctx, int kind, const double* model, double* out,

@@ -25,15 +25,18 @@ constexpr double kGravConst = 6.67428e-11;  // jif3D/Gravity/BasicGravElements.h
 constexpr double kPi = 3.14159265358979323846;
 
 // Component masks of the corner function.  Component order inside kernels:
-// gz, Uxx, Uyy, Uzz, Uxy, Uxz, Uyz.
-enum : int { kGz = 1, kDiag = 2, kOff = 4 };
+// gz, Uxx, Uyy, Uzz, Uxy, Uxz, Uyz.  kZcol alone (never combined with the others) is the
+// tensor's z column, Uzz, Uxz, Uyz: the terms the vertical magnetic gradient needs.
+enum : int { kGz = 1, kDiag = 2, kOff = 4, kZcol = 8 };
 constexpr int kMaskScalar = kGz;
 constexpr int kMaskTensor = kDiag | kOff;
 constexpr int kMaskAll = kGz | kDiag | kOff;
+constexpr int kMaskZcol = kZcol;
 
 template <int MASK>
 struct Ncomp {
-  static constexpr int value = ((MASK & kGz) ? 1 : 0) + ((MASK & kDiag) ? 3 : 0) + ((MASK & kOff) ? 3 : 0);
+  static constexpr int value = ((MASK & kGz) ? 1 : 0) + ((MASK & kDiag) ? 3 : 0) +
+                               ((MASK & kOff) ? 3 : 0) + ((MASK & kZcol) ? 3 : 0);
 };
 
 // Natural logarithm of a finite a >= DBL_EPSILON (the guarded log arguments of the
@@ -163,6 +166,16 @@ __device__ __forceinline__ void diag_terms_f64(double x, double y, double z, dou
 template <int MASK>
 __device__ __forceinline__ void corner_terms_f64(double x, double y, double z, double* t) {
   const double r = sqrt(x * x + y * y + z * z);
+  if constexpr (MASK == kMaskZcol) {
+    // Uzz = atan2(x y, z r), Uxz = log(max(y + r, eps)), Uyz = log(max(x + r, eps)); the
+    // atan2 term evaluated directly (the tensor mask may derive it from its two others)
+    t[0] = (fmin(fmin(fabs(x), fabs(y)), fabs(z)) >= 1e-140 && r >= DBL_EPSILON && r <= 1e140)
+               ? atan2_f64(x * y, z * r)
+               : atan2(x * y, z * r);
+    t[1] = log_pos_f64(fmax(y + r, DBL_EPSILON));
+    t[2] = log_pos_f64(fmax(x + r, DBL_EPSILON));
+    return;
+  }
   int k = 0;
   if constexpr ((MASK & kOff) != 0) {
     const double lz = log_pos_f64(fmax(z + r, DBL_EPSILON));
@@ -214,6 +227,7 @@ __device__ __forceinline__ void corner_terms_f64(double x, double y, double z, d
 template <int MASK>
 __host__ __device__ inline void comp_scales(double* s) {
   int k = 0;
+  if (MASK & kZcol) { s[k++] = kGravConst; s[k++] = -kGravConst; s[k++] = -kGravConst; }
   if (MASK & kGz) s[k++] = -kGravConst;
   if (MASK & kDiag) { s[k++] = kGravConst; s[k++] = kGravConst; s[k++] = kGravConst; }
   if (MASK & kOff) { s[k++] = -kGravConst; s[k++] = -kGravConst; s[k++] = -kGravConst; }
