@@ -1557,9 +1557,9 @@ hipError_t run_jm_f64(jgm_context* ctx, const double* Xt, int P, int nrhs, doubl
   const long colpad = (long)ctx->storedJ_colpad;
   const int rhs_tiles = P / 64;
   const int row_tiles = (int)((ld + 255) / 256);
-  // one round of workgroups (one per CU at 1 wave per SIMD): on 7168 x 65536, 9 splits
-  // run 1.02 ms against 1.12 ms for 16 (448 workgroups: a second pass of workgroups over the
-  // CUs, three quarters full)
+  // one pass of workgroups over the CUs (one per CU at 1 wave per SIMD): on 7168 x 65536,
+  // 9 splits run 1.02 ms against 1.12 ms for 16 (448 workgroups: a second pass three
+  // quarters full)
   long nsplit = std::max<long>(1, (long)ctx->cu_count / ((long)row_tiles * rhs_tiles));
   nsplit = std::min<long>(nsplit, colpad / 32);
   const long per = round_up((colpad + nsplit - 1) / nsplit, 2 * kChunk);
