@@ -222,7 +222,7 @@ jgm_status jgm_destroy(jgm_context* ctx) {
                                &ctx->coef, &ctx->hpart, &ctx->hsum, &ctx->model_stage,
                                &ctx->out_stage, &ctx->misfit_stage, &ctx->grad_stage,
                                &ctx->farpart, &ctx->nearpart, &ctx->coef32, &ctx->storedJ, &ctx->storedJ_vec,
-                               &ctx->storedJ_bf16, &ctx->storedJ_bf16c,
+                               &ctx->storedJ_bf16, &ctx->storedJ_bf16c, &ctx->storedJ_rhs16lo,
                                &ctx->near.col_ptr, &ctx->near.col_sites, &ctx->near.col_ids,
                                &ctx->near.col_kmax, &ctx->near.xcen, &ctx->near.ycen,
                                &ctx->near.zcen, &ctx->near.adj_table, &ctx->near.corner_off,

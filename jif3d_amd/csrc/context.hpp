@@ -253,6 +253,9 @@ struct jgm_context {
   // J^T R, made and dropped like the copy above (same switch)
   jgm::DeviceBuffer storedJ_bf16c, storedJ_rhs16;  // the copy; its BF16 right-hand sides
   size_t storedJ_bf16c_gen = 0, storedJ_bf16c_failed_gen = 0;
+  // the low BF16 parts of the right-hand sides of the BF16-refined J M (storedJ_rhs16 the
+  // high parts), made per call
+  jgm::DeviceBuffer storedJ_rhs16lo;
 };
 
 namespace jgm {

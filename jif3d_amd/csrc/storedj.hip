@@ -281,11 +281,22 @@ struct JmFrag {
 constexpr int kJmStages = 4;
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
-template <int PREC>
+// The BF16-refined product may take M already split (MPRE: its high and low BF16 parts Mh,
+// Ml in the [k / 8][P] layout of k_rows_to_bf16_hilo, two 16-byte loads per tile and chunk
+// instead of sixteen 4-byte loads and their conversions); the same values and MFMA order.
+struct JmFragPre {
+  float a[2][8];
+  bf16x8 bh[2], bl[2];
+};
+
+template <int PREC, bool MPRE = false>
 __global__ __launch_bounds__(kBlock) void k_gemm_jm(const float* __restrict__ J, long ld, long R,
                                                     long colpad, const float* __restrict__ Mt,
-                                                    int P, long per, int row_tiles, int nsplit,
+                                                    const bf16x8* __restrict__ Mh,
+                                                    const bf16x8* __restrict__ Ml, int P, long per,
+                                                    int row_tiles, int nsplit,
                                                     float* __restrict__ Yp) {
+  static_assert(!MPRE || PREC == JGM_GEMM_BF16_REFINED, "split M: the refined product only");
   const int w = blockIdx.x;
   int split, rt;
   xcd_item(w, nsplit * row_tiles, row_tiles, split, rt);
@@ -316,16 +327,29 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm(const float* __restrict__ J,
       for (int r = 0; r < 16; ++r) tot[ta][tb][r] = 0.0f;
   // a ring of kJmStages chunks in registers: chunk q is loaded while q - kJmStages + 1 is
   // multiplied, so kJmStages - 1 chunks of loads are in flight behind each MFMA block
-  JmFrag f[kJmStages];
-  auto load = [&](JmFrag& g, int q) {
+  using Frag = std::conditional_t<MPRE, JmFragPre, JmFrag>;
+  Frag f[kJmStages];
+  auto load = [&](Frag& g, int q) {
     const long cb = c0 + (long)(q < nchunk ? q : nchunk - 1) * kChunk;  // past the end: unused
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
       const long cell = cb + chunk_k<PREC>(s, lk);
       g.a[0][s] = __builtin_nontemporal_load(Jw + cell * ld);
       g.a[1][s] = __builtin_nontemporal_load(Jw + cell * ld + 32);
-      g.b[0][s] = Mw[cell * P];
-      g.b[1][s] = Mw[cell * P + 32];
+      if constexpr (!MPRE) {
+        g.b[0][s] = Mw[cell * P];
+        g.b[1][s] = Mw[cell * P + 32];
+      }
+    }
+    if constexpr (MPRE) {
+      // the lane's eight cells cb + 8 lk .. cb + 8 lk + 7 (k = 8 lk + s) of right-hand
+      // sides b0 + li and b0 + li + 32: one group of the [k / 8][P] copies each
+      const long grp = (cb / 8 + lk) * P + b0 + li;
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        g.bh[t] = Mh[grp + 32 * t];
+        g.bl[t] = Ml[grp + 32 * t];
+      }
     }
   };
   // sched_barrier keeps the scheduler from sinking the loads ahead down to their uses,
@@ -339,7 +363,22 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm(const float* __restrict__ J,
         load(f[(d + kJmStages - 1) % kJmStages], q + d + kJmStages - 1);
         __builtin_amdgcn_sched_barrier(0);
         if (q + d < nchunk) {
-          mma_block<PREC>(f[d].a, f[d].b, acc);
+          if constexpr (MPRE) {
+            // mma_block<BF16_REFINED> with M's parts loaded instead of formed
+            bf16x8 ah[2], al[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) to_bf16<true>(f[d].a[t], ah[t], al[t]);
+#pragma unroll
+            for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+              for (int tb = 0; tb < 2; ++tb) {
+                acc[ta][tb] = mfma_bf16(ah[ta], f[d].bh[tb], acc[ta][tb]);
+                acc[ta][tb] = mfma_bf16(ah[ta], f[d].bl[tb], acc[ta][tb]);
+                acc[ta][tb] = mfma_bf16(al[ta], f[d].bh[tb], acc[ta][tb]);
+              }
+          } else {
+            mma_block<PREC>(f[d].a, f[d].b, acc);
+          }
           if (((q + d + 1) & 15) == 0) flush(acc, tot);  // every 256 cells
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -556,6 +595,25 @@ __global__ __launch_bounds__(kBlock) void k_rows_to_bf16(const float* __restrict
   Rb[idx] = v;
 }
 
+// k_rows_to_bf16 with the low parts too: Rh the rows rounded to BF16, Rl the rounded
+// remainders (to_bf16<true>)
+__global__ __launch_bounds__(kBlock) void k_rows_to_bf16_hilo(const float* __restrict__ Rt,
+                                                              long ngroups, int P,
+                                                              bf16x8* __restrict__ Rh,
+                                                              bf16x8* __restrict__ Rl) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ngroups * P) return;
+  const long g = idx / P;
+  const int b = (int)(idx - g * P);
+  float x[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) x[e] = Rt[(8 * g + e) * P + b];
+  bf16x8 h, l;
+  to_bf16<true>(x, h, l);
+  Rh[idx] = h;
+  Rl[idx] = l;
+}
+
 // BF16 G^T = Rt^T J from the tiled BF16 copy of J (k_pack_bf16_tiles): workgroup = 256 cells (wave: 64
 // cells) x 64 right-hand sides, no LDS.  For each 16-row k-step a lane's eight rows of its
 // cell (B operand, k = 8 (lane / 32) + e) and of its right-hand side (A operand) are one
@@ -673,13 +731,23 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jm_reduce(const float* __restri
 // G^T = Rt^T J over all rows: workgroup = 256 cells (wave: 64 cells) x 64 right-hand
 // sides.  A operand = Rt stage (i = right-hand side, k = row), B operand = J stage
 // (k = row, j = cell), both read from LDS.
-template <int PREC>
+// MPRE (the refined product only): R already split into its BF16 parts Rhi, Rlo in the
+// [k / 8][P] layout of k_rows_to_bf16_hilo; a stage holds them in the R area of LDS as
+// [group][rhs] (high parts, then low parts), a lane's A operand for a tile is one 16-byte
+// read of each, and only J is split in registers; the same values and MFMA order.
+template <int PREC, bool MPRE = false>
 __global__ __launch_bounds__(kBlock) void k_gemm_jtr(const float* __restrict__ J, long ld, long N,
                                                      long nblk, const float* __restrict__ Rt,
-                                                     int P, float* __restrict__ G, long ldg,
-                                                     int nrhs) {
+                                                     const bf16x8* __restrict__ Rhi,
+                                                     const bf16x8* __restrict__ Rlo, int P,
+                                                     float* __restrict__ G, long ldg, int nrhs) {
+  static_assert(!MPRE || PREC == JGM_GEMM_BF16_REFINED, "split R: the refined product only");
+  static_assert(kStage * 64 * sizeof(float) == 2 * (kStage / 8) * 64 * sizeof(bf16x8),
+                "the split R stage fills the R area");
   __shared__ float jt[kStage * kPitch];                           // [row][cell]
   __shared__ __attribute__((aligned(16))) float rt[kStage * 64];  // [row][rhs]
+  bf16x8* const rth = reinterpret_cast<bf16x8*>(rt);              // MPRE: [group][rhs]
+  bf16x8* const rtl = rth + (kStage / 8) * 64;
   const int b0 = blockIdx.y * 64;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -702,8 +770,16 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jtr(const float* __restrict__ J
 #pragma unroll
       for (int q = 0; q < 8; ++q)
         pj[q] = *(const f32x4*)(Jl + (long)q * 32 * ld + r0);
+      if constexpr (MPRE) {
+        // thread t: group t / 64 of the stage's four, right-hand side b0 + t % 64; the two
+        // 16-byte parts travel in pr as raw bits
+        const long gi = (r0 / 8 + (tid >> 6)) * P + b0 + (tid & 63);
+        pr[0] = __builtin_bit_cast(f32x4, Rhi[gi]);
+        pr[1] = __builtin_bit_cast(f32x4, Rlo[gi]);
+      } else {
 #pragma unroll
-      for (int h = 0; h < 2; ++h) pr[h] = *(const f32x4*)(Rl + (r0 + 16 * h) * P);
+        for (int h = 0; h < 2; ++h) pr[h] = *(const f32x4*)(Rl + (r0 + 16 * h) * P);
+      }
     };
     f32x16 acc[2][2];  // [rhs tile][cell tile]
     float tot[2][2][16];
@@ -724,8 +800,13 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jtr(const float* __restrict__ J
         d[2 * kPitch] = pj[q][2];
         d[3 * kPitch] = pj[q][3];
       }
+      if constexpr (MPRE) {
+        rth[tid] = __builtin_bit_cast(bf16x8, pr[0]);
+        rtl[tid] = __builtin_bit_cast(bf16x8, pr[1]);
+      } else {
 #pragma unroll
-      for (int h = 0; h < 2; ++h) *(f32x4*)(rt + (rrow + 16 * h) * 64 + rpart * 4) = pr[h];
+        for (int h = 0; h < 2; ++h) *(f32x4*)(rt + (rrow + 16 * h) * 64 + rpart * 4) = pr[h];
+      }
       __syncthreads();
       gload(st + 2, pj, pr);
       __builtin_amdgcn_sched_barrier(0);
@@ -735,12 +816,35 @@ __global__ __launch_bounds__(kBlock) void k_gemm_jtr(const float* __restrict__ J
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
           const int row = 16 * half + chunk_k<PREC>(s, lk);
-          a[0][s] = rt[row * 64 + li];
-          a[1][s] = rt[row * 64 + 32 + li];
+          if constexpr (!MPRE) {
+            a[0][s] = rt[row * 64 + li];
+            a[1][s] = rt[row * 64 + 32 + li];
+          }
           b[0][s] = jt[row * kPitch + 64 * wave + li];
           b[1][s] = jt[row * kPitch + 64 * wave + 32 + li];
         }
-        mma_block<PREC>(a, b, acc);
+        if constexpr (MPRE) {
+          // mma_block<BF16_REFINED> with R's parts read instead of formed: the lane's rows
+          // 16 half + 8 lk + s are group 2 half + lk
+          const int g = 2 * half + lk;
+          bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+          for (int t = 0; t < 2; ++t) {
+            ah[t] = rth[g * 64 + 32 * t + li];
+            al[t] = rtl[g * 64 + 32 * t + li];
+            to_bf16<true>(b[t], bh[t], bl[t]);
+          }
+#pragma unroll
+          for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+            for (int tb = 0; tb < 2; ++tb) {
+              acc[ta][tb] = mfma_bf16(ah[ta], bh[tb], acc[ta][tb]);
+              acc[ta][tb] = mfma_bf16(ah[ta], bl[tb], acc[ta][tb]);
+              acc[ta][tb] = mfma_bf16(al[ta], bh[tb], acc[ta][tb]);
+            }
+        } else {
+          mma_block<PREC>(a, b, acc);
+        }
       }
       if ((st & 7) == 7) flush(acc, tot);  // every 256 rows
     };
@@ -1301,6 +1405,17 @@ hipError_t run_jm(jgm_context* ctx, int gemm_precision, const float* Xt, int P, 
       hipLaunchKernelGGL(k_rows_to_bf16, dim3(blocks_for(ncols / 8 * P)), dim3(kBlock), 0, st, Mt,
                          ncols / 8, P, ctx->storedJ_rhs16.as<bf16x8>());
     }
+    // the refined product takes M split into its BF16 parts once per block instead of in
+    // every workgroup that reads it
+    const bool mpre = gemm_precision == JGM_GEMM_BF16_REFINED;
+    if (mpre) {
+      if ((e2 = ctx->storedJ_rhs16.ensure(sizeof(__bf16) * (size_t)ncols * P)) != hipSuccess ||
+          (e2 = ctx->storedJ_rhs16lo.ensure(sizeof(__bf16) * (size_t)ncols * P)) != hipSuccess)
+        return e2;
+      hipLaunchKernelGGL(k_rows_to_bf16_hilo, dim3(blocks_for(ncols / 8 * P)), dim3(kBlock), 0, st,
+                         Mt, ncols / 8, P, ctx->storedJ_rhs16.as<bf16x8>(),
+                         ctx->storedJ_rhs16lo.as<bf16x8>());
+    }
     {
       // the packed BF16 product is profiled under its own name, so that timings know
       // whether 2 or 4 bytes of J were read per element
@@ -1309,8 +1424,9 @@ hipError_t run_jm(jgm_context* ctx, int gemm_precision, const float* Xt, int P, 
                                                                     : "storedj_jm_bf16")
                                                           : "storedj_jm_bf16_refined");
 #define JGM_JM(PR)                                                                        \
-  hipLaunchKernelGGL((k_gemm_jm<PR>), grid, dim3(kBlock), 0, st, J, ld, R, ncols, Mt, P, per, \
-                     row_tiles, (int)nsplit, ctx->partial.as<float>())
+  hipLaunchKernelGGL((k_gemm_jm<PR>), grid, dim3(kBlock), 0, st, J, ld, R, ncols, Mt,         \
+                     (const bf16x8*)nullptr, (const bf16x8*)nullptr, P, per, row_tiles,      \
+                     (int)nsplit, ctx->partial.as<float>())
       switch (gemm_precision) {
         case JGM_GEMM_FP32: JGM_JM(JGM_GEMM_FP32); break;
         case JGM_GEMM_BF16:
@@ -1322,7 +1438,12 @@ hipError_t run_jm(jgm_context* ctx, int gemm_precision, const float* Xt, int P, 
           else
             JGM_JM(JGM_GEMM_BF16);
           break;
-        default: JGM_JM(JGM_GEMM_BF16_REFINED); break;
+        default:
+          hipLaunchKernelGGL((k_gemm_jm<JGM_GEMM_BF16_REFINED, true>), grid, dim3(kBlock), 0, st,
+                             J, ld, R, ncols, Mt, ctx->storedJ_rhs16.as<const bf16x8>(),
+                             ctx->storedJ_rhs16lo.as<const bf16x8>(), P, per, row_tiles,
+                             (int)nsplit, ctx->partial.as<float>());
+          break;
       }
 #undef JGM_JM
     }
@@ -1361,6 +1482,17 @@ hipError_t run_jtr(jgm_context* ctx, int gemm_precision, const float* Xt, int P,
                        ctx->storedJ_rhs16.as<const bf16x8>(), P, out, N, nrhs);
     return hipGetLastError();
   }
+  // the refined product takes R split into its BF16 parts once instead of in every
+  // workgroup (R is the same for every block of J)
+  if (gemm_precision == JGM_GEMM_BF16_REFINED) {
+    hipError_t e;
+    if ((e = ctx->storedJ_rhs16.ensure(sizeof(__bf16) * (size_t)ld * P)) != hipSuccess ||
+        (e = ctx->storedJ_rhs16lo.ensure(sizeof(__bf16) * (size_t)ld * P)) != hipSuccess)
+      return e;
+    hipLaunchKernelGGL(k_rows_to_bf16_hilo, dim3(blocks_for(ld / 8 * P)), dim3(kBlock), 0, st, Xt,
+                       ld / 8, P, ctx->storedJ_rhs16.as<bf16x8>(),
+                       ctx->storedJ_rhs16lo.as<bf16x8>());
+  }
   return for_each_block(ctx, [&](const float* J, long c0, long ncols) -> hipError_t {
     const long nblk = ncols / kColPad;
     const long nvalid = std::max<long>(0, std::min(ncols, N - c0));
@@ -1369,13 +1501,14 @@ hipError_t run_jtr(jgm_context* ctx, int gemm_precision, const float* Xt, int P,
     ProfScope ps(ctx, gemm_precision == JGM_GEMM_FP32   ? "storedj_jtr_fp32"
                       : gemm_precision == JGM_GEMM_BF16 ? "storedj_jtr_bf16"
                                                         : "storedj_jtr_bf16_refined");
-#define JGM_JTR(PR)                                                                          \
-  hipLaunchKernelGGL((k_gemm_jtr<PR>), grid, dim3(kBlock), 0, st, J, ld, nvalid, nblk, Xt, P, \
-                     out + c0, N, nrhs)
+#define JGM_JTR(PR, PRE)                                                                     \
+  hipLaunchKernelGGL((k_gemm_jtr<PR, PRE>), grid, dim3(kBlock), 0, st, J, ld, nvalid, nblk, Xt, \
+                     ctx->storedJ_rhs16.as<const bf16x8>(),                                 \
+                     ctx->storedJ_rhs16lo.as<const bf16x8>(), P, out + c0, N, nrhs)
     switch (gemm_precision) {
-      case JGM_GEMM_FP32: JGM_JTR(JGM_GEMM_FP32); break;
-      case JGM_GEMM_BF16: JGM_JTR(JGM_GEMM_BF16); break;
-      default: JGM_JTR(JGM_GEMM_BF16_REFINED); break;
+      case JGM_GEMM_FP32: JGM_JTR(JGM_GEMM_FP32, false); break;
+      case JGM_GEMM_BF16: JGM_JTR(JGM_GEMM_BF16, false); break;
+      default: JGM_JTR(JGM_GEMM_BF16_REFINED, true); break;
     }
 #undef JGM_JTR
     return hipGetLastError();
